@@ -1,0 +1,641 @@
+"""MI355X (gfx950) backend for Autoscoper's 2D-3D registration cost path.
+
+Python binding of the C ABI declared in include/autoscoper_hip.h
+(libautoscoper_hip.so). It mirrors the reference's xromm::gpu operator
+surface (libautoscoper/src/gpu/opencl/*.hpp): RayCaster, RadRenderer,
+BackgroundRenderer, the four filters, ncc, multiply/drr_background,
+merge/composite, plus the batched scene evaluator and the optimizers.
+
+The library must be built (python -m autoscoper_amd.build); importing the
+native layer without it raises, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+__all__ = [
+    "lib", "AscFilter", "AscCamera", "AscVolumeDesc", "AscOptimizeOptions", "AscOptimizeStats",
+    "Context", "Volume", "Image", "Scene", "DeviceArray", "filters", "geometry",
+    "STORAGE_U16", "STORAGE_F16", "STORAGE_BF16", "STORAGE_F32",
+    "FILTER_CONTRAST", "FILTER_SOBEL", "FILTER_MEDIAN", "FILTER_GAUSSIAN", "FILTER_SHARPEN",
+    "AscError",
+]
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libautoscoper_hip.so"
+
+STORAGE_U16, STORAGE_F16, STORAGE_BF16, STORAGE_F32 = 0, 1, 2, 3
+FILTER_CONTRAST, FILTER_SOBEL, FILTER_MEDIAN, FILTER_GAUSSIAN, FILTER_SHARPEN = 0, 1, 2, 3, 4
+
+
+class AscError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"asc error {code}: {msg}")
+        self.code = code
+
+
+class AscFilter(C.Structure):
+    _fields_ = [("type", C.c_int), ("enabled", C.c_int), ("p", C.c_float * 4)]
+
+
+class AscCamera(C.Structure):
+    _fields_ = [
+        ("rotation", C.c_double * 9), ("translation", C.c_double * 3),
+        ("image_plane", C.c_double * 12), ("viewport", C.c_double * 4), ("size", C.c_double * 2),
+    ]
+
+    def to_array(self) -> np.ndarray:
+        return np.frombuffer(bytes(self), dtype=np.float64).copy()
+
+    @classmethod
+    def from_array(cls, a) -> "AscCamera":
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size == 30
+        return cls.from_buffer_copy(a.tobytes())
+
+
+class AscVolumeDesc(C.Structure):
+    _fields_ = [
+        ("dims", C.c_int * 3), ("crop_min", C.c_int * 3), ("crop_max", C.c_int * 3),
+        ("inv_scale", C.c_float * 3), ("inv_trans", C.c_float * 3), ("trans_center", C.c_double * 3),
+        ("min_value", C.c_float), ("max_value", C.c_float), ("storage", C.c_int), ("reserved", C.c_int),
+    ]
+
+
+class AscOptimizeOptions(C.Structure):
+    _fields_ = [
+        ("method", C.c_int), ("max_iter", C.c_uint), ("max_stall", C.c_uint),
+        ("min_limit", C.c_double), ("max_limit", C.c_double), ("offsets", C.c_double * 6),
+        ("seed", C.c_uint), ("particles", C.c_int), ("synchronous", C.c_int), ("volume", C.c_int),
+    ]
+
+
+class AscOptimizeStats(C.Structure):
+    _fields_ = [
+        ("pso_epochs", C.c_int), ("simplex_iterations", C.c_int), ("pso_best_cost", C.c_double),
+        ("final_cost", C.c_double), ("initial_cost", C.c_double), ("evaluations", C.c_int64),
+    ]
+
+
+_vp = C.c_void_p
+_i = C.c_int
+_u = C.c_uint
+_f = C.c_float
+_d = C.c_double
+_sz = C.c_size_t
+_P = C.POINTER
+
+# name -> argtypes (restype is int unless listed in _RES)
+_SIGS = {
+    "asc_version": [],
+    "asc_device_count": [_P(_i)],
+    "asc_context_create": [_i, _P(_vp)],
+    "asc_context_destroy": [_vp],
+    "asc_context_set_stream": [_vp, _vp],
+    "asc_context_stream": [_vp],
+    "asc_synchronize": [_vp],
+    "asc_malloc": [_vp, _sz, _P(_vp)],
+    "asc_free": [_vp, _vp],
+    "asc_memcpy_h2d": [_vp, _vp, _vp, _sz],
+    "asc_memcpy_d2h": [_vp, _vp, _vp, _sz],
+    "asc_memcpy_d2d": [_vp, _vp, _vp, _sz],
+    "asc_fill_f32": [_vp, _vp, _f, _sz],
+    "asc_fill_u8": [_vp, _vp, C.c_ubyte, _sz],
+    "asc_volume_create": [_vp, _vp, _i, _i, _i, _i, _P(_i), _P(_d), _i, _P(_vp), _P(AscVolumeDesc)],
+    "asc_volume_describe": [_vp, _P(AscVolumeDesc)],
+    "asc_volume_destroy": [_vp],
+    "asc_volume_bytes": [_vp, _P(_sz)],
+    "asc_image_create": [_vp, _vp, _i, _i, _i, _P(_vp)],
+    "asc_image_update": [_vp, _vp],
+    "asc_image_destroy": [_vp],
+    "asc_frame_from_xyzypr": [_P(_d), _P(_d)],
+    "asc_frame_from_xyz_axis_angle": [_P(_d), _P(_d)],
+    "asc_frame_to_xyzypr": [_P(_d), _P(_d)],
+    "asc_frame_mul": [_P(_d), _P(_d), _P(_d)],
+    "asc_frame_inverse": [_P(_d), _P(_d)],
+    "asc_camera_from_mayacam2": [_P(_d), _P(_d), _P(_d), _P(_d), _P(AscCamera)],
+    "asc_inv_model_view": [_P(_d), _P(_f), _P(_f), _P(_f)],
+    "asc_view_geometry": [_P(AscCamera), _P(_d), _P(AscVolumeDesc), _i, _i, _P(_f), _P(_d), _P(_u)],
+    "asc_drr_render": [_vp, _vp, _P(_f), _P(_f), _f, _f, _f, _vp, _u, _u],
+    "asc_rad_render": [_vp, _vp, _P(_f), _P(_f), _vp, _u, _u],
+    "asc_background_render": [_vp, _vp, _P(_f), _P(_f), _f, _vp, _u, _u],
+    "asc_sobel": [_vp, _vp, _vp, _i, _i, _f, _f],
+    "asc_contrast": [_vp, _vp, _vp, _i, _i, _f, _f, _i],
+    "asc_gaussian_weights": [_f, _P(_f), _i, _P(_i)],
+    "asc_gaussian": [_vp, _vp, _vp, _i, _i, _f],
+    "asc_sharpen": [_vp, _vp, _vp, _i, _i, _f, _f, _f],
+    "asc_filter_apply": [_vp, _P(AscFilter), _vp, _vp, _i, _i],
+    "asc_filter_chain": [_vp, _P(AscFilter), _i, _vp, _vp, _vp, _i, _i],
+    "asc_drr_background": [_vp, _vp, _vp, _u, _u],
+    "asc_multiply": [_vp, _vp, _vp, _vp, _u, _u],
+    "asc_mask_multiply": [_vp, _vp, _vp, _vp, _vp, _u, _u],
+    "asc_ncc": [_vp, _vp, _vp, _vp, _u, _P(_d), _P(_d)],
+    "asc_ncc_init": [_vp, _u],
+    "asc_ncc_deinit": [_vp],
+    "asc_merge": [_vp, _vp, _vp, _vp, _u, _u],
+    "asc_composite": [_vp, _vp, _vp, _vp, _vp, _vp, _u, _u],
+    "asc_scene_create": [_vp, _i, _i, _P(_vp)],
+    "asc_scene_destroy": [_vp],
+    "asc_scene_add_volume": [_vp, _vp, _f, _f, _f, _P(_d), _P(_i)],
+    "asc_scene_volume_pivot": [_vp, _i, _P(_d)],
+    "asc_scene_set_drr_params": [_vp, _i, _f, _f, _f],
+    "asc_scene_add_camera": [_vp, _P(AscCamera), _vp, _P(_i)],
+    "asc_scene_set_rad": [_vp, _i, _vp],
+    "asc_scene_set_background": [_vp, _i, _vp, _f],
+    "asc_scene_set_filters": [_vp, _i, _i, _P(AscFilter), _i],
+    "asc_scene_eval_poses": [_vp, _P(_i), _P(_d), _i, _P(_d), _P(_d)],
+    "asc_scene_eval_manips": [_vp, _P(_i), _P(_f), _P(_d), _i, _P(_d), _P(_d)],
+    "asc_scene_use_graph": [_vp, _i],
+    "asc_scene_set_debug": [_vp, _i],
+    "asc_scene_last_info": [_vp, _P(_i), _i, _P(C.c_int64)],
+    "asc_scene_set_timing": [_vp, _i],
+    "asc_scene_last_timing": [_vp, _P(_f)],
+    "asc_scene_last_moments": [_vp, _P(_d), _i],
+    "asc_scene_last_images": [_vp, _i, _i, _P(_f), _P(_f), _P(_f)],
+    "asc_scene_track_frame_plain": [_vp, _i, _P(_d), _P(_d), _P(_d)],
+    "asc_pose_store_set": [_P(_f), _P(_d)],
+    "asc_pose_store_get": [_P(_f), _P(_d)],
+    "asc_compose_pose": [_P(_f), _P(_d), _P(_d), _P(_d)],
+    "asc_optimize_frame": [_vp, _P(_f), _P(AscOptimizeOptions), _P(AscOptimizeStats)],
+}
+_RES = {"asc_last_error": C.c_char_p, "asc_context_stream": _vp}
+
+_LIB = None
+
+
+def lib():
+    """Load libautoscoper_hip.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        path = Path(os.environ.get("ASC_LIB", LIB_PATH))
+        if not path.exists():
+            raise RuntimeError(f"{path} is missing: build it with `python -m autoscoper_amd.build`")
+        L = C.CDLL(str(path))
+        L.asc_last_error.restype = C.c_char_p
+        L.asc_last_error.argtypes = []
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = _RES.get(name, C.c_int)
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols() -> list[str]:
+    return ["asc_last_error", *_SIGS.keys()]
+
+
+def _check(rc: int) -> int:
+    if rc < 0:
+        raise AscError(rc, lib().asc_last_error().decode())
+    return rc
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(lib().asc_device_count(C.byref(n)))
+    return n.value
+
+
+# ---------------------------------------------------------------------------
+# geometry (host, double precision)
+# ---------------------------------------------------------------------------
+class geometry:  # namespace
+    @staticmethod
+    def frame_from_xyzypr(p) -> np.ndarray:
+        p = _f64(p)
+        out = np.zeros(12)
+        _check(lib().asc_frame_from_xyzypr(_ptr(p, _d), _ptr(out, _d)))
+        return out
+
+    @staticmethod
+    def frame_from_xyz_axis_angle(p) -> np.ndarray:
+        p = _f64(p)
+        out = np.zeros(12)
+        _check(lib().asc_frame_from_xyz_axis_angle(_ptr(p, _d), _ptr(out, _d)))
+        return out
+
+    @staticmethod
+    def frame_to_xyzypr(f) -> np.ndarray:
+        f = _f64(f)
+        out = np.zeros(6)
+        _check(lib().asc_frame_to_xyzypr(_ptr(f, _d), _ptr(out, _d)))
+        return out
+
+    @staticmethod
+    def frame_mul(a, b) -> np.ndarray:
+        a, b = _f64(a), _f64(b)
+        out = np.zeros(12)
+        _check(lib().asc_frame_mul(_ptr(a, _d), _ptr(b, _d), _ptr(out, _d)))
+        return out
+
+    @staticmethod
+    def frame_inverse(a) -> np.ndarray:
+        a = _f64(a)
+        out = np.zeros(12)
+        _check(lib().asc_frame_inverse(_ptr(a, _d), _ptr(out, _d)))
+        return out
+
+    @staticmethod
+    def camera_from_mayacam2(K, R, t, size) -> AscCamera:
+        K, R, t, size = _f64(K).ravel(), _f64(R).ravel(), _f64(t).ravel(), _f64(size).ravel()
+        cam = AscCamera()
+        _check(lib().asc_camera_from_mayacam2(_ptr(K, _d), _ptr(R, _d), _ptr(t, _d), _ptr(size, _d),
+                                              C.byref(cam)))
+        return cam
+
+    @staticmethod
+    def inv_model_view(imv16, inv_scale, inv_trans) -> np.ndarray:
+        imv16, s, t = _f64(imv16), _f32(inv_scale), _f32(inv_trans)
+        out = np.zeros(16, np.float32)
+        _check(lib().asc_inv_model_view(_ptr(imv16, _d), _ptr(s, _f), _ptr(t, _f), _ptr(out, _f)))
+        return out
+
+    @staticmethod
+    def view_geometry(cam: AscCamera, xyzypr, desc: AscVolumeDesc, rw: int, rh: int):
+        """-> (in_view, imv[12] float32, viewport[4] float64, (w, h))"""
+        p = _f64(xyzypr)
+        imv = np.zeros(12, np.float32)
+        vp = np.zeros(4)
+        sz = np.zeros(2, np.uint32)
+        rc = _check(lib().asc_view_geometry(C.byref(cam), _ptr(p, _d), C.byref(desc), rw, rh, _ptr(imv, _f),
+                                            _ptr(vp, _d), _ptr(sz, _u)))
+        return rc == 1, imv, vp, (int(sz[0]), int(sz[1]))
+
+    @staticmethod
+    def pose_store_set(xyzypr) -> np.ndarray:
+        p = _f64(xyzypr)
+        s = np.zeros(7, np.float32)
+        _check(lib().asc_pose_store_set(_ptr(s, _f), _ptr(p, _d)))
+        return s
+
+    @staticmethod
+    def pose_store_get(store) -> np.ndarray:
+        s = _f32(store)
+        p = np.zeros(6)
+        _check(lib().asc_pose_store_get(_ptr(s, _f), _ptr(p, _d)))
+        return p
+
+    @staticmethod
+    def compose_pose(store, pivot, manip) -> np.ndarray:
+        s, pv, m = _f32(store), _f64(pivot), _f64(manip)
+        out = np.zeros(6)
+        _check(lib().asc_compose_pose(_ptr(s, _f), _ptr(pv, _d), _ptr(m, _d), _ptr(out, _d)))
+        return out
+
+
+def gaussian_weights(radius: float) -> np.ndarray:
+    n = C.c_int(0)
+    _check(lib().asc_gaussian_weights(radius, None, 0, C.byref(n)))
+    out = np.zeros(n.value * n.value, np.float32)
+    _check(lib().asc_gaussian_weights(radius, _ptr(out, _f), out.size, C.byref(n)))
+    return out.reshape(n.value, n.value)
+
+
+class filters:  # namespace: filter records
+    @staticmethod
+    def sobel(scale=1.0, blend=0.5, enabled=True) -> AscFilter:
+        return AscFilter(FILTER_SOBEL, int(enabled), (C.c_float * 4)(scale, blend, 0, 0))
+
+    @staticmethod
+    def contrast(alpha=1.0, beta=1.0, size=3, enabled=True) -> AscFilter:
+        return AscFilter(FILTER_CONTRAST, int(enabled), (C.c_float * 4)(alpha, beta, size, 0))
+
+    @staticmethod
+    def gaussian(radius=1.0, enabled=True) -> AscFilter:
+        return AscFilter(FILTER_GAUSSIAN, int(enabled), (C.c_float * 4)(radius, 0, 0, 0))
+
+    @staticmethod
+    def sharpen(radius=1.0, contrast=1.0, threshold=0.0, enabled=True) -> AscFilter:
+        return AscFilter(FILTER_SHARPEN, int(enabled), (C.c_float * 4)(radius, contrast, threshold, 0))
+
+    @staticmethod
+    def to_records(fs) -> np.ndarray:
+        """6-float records {type, enabled, p0..p3} (the oracle's layout)."""
+        out = np.zeros((len(fs), 6), np.float32)
+        for k, f in enumerate(fs):
+            out[k] = [f.type, f.enabled, *f.p]
+        return out
+
+
+def _filter_array(fs):
+    arr = (AscFilter * max(1, len(fs)))(*fs)
+    return arr
+
+
+# ---------------------------------------------------------------------------
+# device objects
+# ---------------------------------------------------------------------------
+class DeviceArray:
+    """Device buffer owned by a Context (Buffer in the reference facade)."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check(lib().asc_malloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    @property
+    def addr(self) -> int:
+        return self.ptr.value or 0
+
+    def upload(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(lib().asc_memcpy_h2d(self.ctx.handle, self.ptr, a.ctypes.data_as(_vp), a.nbytes))
+        return self
+
+    def download(self, shape, dtype=np.float32) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib().asc_memcpy_d2h(self.ctx.handle, out.ctypes.data_as(_vp), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr and self.ptr.value:
+            lib().asc_free(self.ctx.handle, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().asc_context_create(device, C.byref(h)))
+        self.handle = h
+
+    def set_stream(self, stream_ptr: int | None):
+        _check(lib().asc_context_set_stream(self.handle, C.c_void_p(stream_ptr or 0)))
+
+    @property
+    def stream(self) -> int:
+        return lib().asc_context_stream(self.handle) or 0
+
+    def synchronize(self):
+        _check(lib().asc_synchronize(self.handle))
+
+    def alloc(self, nbytes: int) -> DeviceArray:
+        return DeviceArray(self, nbytes)
+
+    def to_device(self, a: np.ndarray) -> DeviceArray:
+        a = np.ascontiguousarray(a)
+        return DeviceArray(self, max(a.nbytes, 4)).upload(a)
+
+    def close(self):
+        if self.handle:
+            lib().asc_context_destroy(self.handle)
+            self.handle = None
+
+    # -- per-operator calls on device arrays (raw pointers) --
+    def drr_render(self, vol: "Volume", imv, viewport, step, intensity, cutoff, out: DeviceArray, w, h):
+        imv, vp = _f32(imv), _f32(viewport)
+        _check(lib().asc_drr_render(self.handle, vol.handle, _ptr(imv, _f), _ptr(vp, _f), step, intensity, cutoff,
+                                    out.ptr, w, h))
+
+    def rad_render(self, img: "Image", plane, viewport, out: DeviceArray, w, h):
+        pl, vp = _f32(plane), _f32(viewport)
+        _check(lib().asc_rad_render(self.handle, img.handle, _ptr(pl, _f), _ptr(vp, _f), out.ptr, w, h))
+
+    def background_render(self, img: "Image", plane, viewport, threshold, out: DeviceArray, w, h):
+        pl, vp = _f32(plane), _f32(viewport)
+        _check(lib().asc_background_render(self.handle, img.handle, _ptr(pl, _f), _ptr(vp, _f), threshold,
+                                           out.ptr, w, h))
+
+    def filter_apply(self, f: AscFilter, src: DeviceArray, dst: DeviceArray, w, h):
+        _check(lib().asc_filter_apply(self.handle, C.byref(f), src.ptr, dst.ptr, w, h))
+
+    def filter_chain(self, fs, src: DeviceArray, dst: DeviceArray, scratch: DeviceArray | None, w, h):
+        arr = _filter_array(fs)
+        _check(lib().asc_filter_chain(self.handle, arr, len(fs), src.ptr, dst.ptr,
+                                      scratch.ptr if scratch else None, w, h))
+
+    def drr_background(self, src, dst, w, h):
+        _check(lib().asc_drr_background(self.handle, src.ptr, dst.ptr, w, h))
+
+    def multiply(self, a, b, dst, w, h):
+        _check(lib().asc_multiply(self.handle, a.ptr, b.ptr, dst.ptr, w, h))
+
+    def merge(self, a, b, dst, w, h):
+        _check(lib().asc_merge(self.handle, a.ptr, b.ptr, dst.ptr, w, h))
+
+    def composite(self, s1, s2, s3, s4, dst, w, h):
+        _check(lib().asc_composite(self.handle, s1.ptr, s2.ptr, s3.ptr, s4.ptr if s4 else None, dst.ptr, w, h))
+
+    def mask_multiply(self, drr, rad, bg, mask, w, h):
+        _check(lib().asc_mask_multiply(self.handle, drr.ptr, rad.ptr, bg.ptr if bg else None,
+                                       mask.ptr if mask else None, w, h))
+
+    def ncc(self, f: DeviceArray, g: DeviceArray, m: DeviceArray, n: int):
+        """-> (ncc, moments[9])"""
+        v = C.c_double(0)
+        mom = np.zeros(9)
+        _check(lib().asc_ncc(self.handle, f.ptr, g.ptr, m.ptr, n, C.byref(v), _ptr(mom, _d)))
+        return v.value, mom
+
+
+class Volume:
+    def __init__(self, ctx: Context, data: np.ndarray, scale=(1.0, 1.0, 1.0), flip=(0, 0, 0),
+                 storage: int = STORAGE_U16):
+        """data: (depth, height, width) uint8/uint16 array, x fastest."""
+        data = np.ascontiguousarray(data)
+        assert data.ndim == 3 and data.dtype in (np.uint8, np.uint16)
+        bps = 8 * data.dtype.itemsize
+        d, h, w = data.shape
+        fl = np.asarray(flip, np.int32)
+        sc = _f64(scale)
+        self.desc = AscVolumeDesc()
+        hnd = C.c_void_p()
+        _check(lib().asc_volume_create(ctx.handle, data.ctypes.data_as(_vp), w, h, d, bps, _ptr(fl, _i),
+                                       _ptr(sc, _d), storage, C.byref(hnd), C.byref(self.desc)))
+        self.handle = hnd
+        self.ctx = ctx
+        self.scale = tuple(float(s) for s in scale)
+
+    @property
+    def nbytes(self) -> int:
+        n = C.c_size_t(0)
+        _check(lib().asc_volume_bytes(self.handle, C.byref(n)))
+        return n.value
+
+    def close(self):
+        if self.handle:
+            lib().asc_volume_destroy(self.handle)
+            self.handle = None
+
+
+class Image:
+    def __init__(self, ctx: Context, data: np.ndarray):
+        """2-D radiograph/background: uint8, uint16 (UNORM) or float32."""
+        data = np.ascontiguousarray(data)
+        bps = {np.dtype(np.uint8): 8, np.dtype(np.uint16): 16, np.dtype(np.float32): 32}[data.dtype]
+        h, w = data.shape
+        hnd = C.c_void_p()
+        _check(lib().asc_image_create(ctx.handle, data.ctypes.data_as(_vp), w, h, bps, C.byref(hnd)))
+        self.handle = hnd
+        self.shape = (h, w)
+        self.dtype = data.dtype
+
+    def update(self, data: np.ndarray):
+        data = np.ascontiguousarray(data, dtype=self.dtype)
+        assert data.shape == self.shape
+        _check(lib().asc_image_update(self.handle, data.ctypes.data_as(_vp)))
+
+    def close(self):
+        if self.handle:
+            lib().asc_image_destroy(self.handle)
+            self.handle = None
+
+
+class Scene:
+    """Tracker::load state + batched evaluation (Tracker.cpp:244-335, 583-827)."""
+
+    def __init__(self, ctx: Context, render_width: int, render_height: int):
+        h = C.c_void_p()
+        _check(lib().asc_scene_create(ctx.handle, render_width, render_height, C.byref(h)))
+        self.handle = h
+        self.ctx = ctx
+        self.ncam = 0
+        self.nvol = 0
+        self.render = (render_width, render_height)
+        self._keep = []
+
+    def add_volume(self, vol: Volume, step: float, intensity: float, cutoff: float, pivot=None) -> int:
+        idx = C.c_int()
+        pv = None if pivot is None else _f64(pivot)
+        _check(lib().asc_scene_add_volume(self.handle, vol.handle, step, intensity, cutoff,
+                                          None if pv is None else _ptr(pv, _d), C.byref(idx)))
+        self._keep.append(vol)
+        self.nvol += 1
+        return idx.value
+
+    def set_drr_params(self, volume: int, step: float, intensity: float, cutoff: float):
+        _check(lib().asc_scene_set_drr_params(self.handle, volume, step, intensity, cutoff))
+
+    def pivot(self, volume: int) -> np.ndarray:
+        out = np.zeros(12)
+        _check(lib().asc_scene_volume_pivot(self.handle, volume, _ptr(out, _d)))
+        return out
+
+    def add_camera(self, cam: AscCamera, rad: Image) -> int:
+        idx = C.c_int()
+        _check(lib().asc_scene_add_camera(self.handle, C.byref(cam), rad.handle, C.byref(idx)))
+        self._keep.append(rad)
+        self.ncam += 1
+        return idx.value
+
+    def set_rad(self, camera: int, rad: Image):
+        _check(lib().asc_scene_set_rad(self.handle, camera, rad.handle))
+        self._keep.append(rad)
+
+    def set_background(self, camera: int, bg: Image | None, threshold: float):
+        _check(lib().asc_scene_set_background(self.handle, camera, bg.handle if bg else None, threshold))
+        if bg:
+            self._keep.append(bg)
+
+    def set_filters(self, camera: int, which: int, fs):
+        arr = _filter_array(fs)
+        _check(lib().asc_scene_set_filters(self.handle, camera, which, arr, len(fs)))
+
+    def use_graph(self, enable: bool):
+        _check(lib().asc_scene_use_graph(self.handle, int(enable)))
+
+    def set_debug(self, enable: bool):
+        _check(lib().asc_scene_set_debug(self.handle, int(enable)))
+
+    def eval_poses(self, poses, volumes=None):
+        """poses (n, 6) xyzypr -> (costs[n], per_camera[n, ncam])"""
+        p = _f64(poses).reshape(-1, 6)
+        n = p.shape[0]
+        vols = np.zeros(n, np.int32) if volumes is None else np.ascontiguousarray(volumes, np.int32)
+        costs = np.zeros(n)
+        per = np.zeros((n, self.ncam))
+        _check(lib().asc_scene_eval_poses(self.handle, _ptr(vols, _i), _ptr(p, _d), n, _ptr(costs, _d),
+                                          _ptr(per, _d)))
+        return costs, per
+
+    def eval_manips(self, stores, manips, volumes=None):
+        m = _f64(manips).reshape(-1, 6)
+        n = m.shape[0]
+        s = _f32(stores).reshape(-1, 7)
+        if s.shape[0] == 1 and n > 1:
+            s = np.ascontiguousarray(np.repeat(s, n, axis=0))
+        vols = np.zeros(n, np.int32) if volumes is None else np.ascontiguousarray(volumes, np.int32)
+        costs = np.zeros(n)
+        per = np.zeros((n, self.ncam))
+        _check(lib().asc_scene_eval_manips(self.handle, _ptr(vols, _i), _ptr(s, _f), _ptr(m, _d), n,
+                                           _ptr(costs, _d), _ptr(per, _d)))
+        return costs, per
+
+    def set_timing(self, enable: bool):
+        _check(lib().asc_scene_set_timing(self.handle, int(enable)))
+
+    def last_timing(self) -> np.ndarray:
+        """ms of {ray march, filters+moments, finalize} of the last eager batch"""
+        ms = np.zeros(3, np.float32)
+        _check(lib().asc_scene_last_timing(self.handle, _ptr(ms, _f)))
+        return ms
+
+    def last_info(self, n_items: int):
+        info = np.zeros((n_items, 8), np.int32)
+        samples = C.c_int64(0)
+        _check(lib().asc_scene_last_info(self.handle, _ptr(info, _i), info.size, C.byref(samples)))
+        return info, samples.value
+
+    def last_moments(self, n_items: int) -> np.ndarray:
+        m = np.zeros((n_items, 10))
+        _check(lib().asc_scene_last_moments(self.handle, _ptr(m, _d), m.size))
+        return m
+
+    def last_images(self, pose: int, camera: int, w: int, h: int):
+        d = np.zeros((h, w), np.float32)
+        r = np.zeros((h, w), np.float32)
+        m = np.zeros((h, w), np.float32)
+        _check(lib().asc_scene_last_images(self.handle, pose, camera, _ptr(d, _f), _ptr(r, _f), _ptr(m, _f)))
+        return d, r, m
+
+    def track_frame_plain(self, pose, volume: int = 0):
+        p = _f64(pose)
+        per = np.zeros(self.ncam)
+        cost = C.c_double(0)
+        _check(lib().asc_scene_track_frame_plain(self.handle, volume, _ptr(p, _d), _ptr(per, _d), C.byref(cost)))
+        return cost.value, per
+
+    def optimize_frame(self, store, method=0, max_iter=100, max_stall=25, min_limit=-3.0, max_limit=3.0,
+                       offsets=(0.1,) * 6, seed=1, particles=100, synchronous=False, volume=0):
+        s = _f32(store).copy()
+        o = AscOptimizeOptions()
+        o.method, o.max_iter, o.max_stall = method, max_iter, max_stall
+        o.min_limit, o.max_limit = min_limit, max_limit
+        for k in range(6):
+            o.offsets[k] = offsets[k]
+        o.seed, o.particles, o.synchronous, o.volume = seed, particles, int(synchronous), volume
+        st = AscOptimizeStats()
+        _check(lib().asc_optimize_frame(self.handle, _ptr(s, _f), C.byref(o), C.byref(st)))
+        return s, st
+
+    def close(self):
+        if self.handle:
+            lib().asc_scene_destroy(self.handle)
+            self.handle = None

@@ -1,0 +1,92 @@
+"""Build recipe for libautoscoper_hip.so (gfx950) and the CPU oracle.
+
+    python -m autoscoper_amd.build          # build both
+    python -m autoscoper_amd.build --lib    # product library only
+
+HIP sources are compiled by hipcc for gfx950 with:
+  -ffp-contract=off                        no implicit FMA: expressions keep
+                                           the reference's IEEE float order
+  -fhip-fp32-correctly-rounded-divide-sqrt IEEE division and sqrt
+  -Xclang -target-feature -Xclang -packed-fp32-ops
+                                           no packed fp32 instructions
+Plain C++ host sources are compiled by g++ with -ffp-contract=off.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+BUILD = ROOT / "_build"
+LIB = PKG / "libautoscoper_hip.so"
+ORACLE_DIR = ROOT / "oracle"
+
+ARCH = os.environ.get("ASC_OFFLOAD_ARCH", "gfx950")
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fvisibility=hidden", "-Wall", "-Wno-unused-function",
+]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+             "-fvisibility=hidden", "-Wall", "-Wno-unused-function"]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+
+
+def _compile(src: Path) -> Path:
+    obj = BUILD / (src.name + ".o")
+    deps = [src] + sorted(CSRC.glob("*.hpp")) + [ROOT / "include" / "autoscoper_hip.h"]
+    deps += sorted((ROOT / "include").glob("*.hpp"))
+    if obj.exists() and all(obj.stat().st_mtime >= d.stat().st_mtime for d in deps):
+        return obj
+    if src.suffix == ".hip":
+        cmd = [HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)]
+    else:
+        cmd = ["g++", *CXX_FLAGS, f"-I{ROOT / 'include'}", "-c", str(src), "-o", str(obj)]
+    _run(cmd)
+    return obj
+
+
+def build_lib(force: bool = False) -> Path:
+    BUILD.mkdir(exist_ok=True)
+    srcs = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
+    if force:
+        for o in BUILD.glob("*.o"):
+            o.unlink()
+    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
+        objs = list(ex.map(_compile, srcs))
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> Path:
+    _run(["make", "-s", "-C", str(ORACLE_DIR)])
+    return ORACLE_DIR / "liboracle.so"
+
+
+def main(argv: list[str]) -> int:
+    force = "--force" in argv
+    if "--oracle" not in argv:
+        print("built", build_lib(force=force))
+    if "--lib" not in argv:
+        print("built", build_oracle())
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))

@@ -1,0 +1,955 @@
+// Scene + fused batched evaluation of Tracker::trackFrame (Tracker.cpp:583-773)
+// for many poses x cameras at once.
+//
+// One batch = n poses x C cameras = n*C "items". Per item the host computes
+// the exact double-precision geometry (inverse model-view, projected
+// bounding box, render size). The device then runs three kernels:
+//   A  drr_tiles   : persistent; 16x16-pixel tiles of every item's bounding
+//                    box pulled from a global work counter; ray march ->
+//                    DRR scratch (one image per item).
+//   B  post_tiles  : persistent; per 16x16 tile: DRR filter chain and
+//                    radiograph resample + filter chain in LDS with halos,
+//                    DRR mask, multiplies, nine NCC moments -> per-tile
+//                    partial (9 doubles).
+//   C  finalize    : one block per item: fixed-order sum of its partials,
+//                    NCC in double; resets the work counters for the next
+//                    batch.
+// The three launches have fixed grids and read all sizes from device memory,
+// so a batch can be recorded once in a hipGraph and replayed.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "device.hpp"
+
+namespace asc {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTile = 16;
+constexpr int kMaxChain = 8;
+constexpr int kMaxHalo = 12;
+constexpr int kRegion = kTile + 2 * kMaxHalo;  // 40
+
+struct ChainDev {
+  int n;
+  int halo;
+  asc_filter f[kMaxChain];
+  int gn[kMaxChain];    // Gaussian/sharpen size N (1 = copy)
+  int woff[kMaxChain];  // offset of the weights in the weight pool
+  int radius[kMaxChain];
+};
+
+struct CamDev {
+  ImgDev rad;
+  ImgDev bg;
+  int has_bg;
+  float bg_threshold;
+  float plane[4];  // camera viewport = RadRenderer image plane (Tracker.cpp:327-329)
+  ChainDev chain[2];
+};
+
+struct VolParams {
+  VolDev v;
+  float step, intensity, cutoff_raw, step_vscale;
+};
+
+struct ItemDesc {
+  float imv[12];
+  float vp[4];
+  int w, h;
+  int vol, cam;
+  int tiles_x, tiles_y;
+  int tile_base;  // first tile of this item in the batch's tile list
+  int part_off;   // first partial slot
+  long long drr_off;
+};
+
+struct BatchHeader {
+  int n_items;
+  int total_tiles;
+  int debug;
+  int pad;
+};
+
+struct Counters {
+  unsigned int drr_next;
+  unsigned int post_next;
+  unsigned int pad[2];
+};
+
+struct BatchArgs {
+  const BatchHeader* hdr;
+  const ItemDesc* items;
+  const CamDev* cams;
+  const VolParams* vols;
+  const float* weights;
+  float* drr;
+  double* partials;
+  double* results;  // n_items * 10
+  unsigned long long* samples;  // n_items
+  Counters* counters;
+  float* dbg_drr;
+  float* dbg_rad;
+  float* dbg_mask;
+};
+
+// Find the item owning tile t (items sorted by tile_base). Uniform per block.
+__device__ __forceinline__ int find_item(const ItemDesc* __restrict__ items, int n, int t)
+{
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].tile_base <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Pull the next tile for the whole block.
+__device__ __forceinline__ int next_tile(unsigned int* counter, int* slot)
+{
+  __syncthreads();
+  if (threadIdx.x == 0) *slot = (int)atomicAdd(counter, 1u);
+  __syncthreads();
+  return *slot;
+}
+
+template <int ST>
+__device__ __forceinline__ float march_item(const VolParams& vp, const ItemDesc& it, unsigned x, unsigned y,
+                                            int& nsamp)
+{
+  dev::Ray r;
+  if (!dev::ray_setup(it.imv, it.vp, vp.step, x, y, (unsigned)it.w, (unsigned)it.h, r)) return 0.0f;
+  return dev::drr_finish(dev::march<ST>(vp.v, r, vp.cutoff_raw, nsamp), vp.step_vscale, vp.intensity);
+}
+
+__global__ __launch_bounds__(kBlock) void drr_tiles(BatchArgs a)
+{
+  __shared__ int s_tile;
+  __shared__ unsigned long long s_samples;
+  const int total = a.hdr->total_tiles;
+  const int n_items = a.hdr->n_items;
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (;;) {
+    const int t = next_tile(&a.counters->drr_next, &s_tile);
+    if (t >= total) break;
+    const int ii = find_item(a.items, n_items, t);
+    const ItemDesc& it = a.items[ii];
+    const int local = t - it.tile_base;
+    const int tx = local % it.tiles_x, ty = local / it.tiles_x;
+    const unsigned x = tx * kTile + (wave & 1) * 8 + (lane & 7);
+    const unsigned y = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
+    if (threadIdx.x == 0) s_samples = 0;
+    __syncthreads();
+    int nsamp = 0;
+    if (x < (unsigned)it.w && y < (unsigned)it.h) {
+      const VolParams& vp = a.vols[it.vol];
+      float val;
+      switch (vp.v.storage) {
+        case ASC_STORAGE_U16: val = march_item<ASC_STORAGE_U16>(vp, it, x, y, nsamp); break;
+        case ASC_STORAGE_F16: val = march_item<ASC_STORAGE_F16>(vp, it, x, y, nsamp); break;
+        case ASC_STORAGE_BF16: val = march_item<ASC_STORAGE_BF16>(vp, it, x, y, nsamp); break;
+        default: val = march_item<ASC_STORAGE_F32>(vp, it, x, y, nsamp); break;
+      }
+      a.drr[it.drr_off + (long long)y * it.w + x] = val;
+    }
+    // per-tile sample count (for the algorithmic-bytes figure)
+    int ws = nsamp;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) ws += __shfl_xor(ws, off, 64);
+    if (lane == 0) atomicAdd(&s_samples, (unsigned long long)ws);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&a.samples[ii], s_samples);
+  }
+}
+
+// LDS region accessor: region origin (ox, oy) in image coordinates.
+struct LdsGet {
+  const float* buf;
+  int ox, oy;
+  __device__ __forceinline__ float operator()(int x, int y) const { return buf[(y - oy) * kRegion + (x - ox)]; }
+};
+
+// Run a filter chain over the region in LDS. On entry `cur` holds stage-0
+// values at every in-image position of the region grown by the chain's
+// halo; on exit the returned buffer holds the chain output for the tile.
+__device__ float* run_chain(const ChainDev& ch, const float* __restrict__ weights, float* cur, float* nxt, int x0,
+                            int y0, int w, int h)
+{
+  const int H0 = ch.halo;
+  const int ox = x0 - H0, oy = y0 - H0;  // region origin (stride kRegion)
+  int remaining = H0;
+  for (int k = 0; k < ch.n; k++) {
+    const asc_filter& f = ch.f[k];
+    const int r = ch.radius[k];
+    remaining -= r;
+    // output rectangle: tile grown by `remaining`
+    const int rx0 = x0 - remaining, ry0 = y0 - remaining;
+    const int side = kTile + 2 * remaining;
+    const LdsGet get{cur, ox, oy};
+    for (int idx = threadIdx.x; idx < side * side; idx += kBlock) {
+      const int px = rx0 + idx % side, py = ry0 + idx / side;
+      if (px < 0 || py < 0 || px >= w || py >= h) continue;
+      float v;
+      if (!f.enabled) {
+        v = get(px, py);
+      } else if (f.type == ASC_FILTER_SOBEL) {
+        v = dev::sobel_px(get, px, py, w, h, f.p[0], f.p[1]);
+      } else if (f.type == ASC_FILTER_CONTRAST) {
+        v = dev::contrast_px(get, px, py, w, h, f.p[0], f.p[1], (int)f.p[2]);
+      } else if (ch.gn[k] <= 1) {
+        v = get(px, py);
+      } else {
+        const float c = dev::conv_px(get, px, py, w, h, weights + ch.woff[k], ch.gn[k]);
+        v = f.type == ASC_FILTER_SHARPEN ? dev::sharpen_finish(get(px, py), c, f.p[1] < 1.0f ? 1.0f : f.p[1], f.p[2])
+                                         : dev::gaussian_finish(c);
+      }
+      nxt[(py - oy) * kRegion + (px - ox)] = v;
+    }
+    __syncthreads();
+    float* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  return cur;
+}
+
+__global__ __launch_bounds__(kBlock) void post_tiles(BatchArgs a)
+{
+  __shared__ float buf0[kRegion * kRegion];
+  __shared__ float buf1[kRegion * kRegion];
+  __shared__ float sdrr[kTile * kTile];
+  __shared__ double lds_red[4 * 9];
+  __shared__ int s_tile;
+  const int total = a.hdr->total_tiles;
+  const int n_items = a.hdr->n_items;
+  const int debug = a.hdr->debug;
+  for (;;) {
+    const int t = next_tile(&a.counters->post_next, &s_tile);
+    if (t >= total) break;
+    const int ii = find_item(a.items, n_items, t);
+    const ItemDesc& it = a.items[ii];
+    const CamDev& cam = a.cams[it.cam];
+    const int local = t - it.tile_base;
+    const int tx = local % it.tiles_x, ty = local / it.tiles_x;
+    const int x0 = tx * kTile, y0 = ty * kTile;
+    const int w = it.w, h = it.h;
+
+    // --- DRR chain: stage 0 = DRR scratch ---
+    {
+      const ChainDev& ch = cam.chain[0];
+      const int H0 = ch.halo, side = kTile + 2 * H0;
+      const int ox = x0 - H0, oy = y0 - H0;
+      const float* src = a.drr + it.drr_off;
+      for (int idx = threadIdx.x; idx < side * side; idx += kBlock) {
+        const int px = ox + idx % side, py = oy + idx / side;
+        if (px < 0 || py < 0 || px >= w || py >= h) continue;
+        buf0[(py - oy) * kRegion + (px - ox)] = src[(long long)py * w + px];
+      }
+      __syncthreads();
+      const float* outb = run_chain(ch, a.weights, buf0, buf1, x0, y0, w, h);
+      const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+      sdrr[threadIdx.x] = outb[(ly + H0) * kRegion + (lx + H0)];
+      __syncthreads();
+    }
+    // --- radiograph chain: stage 0 = rad resample (RadRenderer.cl) ---
+    const float* radb;
+    int rH0;
+    {
+      const ChainDev& ch = cam.chain[1];
+      rH0 = ch.halo;
+      const int side = kTile + 2 * rH0;
+      const int ox = x0 - rH0, oy = y0 - rH0;
+      for (int idx = threadIdx.x; idx < side * side; idx += kBlock) {
+        const int px = ox + idx % side, py = oy + idx / side;
+        if (px < 0 || py < 0 || px >= w || py >= h) continue;
+        buf0[(py - oy) * kRegion + (px - ox)] =
+          dev::rad_pixel(cam.rad, cam.plane, it.vp, px, py, w, h, false, 0.0f);
+      }
+      __syncthreads();
+      radb = run_chain(ch, a.weights, buf0, buf1, x0, y0, w, h);
+    }
+    // --- mask, multiplies, moments (Tracker.cpp:744-752, Ncc.cpp) ---
+    dev::Moments acc;
+    {
+      const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+      const int px = x0 + lx, py = y0 + ly;
+      if (px < w && py < h) {
+        float d = sdrr[threadIdx.x];
+        float g = radb[(ly + rH0) * kRegion + (lx + rH0)];
+        float m = (d != 0.0f) ? 1.0f : 0.0f;
+        if (cam.has_bg) {
+          const float bgv = dev::rad_pixel(cam.bg, cam.plane, it.vp, px, py, w, h, true, cam.bg_threshold);
+          m = bgv * m;
+        }
+        g = g * m;
+        d = d * m;
+        acc.add(d, g, m);
+        if (debug) {
+          const long long o = it.drr_off + (long long)py * w + px;
+          a.dbg_drr[o] = d;
+          a.dbg_rad[o] = g;
+          a.dbg_mask[o] = m;
+        }
+      }
+    }
+    double v[9];
+    acc.to_double(v);
+    dev::block_sum9(v, lds_red);
+    if (threadIdx.x == 0) {
+      double* dst = a.partials + (size_t)(it.part_off + local) * 9;
+#pragma unroll
+      for (int k = 0; k < 9; k++) dst[k] = v[k];
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void finalize_items(BatchArgs a)
+{
+  __shared__ double lds[4 * 9];
+  const int n_items = a.hdr->n_items;
+  for (int ii = blockIdx.x; ii < n_items; ii += gridDim.x) {
+    const ItemDesc& it = a.items[ii];
+    const int nt = it.tiles_x * it.tiles_y;
+    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = threadIdx.x; k < nt; k += kBlock) {
+      const double* p = a.partials + (size_t)(it.part_off + k) * 9;
+#pragma unroll
+      for (int j = 0; j < 9; j++) v[j] += p[j];
+    }
+    dev::block_sum9(v, lds);
+    if (threadIdx.x == 0) {
+      double* r = a.results + (size_t)ii * 10;
+      for (int j = 0; j < 9; j++) r[j] = v[j];
+      r[9] = ncc_from_moments(v);
+    }
+    __syncthreads();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.counters->drr_next = 0;
+    a.counters->post_next = 0;
+  }
+}
+
+__global__ void zero_u64(unsigned long long* p, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;  // elements
+  int ensure(size_t n)
+  {
+    if (n <= cap) return ASC_OK;
+    if (p) ASC_HIP(hipFree(p));
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max(n, (size_t)16);
+    hipError_t e = hipMalloc((void**)&p, want * sizeof(T));
+    if (e != hipSuccess) return fail(ASC_ERR_NOMEM, std::string("scene scratch allocation: ") + hipGetErrorString(e));
+    cap = want;
+    return ASC_OK;
+  }
+  void release()
+  {
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+template <class T>
+struct PinBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t n)
+  {
+    if (n <= cap) return ASC_OK;
+    if (p) hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max(n, (size_t)16);
+    ASC_HIP(hipHostMalloc((void**)&p, want * sizeof(T), hipHostMallocDefault));
+    cap = want;
+    return ASC_OK;
+  }
+  void release()
+  {
+    if (p) hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+}  // namespace
+}  // namespace asc
+
+using namespace asc;
+
+struct SceneVolume {
+  asc_volume vol;
+  float step, intensity, cutoff;
+  Frame pivot;
+};
+
+struct SceneCamera {
+  asc_camera cam;
+  asc_image rad = nullptr;
+  asc_image bg = nullptr;
+  float bg_threshold = -1.0f;
+  std::vector<asc_filter> chain[2];
+};
+
+struct asc_scene_s {
+  asc_context ctx;
+  int RW, RH;
+  std::vector<SceneVolume> vols;
+  std::vector<SceneCamera> cams;
+  bool dirty = true;
+  int grid_persistent = 0;
+  // device
+  DevBuf<CamDev> d_cams;
+  DevBuf<VolParams> d_vols;
+  DevBuf<float> d_weights;
+  DevBuf<unsigned char> d_batch;  // header + items
+  DevBuf<float> d_drr;
+  DevBuf<double> d_partials;
+  DevBuf<double> d_results;
+  DevBuf<unsigned long long> d_samples;
+  DevBuf<Counters> d_counters;
+  DevBuf<float> d_dbg;
+  // host
+  PinBuf<unsigned char> h_batch;
+  PinBuf<double> h_results;
+  PinBuf<unsigned long long> h_samples;
+  std::vector<ItemDesc> items;
+  std::vector<int> in_view;
+  int n_items = 0;
+  int debug = 0;
+  // graph
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_items = -1;
+  const void* graph_key[8] = {};
+  // plain-path scratch (one eval at a time)
+  DevBuf<float> p_bufs;
+  // per-kernel timing of the last eager batch (HIP events on the scene stream)
+  bool timing = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  float last_ms[3] = {0, 0, 0};
+};
+
+namespace {
+
+int scene_upload_config(asc_scene s)
+{
+  if (!s->dirty) return ASC_OK;
+  std::vector<CamDev> cams(s->cams.size());
+  std::vector<float> pool;
+  for (size_t c = 0; c < s->cams.size(); c++) {
+    const SceneCamera& sc = s->cams[c];
+    CamDev& cd = cams[c];
+    std::memset(&cd, 0, sizeof cd);
+    if (!sc.rad) return fail(ASC_ERR_INVALID, "camera has no radiograph");
+    cd.rad = sc.rad->dev;
+    cd.has_bg = (sc.bg && sc.bg_threshold >= 0.0f) ? 1 : 0;  // View.cpp:173-174
+    if (sc.bg) cd.bg = sc.bg->dev;
+    cd.bg_threshold = sc.bg_threshold;
+    for (int i = 0; i < 4; i++) cd.plane[i] = (float)sc.cam.viewport[i];
+    for (int which = 0; which < 2; which++) {
+      const std::vector<asc_filter>& fs = sc.chain[which];
+      ChainDev& ch = cd.chain[which];
+      if (fs.size() > (size_t)kMaxChain) return fail(ASC_ERR_UNSUPPORTED, "filter chain longer than 8");
+      ch.n = (int)fs.size();
+      ch.halo = 0;
+      for (int k = 0; k < ch.n; k++) {
+        ch.f[k] = fs[k];
+        ch.gn[k] = 1;
+        ch.woff[k] = 0;
+        if (fs[k].type == ASC_FILTER_MEDIAN && fs[k].enabled)
+          return fail(ASC_ERR_UNSUPPORTED, "median filter is not implemented by the reference");
+        if (fs[k].type == ASC_FILTER_GAUSSIAN || fs[k].type == ASC_FILTER_SHARPEN) {
+          std::vector<float> wts;
+          const int n = gauss_weights(fs[k].p[0], wts);
+          ch.gn[k] = n;
+          ch.woff[k] = (int)pool.size();
+          pool.insert(pool.end(), wts.begin(), wts.end());
+        }
+        ch.radius[k] = dev::filter_radius(fs[k], ch.gn[k]);
+        ch.halo += ch.radius[k];
+      }
+      if (ch.halo > kMaxHalo) return fail(ASC_ERR_UNSUPPORTED, "filter chain halo exceeds 12 pixels");
+    }
+  }
+  std::vector<VolParams> vols(s->vols.size());
+  for (size_t i = 0; i < s->vols.size(); i++) {
+    const SceneVolume& sv = s->vols[i];
+    vols[i].v = sv.vol->dev;
+    vols[i].step = sv.step;
+    vols[i].intensity = sv.intensity;
+    vols[i].cutoff_raw = sv.cutoff / sv.vol->dev.vscale;
+    vols[i].step_vscale = sv.step * sv.vol->dev.vscale;
+  }
+  if (pool.empty()) pool.push_back(0.0f);
+  int rc;
+  if ((rc = s->d_cams.ensure(cams.size()))) return rc;
+  if ((rc = s->d_vols.ensure(vols.size()))) return rc;
+  if ((rc = s->d_weights.ensure(pool.size()))) return rc;
+  ASC_HIP(hipStreamSynchronize(s->ctx->stream));
+  ASC_HIP(hipMemcpy(s->d_cams.p, cams.data(), sizeof(CamDev) * cams.size(), hipMemcpyHostToDevice));
+  ASC_HIP(hipMemcpy(s->d_vols.p, vols.data(), sizeof(VolParams) * vols.size(), hipMemcpyHostToDevice));
+  ASC_HIP(hipMemcpy(s->d_weights.p, pool.data(), sizeof(float) * pool.size(), hipMemcpyHostToDevice));
+  s->dirty = false;
+  return ASC_OK;
+}
+
+void destroy_graph(asc_scene s)
+{
+  if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
+  if (s->graph) hipGraphDestroy(s->graph);
+  s->graph_exec = nullptr;
+  s->graph = nullptr;
+  s->graph_items = -1;
+}
+
+// Issue the batch (upload, three kernels, download) on the context stream.
+int issue_batch(asc_scene s, size_t batch_bytes)
+{
+  hipStream_t st = s->ctx->stream;
+  const int n_items = s->n_items;
+  BatchArgs a;
+  a.hdr = (const BatchHeader*)s->d_batch.p;
+  a.items = (const ItemDesc*)(s->d_batch.p + sizeof(BatchHeader));
+  a.cams = s->d_cams.p;
+  a.vols = s->d_vols.p;
+  a.weights = s->d_weights.p;
+  a.drr = s->d_drr.p;
+  a.partials = s->d_partials.p;
+  a.results = s->d_results.p;
+  a.samples = s->d_samples.p;
+  a.counters = s->d_counters.p;
+  const size_t plane = (size_t)s->RW * s->RH;
+  a.dbg_drr = s->debug ? s->d_dbg.p : nullptr;
+  a.dbg_rad = s->debug ? s->d_dbg.p + plane * n_items : nullptr;
+  a.dbg_mask = s->debug ? s->d_dbg.p + 2 * plane * n_items : nullptr;
+  ASC_HIP(hipMemcpyAsync(s->d_batch.p, s->h_batch.p, batch_bytes, hipMemcpyHostToDevice, st));
+  zero_u64<<<(n_items + 255) / 256, 256, 0, st>>>(s->d_samples.p, n_items);
+  const bool timed = s->timing && s->ev[0];
+  if (timed) ASC_HIP(hipEventRecord(s->ev[0], st));
+  drr_tiles<<<s->grid_persistent, kBlock, 0, st>>>(a);
+  if (timed) ASC_HIP(hipEventRecord(s->ev[1], st));
+  post_tiles<<<s->grid_persistent, kBlock, 0, st>>>(a);
+  if (timed) ASC_HIP(hipEventRecord(s->ev[2], st));
+  finalize_items<<<std::min(n_items, 4096), kBlock, 0, st>>>(a);
+  if (timed) ASC_HIP(hipEventRecord(s->ev[3], st));
+  ASC_HIP(hipGetLastError());
+  ASC_HIP(hipMemcpyAsync(s->h_results.p, s->d_results.p, sizeof(double) * 10 * n_items, hipMemcpyDeviceToHost, st));
+  ASC_HIP(hipMemcpyAsync(s->h_samples.p, s->d_samples.p, sizeof(unsigned long long) * n_items,
+                         hipMemcpyDeviceToHost, st));
+  return ASC_OK;
+}
+
+// Geometry for every (pose, camera) item, then the fused launches.
+int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, double* costs, double* per_camera)
+{
+  int rc = scene_upload_config(s);
+  if (rc) return rc;
+  const int C = (int)s->cams.size();
+  ASC_CHECK(C > 0, ASC_ERR_INVALID, "scene has no cameras");
+  ASC_CHECK(!s->vols.empty(), ASC_ERR_INVALID, "scene has no volumes");
+  const int n_items = n * C;
+  s->n_items = n_items;
+  s->items.resize(n_items);
+  s->in_view.assign(n_items, 0);
+  const size_t plane = (size_t)s->RW * s->RH;
+  const int max_tiles = ((s->RW + kTile - 1) / kTile) * ((s->RH + kTile - 1) / kTile);
+  int tile_base = 0;
+  for (int i = 0; i < n; i++) {
+    const int vi = volumes ? volumes[i] : 0;
+    ASC_CHECK(vi >= 0 && vi < (int)s->vols.size(), ASC_ERR_INVALID, "volume index out of range");
+    const asc_volume_desc& d = s->vols[vi].vol->desc;
+    for (int c = 0; c < C; c++) {
+      const int ii = i * C + c;
+      ItemDesc& it = s->items[ii];
+      std::memset(&it, 0, sizeof it);
+      double vp[4];
+      unsigned sz[2] = {0, 0};
+      const int inview = view_geometry(s->cams[c].cam, poses + 6 * i, d.inv_scale, d.inv_trans, s->RW, s->RH,
+                                       it.imv, vp, sz);
+      it.vol = vi;
+      it.cam = c;
+      it.drr_off = (long long)ii * (long long)plane;
+      it.part_off = ii * max_tiles;
+      it.tile_base = tile_base;
+      if (inview) {
+        if (sz[0] > (unsigned)s->RW || sz[1] > (unsigned)s->RH)
+          return fail(ASC_ERR_TOO_LARGE, "Rendered image is larger than the viewport buffer");
+        for (int k = 0; k < 4; k++) it.vp[k] = (float)vp[k];
+        it.w = (int)sz[0];
+        it.h = (int)sz[1];
+        it.tiles_x = (it.w + kTile - 1) / kTile;
+        it.tiles_y = (it.h + kTile - 1) / kTile;
+        tile_base += it.tiles_x * it.tiles_y;
+        s->in_view[ii] = 1;
+      }
+    }
+  }
+  // buffers
+  const size_t batch_bytes = sizeof(BatchHeader) + sizeof(ItemDesc) * (size_t)n_items;
+  const void* before[6] = {s->d_batch.p, s->d_drr.p, s->d_partials.p, s->d_results.p, s->d_samples.p,
+                           s->d_dbg.p};
+  if ((rc = s->d_batch.ensure(batch_bytes))) return rc;
+  if ((rc = s->h_batch.ensure(batch_bytes))) return rc;
+  if ((rc = s->d_drr.ensure(plane * n_items))) return rc;
+  if ((rc = s->d_partials.ensure((size_t)max_tiles * n_items * 9))) return rc;
+  if ((rc = s->d_results.ensure((size_t)n_items * 10))) return rc;
+  if ((rc = s->h_results.ensure((size_t)n_items * 10))) return rc;
+  if ((rc = s->d_samples.ensure((size_t)n_items))) return rc;
+  if ((rc = s->h_samples.ensure((size_t)n_items))) return rc;
+  if (!s->d_counters.p) {
+    if ((rc = s->d_counters.ensure(1))) return rc;
+    ASC_HIP(hipMemset(s->d_counters.p, 0, sizeof(Counters)));
+  }
+  if (s->debug && (rc = s->d_dbg.ensure(plane * n_items * 3))) return rc;
+  const void* after[6] = {s->d_batch.p, s->d_drr.p, s->d_partials.p, s->d_results.p, s->d_samples.p, s->d_dbg.p};
+  if (std::memcmp(before, after, sizeof before) != 0) destroy_graph(s);
+  if (s->grid_persistent == 0) {
+    hipDeviceProp_t prop;
+    ASC_HIP(hipGetDeviceProperties(&prop, s->ctx->device));
+    int per_cu = 0;
+    ASC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, drr_tiles, kBlock, 0));
+    if (per_cu < 1) per_cu = 1;
+    s->grid_persistent = prop.multiProcessorCount * per_cu;
+  }
+  BatchHeader hdr{n_items, tile_base, s->debug, 0};
+  std::memcpy(s->h_batch.p, &hdr, sizeof hdr);
+  std::memcpy(s->h_batch.p + sizeof hdr, s->items.data(), sizeof(ItemDesc) * n_items);
+
+  hipStream_t st = s->ctx->stream;
+  if (s->use_graph) {
+    if (!s->graph_exec || s->graph_items != n_items) {
+      destroy_graph(s);
+      ASC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      rc = issue_batch(s, batch_bytes);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+      s->graph = g;
+      ASC_HIP(hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0));
+      s->graph_items = n_items;
+    }
+    ASC_HIP(hipGraphLaunch(s->graph_exec, st));
+  } else {
+    if ((rc = issue_batch(s, batch_bytes))) return rc;
+  }
+  ASC_HIP(hipStreamSynchronize(st));
+  if (s->timing && s->ev[0] && !s->use_graph)
+    for (int k = 0; k < 3; k++) ASC_HIP(hipEventElapsedTime(&s->last_ms[k], s->ev[k], s->ev[k + 1]));
+
+  // Tracker.cpp:614-618 (9999 when not in view), :769 (1 - NCC),
+  // :814-826 (sum over cameras; negative totals become 9999).
+  for (int i = 0; i < n; i++) {
+    double total = 0.0;
+    for (int c = 0; c < C; c++) {
+      const int ii = i * C + c;
+      const double corr = s->in_view[ii] ? 1.0 - s->h_results.p[(size_t)ii * 10 + 9] : 9999.0;
+      if (per_camera) per_camera[ii] = corr;
+      total = c == 0 ? corr : total + corr;
+    }
+    if (total < 0) total = 9999;
+    if (costs) costs[i] = total;
+  }
+  return ASC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+ASC_API int asc_scene_create(asc_context ctx, int render_width, int render_height, asc_scene* out)
+{
+  ASC_CHECK(ctx && out, ASC_ERR_INVALID, "bad argument");
+  ASC_CHECK(render_width > 0 && render_height > 0, ASC_ERR_INVALID, "bad render size");
+  asc_scene s = new asc_scene_s();
+  s->ctx = ctx;
+  s->RW = render_width;
+  s->RH = render_height;
+  *out = s;
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_destroy(asc_scene s)
+{
+  if (!s) return ASC_OK;
+  hipStreamSynchronize(s->ctx->stream);
+  destroy_graph(s);
+  for (hipEvent_t& e : s->ev)
+    if (e) hipEventDestroy(e);
+  s->d_cams.release();
+  s->d_vols.release();
+  s->d_weights.release();
+  s->d_batch.release();
+  s->d_drr.release();
+  s->d_partials.release();
+  s->d_results.release();
+  s->d_samples.release();
+  s->d_counters.release();
+  s->d_dbg.release();
+  s->p_bufs.release();
+  s->h_batch.release();
+  s->h_results.release();
+  s->h_samples.release();
+  delete s;
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_add_volume(asc_scene s, asc_volume vol, float step, float intensity, float cutoff,
+                                 const double pivot[12], int* index)
+{
+  ASC_CHECK(s && vol, ASC_ERR_INVALID, "bad argument");
+  SceneVolume sv;
+  sv.vol = vol;
+  sv.step = step;
+  sv.intensity = intensity;
+  sv.cutoff = cutoff;
+  sv.pivot = pivot ? frame_load(pivot) : frame_identity();
+  s->vols.push_back(sv);
+  if (index) *index = (int)s->vols.size() - 1;
+  s->dirty = true;
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_volume_pivot(asc_scene s, int volume, double pivot[12])
+{
+  ASC_CHECK(s && pivot && volume >= 0 && volume < (int)s->vols.size(), ASC_ERR_INVALID, "bad volume index");
+  frame_store(s->vols[volume].pivot, pivot);
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_set_drr_params(asc_scene s, int volume, float step, float intensity, float cutoff)
+{
+  ASC_CHECK(s && volume >= 0 && volume < (int)s->vols.size(), ASC_ERR_INVALID, "bad volume index");
+  s->vols[volume].step = step;
+  s->vols[volume].intensity = intensity;
+  s->vols[volume].cutoff = cutoff;
+  s->dirty = true;
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_add_camera(asc_scene s, const asc_camera* cam, asc_image rad, int* index)
+{
+  ASC_CHECK(s && cam, ASC_ERR_INVALID, "bad argument");
+  SceneCamera sc;
+  sc.cam = *cam;
+  sc.rad = rad;
+  s->cams.push_back(sc);
+  if (index) *index = (int)s->cams.size() - 1;
+  s->dirty = true;
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_set_rad(asc_scene s, int camera, asc_image rad)
+{
+  ASC_CHECK(s && camera >= 0 && camera < (int)s->cams.size() && rad, ASC_ERR_INVALID, "bad argument");
+  s->cams[camera].rad = rad;
+  s->dirty = true;
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_set_background(asc_scene s, int camera, asc_image bg, float threshold)
+{
+  ASC_CHECK(s && camera >= 0 && camera < (int)s->cams.size(), ASC_ERR_INVALID, "bad argument");
+  ASC_CHECK(!bg || bg->dev.bps == 32, ASC_ERR_UNSUPPORTED, "background image must be float32 (CL_FLOAT)");
+  s->cams[camera].bg = bg;
+  s->cams[camera].bg_threshold = threshold;
+  s->dirty = true;
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_set_filters(asc_scene s, int camera, int which, const asc_filter* filters, int count)
+{
+  ASC_CHECK(s && camera >= 0 && camera < (int)s->cams.size() && (which == 0 || which == 1), ASC_ERR_INVALID,
+            "bad argument");
+  ASC_CHECK(count >= 0 && (count == 0 || filters), ASC_ERR_INVALID, "bad filter list");
+  s->cams[camera].chain[which].assign(filters, filters + count);
+  s->dirty = true;
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_eval_poses(asc_scene s, const int* volumes, const double* xyzypr, int n, double* costs,
+                                 double* per_camera)
+{
+  ASC_CHECK(s && xyzypr && n > 0, ASC_ERR_INVALID, "bad argument");
+  return eval_batch(s, volumes, xyzypr, n, costs, per_camera);
+}
+
+ASC_API int asc_scene_eval_manips(asc_scene s, const int* volumes, const float* stores, const double* manips, int n,
+                                  double* costs, double* per_camera)
+{
+  ASC_CHECK(s && stores && manips && n > 0, ASC_ERR_INVALID, "bad argument");
+  std::vector<double> poses((size_t)n * 6);
+  for (int i = 0; i < n; i++) {
+    const int vi = volumes ? volumes[i] : 0;
+    ASC_CHECK(vi >= 0 && vi < (int)s->vols.size(), ASC_ERR_INVALID, "volume index out of range");
+    PoseStore ps;
+    std::memcpy(&ps, stores + 7 * i, sizeof ps);
+    compose_pose(ps, s->vols[vi].pivot, manips + 6 * i, poses.data() + 6 * i);
+  }
+  return eval_batch(s, volumes, poses.data(), n, costs, per_camera);
+}
+
+ASC_API int asc_scene_use_graph(asc_scene s, int enable)
+{
+  ASC_CHECK(s, ASC_ERR_INVALID, "null scene");
+  s->use_graph = enable != 0;
+  if (!s->use_graph) destroy_graph(s);
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_set_debug(asc_scene s, int enable)
+{
+  ASC_CHECK(s, ASC_ERR_INVALID, "null scene");
+  s->debug = enable != 0;
+  destroy_graph(s);
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_set_timing(asc_scene s, int enable)
+{
+  ASC_CHECK(s, ASC_ERR_INVALID, "null scene");
+  if (enable && !s->ev[0])
+    for (hipEvent_t& e : s->ev) ASC_HIP(hipEventCreate(&e));
+  s->timing = enable != 0;
+  destroy_graph(s);
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_last_timing(asc_scene s, float ms[3])
+{
+  ASC_CHECK(s && ms, ASC_ERR_INVALID, "bad argument");
+  for (int k = 0; k < 3; k++) ms[k] = s->last_ms[k];
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_last_info(asc_scene s, int* info, int capacity, int64_t* samples)
+{
+  ASC_CHECK(s, ASC_ERR_INVALID, "null scene");
+  int64_t total = 0;
+  for (int ii = 0; ii < s->n_items; ii++) {
+    total += (int64_t)s->h_samples.p[ii];
+    if (info && (ii + 1) * 8 <= capacity) {
+      int* o = info + 8 * ii;
+      const ItemDesc& it = s->items[ii];
+      o[0] = s->in_view[ii];
+      o[1] = it.w;
+      o[2] = it.h;
+      o[3] = (int)s->h_results.p[(size_t)ii * 10 + 3];
+      o[4] = it.tiles_x * it.tiles_y;
+      o[5] = (int)s->h_samples.p[ii];
+      o[6] = o[7] = 0;
+    }
+  }
+  if (samples) *samples = total;
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_last_moments(asc_scene s, double* moments, int capacity)
+{
+  ASC_CHECK(s && moments, ASC_ERR_INVALID, "bad argument");
+  for (int ii = 0; ii < s->n_items && (ii + 1) * 10 <= capacity; ii++)
+    for (int k = 0; k < 10; k++) moments[ii * 10 + k] = s->h_results.p[(size_t)ii * 10 + k];
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_last_images(asc_scene s, int pose, int camera, float* drr, float* rad, float* mask)
+{
+  ASC_CHECK(s && s->debug, ASC_ERR_INVALID, "enable debug images with asc_scene_set_debug first");
+  const int C = (int)s->cams.size();
+  const int ii = pose * C + camera;
+  ASC_CHECK(pose >= 0 && camera >= 0 && camera < C && ii < s->n_items, ASC_ERR_INVALID, "bad item");
+  const ItemDesc& it = s->items[ii];
+  const size_t n = (size_t)it.w * it.h;
+  const size_t plane = (size_t)s->RW * s->RH;
+  if (n == 0) return ASC_OK;
+  float* bases[3] = {drr, rad, mask};
+  for (int k = 0; k < 3; k++)
+    if (bases[k])
+      ASC_HIP(hipMemcpy(bases[k], s->d_dbg.p + k * plane * s->n_items + it.drr_off, n * sizeof(float),
+                        hipMemcpyDeviceToHost));
+  return ASC_OK;
+}
+
+// Tracker::trackFrame one pose at a time with one launch per operator and a
+// blocking NCC per camera (Tracker.cpp:603-771, View.cpp:147-256).
+ASC_API int asc_scene_track_frame_plain(asc_scene s, int volume, const double xyzypr[6], double* per_camera,
+                                        double* cost)
+{
+  ASC_CHECK(s && xyzypr && volume >= 0 && volume < (int)s->vols.size(), ASC_ERR_INVALID, "bad argument");
+  asc_context ctx = s->ctx;
+  const size_t plane = (size_t)s->RW * s->RH;
+  int rc;
+  // rendered_drr, rendered_rad, drr_mask, background_mask, filter scratch
+  if ((rc = s->p_bufs.ensure(plane * 5))) return rc;
+  float* drr_raw = s->p_bufs.p;
+  float* rad_buf = drr_raw + plane;
+  float* mask = rad_buf + plane;
+  float* bgm = mask + plane;
+  float* scratch = bgm + plane;
+  const SceneVolume& sv = s->vols[volume];
+  double total = 0.0;
+  for (size_t c = 0; c < s->cams.size(); c++) {
+    const SceneCamera& sc = s->cams[c];
+    float imv[12];
+    double vp[4];
+    unsigned sz[2];
+    double corr;
+    if (!view_geometry(sc.cam, xyzypr, sv.vol->desc.inv_scale, sv.vol->desc.inv_trans, s->RW, s->RH, imv, vp,
+                       sz)) {
+      corr = 9999.0;
+    } else {
+      if (sz[0] > (unsigned)s->RW || sz[1] > (unsigned)s->RH)
+        return fail(ASC_ERR_TOO_LARGE, "Rendered image is larger than the viewport buffer");
+      const float vpf[4] = {(float)vp[0], (float)vp[1], (float)vp[2], (float)vp[3]};
+      const float plane4[4] = {(float)sc.cam.viewport[0], (float)sc.cam.viewport[1], (float)sc.cam.viewport[2],
+                               (float)sc.cam.viewport[3]};
+      const int w = (int)sz[0], h = (int)sz[1];
+      // renderDrrSingle: DRR then the DRR chain, into `mask` as the filtered image holder
+      float* drr_f = mask;
+      if ((rc = launch_drr(ctx, sv.vol->dev, imv, vpf, sv.step, sv.intensity, sv.cutoff, drr_raw, w, h))) return rc;
+      if ((rc = asc_filter_chain(ctx, sc.chain[0].data(), (int)sc.chain[0].size(), drr_raw, drr_f, scratch, w, h)))
+        return rc;
+      // renderRad: resample then the rad chain
+      if ((rc = launch_rad(ctx, sc.rad->dev, plane4, vpf, drr_raw, w, h, false, 0.0f))) return rc;
+      if ((rc = asc_filter_chain(ctx, sc.chain[1].data(), (int)sc.chain[1].size(), drr_raw, rad_buf, scratch, w,
+                                 h)))
+        return rc;
+      const float* bgp = nullptr;
+      if (sc.bg && sc.bg_threshold >= 0.0f) {
+        if ((rc = launch_rad(ctx, sc.bg->dev, plane4, vpf, bgm, w, h, true, sc.bg_threshold))) return rc;
+        bgp = bgm;
+      }
+      // drr mask + 3 multiplies, then NCC(drr, rad, mask)
+      if ((rc = launch_mask_multiply(ctx, drr_f, rad_buf, bgp, scratch, (size_t)w * h))) return rc;
+      double ncc;
+      if ((rc = ncc_moments(ctx, drr_f, rad_buf, scratch, (unsigned)(w * h), nullptr, &ncc))) return rc;
+      corr = 1.0 - ncc;
+    }
+    if (per_camera) per_camera[c] = corr;
+    total = c == 0 ? corr : total + corr;
+  }
+  if (total < 0) total = 9999;
+  if (cost) *cost = total;
+  return ASC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,365 @@
+// Device helpers shared by the per-operator kernels and the fused batch
+// kernels. Only included by translation units built with -ffp-contract=off:
+// plain `a * b + c` below is two IEEE operations, as in the reference
+// kernels; contraction is written explicitly (fmaf) where it is allowed.
+#pragma once
+
+#include "internal.hpp"
+
+namespace asc {
+namespace dev {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Ray setup, RayCaster.cl:20-57. Exact operation order: the slab distances
+// decide the sample count and hence the DRR's zero pattern (the mask).
+// ---------------------------------------------------------------------------
+struct Ray {
+  float ox, oy, oz;
+  float dx, dy, dz;
+  float t_near, t_far;
+};
+
+__device__ __forceinline__ bool ray_setup(const float* __restrict__ imv, const float* __restrict__ vp, float step,
+                                          unsigned x, unsigned y, unsigned w, unsigned h, Ray& r)
+{
+  const float u = vp[0] + vp[2] * ((float)x / (float)w);
+  const float v = vp[1] + vp[3] * ((float)y / (float)h);
+  // normalize((u, v, -2)) as p / sqrt(dot(p, p))
+  const float len = sqrtf(u * u + v * v + 4.0f);
+  const float lx = step * (u / len);
+  const float ly = step * (v / len);
+  const float lz = step * (-2.0f / len);
+  r.ox = imv[3];
+  r.oy = imv[7];
+  r.oz = imv[11];
+  r.dx = imv[0] * lx + imv[1] * ly + imv[2] * lz;
+  r.dy = imv[4] * lx + imv[5] * ly + imv[6] * lz;
+  r.dz = imv[8] * lx + imv[9] * ly + imv[10] * lz;
+  // box [0,1] x [0,1] x [-1,0]
+  const float bx = (0.0f - r.ox) / r.dx, by = (0.0f - r.oy) / r.dy, bz = (-1.0f - r.oz) / r.dz;
+  const float tx = (1.0f - r.ox) / r.dx, ty = (1.0f - r.oy) / r.dy, tz = (0.0f - r.oz) / r.dz;
+  float t_near = fmaxf(fmaxf(fminf(tx, bx), fminf(ty, by)), fminf(tz, bz));
+  const float t_far = fminf(fminf(fmaxf(tx, bx), fmaxf(ty, by)), fmaxf(tz, bz));
+  if (!(t_far > t_near)) return false;
+  if (t_near < 0.0f) t_near = 0.0f;
+  r.t_near = t_near;
+  r.t_far = t_far;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Volume texel loads (raw value; normalised = raw * VolDev::vscale).
+// ---------------------------------------------------------------------------
+template <int ST>
+__device__ __forceinline__ float texel_raw(const void* __restrict__ p, int idx);
+
+template <>
+__device__ __forceinline__ float texel_raw<ASC_STORAGE_U16>(const void* __restrict__ p, int idx)
+{
+  return (float)((const uint16_t*)p)[idx];
+}
+template <>
+__device__ __forceinline__ float texel_raw<ASC_STORAGE_F16>(const void* __restrict__ p, int idx)
+{
+  return __half2float(((const __half*)p)[idx]);
+}
+template <>
+__device__ __forceinline__ float texel_raw<ASC_STORAGE_BF16>(const void* __restrict__ p, int idx)
+{
+  return __uint_as_float((uint32_t)((const uint16_t*)p)[idx] << 16);
+}
+template <>
+__device__ __forceinline__ float texel_raw<ASC_STORAGE_F32>(const void* __restrict__ p, int idx)
+{
+  return ((const float*)p)[idx];
+}
+
+// Trilinear sample at voxel-space position (X, Y, Z) = (s*W-0.5, t*H-0.5,
+// r*D-0.5) with the sampler's weights (OpenCL 1.2 section 8.2). Each lerp
+// is one FMA: fma(a, T1 - T0, T0) is exact when T1 == 0 or T0 == 0, so the
+// result is > 0 exactly when a texel with positive weight is > 0 (the same
+// zero pattern as the weighted-sum form).
+template <int ST>
+__device__ __forceinline__ float trilinear(const VolDev& v, float X, float Y, float Z)
+{
+  const float fx = floorf(X), fy = floorf(Y), fz = floorf(Z);
+  const float a = X - fx, b = Y - fy, c = Z - fz;
+  // in-box samples satisfy -1 <= i0 <= W-1; the clamp only keeps stray
+  // positions inside the padded allocation
+  const int i = min(max((int)fx, -1), v.W - 1);
+  const int j = min(max((int)fy, -1), v.H - 1);
+  const int k = min(max((int)fz, -1), v.D - 1);
+  const int base = (k + 1) * v.pS + (j + 1) * v.pW + (i + 1);
+  const float t000 = texel_raw<ST>(v.data, base);
+  const float t100 = texel_raw<ST>(v.data, base + 1);
+  const float t010 = texel_raw<ST>(v.data, base + v.pW);
+  const float t110 = texel_raw<ST>(v.data, base + v.pW + 1);
+  const float t001 = texel_raw<ST>(v.data, base + v.pS);
+  const float t101 = texel_raw<ST>(v.data, base + v.pS + 1);
+  const float t011 = texel_raw<ST>(v.data, base + v.pS + v.pW);
+  const float t111 = texel_raw<ST>(v.data, base + v.pS + v.pW + 1);
+  const float c00 = fmaf(a, t100 - t000, t000);
+  const float c10 = fmaf(a, t110 - t010, t010);
+  const float c01 = fmaf(a, t101 - t001, t001);
+  const float c11 = fmaf(a, t111 - t011, t011);
+  const float c0 = fmaf(b, c10 - c00, c00);
+  const float c1 = fmaf(b, c11 - c01, c01);
+  return fmaf(c, c1 - c0, c0);
+}
+
+// Ray march, RayCaster.cl:59-69: t runs from t_far down by 1 while t >
+// t_near (the same float loop, so the same sample count). Returns the sum
+// of raw texel values above the raw cutoff.
+template <int ST>
+__device__ __forceinline__ float march(const VolDev& v, const Ray& r, float cutoff_raw, int& nsamples)
+{
+  const float W = (float)v.W, H = (float)v.H, D = (float)v.D;
+  // sampler coordinates (p.x, 1 - p.y, -p.z) scaled to voxel space
+  const float ox = r.ox * W - 0.5f, dx = r.dx * W;
+  const float oy = (H - 0.5f) - r.oy * H, dy = -(r.dy * H);
+  const float oz = -(r.oz * D) - 0.5f, dz = -(r.dz * D);
+  float acc = 0.0f;
+  int n = 0;
+  for (float t = r.t_far; t > r.t_near; t -= 1.0f) {
+    const float val = trilinear<ST>(v, fmaf(t, dx, ox), fmaf(t, dy, oy), fmaf(t, dz, oz));
+    acc += val > cutoff_raw ? val : 0.0f;
+    ++n;
+  }
+  nsamples += n;
+  return acc;
+}
+
+// density = sum(step * val); out = clamp(density / intensity, 0, 1)
+__device__ __forceinline__ float drr_finish(float acc_raw, float step_vscale, float intensity)
+{
+  const float d = (acc_raw * step_vscale) / intensity;
+  return d < 0.0f ? 0.0f : (d > 1.0f ? 1.0f : d);
+}
+
+// ---------------------------------------------------------------------------
+// 2-D image sampling, RadRenderer.cl / BackgroundRenderer.cl (exact order).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float img_texel(const ImgDev& im, int i, int j)
+{
+  if (i < 0 || j < 0 || i >= im.W || j >= im.H) return 0.0f;
+  const size_t idx = (size_t)j * im.W + i;
+  if (im.bps == 8) return (float)((const uint8_t*)im.data)[idx] / 255.0f;
+  if (im.bps == 16) return (float)((const uint16_t*)im.data)[idx] / 65535.0f;
+  return ((const float*)im.data)[idx];
+}
+
+__device__ __forceinline__ int floor_idx(float x, float& frac)
+{
+  const float f = floorf(x);
+  frac = x - f;
+  if (!(f > -4.0f)) return -4;
+  if (!(f < 1073741824.0f)) return 1073741824;
+  return (int)f;
+}
+
+__device__ __forceinline__ float img_sample(const ImgDev& im, float s, float t)
+{
+  const float x = s * (float)im.W - 0.5f;
+  const float y = t * (float)im.H - 0.5f;
+  float a, b;
+  const int i0 = floor_idx(x, a), j0 = floor_idx(y, b);
+  const float T00 = img_texel(im, i0, j0), T10 = img_texel(im, i0 + 1, j0);
+  const float T01 = img_texel(im, i0, j0 + 1), T11 = img_texel(im, i0 + 1, j0 + 1);
+  return (1.0f - a) * (1.0f - b) * T00 + a * (1.0f - b) * T10 + (1.0f - a) * b * T01 + a * b * T11;
+}
+
+// One pixel of rad_render_kernel (background = false) or
+// background_render_kernel (background = true).
+__device__ __forceinline__ float rad_pixel(const ImgDev& im, const float* __restrict__ plane,
+                                           const float* __restrict__ vp, unsigned x, unsigned y, unsigned w,
+                                           unsigned h, bool background, float threshold)
+{
+  const float u = vp[0] + vp[2] * ((float)x / (float)w);
+  const float v = vp[1] + vp[3] * ((float)y / (float)h);
+  const float s = (u - plane[0]) / plane[2] + 0.5f;
+  const float t = 0.5f - (v - plane[1]) / plane[3];
+  if (s < 0.0f || t < 0.0f || s > 1.0f || t > 1.0f) return 0.0f;
+  const float val = img_sample(im, s, t);
+  if (background) return (threshold <= val) ? 1.0f : 0.0f;
+  return 1.0f - val;
+}
+
+// ---------------------------------------------------------------------------
+// Filter pixel functions over an accessor get(x, y) that returns the input
+// at in-image coordinates. Same operation order as the reference kernels.
+// ---------------------------------------------------------------------------
+// SobelFilter.cl:10-47
+template <class G>
+__device__ __forceinline__ float sobel_px(const G& get, int x1, int y1, int w, int h, float scale, float blend)
+{
+  const int x0 = x1 - 1 < 0 ? 0 : x1 - 1, y0 = y1 - 1 < 0 ? 0 : y1 - 1;
+  const int x2 = x1 + 1 > w - 1 ? w - 1 : x1 + 1, y2 = y1 + 1 > h - 1 ? h - 1 : y1 + 1;
+  const float p00 = get(x0, y0), p01 = get(x1, y0), p02 = get(x2, y0);
+  const float p10 = get(x0, y1), p11 = get(x1, y1), p12 = get(x2, y1);
+  const float p20 = get(x0, y2), p21 = get(x1, y2), p22 = get(x2, y2);
+  const float horz = p02 + 2.0f * p12 + p22 - p00 - 2.0f * p10 - p20;
+  const float vert = p00 + 2.0f * p01 + p02 - p20 - 2.0f * p21 - p22;
+  const float grad = sqrtf(horz * horz + vert * vert);
+  float sum;
+  if (blend < 0.5f)
+    sum = scale * grad + 2.0f * blend * p11;
+  else
+    sum = 2.0f * (1.0f - blend) * scale * grad + p11;
+  if (sum < 0.0f)
+    sum = 0.0f;
+  else if (sum > 1.0f)
+    sum = 1.0f;
+  return sum;
+}
+
+// ContrastFilter.cl:11-38
+template <class G>
+__device__ __forceinline__ float contrast_px(const G& get, int x, int y, int w, int h, float alpha, float beta,
+                                             int size)
+{
+  const float fxy = get(x, y);
+  float n = 0.0f, sum = 0.0f;
+  const int minI = max(y - size / 2, 0), maxI = min(y + (size + 1) / 2, h);
+  const int minJ = max(x - size / 2, 0), maxJ = min(x + (size + 1) / 2, w);
+  for (int i = minI; i < maxI; ++i)
+    for (int j = minJ; j < maxJ; ++j) {
+      n += 1.0f;
+      sum += get(j, i);
+    }
+  const float axy = sum / n;
+  float gxy = 0.0f;
+  if (axy > 0.01f) gxy = powf(axy, alpha - beta) * powf(fxy, beta);
+  return gxy;
+}
+
+// GaussianFilter.cl:15-27 / SharpenFilter.cl:18-30: zero-padded N x N
+// convolution, i walks x (outer), j walks y (inner), weight index i*N+j.
+template <class G>
+__device__ __forceinline__ float conv_px(const G& get, int x, int y, int w, int h, const float* __restrict__ wts,
+                                         int n)
+{
+  const int fr = (n - 1) / 2;
+  float acc = 0.0f;
+  for (int i = 0; i < n; ++i) {
+    const int a = x - fr + i;
+    if (a < 0 || a >= w) continue;
+    for (int j = 0; j < n; ++j) {
+      const int b = y - fr + j;
+      if (!(b < 0 || b >= h)) acc = acc + wts[i * n + j] * get(a, b);
+    }
+  }
+  return acc;
+}
+
+__device__ __forceinline__ float gaussian_finish(float c)
+{
+  if (c > 1.0f) c = 1.0f;
+  if (c < 0.0f) c = 0.0f;
+  return c;
+}
+
+// SharpenFilter.cl:32-45
+__device__ __forceinline__ float sharpen_finish(float f, float blur, float contrast, float threshold)
+{
+  if (fabsf(f - blur) > threshold) {
+    float o = f + contrast * (f - blur);
+    if (o > 1.0f) o = 1.0f;
+    if (o < 0.0f) o = 0.0f;
+    return o;
+  }
+  return f;
+}
+
+// Halo (pixels) a filter needs on each side.
+__host__ __device__ inline int filter_radius(const asc_filter& f, int gauss_n)
+{
+  if (!f.enabled) return 0;
+  switch (f.type) {
+    case ASC_FILTER_SOBEL: return 1;
+    case ASC_FILTER_CONTRAST: {
+      const int size = (int)f.p[2];
+      const int lo = size / 2, hi = (size + 1) / 2 - 1;
+      return lo > hi ? lo : hi;
+    }
+    case ASC_FILTER_GAUSSIAN:
+    case ASC_FILTER_SHARPEN: return gauss_n > 1 ? (gauss_n - 1) / 2 : 0;
+    default: return 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// NCC moments: per-thread fp32 partials with Neumaier compensation, combined
+// in double across lanes, waves and blocks.
+// ---------------------------------------------------------------------------
+struct Comp {
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float x)
+  {
+    const float t = s + x;
+    if (fabsf(s) >= fabsf(x))
+      c += (s - t) + x;
+    else
+      c += (x - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return (double)s + (double)c; }
+};
+
+struct Moments {
+  // {sum mask, sum f, sum g, count, S_f, S_g, S_ff, S_gg, S_fg}
+  Comp m, f, g, sf, sg, sff, sgg, sfg;
+  int count = 0;
+  __device__ __forceinline__ void add(float fv, float gv, float mv)
+  {
+    m.add(mv);
+    f.add(fv);
+    g.add(gv);
+    if (mv > 0.5f) {
+      ++count;
+      sf.add(fv);
+      sg.add(gv);
+      sff.add(fv * fv);
+      sgg.add(gv * gv);
+      sfg.add(fv * gv);
+    }
+  }
+  __device__ __forceinline__ void to_double(double* o) const
+  {
+    o[0] = m.value();
+    o[1] = f.value();
+    o[2] = g.value();
+    o[3] = (double)count;
+    o[4] = sf.value();
+    o[5] = sg.value();
+    o[6] = sff.value();
+    o[7] = sgg.value();
+    o[8] = sfg.value();
+  }
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Block-wide sum of 9 doubles (blockDim.x == 256). Result valid in thread 0.
+__device__ __forceinline__ void block_sum9(double* v, double* lds /* 4*9 */)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 9; k++) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 9; k++) lds[wave * 9 + k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < 9; k++) v[k] = (lds[k] + lds[9 + k]) + (lds[18 + k] + lds[27 + k]);
+}
+
+}  // namespace dev
+}  // namespace asc

@@ -1,0 +1,126 @@
+// Internal declarations shared by the HIP sources of libautoscoper_hip.
+//
+// Floating-point policy: every translation unit is compiled with
+// -ffp-contract=off and correctly rounded fp32 division/sqrt, so code written
+// as `a * b + c` evaluates exactly like the reference kernels' IEEE float
+// operations. Hot loops that may contract use explicit fmaf().
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/autoscoper_hip.h"
+#include "geometry.hpp"
+
+namespace asc {
+
+int hip_fail(hipError_t e, const char* what);
+
+#define ASC_HIP(call)                                \
+  do {                                               \
+    hipError_t e_ = (call);                          \
+    if (e_ != hipSuccess) return asc::hip_fail(e_, #call); \
+  } while (0)
+
+#define ASC_CHECK(cond, code, msg)        \
+  do {                                    \
+    if (!(cond)) return asc::fail(code, msg); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Device-side views of volumes and images.
+// ---------------------------------------------------------------------------
+// Volume: zero-padded by one voxel on every side, x fastest. pW = W + 2,
+// pS = (W + 2) * (H + 2). Texel (i, j, k) of the cropped volume lives at
+// ((k + 1) * (H + 2) + (j + 1)) * pW + (i + 1). The zero border realises the
+// sampler's CLK_ADDRESS_CLAMP border colour for every in-box sample.
+struct VolDev {
+  const void* data;
+  int W, H, D;
+  int pW, pS;
+  int storage;   // ASC_STORAGE_*
+  float vscale;  // raw texel value -> normalised value (1/65535 for U16, else 1)
+};
+
+struct ImgDev {
+  const void* data;
+  int W, H;
+  int bps;  // 8, 16 or 32
+};
+
+}  // namespace asc
+
+// ---------------------------------------------------------------------------
+// Host objects behind the opaque handles.
+// ---------------------------------------------------------------------------
+struct asc_context_s {
+  int device;
+  hipStream_t own_stream;
+  hipStream_t stream;
+  // NCC scratch (block partials + result)
+  double* ncc_partials = nullptr;
+  size_t ncc_partials_cap = 0;  // number of blocks
+  double* ncc_result = nullptr;  // 10 doubles
+  double* ncc_host = nullptr;    // pinned
+  float* conv_wts = nullptr;     // large Gaussian/sharpen weight sets
+};
+
+struct asc_volume_s {
+  asc_context ctx;
+  asc_volume_desc desc;
+  asc::VolDev dev;
+  void* dptr;
+  size_t bytes;
+};
+
+struct asc_image_s {
+  asc_context ctx;
+  asc::ImgDev dev;
+  void* dptr;
+  size_t bytes;
+};
+
+namespace asc {
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (kernels.hip).
+// ---------------------------------------------------------------------------
+int launch_drr(asc_context ctx, const VolDev& v, const float* imv, const float* vp, float step, float intensity,
+               float cutoff, float* out, unsigned w, unsigned h);
+int launch_rad(asc_context ctx, const ImgDev& img, const float* plane, const float* vp, float* out, unsigned w,
+               unsigned h, bool background, float threshold);
+int launch_filter(asc_context ctx, const asc_filter& f, const float* in, float* out, int w, int h);
+int launch_binary(asc_context ctx, int op, const float* a, const float* b, float* out, size_t n);
+int launch_mask_multiply(asc_context ctx, float* drr, float* rad, const float* bg, float* mask, size_t n);
+int launch_composite(asc_context ctx, const float* s1, const float* s2, const float* s3, float* out, size_t n);
+int launch_fill_f32(asc_context ctx, float* dst, float v, size_t n);
+int ncc_moments(asc_context ctx, const float* f, const float* g, const float* m, unsigned n, double* out9,
+                double* ncc);
+
+// NCC from the nine moments, combined in double (Ncc.cpp:180-220 semantics).
+__host__ __device__ inline double ncc_from_moments(const double* m)
+{
+  // m = {sum mask, sum f, sum g, count, S_f, S_g, S_ff, S_gg, S_fg}
+  const double nb = m[0];
+  const double mf = m[1] / nb, mg = m[2] / nb;
+  const double n = m[3];
+  const double num = m[8] - mg * m[4] - mf * m[5] + n * mf * mg;
+  double d1 = m[6] - 2.0 * mf * m[4] + n * mf * mf;
+  double d2 = m[7] - 2.0 * mg * m[5] + n * mg * mg;
+  // both are sums of squares; rounding in the one-pass form must not make
+  // them negative
+  d1 = d1 < 0.0 ? 0.0 : d1;
+  d2 = d2 < 0.0 ? 0.0 : d2;
+  const double den = sqrt(d1 * d2);
+  if (den < 1e-5) return 1e5;
+  return num / den;
+}
+
+enum { BIN_MULTIPLY = 0, BIN_MERGE = 1, BIN_DRR_MASK = 2 };
+
+}  // namespace asc

@@ -1,0 +1,346 @@
+// Per-operator kernels: one launch per reference operator (the "plain"
+// form that gpu::View and Tracker::trackFrame drive one call at a time).
+#include "device.hpp"
+
+namespace asc {
+namespace {
+
+constexpr int kBlock = 256;
+
+// 16 x 16 pixel block made of four 8 x 8 wave tiles: the 64 rays of a wave
+// are spatial neighbours, so their samples hit neighbouring voxels.
+__device__ __forceinline__ void tile_pixel(unsigned bx, unsigned by, unsigned& x, unsigned& y)
+{
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  x = bx * 16 + (wave & 1) * 8 + (lane & 7);
+  y = by * 16 + (wave >> 1) * 8 + (lane >> 3);
+}
+
+struct DrrArgs {
+  float imv[12];
+  float vp[4];
+  float step, intensity, cutoff_raw, step_vscale;
+  unsigned w, h;
+};
+
+template <int ST>
+__global__ __launch_bounds__(kBlock) void drr_kernel(VolDev v, DrrArgs a, float* __restrict__ out)
+{
+  unsigned x, y;
+  tile_pixel(blockIdx.x, blockIdx.y, x, y);
+  if (x >= a.w || y >= a.h) return;
+  dev::Ray r;
+  float val = 0.0f;
+  if (dev::ray_setup(a.imv, a.vp, a.step, x, y, a.w, a.h, r)) {
+    int n = 0;
+    val = dev::drr_finish(dev::march<ST>(v, r, a.cutoff_raw, n), a.step_vscale, a.intensity);
+  }
+  out[(size_t)y * a.w + x] = val;
+}
+
+struct RadArgs {
+  float plane[4];
+  float vp[4];
+  unsigned w, h;
+  float threshold;
+  int background;
+};
+
+__global__ __launch_bounds__(kBlock) void rad_kernel(ImgDev im, RadArgs a, float* __restrict__ out)
+{
+  unsigned x, y;
+  tile_pixel(blockIdx.x, blockIdx.y, x, y);
+  if (x >= a.w || y >= a.h) return;
+  out[(size_t)y * a.w + x] = dev::rad_pixel(im, a.plane, a.vp, x, y, a.w, a.h, a.background != 0, a.threshold);
+}
+
+struct GlobalGet {
+  const float* __restrict__ p;
+  int w;
+  __device__ __forceinline__ float operator()(int x, int y) const { return p[y * w + x]; }
+};
+
+__global__ __launch_bounds__(kBlock) void sobel_kernel(const float* __restrict__ in, float* __restrict__ out, int w,
+                                                       int h, float scale, float blend)
+{
+  unsigned x, y;
+  tile_pixel(blockIdx.x, blockIdx.y, x, y);
+  if ((int)x >= w || (int)y >= h) return;
+  out[y * w + x] = dev::sobel_px(GlobalGet{in, w}, x, y, w, h, scale, blend);
+}
+
+__global__ __launch_bounds__(kBlock) void contrast_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                          int w, int h, float alpha, float beta, int size)
+{
+  unsigned x, y;
+  tile_pixel(blockIdx.x, blockIdx.y, x, y);
+  if ((int)x >= w || (int)y >= h) return;
+  out[y * w + x] = dev::contrast_px(GlobalGet{in, w}, x, y, w, h, alpha, beta, size);
+}
+
+constexpr int kArgTaps = 19 * 19;   // radius <= 3: weights travel in the kernel arguments
+constexpr int kMaxTaps = 61 * 61;   // radius <= 10: weights from a device buffer
+
+struct ConvArgs {
+  int n;
+  int sharpen;
+  float contrast, threshold;
+  const float* wts_dev;  // used when n*n > kArgTaps
+  float wts[kArgTaps];
+};
+
+__global__ __launch_bounds__(kBlock) void conv_kernel(const float* __restrict__ in, float* __restrict__ out, int w,
+                                                      int h, ConvArgs a)
+{
+  __shared__ float sw[kMaxTaps];
+  const int nn = a.n * a.n;
+  for (int i = threadIdx.x; i < nn; i += kBlock) sw[i] = nn > kArgTaps ? a.wts_dev[i] : a.wts[i];
+  __syncthreads();
+  unsigned x, y;
+  tile_pixel(blockIdx.x, blockIdx.y, x, y);
+  if ((int)x >= w || (int)y >= h) return;
+  const float c = dev::conv_px(GlobalGet{in, w}, x, y, w, h, sw, a.n);
+  out[y * w + x] = a.sharpen ? dev::sharpen_finish(in[y * w + x], c, a.contrast, a.threshold)
+                             : dev::gaussian_finish(c);
+}
+
+__global__ __launch_bounds__(kBlock) void binary_kernel(int op, const float* __restrict__ a,
+                                                        const float* __restrict__ b, float* __restrict__ o,
+                                                        size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  switch (op) {
+    case BIN_MULTIPLY: o[i] = a[i] * b[i]; break;                    // Mult.cl:16
+    case BIN_MERGE: o[i] = fminf(a[i] + b[i], 1.0f); break;          // Merger.cl:16
+    default: o[i] = (a[i] != 0.0f) ? 1.0f : 0.0f; break;             // DRRBackground.cl:15
+  }
+}
+
+// Tracker.cpp:747-752 fused: m = bg * (drr != 0); rad = rad * m; drr = drr * m
+__global__ __launch_bounds__(kBlock) void mask_multiply_kernel(float* __restrict__ drr, float* __restrict__ rad,
+                                                               const float* __restrict__ bg,
+                                                               float* __restrict__ mask, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  float m = (drr[i] != 0.0f) ? 1.0f : 0.0f;
+  if (bg) m = bg[i] * m;
+  if (mask) mask[i] = m;
+  rad[i] = rad[i] * m;
+  drr[i] = drr[i] * m;
+}
+
+// Compositor.cl:17-23
+__global__ __launch_bounds__(kBlock) void composite_kernel(const float* __restrict__ s1,
+                                                           const float* __restrict__ s2,
+                                                           const float* __restrict__ s3, float* __restrict__ o,
+                                                           size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float multi = (s3[i] < 0.5f) ? 0.0f : 1.0f;
+  o[3 * i + 0] = s1[i];
+  o[3 * i + 1] = multi * (s1[i] / 2.f + s2[i] / 2.f);
+  o[3 * i + 2] = s2[i];
+}
+
+__global__ __launch_bounds__(kBlock) void fill_kernel(float* __restrict__ o, float v, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) o[i] = v;
+}
+
+// NCC moments: stage 1 writes one 9-double partial per block, stage 2 sums
+// the partials in a fixed order (deterministic) and forms the NCC.
+constexpr int kNccPerThread = 8;
+
+__global__ __launch_bounds__(kBlock) void ncc_partial_kernel(const float* __restrict__ f,
+                                                             const float* __restrict__ g,
+                                                             const float* __restrict__ m, unsigned n,
+                                                             double* __restrict__ partials)
+{
+  __shared__ double lds[4 * 9];
+  dev::Moments acc;
+  const size_t base = (size_t)blockIdx.x * kBlock * kNccPerThread + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < kNccPerThread; k++) {
+    const size_t i = base + (size_t)k * kBlock;
+    if (i < n) acc.add(f[i], g[i], m[i]);
+  }
+  double v[9];
+  acc.to_double(v);
+  dev::block_sum9(v, lds);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 9; k++) partials[(size_t)blockIdx.x * 9 + k] = v[k];
+}
+
+__global__ __launch_bounds__(kBlock) void ncc_final_kernel(const double* __restrict__ partials, unsigned nblocks,
+                                                           double* __restrict__ result)
+{
+  __shared__ double lds[4 * 9];
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (unsigned b = threadIdx.x; b < nblocks; b += kBlock)
+    for (int k = 0; k < 9; k++) v[k] += partials[(size_t)b * 9 + k];
+  dev::block_sum9(v, lds);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 9; k++) result[k] = v[k];
+    result[9] = ncc_from_moments(v);
+  }
+}
+
+dim3 grid2(unsigned w, unsigned h) { return dim3((w + 15) / 16, (h + 15) / 16); }
+
+}  // namespace
+
+int launch_drr(asc_context ctx, const VolDev& v, const float* imv, const float* vp, float step, float intensity,
+               float cutoff, float* out, unsigned w, unsigned h)
+{
+  if (w == 0 || h == 0) return ASC_OK;
+  DrrArgs a;
+  for (int i = 0; i < 12; i++) a.imv[i] = imv[i];
+  for (int i = 0; i < 4; i++) a.vp[i] = vp[i];
+  a.step = step;
+  a.intensity = intensity;
+  // val > cutoff on normalised values  <=>  raw > cutoff / vscale
+  a.cutoff_raw = cutoff / v.vscale;
+  a.step_vscale = step * v.vscale;
+  a.w = w;
+  a.h = h;
+  switch (v.storage) {
+    case ASC_STORAGE_U16: drr_kernel<ASC_STORAGE_U16><<<grid2(w, h), kBlock, 0, ctx->stream>>>(v, a, out); break;
+    case ASC_STORAGE_F16: drr_kernel<ASC_STORAGE_F16><<<grid2(w, h), kBlock, 0, ctx->stream>>>(v, a, out); break;
+    case ASC_STORAGE_BF16: drr_kernel<ASC_STORAGE_BF16><<<grid2(w, h), kBlock, 0, ctx->stream>>>(v, a, out); break;
+    case ASC_STORAGE_F32: drr_kernel<ASC_STORAGE_F32><<<grid2(w, h), kBlock, 0, ctx->stream>>>(v, a, out); break;
+    default: return fail(ASC_ERR_UNSUPPORTED, "drr: unknown volume storage");
+  }
+  ASC_HIP(hipGetLastError());
+  return ASC_OK;
+}
+
+int launch_rad(asc_context ctx, const ImgDev& img, const float* plane, const float* vp, float* out, unsigned w,
+               unsigned h, bool background, float threshold)
+{
+  if (w == 0 || h == 0) return ASC_OK;
+  RadArgs a;
+  for (int i = 0; i < 4; i++) {
+    a.plane[i] = plane[i];
+    a.vp[i] = vp[i];
+  }
+  a.w = w;
+  a.h = h;
+  a.threshold = threshold;
+  a.background = background ? 1 : 0;
+  rad_kernel<<<grid2(w, h), kBlock, 0, ctx->stream>>>(img, a, out);
+  ASC_HIP(hipGetLastError());
+  return ASC_OK;
+}
+
+int launch_filter(asc_context ctx, const asc_filter& f, const float* in, float* out, int w, int h)
+{
+  if (w <= 0 || h <= 0) return ASC_OK;
+  const size_t bytes = sizeof(float) * (size_t)w * h;
+  if (!f.enabled) {
+    ASC_HIP(hipMemcpyAsync(out, in, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return ASC_OK;
+  }
+  switch (f.type) {
+    case ASC_FILTER_SOBEL:
+      sobel_kernel<<<grid2(w, h), kBlock, 0, ctx->stream>>>(in, out, w, h, f.p[0], f.p[1]);
+      break;
+    case ASC_FILTER_CONTRAST:
+      contrast_kernel<<<grid2(w, h), kBlock, 0, ctx->stream>>>(in, out, w, h, f.p[0], f.p[1], (int)f.p[2]);
+      break;
+    case ASC_FILTER_GAUSSIAN:
+    case ASC_FILTER_SHARPEN: {
+      std::vector<float> wts;
+      const int n = gauss_weights(f.p[0], wts);
+      if (n == 1) {  // GaussianFilter.cpp:131-133: filter size 1 is a copy
+        ASC_HIP(hipMemcpyAsync(out, in, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        return ASC_OK;
+      }
+      if (n * n > kMaxTaps) return fail(ASC_ERR_UNSUPPORTED, "filter radius too large (max 10)");
+      ConvArgs a;
+      a.n = n;
+      a.sharpen = f.type == ASC_FILTER_SHARPEN;
+      a.contrast = f.p[1] < 1.0f ? 1.0f : f.p[1];  // SharpenFilter::set_contrast clamps to >= 1
+      a.threshold = f.p[2];
+      a.wts_dev = nullptr;
+      if (n * n <= kArgTaps) {
+        for (int i = 0; i < n * n; i++) a.wts[i] = wts[i];
+      } else {
+        ASC_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->conv_wts) ASC_HIP(hipFree(ctx->conv_wts));
+        ctx->conv_wts = nullptr;
+        ASC_HIP(hipMalloc((void**)&ctx->conv_wts, sizeof(float) * n * n));
+        ASC_HIP(hipMemcpy(ctx->conv_wts, wts.data(), sizeof(float) * n * n, hipMemcpyHostToDevice));
+        a.wts_dev = ctx->conv_wts;
+      }
+      conv_kernel<<<grid2(w, h), kBlock, 0, ctx->stream>>>(in, out, w, h, a);
+      ASC_HIP(hipGetLastError());
+      return ASC_OK;
+    }
+    default: return fail(ASC_ERR_UNSUPPORTED, "filter type not implemented by the reference (median)");
+  }
+  ASC_HIP(hipGetLastError());
+  return ASC_OK;
+}
+
+int launch_binary(asc_context ctx, int op, const float* a, const float* b, float* out, size_t n)
+{
+  if (n == 0) return ASC_OK;
+  binary_kernel<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(op, a, b, out, n);
+  ASC_HIP(hipGetLastError());
+  return ASC_OK;
+}
+
+int launch_mask_multiply(asc_context ctx, float* drr, float* rad, const float* bg, float* mask, size_t n)
+{
+  if (n == 0) return ASC_OK;
+  mask_multiply_kernel<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(drr, rad, bg, mask, n);
+  ASC_HIP(hipGetLastError());
+  return ASC_OK;
+}
+
+int launch_composite(asc_context ctx, const float* s1, const float* s2, const float* s3, float* out, size_t n)
+{
+  if (n == 0) return ASC_OK;
+  composite_kernel<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(s1, s2, s3, out, n);
+  ASC_HIP(hipGetLastError());
+  return ASC_OK;
+}
+
+int launch_fill_f32(asc_context ctx, float* dst, float v, size_t n)
+{
+  if (n == 0) return ASC_OK;
+  fill_kernel<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(dst, v, n);
+  ASC_HIP(hipGetLastError());
+  return ASC_OK;
+}
+
+int ncc_moments(asc_context ctx, const float* f, const float* g, const float* m, unsigned n, double* out9,
+                double* ncc)
+{
+  const unsigned per_block = kBlock * kNccPerThread;
+  const unsigned nblocks = n == 0 ? 1 : (n + per_block - 1) / per_block;
+  if (ctx->ncc_partials_cap < nblocks) {
+    if (ctx->ncc_partials) ASC_HIP(hipFree(ctx->ncc_partials));
+    ctx->ncc_partials = nullptr;
+    ASC_HIP(hipMalloc((void**)&ctx->ncc_partials, sizeof(double) * 9 * nblocks));
+    ctx->ncc_partials_cap = nblocks;
+  }
+  if (!ctx->ncc_result) ASC_HIP(hipMalloc((void**)&ctx->ncc_result, sizeof(double) * 10));
+  if (!ctx->ncc_host) ASC_HIP(hipHostMalloc((void**)&ctx->ncc_host, sizeof(double) * 10, hipHostMallocDefault));
+  ncc_partial_kernel<<<nblocks, kBlock, 0, ctx->stream>>>(f, g, m, n, ctx->ncc_partials);
+  ASC_HIP(hipGetLastError());
+  ncc_final_kernel<<<1, kBlock, 0, ctx->stream>>>(ctx->ncc_partials, nblocks, ctx->ncc_result);
+  ASC_HIP(hipGetLastError());
+  ASC_HIP(hipMemcpyAsync(ctx->ncc_host, ctx->ncc_result, sizeof(double) * 10, hipMemcpyDeviceToHost, ctx->stream));
+  ASC_HIP(hipStreamSynchronize(ctx->stream));
+  if (out9)
+    for (int k = 0; k < 9; k++) out9[k] = ctx->ncc_host[k];
+  if (ncc) *ncc = ctx->ncc_host[9];
+  return ASC_OK;
+}
+
+}  // namespace asc

@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Pose-evaluation throughput of the fused DRR -> filters -> NCC path on one
+MI355X.
+
+Workload (default "cfg2"): seeded long-bone phantom 256x256x320 u16 at
+0.3 mm, two pinhole cameras 90 degrees apart with 1024x1024 detectors,
+targets rendered at a hidden pose. One step = one PSO-sized batch of
+`--particles` poses (uniform +/-3 mm / +/-3 deg around the hidden pose, the
+reference's PSO start range) x 2 cameras evaluated by the fused path, inputs
+resident in HBM. Prints one JSON line.
+
+Beside it: the plain one-eval-at-a-time GPU form (one launch per operator,
+blocking NCC per camera) and the CPU oracle on the host cores (bounded
+sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+STORAGES = {"u16": 0, "f16": 1, "bf16": 2, "f32": 3}
+BYTES_PER_VOXEL = {"u16": 2, "f16": 2, "bf16": 2, "f32": 4}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(A, synth, name: str, storage: str, ctx):
+    t0 = time.time()
+    wl = synth.workload(name, frames=2)
+    log(f"[bench] phantom {wl.volume.shape} generated in {time.time() - t0:.1f}s")
+    vol = A.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3, storage=STORAGES[storage])
+    RW, RH = wl.render
+    cams = [A.geometry.camera_from_mayacam2(*c.mayacam2()) for c in wl.cameras]
+    gt = wl.gt_poses[0]
+    # targets: full-frame DRR at the hidden pose (product path), blur, noise, background
+    rads = []
+    full = ctx.alloc(4 * RW * RH)
+    for i, cam in enumerate(cams):
+        inview, imv, _, _ = A.geometry.view_geometry(cam, gt, vol.desc, RW, RH)
+        assert inview
+        fvp = synth.full_frame_viewport(list(cam.viewport), RW, RH)
+        ctx.drr_render(vol, imv, fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, full, RW, RH)
+        rads.append(synth.target_from_drr(full.download((RH, RW)), 500 + i))
+    full.free()
+    scene = A.Scene(ctx, RW, RH)
+    pivot = A.geometry.frame_from_xyzypr(list(vol.desc.trans_center) + [0, 0, 0])
+    scene.add_volume(vol, synth.DRR_STEP, synth.DRR_INTENSITY, synth.DRR_CUTOFF, pivot)
+    for i, cam in enumerate(cams):
+        c = scene.add_camera(cam, A.Image(ctx, rads[i][1]))
+        scene.set_filters(c, 0, wl.drr_filters)
+        scene.set_filters(c, 1, wl.rad_filters)
+    return wl, vol, cams, rads, scene, gt
+
+
+def cpu_baseline(wl, cams, rads, gt, budget_s: float):
+    """Oracle trackFrame (float-faithful restatement, OpenMP over rows) on the
+    host cores: a bounded sample of the same workload's evaluations."""
+    import oracle as O
+    from autoscoper_amd import filters as F
+    from autoscoper_amd import synth
+
+    prep = O.volume_prep(wl.volume, (wl.voxel_mm,) * 3)
+    recs = [(F.to_records(wl.drr_filters), F.to_records(wl.rad_filters)) for _ in cams]
+    S = O.Scene(prep, (synth.DRR_STEP, synth.DRR_INTENSITY, synth.DRR_CUTOFF), [c.to_array() for c in cams],
+                [r[1] for r in rads], recs, wl.render)
+    rng = np.random.default_rng(77)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        p = gt + np.concatenate([rng.uniform(-3, 3, 3), rng.uniform(-3, 3, 3)])
+        S.track_frame(p)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return n / el, n, el, threads
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--particles", type=int, default=100)
+    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--storage", default="u16", choices=sorted(STORAGES))
+    ap.add_argument("--graph", action="store_true", help="replay the fused launches from a hipGraph")
+    ap.add_argument("--plain-evals", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(jobs and particle batches would shard across GPUs without collectives; see DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    import autoscoper_amd as A
+    from autoscoper_amd import synth
+
+    ctx = A.Context(0)
+    wl, vol, cams, rads, scene, gt = build_workload(A, synth, args.workload, args.storage, ctx)
+    RW, RH = wl.render
+    P = args.particles
+    rng = np.random.default_rng(synth.SEED_START)
+    nb = args.warmup + args.steps
+    batches = [gt + np.concatenate([rng.uniform(-3, 3, (P, 3)), rng.uniform(-3, 3, (P, 3))], 1) for _ in range(nb)]
+
+    scene.set_timing(not args.graph)
+    scene.use_graph(args.graph)
+    log(f"[bench] warmup {args.warmup} x {P} poses")
+    for k in range(args.warmup):
+        scene.eval_poses(batches[k])
+    ctx.synchronize()
+    kern_ms = np.zeros(3)
+    samples = 0
+    log(f"[bench] timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for k in range(args.warmup, nb):
+        scene.eval_poses(batches[k])
+        if not args.graph:
+            kern_ms += scene.last_timing()
+            samples += scene.last_info(P * len(cams))[1]
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    ms_step = 1e3 * el / args.steps
+    evals_s = P * args.steps / el
+    info, last_samples = scene.last_info(P * len(cams))
+    if args.graph:
+        # per-kernel time from an eager pass over the same batches
+        scene.use_graph(False)
+        scene.set_timing(True)
+        for k in range(args.warmup, nb):
+            scene.eval_poses(batches[k])
+            kern_ms += scene.last_timing()
+            samples += scene.last_info(P * len(cams))[1]
+    drr_ms = kern_ms[0] / args.steps
+    bpv = BYTES_PER_VOXEL[args.storage]
+    bytes_per_launch = 8 * bpv * samples / args.steps  # 8 texels per trilinear sample
+    achieved = bytes_per_launch / (drr_ms * 1e-3)
+    peak = 8.0e12
+    log(f"[bench] {evals_s:.1f} evals/s, {ms_step:.2f} ms/step, kernels ms {kern_ms / args.steps}")
+
+    # plain one-eval-at-a-time form
+    t0 = time.perf_counter()
+    for k in range(args.plain_evals):
+        scene.track_frame_plain(batches[args.warmup][k % P])
+    plain_s = args.plain_evals / (time.perf_counter() - t0)
+    log(f"[bench] plain form {plain_s:.1f} evals/s")
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU oracle baseline (~{args.cpu_seconds:.0f}s)")
+        v, n, el_c, threads = cpu_baseline(wl, cams, rads, gt, args.cpu_seconds)
+        cpu = {"value": v, "unit": "pose-evals/s", "cores": threads, "kind": "port",
+               "sample": f"{n} pose evaluations of the same workload (float-faithful oracle trackFrame, "
+                         f"OpenMP over rows, {el_c:.1f}s)"}
+
+    tiles = info[:, 4].sum()
+    out = {
+        "metric": "pose evals/sec (DRR+filters+NCC, 2 cams 1024^2)",
+        "value": evals_s,
+        "unit": "pose-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded phantom, rendered targets)",
+        "config": {
+            "workload": f"{args.workload}: {wl.volume.shape[2]}x{wl.volume.shape[1]}x{wl.volume.shape[0]} "
+                        f"{args.storage} volume @{wl.voxel_mm}mm, {len(cams)} cams {RW}x{RH}, "
+                        f"{P} poses per batch",
+            "volume_storage": args.storage,
+            "particles_per_step": P,
+            "cameras": len(cams),
+            "render": [RW, RH],
+            "graph": bool(args.graph),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved / 1e9,
+            "peak": peak / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / peak,
+            "traffic": None,
+            "kernel": "drr_tiles",
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "ray_samples_per_launch": samples / args.steps,
+            "kernel_ms": {"drr_tiles": drr_ms, "post_tiles": kern_ms[1] / args.steps,
+                          "finalize_items": kern_ms[2] / args.steps},
+        },
+        "cpu_baseline": cpu,
+        "plain_per_eval": {"value": plain_s, "unit": "pose-evals/s",
+                           "speedup_of_fused": evals_s / plain_s if plain_s > 0 else None},
+        "per_eval": {"bbox_pixels": int((info[:, 1] * info[:, 2]).sum() / P),
+                     "ray_samples": float(last_samples) / P, "tiles": int(tiles / P)},
+    }
+    print(json.dumps(out), flush=True)
+    scene.close()
+    vol.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

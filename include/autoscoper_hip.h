@@ -1,0 +1,305 @@
+/*
+ * autoscoper_hip.h -- C ABI of the MI355X (gfx950) backend for Autoscoper's
+ * 2D-3D registration cost path: DRR ray march -> image filters -> DRR mask
+ * -> normalised cross-correlation.
+ *
+ * Plain pointers and sizes only. Every entry point returns an int status
+ * (ASC_OK == 0, negative on error; asc_last_error() gives the message for
+ * the calling thread). Nothing here calls exit(). Device pointers are HIP
+ * device addresses; work is issued on the context's stream (the context's
+ * own stream unless asc_context_set_stream() installed another one).
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the reference checkout, libautoscoper/src/...). The reference binds its
+ * backend at compile time (gpu/opencl or gpu/cuda); the C++ facade in
+ * autoscoper_xromm.hpp maps those class signatures onto this ABI.
+ */
+#ifndef AUTOSCOPER_HIP_H
+#define AUTOSCOPER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASC_API __attribute__((visibility("default")))
+
+/* ---- status codes -------------------------------------------------------- */
+#define ASC_OK 0
+#define ASC_ERR_INVALID (-1)      /* bad argument / handle */
+#define ASC_ERR_HIP (-2)          /* HIP runtime error */
+#define ASC_ERR_NOMEM (-3)        /* allocation failed / over budget */
+#define ASC_ERR_EMPTY_VOLUME (-4) /* VolumeDescription.cpp:196-199 (exit(0) there) */
+#define ASC_ERR_TOO_LARGE (-5)    /* View.cpp:448-455, Tracker.cpp:625-629 (throw there) */
+#define ASC_ERR_UNSUPPORTED (-6)  /* e.g. bit depth, RadRenderer.cpp:93-96 */
+#define ASC_ERR_NO_DEVICE (-7)
+
+/* ---- handles and records ------------------------------------------------- */
+typedef struct asc_context_s* asc_context;
+typedef struct asc_volume_s* asc_volume;
+typedef struct asc_image_s* asc_image;
+typedef struct asc_scene_s* asc_scene;
+
+/* Volume storage on the device. U16 is exact (the reference uploads
+ * CL_UNORM_INT16, VolumeDescription.cpp:287-303); F16/BF16 store the same
+ * normalised value rounded; F32 stores it exactly. */
+enum { ASC_STORAGE_U16 = 0, ASC_STORAGE_F16 = 1, ASC_STORAGE_BF16 = 2, ASC_STORAGE_F32 = 3 };
+
+/* Filter types, numbered as the reference's Filter enum (Filter.hpp:60-66). */
+enum {
+  ASC_FILTER_CONTRAST = 0, /* p = {alpha, beta, size}         ContrastFilter.cpp:57-86 */
+  ASC_FILTER_SOBEL = 1,    /* p = {scale, blend}              SobelFilter.cpp:56-83 */
+  ASC_FILTER_MEDIAN = 2,   /* declared but never implemented by the reference: rejected */
+  ASC_FILTER_GAUSSIAN = 3, /* p = {radius}                    GaussianFilter.cpp:75-150 */
+  ASC_FILTER_SHARPEN = 4   /* p = {radius, contrast, threshold} SharpenFilter.cpp:80-173 */
+};
+
+typedef struct {
+  int type;
+  int enabled; /* Filter::enabled(), Filter.hpp:86-87; a disabled filter is a copy */
+  float p[4];
+} asc_filter;
+
+/* xromm::Camera state (Camera.hpp): coord_frame (rotation columns = camera
+ * x, y, z axes in world space; translation = camera centre), image_plane
+ * (4 world-space corners), viewport (film-plane centre and extent at z=-2),
+ * size (pixels). */
+typedef struct {
+  double rotation[9];
+  double translation[3];
+  double image_plane[12];
+  double viewport[4];
+  double size[2];
+} asc_camera;
+
+/* gpu::VolumeDescription accessors (VolumeDescription.hpp:62-87). */
+typedef struct {
+  int dims[3];     /* cropped extent */
+  int crop_min[3]; /* first non-zero voxel per axis after flipping */
+  int crop_max[3];
+  float inv_scale[3];
+  float inv_trans[3];
+  double trans_center[3];
+  float min_value;
+  float max_value;
+  int storage;
+  int reserved;
+} asc_volume_desc;
+
+/* ---- context, memory (OpenCL.cpp:668-782, 1027-1095) ---------------------- */
+ASC_API const char* asc_last_error(void);
+ASC_API int asc_version(void);
+ASC_API int asc_device_count(int* count);
+ASC_API int asc_context_create(int device, asc_context* out);
+ASC_API int asc_context_destroy(asc_context ctx);
+/* Install a caller-owned hipStream_t (NULL restores the context's own). */
+ASC_API int asc_context_set_stream(asc_context ctx, void* hip_stream);
+ASC_API void* asc_context_stream(asc_context ctx);
+ASC_API int asc_synchronize(asc_context ctx);
+
+ASC_API int asc_malloc(asc_context ctx, size_t bytes, void** dptr);
+ASC_API int asc_free(asc_context ctx, void* dptr);
+/* Buffer::read (host -> device, blocking), OpenCL.cpp:1043-1049. */
+ASC_API int asc_memcpy_h2d(asc_context ctx, void* dst, const void* src, size_t bytes);
+/* Buffer::write (device -> host, blocking), OpenCL.cpp:1051-1057. */
+ASC_API int asc_memcpy_d2h(asc_context ctx, void* dst, const void* src, size_t bytes);
+/* Buffer::copy, OpenCL.cpp:1059-1067 (stream-ordered). */
+ASC_API int asc_memcpy_d2d(asc_context ctx, void* dst, const void* src, size_t bytes);
+/* Buffer::fill(float), OpenCL.cpp:1077-1095; gpu::fill in the CUDA backend. */
+ASC_API int asc_fill_f32(asc_context ctx, float* dst, float value, size_t count);
+/* Buffer::fill(char), OpenCL.cpp:1069-1075. */
+ASC_API int asc_fill_u8(asc_context ctx, void* dst, unsigned char value, size_t bytes);
+
+/* ---- a1: volume (VolumeDescription.cpp:152-305) --------------------------- */
+/* data: width*height*depth voxels, x fastest, bps 8 or 16 (unsigned).
+ * flip: Volume::flipX/Y/Z; scale: mm per voxel. Flips, crops to the
+ * bounding box of non-zero voxels and uploads in `storage` form. */
+ASC_API int asc_volume_create(asc_context ctx, const void* data, int width, int height, int depth, int bps,
+                              const int flip[3], const double scale[3], int storage, asc_volume* out,
+                              asc_volume_desc* desc);
+ASC_API int asc_volume_describe(asc_volume vol, asc_volume_desc* desc);
+ASC_API int asc_volume_destroy(asc_volume vol);
+/* Device bytes held by the volume (for memory-budget admission). */
+ASC_API int asc_volume_bytes(asc_volume vol, size_t* bytes);
+
+/* ---- 2-D images: radiograph (RadRenderer::set_rad, RadRenderer.cpp:82-100)
+ * and background (BackgroundRenderer::set_back, BackgroundRenderer.cpp:76-103).
+ * bps 8 / 16: UNORM; bps 32: float32 as stored. */
+ASC_API int asc_image_create(asc_context ctx, const void* data, int width, int height, int bps, asc_image* out);
+ASC_API int asc_image_update(asc_image img, const void* data);
+ASC_API int asc_image_destroy(asc_image img);
+
+/* ---- a2/a3: geometry (host, double precision, reference operation order) - */
+/* Frames are 12 doubles {rotation[9], translation[3]} as CoordFrame. */
+ASC_API int asc_frame_from_xyzypr(const double xyzypr[6], double frame[12]);      /* CoordFrame.cpp:100-108 */
+ASC_API int asc_frame_from_xyz_axis_angle(const double p[6], double frame[12]);   /* CoordFrame.cpp:163-175 */
+ASC_API int asc_frame_to_xyzypr(const double frame[12], double xyzypr[6]);        /* CoordFrame.cpp:177-189 */
+ASC_API int asc_frame_mul(const double a[12], const double b[12], double out[12]); /* CoordFrame.cpp:478-527 */
+ASC_API int asc_frame_inverse(const double a[12], double out[12]);                 /* CoordFrame.cpp:335-353 */
+/* Camera::loadMayaCam2 arithmetic (Camera.cpp:229-362) from the parsed K, R
+ * (file rows, row-major 3x3) and t, plus image size. */
+ASC_API int asc_camera_from_mayacam2(const double K[9], const double R[9], const double t[3], const double size[2],
+                                     asc_camera* out);
+/* RayCaster::setInvModelView, RayCaster.cpp:92-142. */
+ASC_API int asc_inv_model_view(const double imv16[16], const float inv_scale[3], const float inv_trans[3],
+                               float out16[16]);
+/* Tracker.cpp:604-629 for one camera: model-view, inverse model-view (a2),
+ * calculate_viewport (a3, Tracker.cpp:923-1032) and render size.
+ * Returns 1 in view, 0 not in view (cost 9999), <0 error. */
+ASC_API int asc_view_geometry(const asc_camera* cam, const double xyzypr[6], const asc_volume_desc* desc,
+                              int render_w, int render_h, float imv_out[12], double vp_out[4],
+                              unsigned size_out[2]);
+
+/* ---- a5: DRR (RayCaster::render, RayCaster.cpp:158-193; RayCaster.cl) ----- */
+ASC_API int asc_drr_render(asc_context ctx, asc_volume vol, const float imv[12], const float viewport[4],
+                           float step, float intensity, float cutoff, float* out, unsigned width, unsigned height);
+/* ---- a6: radiograph resample (RadRenderer::render, RadRenderer.cpp:117-145) */
+ASC_API int asc_rad_render(asc_context ctx, asc_image img, const float image_plane[4], const float viewport[4],
+                           float* out, unsigned width, unsigned height);
+/* ---- a12: background mask (BackgroundRenderer::render, BackgroundRenderer.cpp:105-129) */
+ASC_API int asc_background_render(asc_context ctx, asc_image img, const float image_plane[4],
+                                  const float viewport[4], float threshold, float* out, unsigned width,
+                                  unsigned height);
+
+/* ---- a8-a11: filters (Filter::apply) --------------------------------------- */
+ASC_API int asc_sobel(asc_context ctx, const float* in, float* out, int width, int height, float scale,
+                      float blend);                                   /* SobelFilter.cpp:66-83 */
+ASC_API int asc_contrast(asc_context ctx, const float* in, float* out, int width, int height, float alpha,
+                         float beta, int size);                       /* ContrastFilter.cpp:66-86 */
+/* GaussianFilter::set_radius weights (GaussianFilter.cpp:75-127); *size = N
+ * (N*N weights written when capacity allows; N == 1 means "copy"). */
+ASC_API int asc_gaussian_weights(float radius, float* out, int capacity, int* size);
+ASC_API int asc_gaussian(asc_context ctx, const float* in, float* out, int width, int height,
+                         float radius);                               /* GaussianFilter.cpp:129-150 */
+ASC_API int asc_sharpen(asc_context ctx, const float* in, float* out, int width, int height, float radius,
+                        float contrast, float threshold);             /* SharpenFilter.cpp:149-173 */
+ASC_API int asc_filter_apply(asc_context ctx, const asc_filter* f, const float* in, float* out, int width,
+                             int height);
+/* View::filter (View.cpp:475-555): applies the chain with `scratch` as the
+ * ping-pong buffer so that the last stage lands in `out`. */
+ASC_API int asc_filter_chain(asc_context ctx, const asc_filter* filters, int count, const float* in, float* out,
+                             float* scratch, int width, int height);
+
+/* ---- a13: DRR mask and multiplies ----------------------------------------- */
+ASC_API int asc_drr_background(asc_context ctx, const float* src, float* dest, unsigned width,
+                               unsigned height);                      /* DRRBackground.cl */
+ASC_API int asc_multiply(asc_context ctx, const float* a, const float* b, float* dest, unsigned width,
+                         unsigned height);                            /* Mult.cl */
+/* Tracker.cpp:747-752 in one pass: m = bg * (drr != 0); rad *= m; drr *= m.
+ * bg may be NULL (all ones). mask receives m. */
+ASC_API int asc_mask_multiply(asc_context ctx, float* drr, float* rad, const float* bg, float* mask,
+                              unsigned width, unsigned height);
+
+/* ---- a14: NCC (gpu::ncc, Ncc.cpp:180-220) ----------------------------------- */
+/* Intended semantics of the reference (exact sums): returns 1e5 when the
+ * denominator is < 1e-5. moments (optional, 9 doubles): {sum mask, sum f,
+ * sum g, count(mask>0.5), S_f, S_g, S_ff, S_gg, S_fg} with the S_ over
+ * mask > 0.5. Blocking (returns a host value, like the reference). */
+ASC_API int asc_ncc(asc_context ctx, const float* f, const float* g, const float* mask, unsigned n, double* ncc,
+                    double moments[9]);
+/* ncc_init / ncc_deinit (Ncc.cpp:151-178): reserve scratch for n elements. */
+ASC_API int asc_ncc_init(asc_context ctx, unsigned max_n);
+ASC_API int asc_ncc_deinit(asc_context ctx);
+
+/* ---- a19/a20: display helpers ---------------------------------------------- */
+ASC_API int asc_merge(asc_context ctx, const float* a, const float* b, float* dest, unsigned width,
+                      unsigned height);                               /* Merger.cl */
+ASC_API int asc_composite(asc_context ctx, const float* src1, const float* src2, const float* src3,
+                          const float* src4, float* dest, unsigned width, unsigned height); /* Compositor.cl */
+
+/* ---- a15/a16 batched: scene + fused evaluation ----------------------------
+ * A scene holds what Tracker::load sets up (Tracker.cpp:244-335): volumes
+ * with their DRR parameters and pivots, cameras with radiograph, optional
+ * background and DRR/rad filter chains, and the render resolution. */
+ASC_API int asc_scene_create(asc_context ctx, int render_width, int render_height, asc_scene* out);
+ASC_API int asc_scene_destroy(asc_scene scene);
+/* pivot: 12-double frame (Trial::getVolumeMatrix); NULL = identity. */
+ASC_API int asc_scene_add_volume(asc_scene scene, asc_volume vol, float step, float intensity, float cutoff,
+                                 const double pivot[12], int* index);
+ASC_API int asc_scene_set_drr_params(asc_scene scene, int volume, float step, float intensity, float cutoff);
+/* Trial::getVolumeMatrix(volume) as a 12-double frame. */
+ASC_API int asc_scene_volume_pivot(asc_scene scene, int volume, double pivot[12]);
+ASC_API int asc_scene_add_camera(asc_scene scene, const asc_camera* cam, asc_image rad, int* index);
+ASC_API int asc_scene_set_rad(asc_scene scene, int camera, asc_image rad);
+ASC_API int asc_scene_set_background(asc_scene scene, int camera, asc_image bg, float threshold);
+/* which: 0 = DRR chain, 1 = radiograph chain (View::drrFilters/radFilters). */
+ASC_API int asc_scene_set_filters(asc_scene scene, int camera, int which, const asc_filter* filters, int count);
+/* Evaluate n absolute poses (Tracker::trackFrame, Tracker.cpp:583-773).
+ * volumes[n] picks the volume per pose. per_camera (n*ncam, optional) gets
+ * 1 - NCC or 9999 per camera; costs (n, optional) the minimizationFunc
+ * total (Tracker.cpp:814-826). Fused: all poses x cameras in one set of
+ * launches. */
+ASC_API int asc_scene_eval_poses(asc_scene scene, const int* volumes, const double* xyzypr, int n, double* costs,
+                                 double* per_camera);
+/* Evaluate n manipulator offsets relative to stored float poses
+ * (Tracker::minimizationFunc, Tracker.cpp:775-827): stores n*7 floats
+ * {x, y, z, qw, qx, qy, qz}, manips n*6 doubles. */
+ASC_API int asc_scene_eval_manips(asc_scene scene, const int* volumes, const float* stores, const double* manips,
+                                  int n, double* costs, double* per_camera);
+/* Record the fused launches of an n-pose batch in a hipGraph and replay it
+ * for later batches of the same size (1 = on, 0 = off). */
+ASC_API int asc_scene_use_graph(asc_scene scene, int enable);
+/* Diagnostics of the last batch: info[n*ncam*8] = {in_view, w, h,
+ * mask_count, tiles, 0, 0, 0}, samples = total ray samples. */
+ASC_API int asc_scene_last_info(asc_scene scene, int* info, int capacity, int64_t* samples);
+/* Keep the filtered, masked DRR/radiograph and mask of every item of the
+ * following batches for asc_scene_last_images (costs extra stores). */
+ASC_API int asc_scene_set_debug(asc_scene scene, int enable);
+/* Time the fused kernels of eager (non-graph) batches with HIP events on
+ * the scene's stream; ms = {ray march, filters+moments, finalize} of the
+ * last batch. */
+ASC_API int asc_scene_set_timing(asc_scene scene, int enable);
+ASC_API int asc_scene_last_timing(asc_scene scene, float ms[3]);
+/* moments[n*ncam*10]: the nine NCC moments (asc_ncc order) + NCC per item. */
+ASC_API int asc_scene_last_moments(asc_scene scene, double* moments, int capacity);
+/* Copy the last batch's filtered, masked DRR and radiograph and the mask of
+ * (pose i, camera c) to host buffers of w*h floats (any may be NULL). */
+ASC_API int asc_scene_last_images(asc_scene scene, int pose, int camera, float* drr, float* rad, float* mask);
+/* Plain form: one pose evaluated camera by camera with one launch per
+ * operator and a blocking NCC readback per camera, as trackFrame does. */
+ASC_API int asc_scene_track_frame_plain(asc_scene scene, int volume, const double xyzypr[6], double* per_camera,
+                                        double* cost);
+
+/* ---- pose store helpers (KeyCurve<float> / Quatf, KeyCurve.hpp:330-356,
+ * Quaternion.hpp:80-94,220-243) -------------------------------------------- */
+ASC_API int asc_pose_store_set(float store[7], const double xyzypr[6]);
+ASC_API int asc_pose_store_get(const float store[7], double xyzypr[6]);
+/* minimizationFunc pose composition (Tracker.cpp:777-796). */
+ASC_API int asc_compose_pose(const float store[7], const double pivot[12], const double manip[6],
+                             double xyzypr[6]);
+
+/* ---- a17/a18: optimizers (Tracker::optimize, Tracker.cpp:337-580) --------- */
+typedef struct {
+  int method;          /* 0 PSO + downhill simplex, 1 simplex only */
+  unsigned max_iter;   /* PSO MAX_EPOCHS */
+  unsigned max_stall;  /* PSO MAX_STALL */
+  double min_limit;    /* PSO START_RANGE_MIN */
+  double max_limit;    /* PSO START_RANGE_MAX */
+  double offsets[6];   /* simplex vertex offsets (Trial::offsets) */
+  unsigned seed;       /* srand seed (Autoscoper_RANDOM_SEED) */
+  int particles;       /* NUM_OF_PARTICLES (100 in the reference) */
+  int synchronous;     /* 0: reference asynchronous PSO, one eval per call;
+                          1: synchronous PSO, all particles of an epoch in one fused batch */
+  int volume;          /* volume index (Trial::current_volume) */
+} asc_optimize_options;
+
+typedef struct {
+  int pso_epochs;
+  int simplex_iterations;
+  double pso_best_cost;
+  double final_cost;
+  double initial_cost;
+  int64_t evaluations;
+} asc_optimize_stats;
+
+/* One frame of Tracker::optimize with guess 0: store (7 floats) is updated
+ * in place with the optimised pose. */
+ASC_API int asc_optimize_frame(asc_scene scene, float store[7], const asc_optimize_options* opt,
+                               asc_optimize_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AUTOSCOPER_HIP_H */

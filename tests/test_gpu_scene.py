@@ -1,0 +1,154 @@
+"""Fused batched evaluation (Tracker::trackFrame / minimizationFunc for many
+poses at once) through the C ABI, against the oracle's trackFrame.
+
+Tolerances: per-camera 1 - NCC within 1e-5 of the oracle; render sizes,
+in-view flags, mask pixel counts and ray-sample counts bit-exact; the plain
+one-eval-at-a-time path, graph replays and mixed batches agree with the
+fused eager result (bit-exact where the reduction order is the same).
+"""
+import numpy as np
+import pytest
+
+from autoscoper_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def build_scene(asc, ctx, tiny, rad_kind=1, bg=None):
+    wl = tiny["wl"]
+    vol = asc.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3)
+    RW, RH = tiny["render"]
+    sc = asc.Scene(ctx, RW, RH)
+    pivot = asc.geometry.frame_from_xyzypr(list(vol.desc.trans_center) + [0, 0, 0])
+    sc.add_volume(vol, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, pivot)
+    for i, cam in enumerate(tiny["cams"]):
+        img = asc.Image(ctx, tiny["rads"][i][rad_kind])
+        c = sc.add_camera(asc.AscCamera.from_array(cam), img)
+        sc.set_filters(c, 0, wl.drr_filters)
+        sc.set_filters(c, 1, wl.rad_filters)
+        if bg is not None:
+            sc.set_background(c, asc.Image(ctx, bg[i]), 0.3)
+    return sc, vol, pivot
+
+
+def oracle_scene(orc, asc, tiny, rad_kind=1, bg=None):
+    wl = tiny["wl"]
+    recs = [(asc.filters.to_records(wl.drr_filters), asc.filters.to_records(wl.rad_filters)) for _ in tiny["cams"]]
+    return orc.Scene(tiny["prep"], (synth.DRR_STEP, synth.DRR_INTENSITY, 0.0), tiny["cams"],
+                     [r[rad_kind] for r in tiny["rads"]], recs, tiny["render"],
+                     bg_threshold=0.3 if bg is not None else -1.0, bgs=bg)
+
+
+def test_batch_matches_oracle(asc, ctx, orc, tiny):
+    sc, vol, _ = build_scene(asc, ctx, tiny)
+    S = oracle_scene(orc, asc, tiny)
+    rng = np.random.default_rng(21)
+    poses = tiny["wl"].gt_poses[0] + np.concatenate([rng.uniform(-3, 3, (12, 3)), rng.uniform(-6, 6, (12, 3))], 1)
+    poses[5, :3] += 900.0  # out of view for both cameras
+    costs, per = sc.eval_poses(poses)
+    info, samples = sc.last_info(len(poses) * 2)
+    tot = 0
+    for i, p in enumerate(poses):
+        r = S.track_frame(p)
+        for c in range(2):
+            assert info[2 * i + c, 0] == r["info"][c, 0]
+            if r["info"][c, 0] == 1:
+                assert tuple(info[2 * i + c, 1:4]) == tuple(r["info"][c, 1:4])  # w, h, mask count
+                assert abs(per[i, c] - r["corr"][c]) <= 1e-5
+            else:
+                assert per[i, c] == 9999.0
+        ref_cost = r["corr"].sum()
+        assert abs(costs[i] - (ref_cost if ref_cost >= 0 else 9999)) <= 2e-5
+        tot += r["samples"]
+    assert samples == tot  # ray-sample counts identical
+    sc.close()
+    vol.close()
+
+
+def test_batch_images_match_oracle_pipeline(asc, ctx, orc, tiny):
+    sc, vol, _ = build_scene(asc, ctx, tiny)
+    sc.set_debug(True)
+    prep = tiny["prep"]
+    RW, RH = tiny["render"]
+    pose = tiny["wl"].gt_poses[0] + np.array([0.7, -0.4, 0.2, 2.0, -1.0, 3.0])
+    sc.eval_poses(pose[None])
+    for c, cam in enumerate(tiny["cams"]):
+        _, imv, vp, (w, h) = orc.view_geometry(cam, pose, prep["inv_scale"], prep["inv_trans"], RW, RH)
+        d, g, m = sc.last_images(0, c, w, h)
+        drr, _ = orc.drr(prep["volume"], imv[:12], vp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, w, h)
+        drr_f = orc.filter_chain(asc.filters.to_records(tiny["wl"].drr_filters), drr)
+        rad = orc.rad(tiny["rads"][c][1], cam[24:28], vp, w, h)
+        rad_f = orc.filter_chain(asc.filters.to_records(tiny["wl"].rad_filters), rad)
+        mask = (drr_f != 0).astype(np.float32)
+        assert np.array_equal(m, mask)
+        assert np.allclose(d, drr_f * mask, rtol=2e-4, atol=1e-7)
+        assert np.allclose(g, rad_f * mask, rtol=1e-5, atol=1e-7)
+    sc.close()
+    vol.close()
+
+
+def test_background_mask_path(asc, ctx, orc, tiny):
+    rng = np.random.default_rng(4)
+    RW, RH = tiny["render"]
+    bgs = [rng.random((RH, RW)).astype(np.float32) for _ in tiny["cams"]]
+    sc, vol, _ = build_scene(asc, ctx, tiny, bg=bgs)
+    S = oracle_scene(orc, asc, tiny, bg=bgs)
+    pose = tiny["wl"].gt_poses[0] + np.array([0.3, 0.1, -0.2, 1.0, 0.5, -0.5])
+    _, per = sc.eval_poses(pose[None])
+    r = S.track_frame(pose)
+    info, _ = sc.last_info(2)
+    assert np.array_equal(info[:, 3], r["info"][:, 3])
+    assert np.abs(per[0] - r["corr"]).max() <= 1e-5
+    sc.close()
+    vol.close()
+
+
+def test_plain_path_matches_fused(asc, ctx, tiny):
+    sc, vol, _ = build_scene(asc, ctx, tiny)
+    rng = np.random.default_rng(8)
+    poses = tiny["wl"].gt_poses[0] + np.concatenate([rng.uniform(-2, 2, (5, 3)), rng.uniform(-4, 4, (5, 3))], 1)
+    costs, per = sc.eval_poses(poses)
+    for i, p in enumerate(poses):
+        c, pc = sc.track_frame_plain(p)
+        assert np.abs(pc - per[i]).max() <= 1e-9
+        assert abs(c - costs[i]) <= 2e-9
+    sc.close()
+    vol.close()
+
+
+def test_graph_replay_and_mixed_batches(asc, ctx, tiny):
+    sc, vol, _ = build_scene(asc, ctx, tiny)
+    rng = np.random.default_rng(12)
+    A = tiny["wl"].gt_poses[0] + np.concatenate([rng.uniform(-2, 2, (6, 3)), rng.uniform(-4, 4, (6, 3))], 1)
+    B = tiny["wl"].gt_poses[0] + np.concatenate([rng.uniform(-2, 2, (6, 3)), rng.uniform(-4, 4, (6, 3))], 1)
+    ca, pa = sc.eval_poses(A)
+    cb, pb = sc.eval_poses(B)
+    # alone vs in the mix: identical (per-item work and reduction order are independent of the batch)
+    cm, pm = sc.eval_poses(np.concatenate([B[:3], A, B[3:]]))
+    assert np.array_equal(cm[3:9], ca) and np.array_equal(pm[3:9], pa)
+    assert np.array_equal(np.concatenate([cm[:3], cm[9:]]), cb)
+    # graph capture + several replays with eager work in between
+    sc.use_graph(True)
+    for k in range(4):
+        X = A if k % 2 == 0 else B
+        cg, pg = sc.eval_poses(X)
+        assert np.array_equal(cg, ca if k % 2 == 0 else cb)
+        sc.use_graph(False)
+        ce, _ = sc.eval_poses(B)
+        assert np.array_equal(ce, cb)
+        sc.use_graph(True)
+    sc.close()
+    vol.close()
+
+
+def test_manip_eval_matches_oracle_min_func(asc, ctx, orc, tiny):
+    sc, vol, pivot = build_scene(asc, ctx, tiny)
+    S = oracle_scene(orc, asc, tiny)
+    store = asc.geometry.pose_store_set(tiny["wl"].gt_poses[0] + np.array([0.5, -0.3, 0.2, 1.0, -0.5, 0.7]))
+    rng = np.random.default_rng(3)
+    manips = np.concatenate([rng.uniform(-1, 1, (7, 3)), rng.uniform(-2, 2, (7, 3))], 1)
+    costs, _ = sc.eval_manips(store[None], manips)
+    for i, m in enumerate(manips):
+        assert abs(costs[i] - S.min_func(store, pivot, m)) <= 2e-5
+    sc.close()
+    vol.close()
