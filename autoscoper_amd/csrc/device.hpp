@@ -112,18 +112,27 @@ __device__ __forceinline__ float trilinear(const VolDev& v, float X, float Y, fl
 // Ray march, RayCaster.cl:59-69: t runs from t_far down by 1 while t >
 // t_near (the same float loop, so the same sample count). Returns the sum
 // of raw texel values above the raw cutoff.
+//
+// The sample position is formed exactly as the reference does it
+// (p = o + t*d, then s*W - 0.5 for the sampler): the ray origin lies far
+// outside the unit box, so o + t*d cancels and its rounding is ~1e-4 of a
+// DRR pixel at grazing rays; any other association would differ from the
+// reference by that much.
 template <int ST>
 __device__ __forceinline__ float march(const VolDev& v, const Ray& r, float cutoff_raw, int& nsamples)
 {
   const float W = (float)v.W, H = (float)v.H, D = (float)v.D;
-  // sampler coordinates (p.x, 1 - p.y, -p.z) scaled to voxel space
-  const float ox = r.ox * W - 0.5f, dx = r.dx * W;
-  const float oy = (H - 0.5f) - r.oy * H, dy = -(r.dy * H);
-  const float oz = -(r.oz * D) - 0.5f, dz = -(r.dz * D);
   float acc = 0.0f;
   int n = 0;
   for (float t = r.t_far; t > r.t_near; t -= 1.0f) {
-    const float val = trilinear<ST>(v, fmaf(t, dx, ox), fmaf(t, dy, oy), fmaf(t, dz, oz));
+    const float px = r.ox + t * r.dx;
+    const float py = r.oy + t * r.dy;
+    const float pz = r.oz + t * r.dz;
+    // sampler coordinates (p.x, 1 - p.y, -p.z) in voxel space
+    const float X = px * W - 0.5f;
+    const float Y = (1.0f - py) * H - 0.5f;
+    const float Z = (-pz) * D - 0.5f;
+    const float val = trilinear<ST>(v, X, Y, Z);
     acc += val > cutoff_raw ? val : 0.0f;
     ++n;
   }
