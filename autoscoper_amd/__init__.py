@@ -408,6 +408,9 @@ class Context:
             lib().asc_context_destroy(self.handle)
             self.handle = None
 
+    def copy(self, dst: DeviceArray, src: DeviceArray, nbytes: int):
+        _check(lib().asc_memcpy_d2d(self.handle, dst.ptr, src.ptr, nbytes))
+
     # -- per-operator calls on device arrays (raw pointers) --
     def drr_render(self, vol: "Volume", imv, viewport, step, intensity, cutoff, out: DeviceArray, w, h):
         imv, vp = _f32(imv), _f32(viewport)

@@ -5,8 +5,7 @@
 // the exact double-precision geometry (inverse model-view, projected
 // bounding box, render size). The device then runs three kernels:
 //   A  drr_tiles   : persistent; 16x16-pixel tiles of every item's bounding
-//                    box pulled from a global work counter; ray march ->
-//                    DRR scratch (one image per item).
+//                    box; ray march -> DRR scratch (one image per item).
 //   B  post_tiles  : persistent; per 16x16 tile: DRR filter chain and
 //                    radiograph resample + filter chain in LDS with halos,
 //                    DRR mask, multiplies, nine NCC moments -> per-tile
@@ -16,6 +15,15 @@
 //                    batch.
 // The three launches have fixed grids and read all sizes from device memory,
 // so a batch can be recorded once in a hipGraph and replayed.
+//
+// Tile order and L2 locality: tiles are enumerated as (camera, tile row,
+// tile column, pose), so consecutive tiles are the same image region of
+// different poses -- nearly the same rays through the same voxels when the
+// poses are a PSO swarm. The enumeration is cut into 8 contiguous
+// partitions, one per XCD (workgroups are dealt round-robin to the 8 XCDs,
+// so block b runs on XCD b % 8 and pulls from partition b % 8 first); each
+// XCD's L2 therefore holds one slab of the volume. A block whose partition
+// is drained steals from the next ones.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -67,17 +75,24 @@ struct ItemDesc {
   long long drr_off;
 };
 
+constexpr int kMaxCams = 8;
+constexpr int kParts = 8;  // XCDs per MI355X
+
 struct BatchHeader {
   int n_items;
-  int total_tiles;
+  int total_tiles;  // size of the (camera, row, column, pose) enumeration
   int debug;
-  int pad;
+  int n_poses;
+  int n_cams;
+  int pad[3];
+  int cam_base[kMaxCams];  // first enumeration index of each camera
+  int tiles_x[kMaxCams];   // widest item of the camera, in tiles
+  int tiles_y[kMaxCams];
 };
 
 struct Counters {
-  unsigned int drr_next;
-  unsigned int post_next;
-  unsigned int pad[2];
+  unsigned int drr_next[kParts];
+  unsigned int post_next[kParts];
 };
 
 struct BatchArgs {
@@ -96,27 +111,40 @@ struct BatchArgs {
   float* dbg_mask;
 };
 
-// Find the item owning tile t (items sorted by tile_base). Uniform per block.
-__device__ __forceinline__ int find_item(const ItemDesc* __restrict__ items, int n, int t)
+// Next enumeration index for the whole block, or -1 once every partition
+// is drained. `part` counts the partitions this block has exhausted.
+__device__ __forceinline__ int next_tile(unsigned int* counters, int total, int* slot, int& part)
 {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (items[mid].tile_base <= t)
-      lo = mid;
-    else
-      hi = mid - 1;
+  const int chunk = (total + kParts - 1) / kParts;
+  while (part < kParts) {
+    const int p = (int)((blockIdx.x + part) % kParts);
+    __syncthreads();
+    if (threadIdx.x == 0) *slot = p * chunk + (int)atomicAdd(&counters[p], 1u);
+    __syncthreads();
+    // the slot is block-uniform: say so, so that everything derived from it
+    // (item, volume, descriptors) lives in scalar registers
+    const int t = __builtin_amdgcn_readfirstlane(*slot);
+    if (t < min((p + 1) * chunk, total)) return t;
+    ++part;
   }
-  return lo;
+  return -1;
 }
 
-// Pull the next tile for the whole block.
-__device__ __forceinline__ int next_tile(unsigned int* counter, int* slot)
+// Enumeration index -> (item, tile column, tile row); false when the tile
+// lies outside that item's bounding box.
+__device__ __forceinline__ bool decode_tile(const BatchHeader& hd, const ItemDesc* __restrict__ items, int t,
+                                            int& ii, int& tx, int& ty)
 {
-  __syncthreads();
-  if (threadIdx.x == 0) *slot = (int)atomicAdd(counter, 1u);
-  __syncthreads();
-  return *slot;
+  int c = 0;
+  while (c + 1 < hd.n_cams && hd.cam_base[c + 1] <= t) ++c;
+  const int r = t - hd.cam_base[c];
+  const int e = r % hd.n_poses;
+  const int q = r / hd.n_poses;
+  tx = q % hd.tiles_x[c];
+  ty = q / hd.tiles_x[c];
+  ii = e * hd.n_cams + c;
+  const ItemDesc& it = items[ii];
+  return tx < it.tiles_x && ty < it.tiles_y;
 }
 
 template <int ST>
@@ -132,16 +160,16 @@ __global__ __launch_bounds__(kBlock) void drr_tiles(BatchArgs a)
 {
   __shared__ int s_tile;
   __shared__ unsigned long long s_samples;
-  const int total = a.hdr->total_tiles;
-  const int n_items = a.hdr->n_items;
+  const BatchHeader& hd = *a.hdr;
+  const int total = hd.total_tiles;
   const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int part = 0;
   for (;;) {
-    const int t = next_tile(&a.counters->drr_next, &s_tile);
-    if (t >= total) break;
-    const int ii = find_item(a.items, n_items, t);
+    const int t = next_tile(a.counters->drr_next, total, &s_tile, part);
+    if (t < 0) break;
+    int ii, tx, ty;
+    if (!decode_tile(hd, a.items, t, ii, tx, ty)) continue;
     const ItemDesc& it = a.items[ii];
-    const int local = t - it.tile_base;
-    const int tx = local % it.tiles_x, ty = local / it.tiles_x;
     const unsigned x = tx * kTile + (wave & 1) * 8 + (lane & 7);
     const unsigned y = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
     if (threadIdx.x == 0) s_samples = 0;
@@ -226,17 +254,18 @@ __global__ __launch_bounds__(kBlock) void post_tiles(BatchArgs a)
   __shared__ float sdrr[kTile * kTile];
   __shared__ double lds_red[4 * 9];
   __shared__ int s_tile;
-  const int total = a.hdr->total_tiles;
-  const int n_items = a.hdr->n_items;
-  const int debug = a.hdr->debug;
+  const BatchHeader& hd = *a.hdr;
+  const int total = hd.total_tiles;
+  const int debug = hd.debug;
+  int part = 0;
   for (;;) {
-    const int t = next_tile(&a.counters->post_next, &s_tile);
-    if (t >= total) break;
-    const int ii = find_item(a.items, n_items, t);
+    const int t = next_tile(a.counters->post_next, total, &s_tile, part);
+    if (t < 0) break;
+    int ii, tx, ty;
+    if (!decode_tile(hd, a.items, t, ii, tx, ty)) continue;
     const ItemDesc& it = a.items[ii];
     const CamDev& cam = a.cams[it.cam];
-    const int local = t - it.tile_base;
-    const int tx = local % it.tiles_x, ty = local / it.tiles_x;
+    const int local = ty * it.tiles_x + tx;
     const int x0 = tx * kTile, y0 = ty * kTile;
     const int w = it.w, h = it.h;
 
@@ -330,9 +359,9 @@ __global__ __launch_bounds__(kBlock) void finalize_items(BatchArgs a)
     }
     __syncthreads();
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    a.counters->drr_next = 0;
-    a.counters->post_next = 0;
+  if (blockIdx.x == 0 && threadIdx.x < kParts) {
+    a.counters->drr_next[threadIdx.x] = 0;
+    a.counters->post_next[threadIdx.x] = 0;
   }
 }
 
@@ -565,6 +594,7 @@ int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, doub
   if (rc) return rc;
   const int C = (int)s->cams.size();
   ASC_CHECK(C > 0, ASC_ERR_INVALID, "scene has no cameras");
+  ASC_CHECK(C <= kMaxCams, ASC_ERR_UNSUPPORTED, "at most 8 cameras per scene");
   ASC_CHECK(!s->vols.empty(), ASC_ERR_INVALID, "scene has no volumes");
   const int n_items = n * C;
   s->n_items = n_items;
@@ -630,7 +660,28 @@ int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, doub
     if (per_cu < 1) per_cu = 1;
     s->grid_persistent = prop.multiProcessorCount * per_cu;
   }
-  BatchHeader hdr{n_items, tile_base, s->debug, 0};
+  BatchHeader hdr;
+  std::memset(&hdr, 0, sizeof hdr);
+  hdr.n_items = n_items;
+  hdr.debug = s->debug;
+  hdr.n_poses = n;
+  hdr.n_cams = C;
+  {
+    long long base = 0;
+    for (int c = 0; c < C; c++) {
+      int tx = 0, ty = 0;
+      for (int i = 0; i < n; i++) {
+        tx = std::max(tx, s->items[(size_t)i * C + c].tiles_x);
+        ty = std::max(ty, s->items[(size_t)i * C + c].tiles_y);
+      }
+      hdr.cam_base[c] = (int)base;
+      hdr.tiles_x[c] = std::max(tx, 1);
+      hdr.tiles_y[c] = ty;
+      base += (long long)tx * ty * n;
+    }
+    ASC_CHECK(base < (1LL << 31) - (1LL << 20), ASC_ERR_TOO_LARGE, "batch has too many tiles");
+    hdr.total_tiles = (int)base;
+  }
   std::memcpy(s->h_batch.p, &hdr, sizeof hdr);
   std::memcpy(s->h_batch.p + sizeof hdr, s->items.data(), sizeof(ItemDesc) * n_items);
 

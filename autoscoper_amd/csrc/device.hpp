@@ -76,30 +76,75 @@ __device__ __forceinline__ float texel_raw<ASC_STORAGE_F32>(const void* __restri
   return ((const float*)p)[idx];
 }
 
-// Trilinear sample at voxel-space position (X, Y, Z) = (s*W-0.5, t*H-0.5,
-// r*D-0.5) with the sampler's weights (OpenCL 1.2 section 8.2). Each lerp
-// is one FMA: fma(a, T1 - T0, T0) is exact when T1 == 0 or T0 == 0, so the
-// result is > 0 exactly when a texel with positive weight is > 0 (the same
-// zero pattern as the weighted-sum form).
-template <int ST>
-__device__ __forceinline__ float trilinear(const VolDev& v, float X, float Y, float Z)
+// Texel pairs (x, x+1) come from one buffer load: 4 bytes for the 16-bit
+// storages, 8 bytes for F32. Buffer loads take the two y/z strides as
+// scalar offsets, so a sample costs one 32-bit vector offset and four
+// loads; an offset outside the allocation reads 0 instead of faulting.
+struct VolRsrc {
+  __amdgpu_buffer_rsrc_t rsrc;
+  int sy, sz;  // byte strides of +1 row and +1 slice
+};
+
+// The volume of a ray is uniform across the wave (one volume per tile);
+// readfirstlane states that, so the descriptor and strides are scalar.
+__device__ __forceinline__ VolRsrc vol_rsrc(const VolDev& v)
 {
-  const float fx = floorf(X), fy = floorf(Y), fz = floorf(Z);
-  const float a = X - fx, b = Y - fy, c = Z - fz;
-  // in-box samples satisfy -1 <= i0 <= W-1; the clamp only keeps stray
-  // positions inside the padded allocation
-  const int i = min(max((int)fx, -1), v.W - 1);
-  const int j = min(max((int)fy, -1), v.H - 1);
-  const int k = min(max((int)fz, -1), v.D - 1);
-  const int base = (k + 1) * v.pS + (j + 1) * v.pW + (i + 1);
-  const float t000 = texel_raw<ST>(v.data, base);
-  const float t100 = texel_raw<ST>(v.data, base + 1);
-  const float t010 = texel_raw<ST>(v.data, base + v.pW);
-  const float t110 = texel_raw<ST>(v.data, base + v.pW + 1);
-  const float t001 = texel_raw<ST>(v.data, base + v.pS);
-  const float t101 = texel_raw<ST>(v.data, base + v.pS + 1);
-  const float t011 = texel_raw<ST>(v.data, base + v.pS + v.pW);
-  const float t111 = texel_raw<ST>(v.data, base + v.pS + v.pW + 1);
+  const int esize = v.storage == ASC_STORAGE_F32 ? 4 : 2;
+  const uint64_t base = (uint64_t)(uintptr_t)v.data;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
+  void* p = (void*)(uintptr_t)(((uint64_t)hi << 32) | lo);
+  const int bytes = __builtin_amdgcn_readfirstlane(v.pS * (v.D + 2) * esize);
+  VolRsrc r;
+  // dword 3 = 0x00020000: DATA_FORMAT_32, as raw (untyped) buffer loads need
+  r.rsrc = __builtin_amdgcn_make_buffer_rsrc(p, 0, bytes, 0x00020000);
+  r.sy = __builtin_amdgcn_readfirstlane(v.pW * esize);
+  r.sz = __builtin_amdgcn_readfirstlane(v.pS * esize);
+  return r;
+}
+
+template <int ST>
+__device__ __forceinline__ void load_pair(const VolRsrc& r, int voff, int soff, float& lo, float& hi)
+{
+  if constexpr (ST == ASC_STORAGE_F32) {
+    const auto p = __builtin_amdgcn_raw_buffer_load_b64(r.rsrc, voff, soff, 0);
+    lo = __uint_as_float(p[0]);
+    hi = __uint_as_float(p[1]);
+  } else {
+    const uint32_t p = __builtin_amdgcn_raw_buffer_load_b32(r.rsrc, voff, soff, 0);
+    if constexpr (ST == ASC_STORAGE_U16) {
+      lo = (float)(p & 0xFFFFu);
+      hi = (float)(p >> 16);
+    } else if constexpr (ST == ASC_STORAGE_F16) {
+      lo = __half2float(__ushort_as_half((unsigned short)(p & 0xFFFFu)));
+      hi = __half2float(__ushort_as_half((unsigned short)(p >> 16)));
+    } else {
+      lo = __uint_as_float(p << 16);
+      hi = __uint_as_float(p & 0xFFFF0000u);
+    }
+  }
+}
+
+// Trilinear sample at voxel-space position (X, Y, Z) = (s*W-0.5, t*H-0.5,
+// r*D-0.5) with the sampler's weights (OpenCL 1.2 section 8.2): i0 =
+// floor(X), a = X - floor(X). Each lerp is one FMA: fma(a, T1 - T0, T0) is
+// exact when T1 == 0 or T0 == 0, so the result is > 0 exactly when a texel
+// with positive weight is > 0 (the zero pattern of the weighted-sum form).
+template <int ST>
+__device__ __forceinline__ float trilinear(const VolDev& v, const VolRsrc& r, float X, float Y, float Z)
+{
+  const float a = __builtin_amdgcn_fractf(X);
+  const float b = __builtin_amdgcn_fractf(Y);
+  const float c = __builtin_amdgcn_fractf(Z);
+  // in-box samples have -1 <= i0 <= W-1, so the padded index is >= 0
+  const int i = __float2int_rd(X) + 1, j = __float2int_rd(Y) + 1, k = __float2int_rd(Z) + 1;
+  const int esize = ST == ASC_STORAGE_F32 ? 4 : 2;
+  const int voff = (int)(__umul24((unsigned)k, (unsigned)v.pS) + __umul24((unsigned)j, (unsigned)v.pW) + i) * esize;
+  float t000, t100, t010, t110, t001, t101, t011, t111;
+  load_pair<ST>(r, voff, 0, t000, t100);
+  load_pair<ST>(r, voff, r.sy, t010, t110);
+  load_pair<ST>(r, voff, r.sz, t001, t101);
+  load_pair<ST>(r, voff, r.sz + r.sy, t011, t111);
   const float c00 = fmaf(a, t100 - t000, t000);
   const float c10 = fmaf(a, t110 - t010, t010);
   const float c01 = fmaf(a, t101 - t001, t001);
@@ -118,9 +163,12 @@ __device__ __forceinline__ float trilinear(const VolDev& v, float X, float Y, fl
 // outside the unit box, so o + t*d cancels and its rounding is ~1e-4 of a
 // DRR pixel at grazing rays; any other association would differ from the
 // reference by that much.
-template <int ST>
-__device__ __forceinline__ float march(const VolDev& v, const Ray& r, float cutoff_raw, int& nsamples)
+// CUT = false is the cutoff <= 0 case: trilinear values are >= 0, so
+// `val > cutoff ? val : 0` adds val unchanged.
+template <int ST, bool CUT>
+__device__ __forceinline__ float march_loop(const VolDev& v, const Ray& r, float cutoff_raw, int& nsamples)
 {
+  const VolRsrc rs = vol_rsrc(v);
   const float W = (float)v.W, H = (float)v.H, D = (float)v.D;
   float acc = 0.0f;
   int n = 0;
@@ -132,12 +180,22 @@ __device__ __forceinline__ float march(const VolDev& v, const Ray& r, float cuto
     const float X = px * W - 0.5f;
     const float Y = (1.0f - py) * H - 0.5f;
     const float Z = (-pz) * D - 0.5f;
-    const float val = trilinear<ST>(v, X, Y, Z);
-    acc += val > cutoff_raw ? val : 0.0f;
+    const float val = trilinear<ST>(v, rs, X, Y, Z);
+    if constexpr (CUT)
+      acc += val > cutoff_raw ? val : 0.0f;
+    else
+      acc += val;
     ++n;
   }
   nsamples += n;
   return acc;
+}
+
+template <int ST>
+__device__ __forceinline__ float march(const VolDev& v, const Ray& r, float cutoff_raw, int& nsamples)
+{
+  if (cutoff_raw > 0.0f) return march_loop<ST, true>(v, r, cutoff_raw, nsamples);
+  return march_loop<ST, false>(v, r, cutoff_raw, nsamples);
 }
 
 // density = sum(step * val); out = clamp(density / intensity, 0, 1)

@@ -99,6 +99,7 @@ int launch_binary(asc_context ctx, int op, const float* a, const float* b, float
 int launch_mask_multiply(asc_context ctx, float* drr, float* rad, const float* bg, float* mask, size_t n);
 int launch_composite(asc_context ctx, const float* s1, const float* s2, const float* s3, float* out, size_t n);
 int launch_fill_f32(asc_context ctx, float* dst, float v, size_t n);
+int launch_copy(asc_context ctx, void* dst, const void* src, size_t bytes);
 int ncc_moments(asc_context ctx, const float* f, const float* g, const float* m, unsigned n, double* out9,
                 double* ncc);
 

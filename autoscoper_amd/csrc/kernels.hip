@@ -1,5 +1,7 @@
 // Per-operator kernels: one launch per reference operator (the "plain"
 // form that gpu::View and Tracker::trackFrame drive one call at a time).
+#include <algorithm>
+
 #include "device.hpp"
 
 namespace asc {
@@ -145,6 +147,22 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(const float* __restri
   o[3 * i + 2] = s2[i];
 }
 
+// Buffer::copy as a kernel (stream-ordered, capturable in a graph); 16-byte
+// vectors when both pointers and the size allow it.
+__global__ __launch_bounds__(kBlock) void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                      size_t n16)
+{
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n16; i += (size_t)gridDim.x * kBlock)
+    dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(kBlock) void copy_bytes_kernel(const uint8_t* __restrict__ src,
+                                                            uint8_t* __restrict__ dst, size_t n)
+{
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+    dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(kBlock) void fill_kernel(float* __restrict__ o, float v, size_t n)
 {
   const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
@@ -240,10 +258,7 @@ int launch_filter(asc_context ctx, const asc_filter& f, const float* in, float* 
 {
   if (w <= 0 || h <= 0) return ASC_OK;
   const size_t bytes = sizeof(float) * (size_t)w * h;
-  if (!f.enabled) {
-    ASC_HIP(hipMemcpyAsync(out, in, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    return ASC_OK;
-  }
+  if (!f.enabled) return launch_copy(ctx, out, in, bytes);
   switch (f.type) {
     case ASC_FILTER_SOBEL:
       sobel_kernel<<<grid2(w, h), kBlock, 0, ctx->stream>>>(in, out, w, h, f.p[0], f.p[1]);
@@ -255,10 +270,7 @@ int launch_filter(asc_context ctx, const asc_filter& f, const float* in, float* 
     case ASC_FILTER_SHARPEN: {
       std::vector<float> wts;
       const int n = gauss_weights(f.p[0], wts);
-      if (n == 1) {  // GaussianFilter.cpp:131-133: filter size 1 is a copy
-        ASC_HIP(hipMemcpyAsync(out, in, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-        return ASC_OK;
-      }
+      if (n == 1) return launch_copy(ctx, out, in, bytes);  // GaussianFilter.cpp:131-133
       if (n * n > kMaxTaps) return fail(ASC_ERR_UNSUPPORTED, "filter radius too large (max 10)");
       ConvArgs a;
       a.n = n;
@@ -306,6 +318,20 @@ int launch_composite(asc_context ctx, const float* s1, const float* s2, const fl
 {
   if (n == 0) return ASC_OK;
   composite_kernel<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(s1, s2, s3, out, n);
+  ASC_HIP(hipGetLastError());
+  return ASC_OK;
+}
+
+int launch_copy(asc_context ctx, void* dst, const void* src, size_t bytes)
+{
+  if (bytes == 0) return ASC_OK;
+  const bool vec = ((uintptr_t)dst % 16 == 0) && ((uintptr_t)src % 16 == 0) && (bytes % 16 == 0);
+  const size_t n = vec ? bytes / 16 : bytes;
+  const unsigned grid = (unsigned)std::min<size_t>((n + kBlock - 1) / kBlock, 256 * 16);
+  if (vec)
+    copy_kernel<<<grid, kBlock, 0, ctx->stream>>>((const uint4*)src, (uint4*)dst, n);
+  else
+    copy_bytes_kernel<<<grid, kBlock, 0, ctx->stream>>>((const uint8_t*)src, (uint8_t*)dst, n);
   ASC_HIP(hipGetLastError());
   return ASC_OK;
 }

@@ -278,9 +278,7 @@ ASC_API int asc_memcpy_d2h(asc_context ctx, void* dst, const void* src, size_t b
 ASC_API int asc_memcpy_d2d(asc_context ctx, void* dst, const void* src, size_t bytes)
 {
   ASC_CHECK(valid_ctx(ctx), ASC_ERR_INVALID, "null context");
-  if (bytes == 0) return ASC_OK;
-  ASC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-  return ASC_OK;
+  return launch_copy(ctx, dst, src, bytes);
 }
 
 ASC_API int asc_fill_f32(asc_context ctx, float* dst, float value, size_t count)
@@ -631,10 +629,7 @@ ASC_API int asc_filter_chain(asc_context ctx, const asc_filter* filters, int cou
 {
   ASC_CHECK(valid_ctx(ctx) && in && out && (count == 0 || filters), ASC_ERR_INVALID, "bad argument");
   const size_t bytes = sizeof(float) * (size_t)width * height;
-  if (count == 0) {
-    ASC_HIP(hipMemcpyAsync(out, in, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    return ASC_OK;
-  }
+  if (count == 0) return launch_copy(ctx, out, in, bytes);
   ASC_CHECK(count == 1 || scratch, ASC_ERR_INVALID, "scratch buffer required for chains longer than 1");
   float* b1 = (count % 2) ? out : scratch;
   float* b2 = (count % 2) ? scratch : out;

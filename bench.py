@@ -95,6 +95,8 @@ def main(argv=None) -> int:
     ap.add_argument("--plain-evals", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="only run the device copy kernel over MIB MiB 5 times (counter calibration)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -107,6 +109,14 @@ def main(argv=None) -> int:
     from autoscoper_amd import synth
 
     ctx = A.Context(0)
+    if args.calib_copy:
+        nbytes = args.calib_copy << 20
+        src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+        for _ in range(5):
+            ctx.copy(dst, src, nbytes)
+        ctx.synchronize()
+        log(f"[bench] copied {nbytes} bytes x 5")
+        return 0
     wl, vol, cams, rads, scene, gt = build_workload(A, synth, args.workload, args.storage, ctx)
     RW, RH = wl.render
     P = args.particles
