@@ -65,6 +65,7 @@ def lib():
             "orc_pso_bowl": ([_P(_d), _u, _f, _f, _u, _u, _i, _P(_f), _P(_f)], _i),
             "orc_glibc_rand_first": ([_u, _i, _P(_i)], _i),
             "orc_merge_f32": ([_P(_f), _P(_f), _P(_f), C.c_size_t], None),
+            "orc_sample3": ([_vp, _P(_i), _d, _d, _d, _i], _d),
             "orc_composite_f32": ([_P(_f), _P(_f), _P(_f), _P(_f), C.c_size_t], None),
         }
         scene = [_vp, _P(_i), _P(_f), _P(_f), _i, _P(_d), _P(_vp), _P(_i), _P(_f), _P(_i), _i, _i, _f, _P(_vp),
@@ -132,6 +133,12 @@ def drr(vol: np.ndarray, imv, vp, step, intensity, cutoff, w, h, fp64=False):
         n = lib().orc_drr_f32(vol.ctypes.data_as(_vp), _p(vd, _i), _p(imv, _f), _p(vp, _f), step, intensity, cutoff,
                               w, h, _p(out, _f))
     return out, int(n)
+
+
+def sample3(vol: np.ndarray, s, t, r, fp64=False):
+    vol = np.ascontiguousarray(vol, np.uint16)
+    vd = np.array([vol.shape[2], vol.shape[1], vol.shape[0]], np.int32)
+    return float(lib().orc_sample3(vol.ctypes.data_as(_vp), _p(vd, _i), s, t, r, int(fp64)))
 
 
 def rad(img: np.ndarray, plane, vp, w, h):

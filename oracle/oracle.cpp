@@ -1610,3 +1610,10 @@ ORC_API void orc_composite_f32(const float* s1, const float* s2, const float* s3
     o[3 * i + 2] = s2[i];
   }
 }
+
+// Direct access to the sampler (normalised coordinates) for pinning tests.
+ORC_API double orc_sample3(const uint16_t* vol, const int* vdim, double s, double t, double r, int fp64)
+{
+  if (fp64) return sample3<double>(vol, vdim[0], vdim[1], vdim[2], s, t, r);
+  return sample3<float>(vol, vdim[0], vdim[1], vdim[2], (float)s, (float)t, (float)r);
+}
