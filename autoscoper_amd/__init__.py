@@ -107,6 +107,7 @@ _SIGS = {
     "asc_fill_u8": [_vp, _vp, C.c_ubyte, _sz],
     "asc_volume_create": [_vp, _vp, _i, _i, _i, _i, _P(_i), _P(_d), _i, _P(_vp), _P(AscVolumeDesc)],
     "asc_volume_describe": [_vp, _P(AscVolumeDesc)],
+    "asc_volume_set_layout": [_vp, _i],
     "asc_volume_destroy": [_vp],
     "asc_volume_bytes": [_vp, _P(_sz)],
     "asc_image_create": [_vp, _vp, _i, _i, _i, _P(_vp)],
@@ -153,6 +154,7 @@ _SIGS = {
     "asc_scene_set_debug": [_vp, _i],
     "asc_scene_last_info": [_vp, _P(_i), _i, _P(C.c_int64)],
     "asc_scene_set_timing": [_vp, _i],
+    "asc_scene_set_tuning": [_vp, _i, _i],
     "asc_scene_last_timing": [_vp, _P(_f)],
     "asc_scene_last_moments": [_vp, _P(_d), _i],
     "asc_scene_last_images": [_vp, _i, _i, _P(_f), _P(_f), _P(_f)],
@@ -476,6 +478,11 @@ class Volume:
         self.ctx = ctx
         self.scale = tuple(float(s) for s in scale)
 
+    LINEAR, QUAD, OCTET = 0, 1, 2
+
+    def set_layout(self, layout: int):
+        _check(lib().asc_volume_set_layout(self.handle, layout))
+
     @property
     def nbytes(self) -> int:
         n = C.c_size_t(0)
@@ -590,6 +597,11 @@ class Scene:
         _check(lib().asc_scene_eval_manips(self.handle, _ptr(vols, _i), _ptr(s, _f), _ptr(m, _d), n,
                                            _ptr(costs, _d), _ptr(per, _d)))
         return costs, per
+
+    TUNE_UNROLL, TUNE_GRID = 0, 1
+
+    def set_tuning(self, key: int, value: int):
+        _check(lib().asc_scene_set_tuning(self.handle, key, value))
 
     def set_timing(self, enable: bool):
         _check(lib().asc_scene_set_timing(self.handle, int(enable)))

@@ -147,15 +147,16 @@ __device__ __forceinline__ bool decode_tile(const BatchHeader& hd, const ItemDes
   return tx < it.tiles_x && ty < it.tiles_y;
 }
 
-template <int ST>
+template <int UNROLL>
 __device__ __forceinline__ float march_item(const VolParams& vp, const ItemDesc& it, unsigned x, unsigned y,
                                             int& nsamp)
 {
   dev::Ray r;
   if (!dev::ray_setup(it.imv, it.vp, vp.step, x, y, (unsigned)it.w, (unsigned)it.h, r)) return 0.0f;
-  return dev::drr_finish(dev::march<ST>(vp.v, r, vp.cutoff_raw, nsamp), vp.step_vscale, vp.intensity);
+  return dev::drr_finish(dev::march<UNROLL>(vp.v, r, vp.cutoff_raw, nsamp), vp.step_vscale, vp.intensity);
 }
 
+template <int UNROLL>
 __global__ __launch_bounds__(kBlock) void drr_tiles(BatchArgs a)
 {
   __shared__ int s_tile;
@@ -177,13 +178,7 @@ __global__ __launch_bounds__(kBlock) void drr_tiles(BatchArgs a)
     int nsamp = 0;
     if (x < (unsigned)it.w && y < (unsigned)it.h) {
       const VolParams& vp = a.vols[it.vol];
-      float val;
-      switch (vp.v.storage) {
-        case ASC_STORAGE_U16: val = march_item<ASC_STORAGE_U16>(vp, it, x, y, nsamp); break;
-        case ASC_STORAGE_F16: val = march_item<ASC_STORAGE_F16>(vp, it, x, y, nsamp); break;
-        case ASC_STORAGE_BF16: val = march_item<ASC_STORAGE_BF16>(vp, it, x, y, nsamp); break;
-        default: val = march_item<ASC_STORAGE_F32>(vp, it, x, y, nsamp); break;
-      }
+      const float val = march_item<UNROLL>(vp, it, x, y, nsamp);
       a.drr[it.drr_off + (long long)y * it.w + x] = val;
     }
     // per-tile sample count (for the algorithmic-bytes figure)
@@ -425,6 +420,7 @@ using namespace asc;
 
 struct SceneVolume {
   asc_volume vol;
+  int generation_seen = -1;
   float step, intensity, cutoff;
   Frame pivot;
 };
@@ -473,6 +469,7 @@ struct asc_scene_s {
   DevBuf<float> p_bufs;
   // per-kernel timing of the last eager batch (HIP events on the scene stream)
   bool timing = false;
+  int unroll = 2;  // ray samples per loop iteration (1, 2 or 4)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float last_ms[3] = {0, 0, 0};
 };
@@ -481,6 +478,8 @@ namespace {
 
 int scene_upload_config(asc_scene s)
 {
+  for (const SceneVolume& sv : s->vols)
+    if (sv.generation_seen != sv.vol->generation) s->dirty = true;
   if (!s->dirty) return ASC_OK;
   std::vector<CamDev> cams(s->cams.size());
   std::vector<float> pool;
@@ -521,7 +520,8 @@ int scene_upload_config(asc_scene s)
   }
   std::vector<VolParams> vols(s->vols.size());
   for (size_t i = 0; i < s->vols.size(); i++) {
-    const SceneVolume& sv = s->vols[i];
+    SceneVolume& sv = s->vols[i];
+    sv.generation_seen = sv.vol->generation;
     vols[i].v = sv.vol->dev;
     vols[i].step = sv.step;
     vols[i].intensity = sv.intensity;
@@ -574,7 +574,10 @@ int issue_batch(asc_scene s, size_t batch_bytes)
   zero_u64<<<(n_items + 255) / 256, 256, 0, st>>>(s->d_samples.p, n_items);
   const bool timed = s->timing && s->ev[0];
   if (timed) ASC_HIP(hipEventRecord(s->ev[0], st));
-  drr_tiles<<<s->grid_persistent, kBlock, 0, st>>>(a);
+  switch (s->unroll) {
+    case 1: drr_tiles<1><<<s->grid_persistent, kBlock, 0, st>>>(a); break;
+    default: drr_tiles<2><<<s->grid_persistent, kBlock, 0, st>>>(a); break;
+  }
   if (timed) ASC_HIP(hipEventRecord(s->ev[1], st));
   post_tiles<<<s->grid_persistent, kBlock, 0, st>>>(a);
   if (timed) ASC_HIP(hipEventRecord(s->ev[2], st));
@@ -656,7 +659,7 @@ int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, doub
     hipDeviceProp_t prop;
     ASC_HIP(hipGetDeviceProperties(&prop, s->ctx->device));
     int per_cu = 0;
-    ASC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, drr_tiles, kBlock, 0));
+    ASC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, drr_tiles<1>, kBlock, 0));
     if (per_cu < 1) per_cu = 1;
     s->grid_persistent = prop.multiProcessorCount * per_cu;
   }
@@ -871,6 +874,24 @@ ASC_API int asc_scene_set_debug(asc_scene s, int enable)
 {
   ASC_CHECK(s, ASC_ERR_INVALID, "null scene");
   s->debug = enable != 0;
+  destroy_graph(s);
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_set_tuning(asc_scene s, int key, int value)
+{
+  ASC_CHECK(s, ASC_ERR_INVALID, "null scene");
+  switch (key) {
+    case ASC_TUNE_UNROLL:
+      ASC_CHECK(value == 1 || value == 2, ASC_ERR_INVALID, "unroll must be 1 or 2");
+      s->unroll = value;
+      break;
+    case ASC_TUNE_GRID:
+      ASC_CHECK(value >= 0, ASC_ERR_INVALID, "grid must be >= 0 (0 = one wave of blocks per CU slot)");
+      s->grid_persistent = value - value % 8;
+      break;
+    default: return fail(ASC_ERR_INVALID, "unknown tuning key");
+  }
   destroy_graph(s);
   return ASC_OK;
 }

@@ -52,34 +52,9 @@ __device__ __forceinline__ bool ray_setup(const float* __restrict__ imv, const f
 // ---------------------------------------------------------------------------
 // Volume texel loads (raw value; normalised = raw * VolDev::vscale).
 // ---------------------------------------------------------------------------
-template <int ST>
-__device__ __forceinline__ float texel_raw(const void* __restrict__ p, int idx);
-
-template <>
-__device__ __forceinline__ float texel_raw<ASC_STORAGE_U16>(const void* __restrict__ p, int idx)
-{
-  return (float)((const uint16_t*)p)[idx];
-}
-template <>
-__device__ __forceinline__ float texel_raw<ASC_STORAGE_F16>(const void* __restrict__ p, int idx)
-{
-  return __half2float(((const __half*)p)[idx]);
-}
-template <>
-__device__ __forceinline__ float texel_raw<ASC_STORAGE_BF16>(const void* __restrict__ p, int idx)
-{
-  return __uint_as_float((uint32_t)((const uint16_t*)p)[idx] << 16);
-}
-template <>
-__device__ __forceinline__ float texel_raw<ASC_STORAGE_F32>(const void* __restrict__ p, int idx)
-{
-  return ((const float*)p)[idx];
-}
-
-// Texel pairs (x, x+1) come from one buffer load: 4 bytes for the 16-bit
-// storages, 8 bytes for F32. Buffer loads take the two y/z strides as
-// scalar offsets, so a sample costs one 32-bit vector offset and four
-// loads; an offset outside the allocation reads 0 instead of faulting.
+// Volume reads go through buffer loads: the descriptor is scalar, the two
+// y/z strides ride in scalar offsets, and an offset outside the allocation
+// reads 0 instead of faulting.
 struct VolRsrc {
   __amdgpu_buffer_rsrc_t rsrc;
   int sy, sz;  // byte strides of +1 row and +1 slice
@@ -89,38 +64,82 @@ struct VolRsrc {
 // readfirstlane states that, so the descriptor and strides are scalar.
 __device__ __forceinline__ VolRsrc vol_rsrc(const VolDev& v)
 {
-  const int esize = v.storage == ASC_STORAGE_F32 ? 4 : 2;
   const uint64_t base = (uint64_t)(uintptr_t)v.data;
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
   void* p = (void*)(uintptr_t)(((uint64_t)hi << 32) | lo);
-  const int bytes = __builtin_amdgcn_readfirstlane(v.pS * (v.D + 2) * esize);
+  const int bytes = __builtin_amdgcn_readfirstlane(v.pS * (v.D + 2) * v.esz);
   VolRsrc r;
   // dword 3 = 0x00020000: DATA_FORMAT_32, as raw (untyped) buffer loads need
   r.rsrc = __builtin_amdgcn_make_buffer_rsrc(p, 0, bytes, 0x00020000);
-  r.sy = __builtin_amdgcn_readfirstlane(v.pW * esize);
-  r.sz = __builtin_amdgcn_readfirstlane(v.pS * esize);
+  r.sy = __builtin_amdgcn_readfirstlane(v.pW * v.esz);
+  r.sz = __builtin_amdgcn_readfirstlane(v.pS * v.esz);
   return r;
 }
 
+// Two 16-bit texels packed in a dword -> raw floats.
 template <int ST>
-__device__ __forceinline__ void load_pair(const VolRsrc& r, int voff, int soff, float& lo, float& hi)
+__device__ __forceinline__ void unpack2(uint32_t p, float& lo, float& hi)
 {
-  if constexpr (ST == ASC_STORAGE_F32) {
-    const auto p = __builtin_amdgcn_raw_buffer_load_b64(r.rsrc, voff, soff, 0);
-    lo = __uint_as_float(p[0]);
-    hi = __uint_as_float(p[1]);
+  if constexpr (ST == ASC_STORAGE_U16) {
+    lo = (float)(p & 0xFFFFu);
+    hi = (float)(p >> 16);
+  } else if constexpr (ST == ASC_STORAGE_F16) {
+    lo = __half2float(__ushort_as_half((unsigned short)(p & 0xFFFFu)));
+    hi = __half2float(__ushort_as_half((unsigned short)(p >> 16)));
   } else {
-    const uint32_t p = __builtin_amdgcn_raw_buffer_load_b32(r.rsrc, voff, soff, 0);
-    if constexpr (ST == ASC_STORAGE_U16) {
-      lo = (float)(p & 0xFFFFu);
-      hi = (float)(p >> 16);
-    } else if constexpr (ST == ASC_STORAGE_F16) {
-      lo = __half2float(__ushort_as_half((unsigned short)(p & 0xFFFFu)));
-      hi = __half2float(__ushort_as_half((unsigned short)(p >> 16)));
+    lo = __uint_as_float(p << 16);
+    hi = __uint_as_float(p & 0xFFFF0000u);
+  }
+}
+
+// The 8 corner texels {t000, t100, t010, t110, t001, t101, t011, t111} of
+// element voff (byte offset of corner (i0, j0, k0)) in layout LY.
+template <int ST, int LY>
+__device__ __forceinline__ void gather8(const VolRsrc& r, int voff, float* t)
+{
+  constexpr bool F32 = ST == ASC_STORAGE_F32;
+  if constexpr (LY == ASC_LAYOUT_LINEAR) {
+    // x-pairs: 4 loads (y, z), (y+1, z), (y, z+1), (y+1, z+1)
+    const int so[4] = {0, r.sy, r.sz, r.sz + r.sy};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if constexpr (F32) {
+        const auto p = __builtin_amdgcn_raw_buffer_load_b64(r.rsrc, voff, so[q], 0);
+        t[2 * q] = __uint_as_float(p[0]);
+        t[2 * q + 1] = __uint_as_float(p[1]);
+      } else {
+        unpack2<ST>(__builtin_amdgcn_raw_buffer_load_b32(r.rsrc, voff, so[q], 0), t[2 * q], t[2 * q + 1]);
+      }
+    }
+  } else if constexpr (LY == ASC_LAYOUT_QUAD) {
+    // one 2x2 block per slice: slices k and k+1
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int so = q ? r.sz : 0;
+      if constexpr (F32) {
+        const auto p = __builtin_amdgcn_raw_buffer_load_b128(r.rsrc, voff, so, 0);
+#pragma unroll
+        for (int e = 0; e < 4; e++) t[4 * q + e] = __uint_as_float(p[e]);
+      } else {
+        const auto p = __builtin_amdgcn_raw_buffer_load_b64(r.rsrc, voff, so, 0);
+        unpack2<ST>(p[0], t[4 * q], t[4 * q + 1]);
+        unpack2<ST>(p[1], t[4 * q + 2], t[4 * q + 3]);
+      }
+    }
+  } else {
+    if constexpr (F32) {
+      const auto p = __builtin_amdgcn_raw_buffer_load_b128(r.rsrc, voff, 0, 0);
+      const auto q = __builtin_amdgcn_raw_buffer_load_b128(r.rsrc, voff, 16, 0);
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        t[e] = __uint_as_float(p[e]);
+        t[4 + e] = __uint_as_float(q[e]);
+      }
     } else {
-      lo = __uint_as_float(p << 16);
-      hi = __uint_as_float(p & 0xFFFF0000u);
+      const auto p = __builtin_amdgcn_raw_buffer_load_b128(r.rsrc, voff, 0, 0);
+#pragma unroll
+      for (int e = 0; e < 4; e++) unpack2<ST>(p[e], t[2 * e], t[2 * e + 1]);
     }
   }
 }
@@ -130,7 +149,7 @@ __device__ __forceinline__ void load_pair(const VolRsrc& r, int voff, int soff, 
 // floor(X), a = X - floor(X). Each lerp is one FMA: fma(a, T1 - T0, T0) is
 // exact when T1 == 0 or T0 == 0, so the result is > 0 exactly when a texel
 // with positive weight is > 0 (the zero pattern of the weighted-sum form).
-template <int ST>
+template <int ST, int LY>
 __device__ __forceinline__ float trilinear(const VolDev& v, const VolRsrc& r, float X, float Y, float Z)
 {
   const float a = __builtin_amdgcn_fractf(X);
@@ -138,17 +157,13 @@ __device__ __forceinline__ float trilinear(const VolDev& v, const VolRsrc& r, fl
   const float c = __builtin_amdgcn_fractf(Z);
   // in-box samples have -1 <= i0 <= W-1, so the padded index is >= 0
   const int i = __float2int_rd(X) + 1, j = __float2int_rd(Y) + 1, k = __float2int_rd(Z) + 1;
-  const int esize = ST == ASC_STORAGE_F32 ? 4 : 2;
-  const int voff = (int)(__umul24((unsigned)k, (unsigned)v.pS) + __umul24((unsigned)j, (unsigned)v.pW) + i) * esize;
-  float t000, t100, t010, t110, t001, t101, t011, t111;
-  load_pair<ST>(r, voff, 0, t000, t100);
-  load_pair<ST>(r, voff, r.sy, t010, t110);
-  load_pair<ST>(r, voff, r.sz, t001, t101);
-  load_pair<ST>(r, voff, r.sz + r.sy, t011, t111);
-  const float c00 = fmaf(a, t100 - t000, t000);
-  const float c10 = fmaf(a, t110 - t010, t010);
-  const float c01 = fmaf(a, t101 - t001, t001);
-  const float c11 = fmaf(a, t111 - t011, t011);
+  const int voff = (int)(__umul24((unsigned)k, (unsigned)v.pS) + __umul24((unsigned)j, (unsigned)v.pW) + i) * v.esz;
+  float t[8];
+  gather8<ST, LY>(r, voff, t);
+  const float c00 = fmaf(a, t[1] - t[0], t[0]);
+  const float c10 = fmaf(a, t[3] - t[2], t[2]);
+  const float c01 = fmaf(a, t[5] - t[4], t[4]);
+  const float c11 = fmaf(a, t[7] - t[6], t[6]);
   const float c0 = fmaf(b, c10 - c00, c00);
   const float c1 = fmaf(b, c11 - c01, c01);
   return fmaf(c, c1 - c0, c0);
@@ -165,37 +180,83 @@ __device__ __forceinline__ float trilinear(const VolDev& v, const VolRsrc& r, fl
 // reference by that much.
 // CUT = false is the cutoff <= 0 case: trilinear values are >= 0, so
 // `val > cutoff ? val : 0` adds val unchanged.
-template <int ST, bool CUT>
+// Samples visited by `for (t = far; t > near; t -= 1)`: t_k = far - k is
+// exact in float for every visited k (far < 2^24, near >= 0), so the count
+// is #{k >= 0 : far - k > near} = ceil(far - near) in exact arithmetic.
+__device__ __forceinline__ int sample_count(const Ray& r)
+{
+  // (the cap only matters for degenerate geometry, where the reference's
+  // own loop would not terminate)
+  return (int)fmin(ceil((double)r.t_far - (double)r.t_near), 4194304.0);
+}
+
+template <int ST, int LY, bool CUT>
+__device__ __forceinline__ void accumulate(const VolDev& v, const VolRsrc& rs, const Ray& r, float t, float W,
+                                           float H, float D, float cutoff_raw, float& acc)
+{
+  const float px = r.ox + t * r.dx;
+  const float py = r.oy + t * r.dy;
+  const float pz = r.oz + t * r.dz;
+  // sampler coordinates (p.x, 1 - p.y, -p.z) in voxel space
+  const float X = px * W - 0.5f;
+  const float Y = (1.0f - py) * H - 0.5f;
+  const float Z = (-pz) * D - 0.5f;
+  const float val = trilinear<ST, LY>(v, rs, X, Y, Z);
+  if constexpr (CUT)
+    acc += val > cutoff_raw ? val : 0.0f;
+  else
+    acc += val;
+}
+
+// UNROLL samples per iteration (t, t-1, ...): their loads are independent,
+// so a wave keeps 4*UNROLL loads in flight; the sum keeps the reference's
+// sample order.
+template <int ST, int LY, bool CUT, int UNROLL>
 __device__ __forceinline__ float march_loop(const VolDev& v, const Ray& r, float cutoff_raw, int& nsamples)
 {
   const VolRsrc rs = vol_rsrc(v);
   const float W = (float)v.W, H = (float)v.H, D = (float)v.D;
+  const int n = sample_count(r);
   float acc = 0.0f;
-  int n = 0;
-  for (float t = r.t_far; t > r.t_near; t -= 1.0f) {
-    const float px = r.ox + t * r.dx;
-    const float py = r.oy + t * r.dy;
-    const float pz = r.oz + t * r.dz;
-    // sampler coordinates (p.x, 1 - p.y, -p.z) in voxel space
-    const float X = px * W - 0.5f;
-    const float Y = (1.0f - py) * H - 0.5f;
-    const float Z = (-pz) * D - 0.5f;
-    const float val = trilinear<ST>(v, rs, X, Y, Z);
-    if constexpr (CUT)
-      acc += val > cutoff_raw ? val : 0.0f;
-    else
-      acc += val;
-    ++n;
+  float t = r.t_far;
+  int k = 0;
+  for (; k + UNROLL <= n; k += UNROLL) {
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) accumulate<ST, LY, CUT>(v, rs, r, t - (float)u, W, H, D, cutoff_raw, acc);
+    t -= (float)UNROLL;
+  }
+  for (; k < n; k++) {
+    accumulate<ST, LY, CUT>(v, rs, r, t, W, H, D, cutoff_raw, acc);
+    t -= 1.0f;
   }
   nsamples += n;
   return acc;
 }
 
-template <int ST>
+template <int ST, int LY, int UNROLL>
+__device__ __forceinline__ float march_st(const VolDev& v, const Ray& r, float cutoff_raw, int& nsamples)
+{
+  if (cutoff_raw > 0.0f) return march_loop<ST, LY, true, UNROLL>(v, r, cutoff_raw, nsamples);
+  return march_loop<ST, LY, false, UNROLL>(v, r, cutoff_raw, nsamples);
+}
+
+// Dispatch on the volume's storage and layout (uniform per wave).
+template <int UNROLL>
 __device__ __forceinline__ float march(const VolDev& v, const Ray& r, float cutoff_raw, int& nsamples)
 {
-  if (cutoff_raw > 0.0f) return march_loop<ST, true>(v, r, cutoff_raw, nsamples);
-  return march_loop<ST, false>(v, r, cutoff_raw, nsamples);
+#define ASC_MARCH_ST(ST)                                                                  \
+  switch (v.layout) {                                                                     \
+    case ASC_LAYOUT_QUAD: return march_st<ST, ASC_LAYOUT_QUAD, UNROLL>(v, r, cutoff_raw, nsamples);   \
+    case ASC_LAYOUT_OCTET: return march_st<ST, ASC_LAYOUT_OCTET, UNROLL>(v, r, cutoff_raw, nsamples); \
+    default: return march_st<ST, ASC_LAYOUT_LINEAR, UNROLL>(v, r, cutoff_raw, nsamples);              \
+  }
+  switch (v.storage) {
+    case ASC_STORAGE_U16: ASC_MARCH_ST(ASC_STORAGE_U16)
+    case ASC_STORAGE_F16: ASC_MARCH_ST(ASC_STORAGE_F16)
+    case ASC_STORAGE_BF16: ASC_MARCH_ST(ASC_STORAGE_BF16)
+    default: ASC_MARCH_ST(ASC_STORAGE_F32)
+  }
+#undef ASC_MARCH_ST
 }
 
 // density = sum(step * val); out = clamp(density / intensity, 0, 1)

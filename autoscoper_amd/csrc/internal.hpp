@@ -39,12 +39,17 @@ int hip_fail(hipError_t e, const char* what);
 // pS = (W + 2) * (H + 2). Texel (i, j, k) of the cropped volume lives at
 // ((k + 1) * (H + 2) + (j + 1)) * pW + (i + 1). The zero border realises the
 // sampler's CLK_ADDRESS_CLAMP border colour for every in-box sample.
+// Layouts (ASC_LAYOUT_*): LINEAR stores one texel per element; QUAD stores
+// the 2x2 (x, y) corner block of each element (4 texels), OCTET the 2x2x2
+// block (8 texels), so a trilinear sample reads 4, 2 or 1 aligned vectors.
 struct VolDev {
   const void* data;
   int W, H, D;
   int pW, pS;
   int storage;   // ASC_STORAGE_*
   float vscale;  // raw texel value -> normalised value (1/65535 for U16, else 1)
+  int layout;    // ASC_LAYOUT_*
+  int esz;       // bytes per element of the layout
 };
 
 struct ImgDev {
@@ -74,8 +79,11 @@ struct asc_volume_s {
   asc_context ctx;
   asc_volume_desc desc;
   asc::VolDev dev;
-  void* dptr;
+  void* dptr;       // LINEAR copy (always kept)
   size_t bytes;
+  void* packed = nullptr;  // QUAD / OCTET copy when selected
+  size_t packed_bytes = 0;
+  int generation = 0;      // bumped whenever `dev` changes
 };
 
 struct asc_image_s {
@@ -100,6 +108,7 @@ int launch_mask_multiply(asc_context ctx, float* drr, float* rad, const float* b
 int launch_composite(asc_context ctx, const float* s1, const float* s2, const float* s3, float* out, size_t n);
 int launch_fill_f32(asc_context ctx, float* dst, float v, size_t n);
 int launch_copy(asc_context ctx, void* dst, const void* src, size_t bytes);
+int launch_pack_volume(asc_context ctx, const VolDev& linear, void* dst, int layout);
 int ncc_moments(asc_context ctx, const float* f, const float* g, const float* m, unsigned n, double* out9,
                 double* ncc);
 

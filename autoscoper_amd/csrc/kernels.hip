@@ -25,7 +25,6 @@ struct DrrArgs {
   unsigned w, h;
 };
 
-template <int ST>
 __global__ __launch_bounds__(kBlock) void drr_kernel(VolDev v, DrrArgs a, float* __restrict__ out)
 {
   unsigned x, y;
@@ -35,7 +34,7 @@ __global__ __launch_bounds__(kBlock) void drr_kernel(VolDev v, DrrArgs a, float*
   float val = 0.0f;
   if (dev::ray_setup(a.imv, a.vp, a.step, x, y, a.w, a.h, r)) {
     int n = 0;
-    val = dev::drr_finish(dev::march<ST>(v, r, a.cutoff_raw, n), a.step_vscale, a.intensity);
+    val = dev::drr_finish(dev::march<2>(v, r, a.cutoff_raw, n), a.step_vscale, a.intensity);
   }
   out[(size_t)y * a.w + x] = val;
 }
@@ -207,6 +206,26 @@ __global__ __launch_bounds__(kBlock) void ncc_final_kernel(const double* __restr
   }
 }
 
+// QUAD / OCTET packing: element (i, j, k) of the padded grid receives the
+// texels (i+di, j+dj, k+dk), di, dj (, dk) in {0, 1}, in the order
+// gather8() expects; texels beyond the padded grid are 0.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void pack_kernel(const T* __restrict__ src, T* __restrict__ dst, int pW, int pH,
+                                                      int pD, int per)
+{
+  const size_t n = (size_t)pW * pH * pD;
+  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (size_t)gridDim.x * kBlock) {
+    const int i = (int)(e % pW), j = (int)((e / pW) % pH), k = (int)(e / ((size_t)pW * pH));
+    for (int q = 0; q < per; q++) {
+      const int di = q & 1, dj = (q >> 1) & 1, dk = (q >> 2) & 1;
+      const int ii = i + di, jj = j + dj, kk = k + dk;
+      T v = T(0);
+      if (ii < pW && jj < pH && kk < pD) v = src[((size_t)kk * pH + jj) * pW + ii];
+      dst[e * per + q] = v;
+    }
+  }
+}
+
 dim3 grid2(unsigned w, unsigned h) { return dim3((w + 15) / 16, (h + 15) / 16); }
 
 }  // namespace
@@ -225,13 +244,9 @@ int launch_drr(asc_context ctx, const VolDev& v, const float* imv, const float* 
   a.step_vscale = step * v.vscale;
   a.w = w;
   a.h = h;
-  switch (v.storage) {
-    case ASC_STORAGE_U16: drr_kernel<ASC_STORAGE_U16><<<grid2(w, h), kBlock, 0, ctx->stream>>>(v, a, out); break;
-    case ASC_STORAGE_F16: drr_kernel<ASC_STORAGE_F16><<<grid2(w, h), kBlock, 0, ctx->stream>>>(v, a, out); break;
-    case ASC_STORAGE_BF16: drr_kernel<ASC_STORAGE_BF16><<<grid2(w, h), kBlock, 0, ctx->stream>>>(v, a, out); break;
-    case ASC_STORAGE_F32: drr_kernel<ASC_STORAGE_F32><<<grid2(w, h), kBlock, 0, ctx->stream>>>(v, a, out); break;
-    default: return fail(ASC_ERR_UNSUPPORTED, "drr: unknown volume storage");
-  }
+  if (v.storage < ASC_STORAGE_U16 || v.storage > ASC_STORAGE_F32)
+    return fail(ASC_ERR_UNSUPPORTED, "drr: unknown volume storage");
+  drr_kernel<<<grid2(w, h), kBlock, 0, ctx->stream>>>(v, a, out);
   ASC_HIP(hipGetLastError());
   return ASC_OK;
 }
@@ -318,6 +333,21 @@ int launch_composite(asc_context ctx, const float* s1, const float* s2, const fl
 {
   if (n == 0) return ASC_OK;
   composite_kernel<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, ctx->stream>>>(s1, s2, s3, out, n);
+  ASC_HIP(hipGetLastError());
+  return ASC_OK;
+}
+
+int launch_pack_volume(asc_context ctx, const VolDev& v, void* dst, int layout)
+{
+  const int per = layout == ASC_LAYOUT_QUAD ? 4 : 8;
+  const int pH = v.pS / v.pW, pD = v.D + 2;
+  const unsigned grid = 256 * 8;
+  if (v.storage == ASC_STORAGE_F32)
+    pack_kernel<uint32_t><<<grid, kBlock, 0, ctx->stream>>>((const uint32_t*)v.data, (uint32_t*)dst, v.pW, pH, pD,
+                                                            per);
+  else
+    pack_kernel<uint16_t><<<grid, kBlock, 0, ctx->stream>>>((const uint16_t*)v.data, (uint16_t*)dst, v.pW, pH, pD,
+                                                            per);
   ASC_HIP(hipGetLastError());
   return ASC_OK;
 }

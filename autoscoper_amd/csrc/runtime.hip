@@ -163,6 +163,8 @@ int upload_volume(asc_context ctx, const Cropped<T>& c, float full, int storage,
   v->dev.pS = (int)(pW * pH);
   v->dev.storage = storage;
   v->dev.vscale = storage == ASC_STORAGE_U16 ? 1.0f / 65535.0f : 1.0f;
+  v->dev.layout = ASC_LAYOUT_LINEAR;
+  v->dev.esz = (int)esize;
   return ASC_OK;
 }
 
@@ -295,6 +297,8 @@ ASC_API int asc_fill_u8(asc_context ctx, void* dst, unsigned char value, size_t 
   return ASC_OK;
 }
 
+ASC_API int asc_volume_set_layout(asc_volume vol, int layout);
+
 ASC_API int asc_volume_create(asc_context ctx, const void* data, int width, int height, int depth, int bps,
                               const int flip[3], const double scale[3], int storage, asc_volume* out,
                               asc_volume_desc* desc)
@@ -358,6 +362,12 @@ ASC_API int asc_volume_create(asc_context ctx, const void* data, int width, int 
   d.inv_trans[1] = -((height - mx[1] - 1) / (float)dim[1]);
   d.inv_trans[2] = mn[2] / (float)dim[2];
   d.storage = storage;
+  // corner-replicated layout by default: one aligned vector load per sample
+  rc = asc_volume_set_layout(v, ASC_LAYOUT_OCTET);
+  if (rc != ASC_OK) {
+    asc_volume_destroy(v);
+    return rc;
+  }
   if (desc) *desc = d;
   *out = v;
   return ASC_OK;
@@ -373,7 +383,45 @@ ASC_API int asc_volume_describe(asc_volume vol, asc_volume_desc* desc)
 ASC_API int asc_volume_bytes(asc_volume vol, size_t* bytes)
 {
   ASC_CHECK(vol && bytes, ASC_ERR_INVALID, "bad argument");
-  *bytes = vol->bytes;
+  *bytes = vol->bytes + vol->packed_bytes;
+  return ASC_OK;
+}
+
+ASC_API int asc_volume_set_layout(asc_volume vol, int layout)
+{
+  ASC_CHECK(vol, ASC_ERR_INVALID, "null volume");
+  ASC_CHECK(layout >= ASC_LAYOUT_LINEAR && layout <= ASC_LAYOUT_OCTET, ASC_ERR_INVALID, "bad layout");
+  if (layout == vol->dev.layout) return ASC_OK;
+  asc_context ctx = vol->ctx;
+  vol->generation++;
+  ASC_HIP(hipStreamSynchronize(ctx->stream));
+  if (vol->packed) ASC_HIP(hipFree(vol->packed));
+  vol->packed = nullptr;
+  vol->packed_bytes = 0;
+  VolDev lin = vol->dev;
+  lin.data = vol->dptr;
+  lin.layout = ASC_LAYOUT_LINEAR;
+  lin.esz = lin.storage == ASC_STORAGE_F32 ? 4 : 2;
+  if (layout == ASC_LAYOUT_LINEAR) {
+    vol->dev = lin;
+    return ASC_OK;
+  }
+  const int per = layout == ASC_LAYOUT_QUAD ? 4 : 8;
+  const size_t n = (size_t)lin.pS * (lin.D + 2);
+  const size_t bytes = n * per * lin.esz;
+  hipError_t e = hipMalloc(&vol->packed, bytes);
+  if (e != hipSuccess) {
+    vol->packed = nullptr;
+    return fail(ASC_ERR_NOMEM, "packed volume allocation failed");
+  }
+  int rc = launch_pack_volume(ctx, lin, vol->packed, layout);
+  if (rc) return rc;
+  ASC_HIP(hipStreamSynchronize(ctx->stream));
+  vol->packed_bytes = bytes;
+  vol->dev = lin;
+  vol->dev.data = vol->packed;
+  vol->dev.layout = layout;
+  vol->dev.esz = per * lin.esz;
   return ASC_OK;
 }
 
@@ -383,6 +431,7 @@ ASC_API int asc_volume_destroy(asc_volume vol)
   hipSetDevice(vol->ctx->device);
   hipStreamSynchronize(vol->ctx->stream);
   if (vol->dptr) hipFree(vol->dptr);
+  if (vol->packed) hipFree(vol->packed);
   delete vol;
   return ASC_OK;
 }

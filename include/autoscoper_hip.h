@@ -120,6 +120,13 @@ ASC_API int asc_volume_create(asc_context ctx, const void* data, int width, int 
                               const int flip[3], const double scale[3], int storage, asc_volume* out,
                               asc_volume_desc* desc);
 ASC_API int asc_volume_describe(asc_volume vol, asc_volume_desc* desc);
+/* Device layout used by the ray march: LINEAR (one texel per element,
+ * default), QUAD (each element carries its 2x2 corner block in x/y: 4x
+ * memory, 2 loads per sample) or OCTET (2x2x2 block: 8x memory, 1 load per
+ * sample; the default). Values and results are identical; only the access
+ * pattern changes. */
+enum { ASC_LAYOUT_LINEAR = 0, ASC_LAYOUT_QUAD = 1, ASC_LAYOUT_OCTET = 2 };
+ASC_API int asc_volume_set_layout(asc_volume vol, int layout);
 ASC_API int asc_volume_destroy(asc_volume vol);
 /* Device bytes held by the volume (for memory-budget admission). */
 ASC_API int asc_volume_bytes(asc_volume vol, size_t* bytes);
@@ -247,6 +254,12 @@ ASC_API int asc_scene_last_info(asc_scene scene, int* info, int capacity, int64_
 /* Keep the filtered, masked DRR/radiograph and mask of every item of the
  * following batches for asc_scene_last_images (costs extra stores). */
 ASC_API int asc_scene_set_debug(asc_scene scene, int enable);
+/* Launch-shape knobs of the fused path (an autotuner sets them):
+ * ASC_TUNE_UNROLL: ray samples per loop iteration (1 or 2; default 2);
+ * ASC_TUNE_GRID: persistent grid size in blocks, multiple of 8 (0 = CUs x
+ * occupancy). */
+enum { ASC_TUNE_UNROLL = 0, ASC_TUNE_GRID = 1 };
+ASC_API int asc_scene_set_tuning(asc_scene scene, int key, int value);
 /* Time the fused kernels of eager (non-graph) batches with HIP events on
  * the scene's stream; ms = {ray march, filters+moments, finalize} of the
  * last batch. */

@@ -73,6 +73,27 @@ def test_drr_matches_oracle(asc, ctx, orc, tiny, storage):
     v.close()
 
 
+@pytest.mark.parametrize("storage", [0, 1, 2, 3])
+def test_layouts_are_bit_identical(asc, ctx, orc, tiny, storage):
+    """QUAD/OCTET corner-replicated layouts change only the access pattern."""
+    wl, prep = tiny["wl"], tiny["prep"]
+    v = asc.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3, storage=storage)
+    RW, RH = tiny["render"]
+    cam = tiny["cams"][0]
+    _, imv, _, _ = orc.view_geometry(cam, wl.gt_poses[0], prep["inv_scale"], prep["inv_trans"], RW, RH)
+    fvp = synth.full_frame_viewport(cam[24:28], RW, RH)
+    out = ctx.alloc(4 * RW * RH)
+    imgs = []
+    for layout in (v.LINEAR, v.QUAD, v.OCTET, v.LINEAR):
+        v.set_layout(layout)
+        ctx.drr_render(v, imv[:12], fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, out, RW, RH)
+        imgs.append(out.download((RH, RW)))
+    assert imgs[0].max() > 0
+    for im in imgs[1:]:
+        assert np.array_equal(im, imgs[0])
+    v.close()
+
+
 def test_drr_cutoff_and_full_frame(asc, ctx, orc, tiny, vol_pair):
     v, prep = vol_pair
     RW, RH = tiny["render"]

@@ -103,6 +103,19 @@ def test_background_mask_path(asc, ctx, orc, tiny):
     vol.close()
 
 
+def test_scene_follows_volume_layout(asc, ctx, tiny):
+    sc, vol, _ = build_scene(asc, ctx, tiny)
+    rng = np.random.default_rng(13)
+    poses = tiny["wl"].gt_poses[0] + np.concatenate([rng.uniform(-2, 2, (4, 3)), rng.uniform(-4, 4, (4, 3))], 1)
+    c0, _ = sc.eval_poses(poses)
+    for layout in (vol.QUAD, vol.OCTET):
+        vol.set_layout(layout)
+        c, _ = sc.eval_poses(poses)
+        assert np.array_equal(c, c0)
+    sc.close()
+    vol.close()
+
+
 def test_plain_path_matches_fused(asc, ctx, tiny):
     sc, vol, _ = build_scene(asc, ctx, tiny)
     rng = np.random.default_rng(8)
