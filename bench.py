@@ -174,6 +174,13 @@ def main(argv=None) -> int:
                "sample": f"{n} pose evaluations of the same workload (float-faithful oracle trackFrame, "
                          f"OpenMP over rows, {el_c:.1f}s)"}
 
+    # HBM traffic per launch of the ray-march kernel, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated with a copy kernel)
+    traffic = None
+    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "drr_tiles_cfg2_traffic.json")
+    if args.workload == "cfg2" and args.storage == "u16" and P == 100 and os.path.exists(prof):
+        with open(prof) as fh:
+            traffic = json.load(fh)["traffic_bytes_per_launch"]
     tiles = info[:, 4].sum()
     out = {
         "metric": "pose evals/sec (DRR+filters+NCC, 2 cams 1024^2)",
@@ -204,7 +211,7 @@ def main(argv=None) -> int:
             "peak": peak / 1e9,
             "unit": "GB/s",
             "frac": achieved / peak,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "drr_tiles",
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "ray_samples_per_launch": samples / args.steps,
