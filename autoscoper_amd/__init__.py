@@ -121,6 +121,7 @@ _SIGS = {
     "asc_camera_from_mayacam2": [_P(_d), _P(_d), _P(_d), _P(_d), _P(AscCamera)],
     "asc_inv_model_view": [_P(_d), _P(_f), _P(_f), _P(_f)],
     "asc_view_geometry": [_P(AscCamera), _P(_d), _P(AscVolumeDesc), _i, _i, _P(_f), _P(_d), _P(_u)],
+    "asc_calculate_viewport": [_P(_d), _P(AscCamera), _P(AscVolumeDesc), _P(_d)],
     "asc_drr_render": [_vp, _vp, _P(_f), _P(_f), _f, _f, _f, _vp, _u, _u],
     "asc_rad_render": [_vp, _vp, _P(_f), _P(_f), _vp, _u, _u],
     "asc_background_render": [_vp, _vp, _P(_f), _P(_f), _f, _vp, _u, _u],

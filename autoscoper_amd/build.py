@@ -74,6 +74,17 @@ def build_lib(force: bool = False) -> Path:
     return LIB
 
 
+FACADE_CHECK = ROOT / "tests" / "cpp" / "facade_check.bin"
+
+
+def build_facade_check() -> Path:
+    """C++ program that drives the xromm::gpu facade (used by a GPU test)."""
+    src = ROOT / "tests" / "cpp" / "facade_check.cpp"
+    _run(["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(src), "-o", str(FACADE_CHECK),
+          f"-L{PKG}", "-lautoscoper_hip", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../autoscoper_amd"])
+    return FACADE_CHECK
+
+
 def build_oracle() -> Path:
     _run(["make", "-s", "-C", str(ORACLE_DIR)])
     return ORACLE_DIR / "liboracle.so"
@@ -83,6 +94,7 @@ def main(argv: list[str]) -> int:
     force = "--force" in argv
     if "--oracle" not in argv:
         print("built", build_lib(force=force))
+        print("built", build_facade_check())
     if "--lib" not in argv:
         print("built", build_oracle())
     return 0

@@ -1,0 +1,711 @@
+// xromm / xromm::gpu facade over the C ABI (see include/autoscoper_xromm.hpp).
+// Every device operation goes through an asc_* entry point; this file holds
+// no device code.
+#include "../../include/autoscoper_xromm.hpp"
+
+#include <cstring>
+#include <iostream>
+#include <mutex>
+#include <sstream>
+#include <utility>
+
+namespace xromm {
+
+namespace {
+
+void check(int rc)
+{
+  if (rc < 0) throw gpu::Error(rc, asc_last_error());
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// CoordFrame (CoordFrame.cpp) through the C ABI's double-precision helpers.
+// ---------------------------------------------------------------------------
+CoordFrame::CoordFrame()
+{
+  std::memset(f_, 0, sizeof f_);
+  f_[0] = f_[4] = f_[8] = 1.0;
+}
+
+CoordFrame::CoordFrame(const double* rotation, const double* translation)
+{
+  std::memcpy(f_, rotation, 9 * sizeof(double));
+  std::memcpy(f_ + 9, translation, 3 * sizeof(double));
+}
+
+CoordFrame CoordFrame::from_xyzypr(const double* xyzypr)
+{
+  CoordFrame c;
+  check(asc_frame_from_xyzypr(xyzypr, c.f_));
+  return c;
+}
+
+CoordFrame CoordFrame::from_xyzAxis_angle(const double* xyzijk)
+{
+  CoordFrame c;
+  check(asc_frame_from_xyz_axis_angle(xyzijk, c.f_));
+  return c;
+}
+
+void CoordFrame::to_xyzypr(double* xyzypr) const { check(asc_frame_to_xyzypr(f_, xyzypr)); }
+
+void CoordFrame::to_matrix_row_order(double* m) const
+{
+  // CoordFrame.cpp:223-241
+  for (int row = 0; row < 3; row++) {
+    for (int col = 0; col < 3; col++) m[4 * row + col] = f_[3 * col + row];
+    m[4 * row + 3] = f_[9 + row];
+  }
+  m[12] = m[13] = m[14] = 0.0;
+  m[15] = 1.0;
+}
+
+CoordFrame CoordFrame::inverse() const
+{
+  CoordFrame c;
+  check(asc_frame_inverse(f_, c.f_));
+  return c;
+}
+
+CoordFrame CoordFrame::operator*(const CoordFrame& o) const
+{
+  CoordFrame c;
+  check(asc_frame_mul(f_, o.f_, c.f_));
+  return c;
+}
+
+void CoordFrame::point_to_world_space(const double* p, double* q) const
+{
+  // CoordFrame.cpp:355-370
+  for (int i = 0; i < 3; i++) q[i] = f_[i] * p[0] + f_[3 + i] * p[1] + f_[6 + i] * p[2] + f_[9 + i];
+}
+
+void CoordFrame::translate(const double* v)
+{
+  for (int i = 0; i < 3; i++) f_[9 + i] += v[i];
+}
+
+Volume::Volume(const void* data, size_t width, size_t height, size_t depth, int bps, double scaleX, double scaleY,
+               double scaleZ, bool flipX, bool flipY, bool flipZ)
+  : w_(width), h_(height), d_(depth), bps_(bps)
+{
+  if (bps != 8 && bps != 16) throw gpu::Error(ASC_ERR_UNSUPPORTED, "Volume: unsupported bit depth");
+  const size_t n = width * height * depth * (bps / 8);
+  data_.assign((const unsigned char*)data, (const unsigned char*)data + n);
+  s_[0] = scaleX;
+  s_[1] = scaleY;
+  s_[2] = scaleZ;
+  f_[0] = flipX;
+  f_[1] = flipY;
+  f_[2] = flipZ;
+}
+
+Camera::Camera(const asc_camera& c) : cam_(c), frame_(c.rotation, c.translation) {}
+
+Camera Camera::fromMayaCam2(const double K[9], const double R[9], const double t[3], const double size[2])
+{
+  asc_camera c;
+  check(asc_camera_from_mayacam2(K, R, t, size, &c));
+  return Camera(c);
+}
+
+namespace gpu {
+
+namespace {
+std::mutex g_ctx_mutex;
+asc_context g_ctx = nullptr;
+int g_device = 0;
+int g_ray_casters = 0, g_rad_renderers = 0, g_sobel = 0, g_contrast = 0, g_gauss = 0, g_sharpen = 0;
+
+std::string numbered(const char* base, int& counter)
+{
+  std::ostringstream s;
+  s << base << (++counter);
+  return s.str();
+}
+}  // namespace
+
+asc_context context()
+{
+  std::lock_guard<std::mutex> lock(g_ctx_mutex);
+  if (!g_ctx) check(asc_context_create(g_device, &g_ctx));
+  return g_ctx;
+}
+
+void set_device(int device)
+{
+  std::lock_guard<std::mutex> lock(g_ctx_mutex);
+  if (g_ctx) throw Error(ASC_ERR_INVALID, "set_device must precede the first GPU call");
+  g_device = device;
+}
+
+// ---- Buffer (OpenCL.cpp:1027-1095) ----------------------------------------
+Buffer::Buffer(size_t size, int) : ptr_(nullptr), size_(size) { check(asc_malloc(context(), size, &ptr_)); }
+
+Buffer::~Buffer()
+{
+  if (ptr_) asc_free(context(), ptr_);
+}
+
+void Buffer::read(const void* buf, size_t size) const
+{
+  check(asc_memcpy_h2d(context(), ptr_, buf, size ? size : size_));
+}
+
+void Buffer::write(void* buf, size_t size) const { check(asc_memcpy_d2h(context(), buf, ptr_, size ? size : size_)); }
+
+void Buffer::copy(const Buffer* dst, size_t size) const
+{
+  check(asc_memcpy_d2d(context(), dst->ptr_, ptr_, size ? size : size_));
+}
+
+void Buffer::fill(const char c) const { check(asc_fill_u8(context(), ptr_, (unsigned char)c, size_)); }
+
+void Buffer::fill(const float v) const { check(asc_fill_f32(context(), (float*)ptr_, v, size_ / sizeof(float))); }
+
+// ---- VolumeDescription -----------------------------------------------------
+VolumeDescription::VolumeDescription(const Volume& v, int storage) : vol_(nullptr)
+{
+  const int flip[3] = {v.flipX(), v.flipY(), v.flipZ()};
+  const double scale[3] = {v.scaleX(), v.scaleY(), v.scaleZ()};
+  check(asc_volume_create(context(), v.data(), (int)v.width(), (int)v.height(), (int)v.depth(), v.bps(), flip, scale,
+                          storage, &vol_, &desc_));
+}
+
+VolumeDescription::~VolumeDescription()
+{
+  if (vol_) asc_volume_destroy(vol_);
+}
+
+// ---- RayCaster (RayCaster.cpp:60-193) ---------------------------------------
+RayCaster::RayCaster()
+  : volumeDescription_(nullptr), sampleDistance_(0.5f), rayIntensity_(10.f), cutoff_(0.0f), visible_(true)
+{
+  std::memset(imv_, 0, sizeof imv_);
+  viewport_[0] = -1.0f;
+  viewport_[1] = -1.0f;
+  viewport_[2] = 2.0f;
+  viewport_[3] = 2.0f;
+  name_ = numbered("DrrRenderer", g_ray_casters);
+}
+
+void RayCaster::setVolume(VolumeDescription& volume) { volumeDescription_ = &volume; }
+
+void RayCaster::setInvModelView(const double* invModelView)
+{
+  if (!volumeDescription_) throw Error(ASC_ERR_INVALID, "RayCaster: ERROR: Unable to calculate matrix.");
+  check(asc_inv_model_view(invModelView, volumeDescription_->invScale(), volumeDescription_->invTrans(), imv_));
+}
+
+void RayCaster::setViewport(float x, float y, float width, float height)
+{
+  viewport_[0] = x;
+  viewport_[1] = y;
+  viewport_[2] = width;
+  viewport_[3] = height;
+}
+
+void RayCaster::render(const Buffer* buffer, unsigned width, unsigned height)
+{
+  if (!volumeDescription_) {
+    std::cerr << "RayCaster: WARNING: No volume loaded." << std::endl;
+    return;
+  }
+  if (!visible_) {
+    buffer->fill((char)0x00);
+    return;
+  }
+  check(asc_drr_render(context(), volumeDescription_->image(), imv_, viewport_, sampleDistance_, rayIntensity_, cutoff_,
+                       buffer->get(), width, height));
+}
+
+float RayCaster::getMinCutoff() const { return volumeDescription_ ? volumeDescription_->minValue() : 0.0f; }
+float RayCaster::getMaxCutoff() const { return volumeDescription_ ? volumeDescription_->maxValue() : 1.0f; }
+
+// ---- RadRenderer / BackgroundRenderer -----------------------------------------
+RadRenderer::RadRenderer() : image_(nullptr)
+{
+  const float d[4] = {-1.0f, -1.0f, 2.0f, 2.0f};
+  std::memcpy(image_plane_, d, sizeof d);
+  std::memcpy(viewport_, d, sizeof d);
+  name_ = numbered("RadRenderer", g_rad_renderers);
+}
+
+RadRenderer::~RadRenderer()
+{
+  if (image_) asc_image_destroy(image_);
+}
+
+void RadRenderer::set_rad(const void* data, size_t width, size_t height, size_t bps)
+{
+  if (bps != 8 && bps != 16 && bps != 32) throw Error(ASC_ERR_UNSUPPORTED, "RadRenderer::rad(): Unsupported bit depth");
+  if (image_) asc_image_destroy(image_);
+  image_ = nullptr;
+  check(asc_image_create(context(), data, (int)width, (int)height, (int)bps, &image_));
+}
+
+void RadRenderer::set_image_plane(float x, float y, float width, float height)
+{
+  image_plane_[0] = x;
+  image_plane_[1] = y;
+  image_plane_[2] = width;
+  image_plane_[3] = height;
+}
+
+void RadRenderer::set_viewport(float x, float y, float width, float height)
+{
+  viewport_[0] = x;
+  viewport_[1] = y;
+  viewport_[2] = width;
+  viewport_[3] = height;
+}
+
+void RadRenderer::render(const Buffer* buffer, unsigned width, unsigned height) const
+{
+  if (!image_) throw Error(ASC_ERR_INVALID, "RadRenderer: no radiograph set");
+  check(asc_rad_render(context(), image_, image_plane_, viewport_, buffer->get(), width, height));
+}
+
+BackgroundRenderer::BackgroundRenderer() : image_(nullptr)
+{
+  const float d[4] = {-1.0f, -1.0f, 2.0f, 2.0f};
+  std::memcpy(image_plane_, d, sizeof d);
+  std::memcpy(viewport_, d, sizeof d);
+}
+
+BackgroundRenderer::~BackgroundRenderer()
+{
+  if (image_) asc_image_destroy(image_);
+}
+
+void BackgroundRenderer::set_back(const void* data, size_t width, size_t height)
+{
+  if (image_) asc_image_destroy(image_);
+  image_ = nullptr;
+  check(asc_image_create(context(), data, (int)width, (int)height, 32, &image_));
+}
+
+void BackgroundRenderer::set_image_plane(float x, float y, float width, float height)
+{
+  image_plane_[0] = x;
+  image_plane_[1] = y;
+  image_plane_[2] = width;
+  image_plane_[3] = height;
+}
+
+void BackgroundRenderer::set_viewport(float x, float y, float width, float height)
+{
+  viewport_[0] = x;
+  viewport_[1] = y;
+  viewport_[2] = width;
+  viewport_[3] = height;
+}
+
+void BackgroundRenderer::render(const Buffer* buffer, unsigned width, unsigned height, float threshold) const
+{
+  if (!image_) throw Error(ASC_ERR_INVALID, "BackgroundRenderer: no background set");
+  check(asc_background_render(context(), image_, image_plane_, viewport_, threshold, buffer->get(), width, height));
+}
+
+// ---- filters ---------------------------------------------------------------
+SobelFilter::SobelFilter() : Filter(XROMM_GPU_SOBEL_FILTER, ""), scale_(1.0f), blend_(0.5f)
+{
+  name_ = numbered("SobelFilter", g_sobel);
+}
+
+void SobelFilter::apply(const Buffer* in, Buffer* out, int w, int h)
+{
+  check(asc_sobel(context(), in->get(), out->get(), w, h, scale_, blend_));
+}
+
+asc_filter SobelFilter::record() const { return asc_filter{ASC_FILTER_SOBEL, enabled_, {scale_, blend_, 0, 0}}; }
+
+ContrastFilter::ContrastFilter() : Filter(XROMM_GPU_CONTRAST_FILTER, ""), alpha_(1.0f), beta_(1.0f), size_(3)
+{
+  name_ = numbered("ContrastFilter", g_contrast);
+}
+
+void ContrastFilter::apply(const Buffer* in, Buffer* out, int w, int h)
+{
+  check(asc_contrast(context(), in->get(), out->get(), w, h, alpha_, beta_, size_));
+}
+
+asc_filter ContrastFilter::record() const
+{
+  return asc_filter{ASC_FILTER_CONTRAST, enabled_, {alpha_, beta_, (float)size_, 0}};
+}
+
+GaussianFilter::GaussianFilter() : Filter(XROMM_GPU_GAUSSIAN_FILTER, ""), radius_(1.0f)
+{
+  name_ = numbered("GaussianFilter", g_gauss);
+}
+
+void GaussianFilter::apply(const Buffer* in, Buffer* out, int w, int h)
+{
+  check(asc_gaussian(context(), in->get(), out->get(), w, h, radius_));
+}
+
+asc_filter GaussianFilter::record() const { return asc_filter{ASC_FILTER_GAUSSIAN, enabled_, {radius_, 0, 0, 0}}; }
+
+SharpenFilter::SharpenFilter()
+  : Filter(XROMM_GPU_SHARPEN_FILTER, ""), radius_(1.0f), contrast_(1.0f), threshold_(0.0f)
+{
+  name_ = numbered("SharpenFilter", g_sharpen);
+}
+
+void SharpenFilter::apply(const Buffer* in, Buffer* out, int w, int h)
+{
+  check(asc_sharpen(context(), in->get(), out->get(), w, h, radius_, contrast_, threshold_));
+}
+
+asc_filter SharpenFilter::record() const
+{
+  return asc_filter{ASC_FILTER_SHARPEN, enabled_, {radius_, contrast_, threshold_, 0}};
+}
+
+// ---- free functions ----------------------------------------------------------
+void ncc_init(unsigned max_n) { check(asc_ncc_init(context(), max_n)); }
+void ncc_deinit() { check(asc_ncc_deinit(context())); }
+
+float ncc(Buffer* f, Buffer* g, Buffer* mask, unsigned n)
+{
+  double v = 0.0;
+  check(asc_ncc(context(), f->get(), g->get(), mask->get(), n, &v, nullptr));
+  return (float)v;
+}
+
+void multiply(const Buffer* a, const Buffer* b, const Buffer* dest, unsigned w, unsigned h)
+{
+  check(asc_multiply(context(), a->get(), b->get(), dest->get(), w, h));
+}
+
+void drr_background(const Buffer* src, const Buffer* dest, unsigned w, unsigned h)
+{
+  check(asc_drr_background(context(), src->get(), dest->get(), w, h));
+}
+
+void merge(const Buffer* a, const Buffer* b, const Buffer* dest, unsigned w, unsigned h)
+{
+  check(asc_merge(context(), a->get(), b->get(), dest->get(), w, h));
+}
+
+void composite(const Buffer* s1, const Buffer* s2, const Buffer* s3, const Buffer* s4, const Buffer* dest, unsigned w,
+               unsigned h)
+{
+  check(asc_composite(context(), s1->get(), s2->get(), s3->get(), s4 ? s4->get() : nullptr, dest->get(), w, h));
+}
+
+// ---- View (View.cpp) -----------------------------------------------------------
+View::View(Camera& camera)
+  : drr_enabled(true),
+    rad_enabled(true),
+    camera_(&camera),
+    radRenderer_(new RadRenderer()),
+    backgroundRenderer_(new BackgroundRenderer()),
+    maxWidth_(3000),
+    maxHeight_(3000),
+    backgroundThreshold_(-1.0f),
+    inited_(false)
+{
+}
+
+View::~View()
+{
+  for (Filter* f : drrFilters_) delete f;
+  for (Filter* f : radFilters_) delete f;
+}
+
+void View::addDrrRenderer()
+{
+  drrRenderer_.emplace_back(new RayCaster());
+  drrBuffer_.emplace_back(nullptr);
+}
+
+// View.cpp:446-472 (OpenCL flavour): 3000 x 3000 scratch images, masks = 1
+void View::init(unsigned width, unsigned height)
+{
+  if ((size_t)width * height > maxWidth_ * maxHeight_)
+    throw Error(ASC_ERR_TOO_LARGE, "GPU Buffers too small.");
+  if (inited_) return;
+  const size_t bytes = maxWidth_ * maxHeight_ * sizeof(float);
+  filterBuffer_.reset(new Buffer(bytes));
+  for (auto& b : drrBuffer_) b.reset(new Buffer(bytes));
+  drrBufferMerged_.reset(new Buffer(bytes));
+  drrFilterBuffer_.reset(new Buffer(bytes));
+  radBuffer_.reset(new Buffer(bytes));
+  radFilterBuffer_.reset(new Buffer(bytes));
+  backgroundmask_.reset(new Buffer(bytes));
+  drr_mask_.reset(new Buffer(bytes));
+  backgroundmask_->fill(1.0f);
+  drr_mask_->fill(1.0f);
+  inited_ = true;
+}
+
+void View::renderRad(Buffer* buffer, unsigned width, unsigned height)
+{
+  init(width, height);
+  radRenderer_->render(radBuffer_.get(), width, height);
+  filter(radFilters_, radBuffer_.get(), buffer, width, height);
+}
+
+void View::renderBackground(Buffer* buffer, unsigned width, unsigned height)
+{
+  if (backgroundThreshold_ < 0.0) return;
+  backgroundRenderer_->render(buffer, width, height, backgroundThreshold_);
+}
+
+void View::renderDRRMask(Buffer* in_buffer, Buffer* out_buffer, unsigned width, unsigned height)
+{
+  drr_background(in_buffer, out_buffer, width, height);
+}
+
+void View::renderDrr(Buffer* buffer, unsigned width, unsigned height)
+{
+  init(width, height);
+  drrBufferMerged_->fill((char)0x00);
+  for (size_t i = 0; i < drrRenderer_.size(); i++) {
+    drrRenderer_[i]->render(drrBuffer_[i].get(), width, height);
+    merge(drrBufferMerged_.get(), drrBuffer_[i].get(), drrBufferMerged_.get(), width, height);
+  }
+  filter(drrFilters_, drrBufferMerged_.get(), buffer, width, height);
+}
+
+void View::renderDrrSingle(int volume, Buffer* buffer, unsigned width, unsigned height)
+{
+  init(width, height);
+  drrRenderer_[volume]->render(drrBuffer_[volume].get(), width, height);
+  filter(drrFilters_, drrBuffer_[volume].get(), buffer, width, height);
+}
+
+void View::render(Buffer* rgb, unsigned width, unsigned height)
+{
+  init(width, height);
+  if (drr_enabled)
+    renderDrr(drrFilterBuffer_.get(), width, height);
+  else
+    drrFilterBuffer_->fill((char)0x00);
+  if (rad_enabled)
+    renderRad(radFilterBuffer_.get(), width, height);
+  else
+    radFilterBuffer_->fill((char)0x00);
+  renderBackground(backgroundmask_.get(), width, height);
+  renderDRRMask(drrFilterBuffer_.get(), drr_mask_.get(), width, height);
+  composite(drrFilterBuffer_.get(), radFilterBuffer_.get(), backgroundmask_.get(), drr_mask_.get(), rgb, width, height);
+}
+
+void View::updateBackground(const float* buffer, unsigned width, unsigned height)
+{
+  backgroundRenderer_->set_back(buffer, width, height);
+}
+
+void View::filter(const std::vector<Filter*>& filters, const Buffer* input, Buffer* output, unsigned width,
+                  unsigned height)
+{
+  const size_t bytes = (size_t)width * height * sizeof(float);
+  if (filters.empty()) {
+    input->copy(output, bytes);
+    return;
+  }
+  Buffer* b1 = (filters.size() % 2) ? output : filterBuffer_.get();
+  Buffer* b2 = (filters.size() % 2) ? filterBuffer_.get() : output;
+  if (filters[0]->enabled())
+    filters[0]->apply(input, b1, (int)width, (int)height);
+  else
+    input->copy(b1, bytes);
+  for (size_t k = 1; k < filters.size(); k++) {
+    if (filters[k]->enabled())
+      filters[k]->apply(b1, b2, (int)width, (int)height);
+    else
+      b1->copy(b2, bytes);
+    std::swap(b1, b2);
+  }
+}
+
+}  // namespace gpu
+
+// ---------------------------------------------------------------------------
+// Tracker
+// ---------------------------------------------------------------------------
+Tracker::Tracker() : scene_(nullptr), render_width_(0), render_height_(0), current_volume_(0) {}
+
+Tracker::~Tracker()
+{
+  if (scene_) asc_scene_destroy(scene_);
+  for (asc_image im : scene_rads_) asc_image_destroy(im);
+}
+
+void Tracker::load(const std::vector<VolumeSpec>& volumes, std::vector<Camera>& cameras,
+                   const std::vector<const void*>& rads, const std::vector<std::pair<int, int>>& rad_sizes,
+                   int rad_bps, int render_width, int render_height)
+{
+  if (cameras.size() != rads.size() || rads.size() != rad_sizes.size())
+    throw gpu::Error(ASC_ERR_INVALID, "Tracker::load: one radiograph per camera");
+  views_.clear();
+  volumeDescription_.clear();
+  pivots_.clear();
+  cameras_ = cameras;
+  render_width_ = render_width;
+  render_height_ = render_height;
+  // Tracker.cpp:281-290: describe each volume and centre its pivot
+  for (const VolumeSpec& vs : volumes) {
+    volumeDescription_.emplace_back(new gpu::VolumeDescription(*vs.volume));
+    CoordFrame pivot = vs.pivot;
+    pivot.translate(volumeDescription_.back()->transCenter());
+    pivots_.push_back(pivot);
+  }
+  const size_t bytes = (size_t)render_width * render_height * sizeof(float);
+  rendered_drr_.reset(new gpu::Buffer(bytes));
+  rendered_rad_.reset(new gpu::Buffer(bytes));
+  drr_mask_.reset(new gpu::Buffer(bytes));
+  background_mask_.reset(new gpu::Buffer(bytes));
+  drr_mask_->fill(1.0f);
+  background_mask_->fill(1.0f);
+  gpu::ncc_init((unsigned)(render_width * render_height));
+  // Tracker.cpp:310-334: one View per camera
+  for (size_t i = 0; i < cameras_.size(); i++) {
+    std::unique_ptr<gpu::View> view(new gpu::View(cameras_[i]));
+    for (size_t v = 0; v < volumeDescription_.size(); v++) {
+      view->addDrrRenderer();
+      view->drrRenderer((int)v)->setVolume(*volumeDescription_[v]);
+    }
+    const double* vp = cameras_[i].viewport();
+    view->radRenderer()->set_image_plane((float)vp[0], (float)vp[1], (float)vp[2], (float)vp[3]);
+    view->radRenderer()->set_rad(rads[i], rad_sizes[i].first, rad_sizes[i].second, rad_bps);
+    view->backgroundRenderer()->set_image_plane((float)vp[0], (float)vp[1], (float)vp[2], (float)vp[3]);
+    views_.push_back(std::move(view));
+  }
+  // the fused batch path sees the same device volumes and radiographs
+  if (scene_) asc_scene_destroy(scene_);
+  for (asc_image im : scene_rads_) asc_image_destroy(im);
+  scene_rads_.clear();
+  check(asc_scene_create(gpu::context(), render_width, render_height, &scene_));
+  for (size_t v = 0; v < volumeDescription_.size(); v++) {
+    double pv[12];
+    std::memcpy(pv, pivots_[v].rotation(), 9 * sizeof(double));
+    std::memcpy(pv + 9, pivots_[v].translation(), 3 * sizeof(double));
+    check(asc_scene_add_volume(scene_, volumeDescription_[v]->image(), 0.5f, 10.0f, 0.0f, pv, nullptr));
+  }
+  for (size_t i = 0; i < cameras_.size(); i++) {
+    asc_image im = nullptr;
+    check(asc_image_create(gpu::context(), rads[i], rad_sizes[i].first, rad_sizes[i].second, rad_bps, &im));
+    scene_rads_.push_back(im);
+    check(asc_scene_add_camera(scene_, &cameras_[i].record(), im, nullptr));
+  }
+}
+
+void Tracker::setDrrParams(int volume, float step, float intensity, float cutoff)
+{
+  for (auto& v : views_) {
+    v->drrRenderer(volume)->setSampleDistance(step);
+    v->drrRenderer(volume)->setRayIntensity(intensity);
+    v->drrRenderer(volume)->setCutoff(cutoff);
+  }
+}
+
+bool Tracker::calculate_viewport(const CoordFrame& modelview, const Camera& camera, double* viewport) const
+{
+  double mv[12];
+  std::memcpy(mv, modelview.rotation(), 9 * sizeof(double));
+  std::memcpy(mv + 9, modelview.translation(), 3 * sizeof(double));
+  const int rc = asc_calculate_viewport(mv, &camera.record(), &volumeDescription_[current_volume_]->desc(), viewport);
+  check(rc);
+  return rc == 1;
+}
+
+std::vector<double> Tracker::trackFrame(unsigned volumeID, const double* xyzypr) const
+{
+  std::vector<double> correlations;
+  const CoordFrame xcframe = CoordFrame::from_xyzypr(xyzypr);
+  for (size_t i = 0; i < views_.size(); ++i) {
+    gpu::View* view = views_[i].get();
+    const CoordFrame modelview = view->camera()->coord_frame().inverse() * xcframe;
+    double imv[16];
+    modelview.inverse().to_matrix_row_order(imv);
+    view->drrRenderer(volumeID)->setInvModelView(imv);
+    double viewport[4];
+    double mv[12];
+    std::memcpy(mv, modelview.rotation(), 9 * sizeof(double));
+    std::memcpy(mv + 9, modelview.translation(), 3 * sizeof(double));
+    const int inview = asc_calculate_viewport(mv, &view->camera()->record(), &volumeDescription_[volumeID]->desc(),
+                                              viewport);
+    check(inview);
+    if (!inview) {
+      correlations.push_back(9999.0);
+      continue;
+    }
+    const unsigned rw = (unsigned)(viewport[2] * render_width_ / view->camera()->viewport()[2]);
+    const unsigned rh = (unsigned)(viewport[3] * render_height_ / view->camera()->viewport()[3]);
+    if (rw > (unsigned)render_width_ || rh > (unsigned)render_height_)
+      throw gpu::Error(ASC_ERR_TOO_LARGE, "Tracker::trackFrame(): Rendered image is larger than the viewport buffer!");
+    view->drrRenderer(volumeID)->setViewport((float)viewport[0], (float)viewport[1], (float)viewport[2],
+                                             (float)viewport[3]);
+    view->radRenderer()->set_viewport((float)viewport[0], (float)viewport[1], (float)viewport[2], (float)viewport[3]);
+    view->renderDrrSingle((int)volumeID, rendered_drr_.get(), rw, rh);
+    view->renderRad(rendered_rad_.get(), rw, rh);
+    view->backgroundRenderer()->set_viewport((float)viewport[0], (float)viewport[1], (float)viewport[2],
+                                             (float)viewport[3]);
+    view->renderBackground(background_mask_.get(), rw, rh);
+    view->renderDRRMask(rendered_drr_.get(), drr_mask_.get(), rw, rh);
+    gpu::multiply(background_mask_.get(), drr_mask_.get(), drr_mask_.get(), rw, rh);
+    gpu::multiply(rendered_rad_.get(), drr_mask_.get(), rendered_rad_.get(), rw, rh);
+    gpu::multiply(rendered_drr_.get(), drr_mask_.get(), rendered_drr_.get(), rw, rh);
+    correlations.push_back(1.0 - gpu::ncc(rendered_drr_.get(), rendered_rad_.get(), drr_mask_.get(), rw * rh));
+  }
+  return correlations;
+}
+
+double Tracker::minimizationFunc(const float store[7], const double* values) const
+{
+  double pv[12];
+  std::memcpy(pv, pivots_[current_volume_].rotation(), 9 * sizeof(double));
+  std::memcpy(pv + 9, pivots_[current_volume_].translation(), 3 * sizeof(double));
+  double xyzypr[6];
+  check(asc_compose_pose(store, pv, values, xyzypr));
+  const std::vector<double> c = trackFrame((unsigned)current_volume_, xyzypr);
+  double total = c[0];
+  for (size_t i = 1; i < c.size(); i++) total += c[i];
+  if (total < 0) total = 9999;
+  return total;
+}
+
+void Tracker::sync_scene_filters()
+{
+  for (size_t i = 0; i < views_.size(); i++) {
+    for (int which = 0; which < 2; which++) {
+      const std::vector<gpu::Filter*>& fs = which ? views_[i]->radFilters() : views_[i]->drrFilters();
+      std::vector<asc_filter> recs;
+      for (const gpu::Filter* f : fs) recs.push_back(f->record());
+      check(asc_scene_set_filters(scene_, (int)i, which, recs.data(), (int)recs.size()));
+    }
+    gpu::BackgroundRenderer* bg = views_[i]->backgroundRenderer();
+    check(asc_scene_set_background(scene_, (int)i, bg->image(), views_[i]->backgroundThreshold()));
+  }
+  for (size_t v = 0; v < volumeDescription_.size() && !views_.empty(); v++) {
+    gpu::RayCaster* rc = views_[0]->drrRenderer((int)v);
+    check(asc_scene_set_drr_params(scene_, (int)v, rc->getSampleDistance(), rc->getRayIntensity(), rc->getCutoff()));
+  }
+}
+
+std::vector<double> Tracker::minimizationFuncBatch(const float store[7], const std::vector<double>& values) const
+{
+  const int n = (int)(values.size() / 6);
+  const_cast<Tracker*>(this)->sync_scene_filters();
+  std::vector<float> stores((size_t)n * 7);
+  for (int i = 0; i < n; i++) std::memcpy(&stores[(size_t)i * 7], store, 7 * sizeof(float));
+  std::vector<int> vols(n, current_volume_);
+  std::vector<double> costs(n);
+  check(asc_scene_eval_manips(scene_, vols.data(), stores.data(), values.data(), n, costs.data(), nullptr));
+  return costs;
+}
+
+asc_optimize_stats Tracker::optimize(float store[7], const asc_optimize_options& opt)
+{
+  sync_scene_filters();
+  asc_optimize_stats st;
+  check(asc_optimize_frame(scene_, store, &opt, &st));
+  return st;
+}
+
+}  // namespace xromm

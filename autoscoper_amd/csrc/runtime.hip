@@ -589,6 +589,13 @@ ASC_API int asc_inv_model_view(const double imv16[16], const float inv_scale[3],
   return ASC_OK;
 }
 
+ASC_API int asc_calculate_viewport(const double modelview[12], const asc_camera* cam, const asc_volume_desc* desc,
+                                   double vp_out[4])
+{
+  ASC_CHECK(modelview && cam && desc && vp_out, ASC_ERR_INVALID, "null pointer");
+  return calc_viewport(frame_load(modelview), *cam, desc->inv_trans, desc->inv_scale, vp_out) ? 1 : 0;
+}
+
 ASC_API int asc_view_geometry(const asc_camera* cam, const double xyzypr[6], const asc_volume_desc* desc,
                               int render_w, int render_h, float imv_out[12], double vp_out[4],
                               unsigned size_out[2])

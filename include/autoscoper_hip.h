@@ -152,6 +152,11 @@ ASC_API int asc_camera_from_mayacam2(const double K[9], const double R[9], const
 /* RayCaster::setInvModelView, RayCaster.cpp:92-142. */
 ASC_API int asc_inv_model_view(const double imv16[16], const float inv_scale[3], const float inv_trans[3],
                                float out16[16]);
+/* Tracker::calculate_viewport (Tracker.cpp:923-1032) for a model-view
+ * frame (12 doubles): film-plane bounding box {x, y, w, h} of the volume's
+ * box, clipped to the radiograph. Returns 1 in view, 0 not in view. */
+ASC_API int asc_calculate_viewport(const double modelview[12], const asc_camera* cam, const asc_volume_desc* desc,
+                                   double vp_out[4]);
 /* Tracker.cpp:604-629 for one camera: model-view, inverse model-view (a2),
  * calculate_viewport (a3, Tracker.cpp:923-1032) and render size.
  * Returns 1 in view, 0 not in view (cost 9999), <0 error. */
