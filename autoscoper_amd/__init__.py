@@ -164,6 +164,8 @@ _SIGS = {
     "asc_pose_store_get": [_P(_f), _P(_d)],
     "asc_compose_pose": [_P(_f), _P(_d), _P(_d), _P(_d)],
     "asc_optimize_frame": [_vp, _P(_f), _P(AscOptimizeOptions), _P(AscOptimizeStats)],
+    "asc_pso": [_vp, _vp, _u, _f, _f, _u, _u, _i, _i, _P(_f), _P(_f), _P(_i)],
+    "asc_simplex": [_vp, _vp, _P(_d), _P(_d), _i, _d, _P(_i)],
 }
 _RES = {"asc_last_error": C.c_char_p, "asc_context_stream": _vp}
 
@@ -309,6 +311,54 @@ def gaussian_weights(radius: float) -> np.ndarray:
     out = np.zeros(n.value * n.value, np.float32)
     _check(lib().asc_gaussian_weights(radius, _ptr(out, _f), out.size, C.byref(n)))
     return out.reshape(n.value, n.value)
+
+
+ERR_INVALID = -1  # ASC_ERR_INVALID
+COST_FN = C.CFUNCTYPE(C.c_int, _P(_d), C.c_int, _P(_d), C.c_void_p)
+
+
+def _cost_callback(cost, ndim):
+    """cost: a Python callable mapping an (n, ndim) float64 array to n costs,
+    or the address of a C function of the asc_cost_fn type."""
+    if isinstance(cost, int):
+        return COST_FN(cost)
+
+    def cb(x, n, out, _user):
+        try:
+            pts = np.ctypeslib.as_array(x, (n * ndim,)).reshape(n, ndim).copy()
+            res = np.asarray(cost(pts), np.float64).reshape(n)
+            np.ctypeslib.as_array(out, (n,))[:] = res
+            return 0
+        except Exception:  # noqa: BLE001 - reported as a status to the optimiser
+            return ERR_INVALID
+
+    return COST_FN(cb)
+
+
+class optimizers:  # namespace: the reference's optimisers on any cost function
+    @staticmethod
+    def pso(cost, seed, min_limit, max_limit, max_epochs, max_stall, particles=100, synchronous=False, user=None):
+        """pso() of PSO.cpp:198-295 (asynchronous order unless synchronous):
+        returns (best position float32[6], best cost, epochs)."""
+        cb = _cost_callback(cost, 6)
+        best = np.zeros(6, np.float32)
+        bc = C.c_float()
+        ep = C.c_int()
+        _check(lib().asc_pso(C.cast(cb, _vp), user, seed, min_limit, max_limit, max_epochs, max_stall, particles,
+                             int(bool(synchronous)), _ptr(best, _f), C.byref(bc), C.byref(ep)))
+        return best, bc.value, ep.value
+
+    @staticmethod
+    def simplex(cost, P, Y, ftol, user=None):
+        """AMOEBA of DownhillSimplex.cpp:39-172 on the start simplex P
+        ((ndim+1) x ndim) with row costs Y: returns (P, Y, iterations)."""
+        P = np.ascontiguousarray(P, np.float64).copy()
+        Y = np.ascontiguousarray(Y, np.float64).copy()
+        ndim = P.shape[1]
+        cb = _cost_callback(cost, ndim)
+        it = C.c_int()
+        _check(lib().asc_simplex(C.cast(cb, _vp), user, _ptr(P, _d), _ptr(Y, _d), ndim, ftol, C.byref(it)))
+        return P, Y, it.value
 
 
 class filters:  # namespace: filter records

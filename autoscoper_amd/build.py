@@ -86,7 +86,10 @@ def build_facade_check() -> Path:
 
 
 def build_oracle() -> Path:
+    """The CPU oracle, and, where the reference sources are present, its
+    host C++ for the pose math and optimisers into oracle/_ref."""
     _run(["make", "-s", "-C", str(ORACLE_DIR)])
+    _run(["make", "-s", "-C", str(ORACLE_DIR), "ref"])
     return ORACLE_DIR / "liboracle.so"
 
 

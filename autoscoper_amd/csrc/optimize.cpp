@@ -55,20 +55,16 @@ struct Particle {
   float vel[6] = {0, 0, 0, 0, 0, 0};
 };
 
-// Cost of manipulator offsets relative to the stored pose
-// (Tracker::minimizationFunc through the fused batch).
+// A cost function of n points of ndim values (n = 1 for the reference's
+// one-at-a-time calls, n = particles for a synchronous PSO epoch).
 struct Objective {
-  asc_scene scene;
-  const float* store;
-  int volume;
+  asc_cost_fn fn;
+  void* user;
   int64_t evals = 0;
   int status = ASC_OK;
-  int eval(const double* manips, int n, double* costs)
+  int eval(const double* x, int n, double* costs)
   {
-    std::vector<float> stores((size_t)n * 7);
-    for (int i = 0; i < n; i++) std::memcpy(&stores[(size_t)i * 7], store, 7 * sizeof(float));
-    std::vector<int> vols(n, volume);
-    const int rc = asc_scene_eval_manips(scene, vols.data(), stores.data(), manips, n, costs, nullptr);
+    const int rc = fn(x, n, costs, user);
     if (rc != ASC_OK) {
       status = rc;
       for (int i = 0; i < n; i++) costs[i] = NAN;
@@ -76,13 +72,30 @@ struct Objective {
     evals += n;
     return rc;
   }
-  double one(const double* m)
+  double one(const double* x)
   {
     double c;
-    eval(m, 1, &c);
+    eval(x, 1, &c);
     return c;
   }
 };
+
+// Tracker::minimizationFunc on manipulator offsets relative to the stored
+// pose, through the scene's fused batch path.
+struct SceneCost {
+  asc_scene scene;
+  const float* store;
+  int volume;
+};
+
+int scene_cost(const double* manips, int n, double* costs, void* user)
+{
+  const SceneCost* sc = (const SceneCost*)user;
+  std::vector<float> stores((size_t)n * 7);
+  for (int i = 0; i < n; i++) std::memcpy(&stores[(size_t)i * 7], sc->store, 7 * sizeof(float));
+  std::vector<int> vols(n, sc->volume);
+  return asc_scene_eval_manips(sc->scene, vols.data(), stores.data(), manips, n, costs, nullptr);
+}
 
 int pso(Objective& obj, GlibcRand& rng, float lo, float hi, unsigned max_epochs, unsigned max_stall, int np,
         bool synchronous, Particle& g_out)
@@ -260,7 +273,8 @@ extern "C" ASC_API int asc_optimize_frame(asc_scene scene, float store[7], const
   if (rc) return rc;
   const Frame pivot = asc::frame_load(pv);
   GlibcRand rng(opt->seed);  // initializeRandom, PSO.cpp:131-149
-  Objective obj{scene, store, opt->volume};
+  SceneCost sc{scene, store, opt->volume};
+  Objective obj{scene_cost, &sc};
   double xyzypr[6];
   asc_pose_store_get(store, xyzypr);
   const double zero[6] = {0, 0, 0, 0, 0, 0};
@@ -312,5 +326,35 @@ extern "C" ASC_API int asc_optimize_frame(asc_scene scene, float store[7], const
   asc_pose_store_set(store, xyzypr);
   st.evaluations = obj.evals;
   if (stats) *stats = st;
+  return ASC_OK;
+}
+
+extern "C" ASC_API int asc_pso(asc_cost_fn fn, void* user, unsigned seed, float min_limit, float max_limit,
+                               unsigned max_epochs, unsigned max_stall, int particles, int synchronous,
+                               float best[6], float* best_cost, int* epochs)
+{
+  if (!fn || !best || particles < 1) return asc::fail(ASC_ERR_INVALID, "asc_pso: bad argument");
+  GlibcRand rng(seed);
+  Objective obj{fn, user};
+  Particle g;
+  const int ep = pso(obj, rng, min_limit, max_limit, max_epochs, max_stall, particles, synchronous != 0, g);
+  if (obj.status) return obj.status;
+  std::memcpy(best, g.pos, sizeof g.pos);
+  if (best_cost) *best_cost = g.ncc;
+  if (epochs) *epochs = ep;
+  return ASC_OK;
+}
+
+extern "C" ASC_API int asc_simplex(asc_cost_fn fn, void* user, double* P, double* Y, int ndim, double ftol,
+                                   int* iterations)
+{
+  if (!fn || !P || !Y || ndim < 1 || ndim > 20) return asc::fail(ASC_ERR_INVALID, "asc_simplex: bad argument");
+  Objective obj{fn, user};
+  std::vector<double> PP(P, P + (size_t)(ndim + 1) * ndim), YY(Y, Y + ndim + 1);
+  const int it = amoeba(obj, PP, YY, ndim, ftol);
+  if (obj.status) return obj.status;
+  std::memcpy(P, PP.data(), PP.size() * sizeof(double));
+  std::memcpy(Y, YY.data(), YY.size() * sizeof(double));
+  if (iterations) *iterations = it;
   return ASC_OK;
 }

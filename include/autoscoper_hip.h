@@ -317,6 +317,28 @@ typedef struct {
 ASC_API int asc_optimize_frame(asc_scene scene, float store[7], const asc_optimize_options* opt,
                                asc_optimize_stats* stats);
 
+/* Cost of n points x[n][ndim] into costs[n]; returns ASC_OK or an error
+ * that stops the optimiser. The reference binds its cost through the
+ * global callbacks PSO_FUNC / FUNC (PSO.hpp:23, DownhillSimplex.hpp:27,
+ * Tracker.cpp:88-96); n is 1 except for a synchronous PSO epoch. */
+typedef int (*asc_cost_fn)(const double* x, int n, double* costs, void* user);
+
+/* Replaces pso(start_range_min, start_range_max, MAX_EPOCHS, MAX_STALL)
+ * (PSO.hpp:58, PSO.cpp:198-295) with srand(seed) (PSO.cpp:131-149) on a
+ * private generator. particles = NUM_OF_PARTICLES (100 in the reference);
+ * synchronous = 0 is the reference's order. best = gBest.Position,
+ * best_cost = gBest.NCC, epochs = the epoch count the reference prints. */
+ASC_API int asc_pso(asc_cost_fn fn, void* user, unsigned seed, float min_limit, float max_limit,
+                    unsigned max_epochs, unsigned max_stall, int particles, int synchronous, float best[6],
+                    float* best_cost, int* epochs);
+
+/* Replaces AMOEBA(P, Y, NDIM, FTOL, ITER) (DownhillSimplex.hpp:28,
+ * DownhillSimplex.cpp:39-172): P is (ndim+1) x ndim row-major and Y the
+ * costs of its rows, both 0-indexed here (1-indexed MAT in the reference);
+ * ndim <= 20 as the reference's MAT. */
+ASC_API int asc_simplex(asc_cost_fn fn, void* user, double* P, double* Y, int ndim, double ftol,
+                        int* iterations);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,11 @@ def lib():
             "orc_merge_f32": ([_P(_f), _P(_f), _P(_f), C.c_size_t], None),
             "orc_sample3": ([_vp, _P(_i), _d, _d, _d, _i], _d),
             "orc_composite_f32": ([_P(_f), _P(_f), _P(_f), _P(_f), C.c_size_t], None),
+            "orc_cost_ripple": ([_P(_d), _vp], _d),
+            "orc_cost_bowl": ([_P(_d), _vp], _d),
+            "orc_cost_sinc": ([_P(_d), _vp], _d),
+            "orc_pso_fn": ([_vp, _vp, _u, _f, _f, _u, _u, _i, _P(_f), _P(_f)], _i),
+            "orc_amoeba_fn": ([_vp, _vp, _P(_d), _P(_d), _i, _d], _i),
         }
         scene = [_vp, _P(_i), _P(_f), _P(_f), _i, _P(_d), _P(_vp), _P(_i), _P(_f), _P(_i), _i, _i, _f, _P(_vp),
                  _P(_i)]
@@ -315,6 +320,71 @@ def pso_bowl(centre, seed, lo, hi, max_epochs, max_stall, nparticles=100):
     bn = np.zeros(1, np.float32)
     ep = lib().orc_pso_bowl(_p(c, _d), seed, lo, hi, max_epochs, max_stall, nparticles, _p(best, _f), _p(bn, _f))
     return best, float(bn[0]), ep
+
+
+def cost_fn(name):
+    """Address of one of the oracle's exported test objectives
+    (orc_cost_ripple / orc_cost_bowl / orc_cost_sinc)."""
+    return C.cast(getattr(lib(), "orc_cost_" + name), C.c_void_p).value
+
+
+def pso_fn(name, centre, seed, lo, hi, max_epochs, max_stall, nparticles=100):
+    c = f64(centre).copy()
+    best = np.zeros(6, np.float32)
+    bn = np.zeros(1, np.float32)
+    ep = lib().orc_pso_fn(cost_fn(name), c.ctypes.data, seed, lo, hi, max_epochs, max_stall, nparticles,
+                          _p(best, _f), _p(bn, _f))
+    return best, float(bn[0]), ep
+
+
+def amoeba_fn(name, centre, P, ftol):
+    """Simplex from the (ndim+1) x ndim start P; Y is evaluated here."""
+    c = f64(centre).copy()
+    P = f64(P).copy()
+    n = P.shape[1]
+    Y = np.array([_cost(name, P[i], c) for i in range(n + 1)])
+    it = lib().orc_amoeba_fn(cost_fn(name), c.ctypes.data, _p(P, _d), _p(Y, _d), n, ftol)
+    return P, Y, it
+
+
+def _cost(name, x, centre):
+    x = f64(x)
+    return getattr(lib(), "orc_cost_" + name)(_p(x, _d), centre.ctypes.data)
+
+
+# ---------------------------------------------------------------------------
+# The reference's own host C++ (CoordFrame, PSO, DownhillSimplex, Quatf),
+# compiled from its sources into oracle/_ref/libref_host.so by `make ref`
+# where the reference is present. None when it is not built.
+# ---------------------------------------------------------------------------
+REF_LIB_PATH = HERE / "_ref" / "libref_host.so"
+_R = None
+
+
+def ref_lib():
+    global _R
+    if _R is None and REF_LIB_PATH.exists():
+        R = C.CDLL(str(REF_LIB_PATH))
+        sig = {
+            "ref_frame_from_xyzypr": ([_P(_d), _P(_d)], None),
+            "ref_frame_from_xyz_axis_angle": ([_P(_d), _P(_d)], None),
+            "ref_frame_to_xyzypr": ([_P(_d), _P(_d)], None),
+            "ref_frame_mul": ([_P(_d), _P(_d), _P(_d)], None),
+            "ref_frame_inverse": ([_P(_d), _P(_d)], None),
+            "ref_frame_linear_extrap": ([_P(_d), _P(_d), _P(_d)], None),
+            "ref_frame_matrix_row_order": ([_P(_d), _P(_d)], None),
+            "ref_pose_store_roundtrip": ([_P(_d), _P(_d)], None),
+            "ref_compose_pose": ([_P(_d), _P(_d), _P(_d), _P(_d)], None),
+            "ref_pso": ([_vp, _vp, _u, _f, _f, _u, _u, _P(_f), _P(_f)], C.c_long),
+            "ref_amoeba": ([_vp, _vp, _P(_d), _P(_d), _i, _d], _i),
+            "ref_calls": ([], C.c_long),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(R, name)
+            fn.argtypes = args
+            fn.restype = res
+        _R = R
+    return _R
 
 
 def glibc_rand(seed, n):

@@ -1575,6 +1575,33 @@ static double cost_bowl(const double* x, void* user)
   return s;
 }
 
+// Test objectives, exported so that the reference's own optimisers (built
+// into oracle/_ref) can be driven with the very same function.
+// user = double[6] centre; a weighted bowl with a ripple, and a 9999 plateau
+// outside |x_0 - c_0| > 4 as the tracker returns for out-of-view poses.
+ORC_API double orc_cost_ripple(const double* x, void* user)
+{
+  const double* c = (const double*)user;
+  if (std::fabs(x[0] - c[0]) > 4.0) return 9999.0;
+  double s = 0;
+  for (int i = 0; i < 6; i++) s += (1.0 + 0.5 * i) * (x[i] - c[i]) * (x[i] - c[i]);
+  return s + 0.3 * sin(3.0 * x[0]) * cos(2.0 * x[1]);
+}
+ORC_API double orc_cost_bowl(const double* x, void* user) { return cost_bowl(x, user); }
+ORC_API double orc_cost_sinc(const double* x, void* user) { return cost_sinc(x, user); }
+
+ORC_API int orc_pso_fn(cost_fn fn, void* user, unsigned seed, float lo, float hi, unsigned max_epochs,
+                       unsigned max_stall, int nparticles, float* best, float* best_ncc)
+{
+  srand(seed);
+  return pso(fn, user, lo, hi, max_epochs, max_stall, nparticles, best, best_ncc);
+}
+
+ORC_API int orc_amoeba_fn(cost_fn fn, void* user, double* P, double* Y, int ndim, double ftol)
+{
+  return amoeba(fn, user, P, Y, ndim, ftol);
+}
+
 ORC_API int orc_amoeba_sinc(double* P, double* Y, double ftol)
 {
   for (int i = 0; i < 3; i++) Y[i] = cost_sinc(P + 2 * i, nullptr);
