@@ -25,6 +25,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = ROOT / "_build"
 LIB = PKG / "libautoscoper_hip.so"
+FACADE_LIB = PKG / "libautoscoper_xromm.so"
 ORACLE_DIR = ROOT / "oracle"
 
 ARCH = os.environ.get("ASC_OFFLOAD_ARCH", "gfx950")
@@ -60,7 +61,15 @@ def _compile(src: Path) -> Path:
     return obj
 
 
+FACADE_SRCS = ("facade_gpu.cpp", "facade.cpp")
+
+
 def build_lib(force: bool = False) -> Path:
+    """libautoscoper_hip.so (the C ABI: kernels, host geometry, optimisers)
+    and libautoscoper_xromm.so (the standalone C++ facade on top of it).
+    The facade is a library of its own so that libautoscoper can compile
+    facade_gpu.cpp against its own Volume/Filter without two definitions of
+    the xromm::gpu classes in one process."""
     BUILD.mkdir(exist_ok=True)
     srcs = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
     if force:
@@ -68,9 +77,15 @@ def build_lib(force: bool = False) -> Path:
             o.unlink()
     with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
         objs = list(ex.map(_compile, srcs))
+    core = [o for s, o in zip(srcs, objs) if s.name not in FACADE_SRCS]
+    facade = [o for s, o in zip(srcs, objs) if s.name in FACADE_SRCS]
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, core)])
     os.replace(tmp, LIB)
+    tmp = FACADE_LIB.with_suffix(".so.tmp")
+    _run(["g++", "-shared", "-fPIC", "-o", str(tmp), *map(str, facade), f"-L{PKG}", "-lautoscoper_hip",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, FACADE_LIB)
     return LIB
 
 
@@ -81,7 +96,8 @@ def build_facade_check() -> Path:
     """C++ program that drives the xromm::gpu facade (used by a GPU test)."""
     src = ROOT / "tests" / "cpp" / "facade_check.cpp"
     _run(["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(src), "-o", str(FACADE_CHECK),
-          f"-L{PKG}", "-lautoscoper_hip", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../autoscoper_amd"])
+          f"-L{PKG}", "-lautoscoper_xromm", "-lautoscoper_hip", f"-Wl,-rpath,{PKG}",
+          "-Wl,-rpath,$ORIGIN/../../autoscoper_amd"])
     return FACADE_CHECK
 
 
