@@ -70,14 +70,14 @@ def lib():
             "orc_cost_ripple": ([_P(_d), _vp], _d),
             "orc_cost_bowl": ([_P(_d), _vp], _d),
             "orc_cost_sinc": ([_P(_d), _vp], _d),
-            "orc_pso_fn": ([_vp, _vp, _u, _f, _f, _u, _u, _i, _P(_f), _P(_f)], _i),
+            "orc_pso_fn": ([_vp, _vp, _u, _f, _f, _u, _u, _i, _i, _P(_f), _P(_f)], _i),
             "orc_amoeba_fn": ([_vp, _vp, _P(_d), _P(_d), _i, _d], _i),
         }
         scene = [_vp, _P(_i), _P(_f), _P(_f), _i, _P(_d), _P(_vp), _P(_i), _P(_f), _P(_i), _i, _i, _f, _P(_vp),
                  _P(_i)]
         sig["orc_track_frame"] = (scene + [_P(_d), _i, _P(_d), _P(_i), _P(_d), _P(_d)], C.c_int64)
         sig["orc_min_func"] = (scene + [_P(_f), _P(_d), _P(_d), _i], _d)
-        sig["orc_optimize_frame"] = (scene + [_P(_f), _P(_d), _i, _u, _d, _d, _u, _P(_d), _u, _i, _i, _P(_d)],
+        sig["orc_optimize_frame"] = (scene + [_P(_f), _P(_d), _i, _u, _d, _d, _u, _P(_d), _u, _i, _i, _i, _P(_d)],
                                      C.c_int64)
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -328,12 +328,12 @@ def cost_fn(name):
     return C.cast(getattr(lib(), "orc_cost_" + name), C.c_void_p).value
 
 
-def pso_fn(name, centre, seed, lo, hi, max_epochs, max_stall, nparticles=100):
+def pso_fn(name, centre, seed, lo, hi, max_epochs, max_stall, nparticles=100, synchronous=False):
     c = f64(centre).copy()
     best = np.zeros(6, np.float32)
     bn = np.zeros(1, np.float32)
     ep = lib().orc_pso_fn(cost_fn(name), c.ctypes.data, seed, lo, hi, max_epochs, max_stall, nparticles,
-                          _p(best, _f), _p(bn, _f))
+                          int(synchronous), _p(best, _f), _p(bn, _f))
     return best, float(bn[0]), ep
 
 
@@ -467,10 +467,11 @@ class Scene:
         return float(lib().orc_min_func(*self._args(), _p(s, _f), _p(pv, _d), _p(m, _d), int(fp64)))
 
     def optimize_frame(self, store, pivot, method=0, max_iter=100, min_limit=-3.0, max_limit=3.0, max_stall=25,
-                       offsets=(0.1,) * 6, seed=1, nparticles=100, fp64=False):
+                       offsets=(0.1,) * 6, seed=1, nparticles=100, fp64=False, synchronous=False):
         s = f32(store).copy()
         pv, off = f64(pivot), f64(offsets)
         stats = np.zeros(4)
         n = lib().orc_optimize_frame(*self._args(), _p(s, _f), _p(pv, _d), method, max_iter, min_limit, max_limit,
-                                     max_stall, _p(off, _d), seed, nparticles, int(fp64), _p(stats, _d))
+                                     max_stall, _p(off, _d), seed, nparticles, int(fp64), int(synchronous),
+                                     _p(stats, _d))
         return s, stats, int(n)

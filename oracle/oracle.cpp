@@ -1028,8 +1028,13 @@ struct Particle {
 float rand_clamped() { return (float)rand() / (float)RAND_MAX; }
 float rand_range(float lo, float hi) { return lo + rand_clamped() * (hi - lo); }
 
+// synchronous = true is the batched variant of the product (not in the
+// reference): every particle of an epoch moves against the gBest of the
+// previous epoch, all are evaluated, then pBest/gBest are updated in
+// particle order. The random draws are taken in the same order as the
+// reference's.
 int pso(cost_fn fn, void* user, float lo, float hi, unsigned max_epochs, unsigned max_stall, int nparticles,
-        float* best_pos, float* best_ncc)
+        float* best_pos, float* best_ncc, bool synchronous = false)
 {
   const float c1 = 1.5f, c2 = 1.5f;
   int stall = 0;
@@ -1053,12 +1058,14 @@ int pso(cost_fn fn, void* user, float lo, float hi, unsigned max_epochs, unsigne
   while (go) {
     if (counter >= max_epochs) go = false;
     Particle cur = g;
+    const Particle g_epoch = g;
     for (int i = 0; i < nparticles; i++) {
+      const Particle& gb = synchronous ? g_epoch : g;
       for (int d = 0; d < 6; d++) {
         float rp = rand_clamped();
         float rg = rand_clamped();
         p[i].vel[d] = omega * p[i].vel[d] + c1 * rp * (pb[i].pos[d] - p[i].pos[d]) +
-                      c2 * rg * (g.pos[d] - p[i].pos[d]);
+                      c2 * rg * (gb.pos[d] - p[i].pos[d]);
         p[i].pos[d] += p[i].vel[d];
       }
       double x[6];
@@ -1504,7 +1511,7 @@ ORC_API double orc_min_func(SCENE_ARGS, const float* store, const double* pivot,
 // simplex iterations, pso best cost, final cost}.
 ORC_API int64_t orc_optimize_frame(SCENE_ARGS, float* store, const double* pivot, int method, unsigned max_iter,
                                    double min_limit, double max_limit, unsigned max_stall, const double* offsets,
-                                   unsigned seed, int nparticles, int fp64, double* stats)
+                                   unsigned seed, int nparticles, int fp64, int synchronous, double* stats)
 {
   Scene S = make_scene(SCENE_PASS);
   srand(seed);
@@ -1520,7 +1527,7 @@ ORC_API int64_t orc_optimize_frame(SCENE_ARGS, float* store, const double* pivot
   if (method == 0) {
     float best[6], best_ncc;
     int epochs = pso(min_func_cb, &ctx, (float)min_limit, (float)max_limit, max_iter, max_stall, nparticles, best,
-                     &best_ncc);
+                     &best_ncc, synchronous != 0);
     double bm[6];
     for (int d = 0; d < 6; d++) bm[d] = best[d];
     manip = frame_from_xyz_axis_angle(bm);
@@ -1559,9 +1566,8 @@ ORC_API int64_t orc_optimize_frame(SCENE_ARGS, float* store, const double* pivot
   return ctx.evals;
 }
 
-// Generic optimizer entry points over a Python-free C callback are not
-// needed by the tests; these two expose PSO and AMOEBA on closed-form cost
-// functions for pinning (sphere bowl, sin(R)/R).
+// PSO and AMOEBA on closed-form cost functions for pinning (sphere bowl,
+// sin(R)/R), and on the exported test objectives below.
 static double cost_sinc(const double* x, void*)
 {
   double r = sqrt(x[0] * x[0] + x[1] * x[1]);
@@ -1591,10 +1597,10 @@ ORC_API double orc_cost_bowl(const double* x, void* user) { return cost_bowl(x, 
 ORC_API double orc_cost_sinc(const double* x, void* user) { return cost_sinc(x, user); }
 
 ORC_API int orc_pso_fn(cost_fn fn, void* user, unsigned seed, float lo, float hi, unsigned max_epochs,
-                       unsigned max_stall, int nparticles, float* best, float* best_ncc)
+                       unsigned max_stall, int nparticles, int synchronous, float* best, float* best_ncc)
 {
   srand(seed);
-  return pso(fn, user, lo, hi, max_epochs, max_stall, nparticles, best, best_ncc);
+  return pso(fn, user, lo, hi, max_epochs, max_stall, nparticles, best, best_ncc, synchronous != 0);
 }
 
 ORC_API int orc_amoeba_fn(cost_fn fn, void* user, double* P, double* Y, int ndim, double ftol)

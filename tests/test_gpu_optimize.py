@@ -1,0 +1,53 @@
+"""Tracker::optimize for one frame (PSO then downhill simplex, or the simplex
+alone) on the GPU scene (asc_optimize_frame) against the oracle's
+optimize_frame with the same seed, start pose and options.
+
+Bar: final poses within 0.05 mm (translation) and 0.05 deg (rotation angle
+between the two orientations). The reference-order (asynchronous) PSO is
+judged against the float-faithful oracle; the synchronous batched-epoch PSO
+against the oracle's synchronous mode in float and in FP64.
+"""
+import numpy as np
+import pytest
+
+from autoscoper_amd import synth
+
+from test_gpu_scene import build_scene, oracle_scene
+
+pytestmark = pytest.mark.gpu
+
+START = np.array([1.5, -1.2, 0.8, 2.0, -1.5, 1.0])  # ~2 mm / 2 deg off the hidden pose
+
+
+def pose_gap(orc, a, b):
+    """(translation distance mm, rotation angle deg) between two xyzypr poses."""
+    fa, fb = orc.frame_from_xyzypr(a), orc.frame_from_xyzypr(b)
+    Ra, Rb = fa[:9].reshape(3, 3), fb[:9].reshape(3, 3)
+    c = (np.trace(Ra.T @ Rb) - 1.0) / 2.0
+    return float(np.linalg.norm(fa[9:] - fb[9:])), float(np.degrees(np.arccos(np.clip(c, -1.0, 1.0))))
+
+
+@pytest.mark.parametrize("method,max_iter,synchronous", [(1, 100, False), (0, 6, False), (0, 6, True)])
+def test_optimize_frame_matches_oracle(asc, ctx, orc, tiny, method, max_iter, synchronous):
+    sc, vol, pivot = build_scene(asc, ctx, tiny)
+    S = oracle_scene(orc, asc, tiny)
+    start = tiny["wl"].gt_poses[0] + START
+    store0 = asc.geometry.pose_store_set(start)
+    opts = dict(method=method, max_iter=max_iter, max_stall=5, seed=7)
+    s_gpu, st = sc.optimize_frame(store0, synchronous=synchronous, **opts)
+    s_orc, ost, n = S.optimize_frame(store0, pivot, nparticles=100, synchronous=synchronous, **opts)
+    p_gpu = asc.geometry.pose_store_get(s_gpu)
+    p_orc = orc.pose_store_get(s_orc)
+    dt, dr = pose_gap(orc, p_gpu, p_orc)
+    print(f"method={method} sync={synchronous}: gpu evals={st.evaluations} epochs={st.pso_epochs} "
+          f"simplex={st.simplex_iterations} final={st.final_cost:.6f} | oracle evals={n} epochs={ost[0]:.0f} "
+          f"simplex={ost[1]:.0f} final={ost[3]:.6f} | gap {dt:.2e} mm {dr:.2e} deg")
+    assert dt <= 0.05 and dr <= 0.05
+    assert abs(st.final_cost - ost[3]) <= 1e-4
+    if synchronous:
+        s64, ost64, _ = S.optimize_frame(store0, pivot, nparticles=100, synchronous=True, fp64=True, **opts)
+        dt64, dr64 = pose_gap(orc, p_gpu, orc.pose_store_get(s64))
+        print(f"  vs FP64 oracle: {dt64:.2e} mm {dr64:.2e} deg")
+        assert dt64 <= 0.05 and dr64 <= 0.05
+    sc.close()
+    vol.close()

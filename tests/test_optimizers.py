@@ -1,0 +1,67 @@
+"""The product's optimisers (asc_pso, asc_simplex; C ABI, CPU) against the
+oracle's restatement, bit-exact, on the oracle's exported test objectives.
+The reference order (asynchronous PSO) is also pinned against the
+reference's own compiled code in test_oracle_vs_reference.py; the
+synchronous (batched-epoch) PSO is this build's variant and is pinned here
+against the oracle's synchronous mode and by its single-particle identity.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+CENTRE = np.array([0.7, -1.1, 0.4, 2.0, -1.5, 0.9])
+
+
+def bits(a):
+    a = np.asarray(a)
+    return a.view(np.uint64 if a.dtype == np.float64 else np.uint32)
+
+
+def point_cost(name):
+    f = getattr(O.lib(), "orc_cost_" + name)
+
+    def cost(pts):
+        return [f(np.ascontiguousarray(p).ctypes.data_as(C.POINTER(C.c_double)), CENTRE.ctypes.data) for p in pts]
+
+    return cost
+
+
+@pytest.mark.parametrize("synchronous", [False, True])
+@pytest.mark.parametrize("name,seed,epochs,stall,particles", [("ripple", 3, 60, 10, 100), ("bowl", 11, 40, 40, 37),
+                                                              ("ripple", 5, 0, 10, 8)])
+def test_pso_matches_oracle(asc, name, seed, epochs, stall, particles, synchronous):
+    best, cost, ep = asc.optimizers.pso(point_cost(name), seed, -3.0, 3.0, epochs, stall, particles,
+                                        synchronous=synchronous)
+    ob, oc, oep = O.pso_fn(name, CENTRE, seed, -3.0, 3.0, epochs, stall, particles, synchronous=synchronous)
+    assert ep == oep
+    assert (bits(best) == bits(ob)).all()
+    assert bits(np.float32(cost)) == bits(np.float32(oc))
+
+
+def test_synchronous_pso_batches_whole_epochs(asc):
+    calls = []
+
+    def cost(pts):
+        calls.append(len(pts))
+        return point_cost("bowl")(pts)
+
+    _, _, ep = asc.optimizers.pso(cost, 2, -3.0, 3.0, 5, 100, 64, synchronous=True)
+    assert calls == [64] * ep
+
+
+@pytest.mark.parametrize("name,ndim,ftol", [("ripple", 6, 1e-4), ("sinc", 2, 1e-9)])
+def test_simplex_matches_oracle(asc, name, ndim, ftol):
+    P0 = np.zeros((ndim + 1, ndim))
+    for i in range(ndim):
+        P0[i + 1, i] = 0.3
+    if name == "sinc":
+        P0 += 2.5
+    cost = point_cost(name)
+    Y0 = np.array(cost(P0))
+    P, Y, it = asc.optimizers.simplex(cost, P0, Y0, ftol)
+    Po, Yo, ito = O.amoeba_fn(name, CENTRE, P0, ftol)
+    assert it == ito
+    assert (bits(P) == bits(Po)).all() and (bits(Y) == bits(Yo)).all()
