@@ -529,7 +529,7 @@ class Volume:
         self.ctx = ctx
         self.scale = tuple(float(s) for s in scale)
 
-    LINEAR, QUAD, OCTET = 0, 1, 2
+    LINEAR, QUAD, OCTET, BRICK = 0, 1, 2, 3
 
     def set_layout(self, layout: int):
         _check(lib().asc_volume_set_layout(self.handle, layout))

@@ -57,8 +57,33 @@ __device__ __forceinline__ bool ray_setup(const float* __restrict__ imv, const f
 // reads 0 instead of faulting.
 struct VolRsrc {
   __amdgpu_buffer_rsrc_t rsrc;
-  int sy, sz;  // byte strides of +1 row and +1 slice
+  int sy, sz;  // byte strides of +1 row and +1 slice (< 2^23, checked on the host)
+  int c0;      // byte offset of padded element (1, 1, 1): voxel (0, 0, 0)
 };
+
+// log2 of the bytes per element of layout LY over storage ST: one texel
+// (LINEAR), a 2x2 block (QUAD) or a 2x2x2 cell (OCTET, BRICK).
+template <int ST, int LY>
+constexpr int elem_shift()
+{
+  return (ST == ASC_STORAGE_F32 ? 2 : 1) + (LY == ASC_LAYOUT_LINEAR ? 0 : LY == ASC_LAYOUT_QUAD ? 2 : 3);
+}
+
+// floor(x) as an int in one instruction (finite x within int range).
+__device__ __forceinline__ int cvt_flr(float x)
+{
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
+// a * b + c with a, b signed 24-bit (b wave-uniform) and c 32-bit.
+__device__ __forceinline__ int mad_i24(int a, int b, int c)
+{
+  int r;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+  return r;
+}
 
 // The volume of a ray is uniform across the wave (one volume per tile);
 // readfirstlane states that, so the descriptor and strides are scalar.
@@ -68,12 +93,15 @@ __device__ __forceinline__ VolRsrc vol_rsrc(const VolDev& v)
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
   void* p = (void*)(uintptr_t)(((uint64_t)hi << 32) | lo);
-  const int bytes = __builtin_amdgcn_readfirstlane(v.pS * (v.D + 2) * v.esz);
+  const int bytes = __builtin_amdgcn_readfirstlane(v.nbytes);
   VolRsrc r;
   // dword 3 = 0x00020000: DATA_FORMAT_32, as raw (untyped) buffer loads need
   r.rsrc = __builtin_amdgcn_make_buffer_rsrc(p, 0, bytes, 0x00020000);
   r.sy = __builtin_amdgcn_readfirstlane(v.pW * v.esz);
   r.sz = __builtin_amdgcn_readfirstlane(v.pS * v.esz);
+  // byte offset of the cell of voxel (0, 0, 0): padded element (1, 1, 1),
+  // or brick (1, 1, 1) cell 0 in the BRICK layout (8 cells per brick)
+  r.c0 = r.sy + r.sz + __builtin_amdgcn_readfirstlane(v.layout == ASC_LAYOUT_BRICK ? 8 * v.esz : v.esz);
   return r;
 }
 
@@ -155,9 +183,25 @@ __device__ __forceinline__ float trilinear(const VolDev& v, const VolRsrc& r, fl
   const float a = __builtin_amdgcn_fractf(X);
   const float b = __builtin_amdgcn_fractf(Y);
   const float c = __builtin_amdgcn_fractf(Z);
-  // in-box samples have -1 <= i0 <= W-1, so the padded index is >= 0
-  const int i = __float2int_rd(X) + 1, j = __float2int_rd(Y) + 1, k = __float2int_rd(Z) + 1;
-  const int voff = (int)(__umul24((unsigned)k, (unsigned)v.pS) + __umul24((unsigned)j, (unsigned)v.pW) + i) * v.esz;
+  // in-box samples have -1 <= i0 <= W-1, so the padded element
+  // (i0+1, j0+1, k0+1) is in the array and its byte offset
+  // i0*esz + j0*sy + k0*sz + c0 is >= 0; the +1s are folded into c0
+  const int i = cvt_flr(X), j = cvt_flr(Y), k = cvt_flr(Z);
+  int voff;
+  if constexpr (LY == ASC_LAYOUT_BRICK) {
+    // cell (i, j, k) lives in brick (i/2+1, j/2+1, k/2+1) (floor division),
+    // at cell (i&1, j&1, k&1) of the brick; with E bytes per cell, a brick
+    // of 8E bytes, and row/layer strides SY = r.sy, SZ = r.sz:
+    //   E*(i&1) + 8E*((i>>1)+1) = E*i + 6E*(i>>1) + 8E, and likewise
+    //   2E*(j&1) + SY*((j>>1)+1) = 2E*j + (SY-4E)*(j>>1) + SY,
+    //   4E*(k&1) + SZ*((k>>1)+1) = 4E*k + (SZ-8E)*(k>>1) + SZ;
+    // the constants sum to r.c0
+    constexpr int S = elem_shift<ST, LY>(), E = 1 << S;
+    const int base = (i << S) + (j << (S + 1)) + (k << (S + 2)) + r.c0;
+    voff = mad_i24(i >> 1, 6 * E, mad_i24(j >> 1, r.sy - 4 * E, mad_i24(k >> 1, r.sz - 8 * E, base)));
+  } else {
+    voff = mad_i24(k, r.sz, mad_i24(j, r.sy, (i << elem_shift<ST, LY>()) + r.c0));
+  }
   float t[8];
   gather8<ST, LY>(r, voff, t);
   const float c00 = fmaf(a, t[1] - t[0], t[0]);
@@ -248,6 +292,7 @@ __device__ __forceinline__ float march(const VolDev& v, const Ray& r, float cuto
   switch (v.layout) {                                                                     \
     case ASC_LAYOUT_QUAD: return march_st<ST, ASC_LAYOUT_QUAD, UNROLL>(v, r, cutoff_raw, nsamples);   \
     case ASC_LAYOUT_OCTET: return march_st<ST, ASC_LAYOUT_OCTET, UNROLL>(v, r, cutoff_raw, nsamples); \
+    case ASC_LAYOUT_BRICK: return march_st<ST, ASC_LAYOUT_BRICK, UNROLL>(v, r, cutoff_raw, nsamples); \
     default: return march_st<ST, ASC_LAYOUT_LINEAR, UNROLL>(v, r, cutoff_raw, nsamples);              \
   }
   switch (v.storage) {

@@ -50,6 +50,7 @@ struct VolDev {
   float vscale;  // raw texel value -> normalised value (1/65535 for U16, else 1)
   int layout;    // ASC_LAYOUT_*
   int esz;       // bytes per element of the layout
+  int nbytes;    // bytes of the layout's array (the buffer's extent)
 };
 
 struct ImgDev {

@@ -226,6 +226,30 @@ __global__ __launch_bounds__(kBlock) void pack_kernel(const T* __restrict__ src,
   }
 }
 
+// BRICK layout from the padded linear volume (pW = W+2, pH = H+2, pD = D+2,
+// voxel x at padded x+1): brick (bx, by, bz), cell c = px + 2py + 4pz holds
+// the 2x2x2 corner texels of voxel cell (2bx+px-2, 2by+py-2, 2bz+pz-2);
+// texels outside the padded grid are 0.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void pack_brick_kernel(const T* __restrict__ src, T* __restrict__ dst, int pW,
+                                                            int pH, int pD, int BX, int BY, int BZ)
+{
+  const size_t n = (size_t)BX * BY * BZ * 8;
+  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (size_t)gridDim.x * kBlock) {
+    const size_t b = e >> 3;
+    const int c = (int)(e & 7);
+    const int bx = (int)(b % BX), by = (int)((b / BX) % BY), bz = (int)(b / ((size_t)BX * BY));
+    const int ci = 2 * bx + (c & 1) - 2, cj = 2 * by + ((c >> 1) & 1) - 2, ck = 2 * bz + (c >> 2) - 2;
+    for (int q = 0; q < 8; q++) {
+      // padded coordinates of corner q
+      const int ii = ci + (q & 1) + 1, jj = cj + ((q >> 1) & 1) + 1, kk = ck + (q >> 2) + 1;
+      T v = T(0);
+      if (ii >= 0 && jj >= 0 && kk >= 0 && ii < pW && jj < pH && kk < pD) v = src[((size_t)kk * pH + jj) * pW + ii];
+      dst[e * 8 + q] = v;
+    }
+  }
+}
+
 dim3 grid2(unsigned w, unsigned h) { return dim3((w + 15) / 16, (h + 15) / 16); }
 
 }  // namespace
@@ -342,6 +366,17 @@ int launch_pack_volume(asc_context ctx, const VolDev& v, void* dst, int layout)
   const int per = layout == ASC_LAYOUT_QUAD ? 4 : 8;
   const int pH = v.pS / v.pW, pD = v.D + 2;
   const unsigned grid = 256 * 8;
+  if (layout == ASC_LAYOUT_BRICK) {
+    const int BX = (v.W + 3) / 2, BY = (v.H + 3) / 2, BZ = (v.D + 3) / 2;
+    if (v.storage == ASC_STORAGE_F32)
+      pack_brick_kernel<uint32_t><<<grid, kBlock, 0, ctx->stream>>>((const uint32_t*)v.data, (uint32_t*)dst, v.pW,
+                                                                    pH, pD, BX, BY, BZ);
+    else
+      pack_brick_kernel<uint16_t><<<grid, kBlock, 0, ctx->stream>>>((const uint16_t*)v.data, (uint16_t*)dst, v.pW,
+                                                                    pH, pD, BX, BY, BZ);
+    ASC_HIP(hipGetLastError());
+    return ASC_OK;
+  }
   if (v.storage == ASC_STORAGE_F32)
     pack_kernel<uint32_t><<<grid, kBlock, 0, ctx->stream>>>((const uint32_t*)v.data, (uint32_t*)dst, v.pW, pH, pD,
                                                             per);

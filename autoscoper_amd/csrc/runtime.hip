@@ -165,6 +165,7 @@ int upload_volume(asc_context ctx, const Cropped<T>& c, float full, int storage,
   v->dev.vscale = storage == ASC_STORAGE_U16 ? 1.0f / 65535.0f : 1.0f;
   v->dev.layout = ASC_LAYOUT_LINEAR;
   v->dev.esz = (int)esize;
+  v->dev.nbytes = (int)std::min<size_t>(host.size(), 0x7fffffff);
   return ASC_OK;
 }
 
@@ -299,6 +300,44 @@ ASC_API int asc_fill_u8(asc_context ctx, void* dst, unsigned char value, size_t 
 
 ASC_API int asc_volume_set_layout(asc_volume vol, int layout);
 
+// The ray march addresses an element with signed 24-bit multiplies of the
+// row and slice byte strides and a 32-bit buffer offset: a layout is usable
+// when both strides are below 2^23 bytes and the whole array below 2^31.
+struct LayoutShape {
+  int64_t esz;    // bytes per element
+  int64_t pW;     // elements per row step (bytes = pW * esz)
+  int64_t pS;     // elements per slice (or brick-layer) step
+  int64_t bytes;  // whole array
+};
+
+static LayoutShape layout_shape(const VolDev& lin, int layout)
+{
+  const int64_t tex = lin.storage == ASC_STORAGE_F32 ? 4 : 2;
+  LayoutShape s;
+  if (layout == ASC_LAYOUT_BRICK) {
+    // bricks of 2x2x2 cells over cells -2 .. 2B-3 per axis (cells -1 .. n-1 are sampled)
+    const int64_t BX = (lin.W + 3) / 2, BY = (lin.H + 3) / 2, BZ = (lin.D + 3) / 2;
+    s.esz = 8 * tex;
+    s.pW = 8 * BX;
+    s.pS = 8 * BX * BY;
+    s.bytes = s.pS * BZ * s.esz;
+    return s;
+  }
+  const int64_t per = layout == ASC_LAYOUT_LINEAR ? 1 : layout == ASC_LAYOUT_QUAD ? 4 : 8;
+  s.esz = per * tex;
+  s.pW = lin.W + 2;
+  s.pS = (int64_t)(lin.W + 2) * (lin.H + 2);
+  s.bytes = s.pS * (lin.D + 2) * s.esz;
+  return s;
+}
+
+static bool layout_fits(const VolDev& lin, int layout)
+{
+  const LayoutShape s = layout_shape(lin, layout);
+  return s.pS * s.esz < (int64_t(1) << 23) && s.bytes < (int64_t(1) << 31);
+}
+
+
 ASC_API int asc_volume_create(asc_context ctx, const void* data, int width, int height, int depth, int bps,
                               const int flip[3], const double scale[3], int storage, asc_volume* out,
                               asc_volume_desc* desc)
@@ -363,7 +402,10 @@ ASC_API int asc_volume_create(asc_context ctx, const void* data, int width, int 
   d.inv_trans[2] = mn[2] / (float)dim[2];
   d.storage = storage;
   // corner-replicated layout by default: one aligned vector load per sample
-  rc = asc_volume_set_layout(v, ASC_LAYOUT_OCTET);
+  // the densest layout whose addressing fits (OCTET: one load per sample)
+  int layout = ASC_LAYOUT_BRICK;
+  while (layout > ASC_LAYOUT_LINEAR && !layout_fits(v->dev, layout)) --layout;
+  rc = asc_volume_set_layout(v, layout);
   if (rc != ASC_OK) {
     asc_volume_destroy(v);
     return rc;
@@ -390,7 +432,9 @@ ASC_API int asc_volume_bytes(asc_volume vol, size_t* bytes)
 ASC_API int asc_volume_set_layout(asc_volume vol, int layout)
 {
   ASC_CHECK(vol, ASC_ERR_INVALID, "null volume");
-  ASC_CHECK(layout >= ASC_LAYOUT_LINEAR && layout <= ASC_LAYOUT_OCTET, ASC_ERR_INVALID, "bad layout");
+  ASC_CHECK(layout >= ASC_LAYOUT_LINEAR && layout <= ASC_LAYOUT_BRICK, ASC_ERR_INVALID, "bad layout");
+  ASC_CHECK(layout_fits(vol->dev, layout), ASC_ERR_TOO_LARGE,
+            "volume too large for this layout (slice >= 8 MiB or array >= 2 GiB)");
   if (layout == vol->dev.layout) return ASC_OK;
   asc_context ctx = vol->ctx;
   vol->generation++;
@@ -399,16 +443,19 @@ ASC_API int asc_volume_set_layout(asc_volume vol, int layout)
   vol->packed = nullptr;
   vol->packed_bytes = 0;
   VolDev lin = vol->dev;
+  const LayoutShape lin_shape = layout_shape(lin, ASC_LAYOUT_LINEAR);
   lin.data = vol->dptr;
   lin.layout = ASC_LAYOUT_LINEAR;
-  lin.esz = lin.storage == ASC_STORAGE_F32 ? 4 : 2;
+  lin.esz = (int)lin_shape.esz;
+  lin.pW = (int)lin_shape.pW;
+  lin.pS = (int)lin_shape.pS;
+  lin.nbytes = (int)lin_shape.bytes;
   if (layout == ASC_LAYOUT_LINEAR) {
     vol->dev = lin;
     return ASC_OK;
   }
-  const int per = layout == ASC_LAYOUT_QUAD ? 4 : 8;
-  const size_t n = (size_t)lin.pS * (lin.D + 2);
-  const size_t bytes = n * per * lin.esz;
+  const LayoutShape shape = layout_shape(lin, layout);
+  const size_t bytes = (size_t)shape.bytes;
   hipError_t e = hipMalloc(&vol->packed, bytes);
   if (e != hipSuccess) {
     vol->packed = nullptr;
@@ -421,7 +468,10 @@ ASC_API int asc_volume_set_layout(asc_volume vol, int layout)
   vol->dev = lin;
   vol->dev.data = vol->packed;
   vol->dev.layout = layout;
-  vol->dev.esz = per * lin.esz;
+  vol->dev.esz = (int)shape.esz;
+  vol->dev.pW = (int)shape.pW;
+  vol->dev.pS = (int)shape.pS;
+  vol->dev.nbytes = (int)shape.bytes;
   return ASC_OK;
 }
 

@@ -31,11 +31,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(A, synth, name: str, storage: str, ctx):
+LAYOUTS = {"linear": 0, "quad": 1, "octet": 2, "brick": 3}
+
+
+def build_workload(A, synth, name: str, storage: str, ctx, layout: str = "auto"):
     t0 = time.time()
     wl = synth.workload(name, frames=2)
     log(f"[bench] phantom {wl.volume.shape} generated in {time.time() - t0:.1f}s")
     vol = A.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3, storage=STORAGES[storage])
+    if layout != "auto":
+        vol.set_layout(LAYOUTS[layout])
     RW, RH = wl.render
     cams = [A.geometry.camera_from_mayacam2(*c.mayacam2()) for c in wl.cameras]
     gt = wl.gt_poses[0]
@@ -91,6 +96,8 @@ def main(argv=None) -> int:
     ap.add_argument("--particles", type=int, default=100)
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--storage", default="u16", choices=sorted(STORAGES))
+    ap.add_argument("--layout", default="auto", choices=["auto"] + sorted(LAYOUTS),
+                    help="device volume layout (auto: the densest that fits, BRICK for the workloads here)")
     ap.add_argument("--graph", action="store_true", help="replay the fused launches from a hipGraph")
     ap.add_argument("--plain-evals", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -117,7 +124,7 @@ def main(argv=None) -> int:
         ctx.synchronize()
         log(f"[bench] copied {nbytes} bytes x 5")
         return 0
-    wl, vol, cams, rads, scene, gt = build_workload(A, synth, args.workload, args.storage, ctx)
+    wl, vol, cams, rads, scene, gt = build_workload(A, synth, args.workload, args.storage, ctx, args.layout)
     RW, RH = wl.render
     P = args.particles
     rng = np.random.default_rng(synth.SEED_START)
@@ -200,6 +207,7 @@ def main(argv=None) -> int:
                         f"{args.storage} volume @{wl.voxel_mm}mm, {len(cams)} cams {RW}x{RH}, "
                         f"{P} poses per batch",
             "volume_storage": args.storage,
+            "volume_layout": args.layout,
             "particles_per_step": P,
             "cameras": len(cams),
             "render": [RW, RH],

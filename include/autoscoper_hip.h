@@ -120,12 +120,17 @@ ASC_API int asc_volume_create(asc_context ctx, const void* data, int width, int 
                               const int flip[3], const double scale[3], int storage, asc_volume* out,
                               asc_volume_desc* desc);
 ASC_API int asc_volume_describe(asc_volume vol, asc_volume_desc* desc);
-/* Device layout used by the ray march: LINEAR (one texel per element,
- * default), QUAD (each element carries its 2x2 corner block in x/y: 4x
- * memory, 2 loads per sample) or OCTET (2x2x2 block: 8x memory, 1 load per
- * sample; the default). Values and results are identical; only the access
- * pattern changes. */
-enum { ASC_LAYOUT_LINEAR = 0, ASC_LAYOUT_QUAD = 1, ASC_LAYOUT_OCTET = 2 };
+/* Device layout used by the ray march: LINEAR (one texel per element),
+ * QUAD (each element carries its 2x2 corner block in x/y: 4x memory, 2 loads
+ * per sample), OCTET (each element carries its 2x2x2 cell: 8x memory, 1 load
+ * per sample, x-fastest rows) or BRICK (OCTET elements grouped in 2x2x2
+ * bricks of one 128-byte line, so the cells a wave samples share fewer
+ * cache lines whatever the ray direction). Values and results are identical;
+ * only the access pattern changes. asc_volume_create picks the densest that
+ * fits (BRICK, else OCTET, QUAD, LINEAR): the march addresses with 24-bit
+ * products, so a layout needs slice/brick-layer strides below 8 MiB and the
+ * whole array below 2 GiB (ASC_ERR_TOO_LARGE otherwise). */
+enum { ASC_LAYOUT_LINEAR = 0, ASC_LAYOUT_QUAD = 1, ASC_LAYOUT_OCTET = 2, ASC_LAYOUT_BRICK = 3 };
 ASC_API int asc_volume_set_layout(asc_volume vol, int layout);
 ASC_API int asc_volume_destroy(asc_volume vol);
 /* Device bytes held by the volume (for memory-budget admission). */

@@ -75,7 +75,7 @@ def test_drr_matches_oracle(asc, ctx, orc, tiny, storage):
 
 @pytest.mark.parametrize("storage", [0, 1, 2, 3])
 def test_layouts_are_bit_identical(asc, ctx, orc, tiny, storage):
-    """QUAD/OCTET corner-replicated layouts change only the access pattern."""
+    """QUAD/OCTET/BRICK corner-replicated layouts change only the access pattern."""
     wl, prep = tiny["wl"], tiny["prep"]
     v = asc.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3, storage=storage)
     RW, RH = tiny["render"]
@@ -84,7 +84,7 @@ def test_layouts_are_bit_identical(asc, ctx, orc, tiny, storage):
     fvp = synth.full_frame_viewport(cam[24:28], RW, RH)
     out = ctx.alloc(4 * RW * RH)
     imgs = []
-    for layout in (v.LINEAR, v.QUAD, v.OCTET, v.LINEAR):
+    for layout in (v.LINEAR, v.QUAD, v.OCTET, v.BRICK, v.LINEAR):
         v.set_layout(layout)
         ctx.drr_render(v, imv[:12], fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, out, RW, RH)
         imgs.append(out.download((RH, RW)))
@@ -248,3 +248,17 @@ def test_ncc_degenerate(asc, ctx):
     assert v == 1e5  # den < 1e-5 (Ncc.cpp:215-217)
     v, _ = ctx.ncc(ctx.to_device(g), ctx.to_device(2 * g + 0.1), ctx.to_device(m), n)
     assert abs(v - 1.0) <= 1e-6
+
+
+def test_wide_slices_fall_back_to_a_layout_that_fits(asc, ctx):
+    """A 1100 x 1100 slice is too wide for BRICK/OCTET/QUAD addressing
+    (>= 8 MiB per slice step): creation falls back to LINEAR and asking for
+    a packed layout is an error, not a wrong result."""
+    vol = np.full((2, 1100, 1100), 3000, np.uint16)
+    v = asc.Volume(ctx, vol, scale=(0.5, 0.5, 0.5))
+    for layout in (v.QUAD, v.OCTET, v.BRICK):
+        with pytest.raises(asc.AscError):
+            v.set_layout(layout)
+    v.set_layout(v.LINEAR)
+    v.close()
+

@@ -108,7 +108,7 @@ def test_scene_follows_volume_layout(asc, ctx, tiny):
     rng = np.random.default_rng(13)
     poses = tiny["wl"].gt_poses[0] + np.concatenate([rng.uniform(-2, 2, (4, 3)), rng.uniform(-4, 4, (4, 3))], 1)
     c0, _ = sc.eval_poses(poses)
-    for layout in (vol.QUAD, vol.OCTET):
+    for layout in (vol.QUAD, vol.OCTET, vol.BRICK):
         vol.set_layout(layout)
         c, _ = sc.eval_poses(poses)
         assert np.array_equal(c, c0)
