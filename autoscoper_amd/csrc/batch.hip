@@ -6,10 +6,11 @@
 // bounding box, render size). The device then runs three kernels:
 //   A  drr_tiles   : persistent; 16x16-pixel tiles of every item's bounding
 //                    box; ray march -> DRR scratch (one image per item).
-//   B  post_tiles  : persistent; per 16x16 tile: DRR filter chain and
+//   B  post_tiles  : persistent; per 32x32 tile: DRR filter chain and
 //                    radiograph resample + filter chain in LDS with halos,
-//                    DRR mask, multiplies, nine NCC moments -> per-tile
-//                    partial (9 doubles).
+//                    DRR mask, multiplies, nine NCC moments (four pixels
+//                    per thread, compensated) -> per-tile partial
+//                    (9 doubles).
 //   C  finalize    : one block per item: fixed-order sum of its partials,
 //                    NCC in double; resets the work counters for the next
 //                    batch.
@@ -36,10 +37,11 @@ namespace asc {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kTile = 16;
+constexpr int kTile = 16;   // ray-march tile side
+constexpr int kPTile = 32;  // post-processing tile side (4 pixels per thread)
 constexpr int kMaxChain = 8;
 constexpr int kMaxHalo = 12;
-constexpr int kRegion = kTile + 2 * kMaxHalo;  // 40
+constexpr int kSmallHalo = 4;  // chains up to this halo use the smaller-LDS post kernel
 
 struct ChainDev {
   int n;
@@ -69,25 +71,32 @@ struct ItemDesc {
   float vp[4];
   int w, h;
   int vol, cam;
-  int tiles_x, tiles_y;
+  int tiles_x, tiles_y;    // ray-march tiles (kTile)
+  int ptiles_x, ptiles_y;  // post tiles (kPTile)
   int tile_base;  // first tile of this item in the batch's tile list
-  int part_off;   // first partial slot
+  int part_off;   // first partial slot (one per post tile)
   long long drr_off;
 };
 
 constexpr int kMaxCams = 8;
 constexpr int kParts = 8;  // XCDs per MI355X
 
-struct BatchHeader {
-  int n_items;
-  int total_tiles;  // size of the (camera, row, column, pose) enumeration
-  int debug;
-  int n_poses;
-  int n_cams;
-  int pad[3];
+// A (camera, tile row, tile column, pose) enumeration of the batch's tiles.
+struct TileEnum {
+  int total;
+  int pad;
   int cam_base[kMaxCams];  // first enumeration index of each camera
   int tiles_x[kMaxCams];   // widest item of the camera, in tiles
   int tiles_y[kMaxCams];
+};
+
+struct BatchHeader {
+  int n_items;
+  int debug;
+  int n_poses;
+  int n_cams;
+  TileEnum drr;   // kTile tiles
+  TileEnum post;  // kPTile tiles
 };
 
 struct Counters {
@@ -132,19 +141,21 @@ __device__ __forceinline__ int next_tile(unsigned int* counters, int total, int*
 
 // Enumeration index -> (item, tile column, tile row); false when the tile
 // lies outside that item's bounding box.
+template <bool POST>
 __device__ __forceinline__ bool decode_tile(const BatchHeader& hd, const ItemDesc* __restrict__ items, int t,
                                             int& ii, int& tx, int& ty)
 {
+  const TileEnum& en = POST ? hd.post : hd.drr;
   int c = 0;
-  while (c + 1 < hd.n_cams && hd.cam_base[c + 1] <= t) ++c;
-  const int r = t - hd.cam_base[c];
+  while (c + 1 < hd.n_cams && en.cam_base[c + 1] <= t) ++c;
+  const int r = t - en.cam_base[c];
   const int e = r % hd.n_poses;
   const int q = r / hd.n_poses;
-  tx = q % hd.tiles_x[c];
-  ty = q / hd.tiles_x[c];
+  tx = q % en.tiles_x[c];
+  ty = q / en.tiles_x[c];
   ii = e * hd.n_cams + c;
   const ItemDesc& it = items[ii];
-  return tx < it.tiles_x && ty < it.tiles_y;
+  return POST ? (tx < it.ptiles_x && ty < it.ptiles_y) : (tx < it.tiles_x && ty < it.tiles_y);
 }
 
 template <int UNROLL>
@@ -162,14 +173,14 @@ __global__ __launch_bounds__(kBlock) void drr_tiles(BatchArgs a)
   __shared__ int s_tile;
   __shared__ unsigned long long s_samples;
   const BatchHeader& hd = *a.hdr;
-  const int total = hd.total_tiles;
+  const int total = hd.drr.total;
   const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int part = 0;
   for (;;) {
     const int t = next_tile(a.counters->drr_next, total, &s_tile, part);
     if (t < 0) break;
     int ii, tx, ty;
-    if (!decode_tile(hd, a.items, t, ii, tx, ty)) continue;
+    if (!decode_tile<false>(hd, a.items, t, ii, tx, ty)) continue;
     const ItemDesc& it = a.items[ii];
     const unsigned x = tx * kTile + (wave & 1) * 8 + (lane & 7);
     const unsigned y = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
@@ -191,21 +202,27 @@ __global__ __launch_bounds__(kBlock) void drr_tiles(BatchArgs a)
   }
 }
 
-// LDS region accessor: region origin (ox, oy) in image coordinates.
+// LDS region accessor: region origin (ox, oy) in image coordinates, row
+// stride R.
+template <int R>
 struct LdsGet {
   const float* buf;
   int ox, oy;
-  __device__ __forceinline__ float operator()(int x, int y) const { return buf[(y - oy) * kRegion + (x - ox)]; }
+  __device__ __forceinline__ float operator()(int x, int y) const { return buf[(y - oy) * R + (x - ox)]; }
 };
 
-// Run a filter chain over the region in LDS. On entry `cur` holds stage-0
-// values at every in-image position of the region grown by the chain's
-// halo; on exit the returned buffer holds the chain output for the tile.
+// Run a filter chain over the region in LDS (row stride R). On entry `cur`
+// holds stage-0 values at every in-image position of the post tile grown by
+// the chain's halo; on exit the returned buffer holds the chain output for
+// the tile. Thread (tx, ty) = (tid % 32, tid / 32) walks the output
+// rectangle in 32 x 8 steps (no division).
+template <int R>
 __device__ float* run_chain(const ChainDev& ch, const float* __restrict__ weights, float* cur, float* nxt, int x0,
                             int y0, int w, int h)
 {
   const int H0 = ch.halo;
-  const int ox = x0 - H0, oy = y0 - H0;  // region origin (stride kRegion)
+  const int ox = x0 - H0, oy = y0 - H0;  // region origin
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   int remaining = H0;
   for (int k = 0; k < ch.n; k++) {
     const asc_filter& f = ch.f[k];
@@ -213,26 +230,31 @@ __device__ float* run_chain(const ChainDev& ch, const float* __restrict__ weight
     remaining -= r;
     // output rectangle: tile grown by `remaining`
     const int rx0 = x0 - remaining, ry0 = y0 - remaining;
-    const int side = kTile + 2 * remaining;
-    const LdsGet get{cur, ox, oy};
-    for (int idx = threadIdx.x; idx < side * side; idx += kBlock) {
-      const int px = rx0 + idx % side, py = ry0 + idx / side;
-      if (px < 0 || py < 0 || px >= w || py >= h) continue;
-      float v;
-      if (!f.enabled) {
-        v = get(px, py);
-      } else if (f.type == ASC_FILTER_SOBEL) {
-        v = dev::sobel_px(get, px, py, w, h, f.p[0], f.p[1]);
-      } else if (f.type == ASC_FILTER_CONTRAST) {
-        v = dev::contrast_px(get, px, py, w, h, f.p[0], f.p[1], (int)f.p[2]);
-      } else if (ch.gn[k] <= 1) {
-        v = get(px, py);
-      } else {
-        const float c = dev::conv_px(get, px, py, w, h, weights + ch.woff[k], ch.gn[k]);
-        v = f.type == ASC_FILTER_SHARPEN ? dev::sharpen_finish(get(px, py), c, f.p[1] < 1.0f ? 1.0f : f.p[1], f.p[2])
-                                         : dev::gaussian_finish(c);
+    const int side = kPTile + 2 * remaining;
+    const LdsGet<R> get{cur, ox, oy};
+    for (int yy = ty; yy < side; yy += 8) {
+      const int py = ry0 + yy;
+      if (py < 0 || py >= h) continue;
+      for (int xx = tx; xx < side; xx += 32) {
+        const int px = rx0 + xx;
+        if (px < 0 || px >= w) continue;
+        float v;
+        if (!f.enabled) {
+          v = get(px, py);
+        } else if (f.type == ASC_FILTER_SOBEL) {
+          v = dev::sobel_px(get, px, py, w, h, f.p[0], f.p[1]);
+        } else if (f.type == ASC_FILTER_CONTRAST) {
+          v = dev::contrast_px(get, px, py, w, h, f.p[0], f.p[1], (int)f.p[2]);
+        } else if (ch.gn[k] <= 1) {
+          v = get(px, py);
+        } else {
+          const float c = dev::conv_px(get, px, py, w, h, weights + ch.woff[k], ch.gn[k]);
+          v = f.type == ASC_FILTER_SHARPEN
+                ? dev::sharpen_finish(get(px, py), c, f.p[1] < 1.0f ? 1.0f : f.p[1], f.p[2])
+                : dev::gaussian_finish(c);
+        }
+        nxt[(py - oy) * R + (px - ox)] = v;
       }
-      nxt[(py - oy) * kRegion + (px - ox)] = v;
     }
     __syncthreads();
     float* t = cur;
@@ -242,73 +264,121 @@ __device__ float* run_chain(const ChainDev& ch, const float* __restrict__ weight
   return cur;
 }
 
+// Radiograph sampler coordinate of pixel column x / row y of a w x h render
+// (RadRenderer.cl:8-16): s depends on x only and t on y only, so a tile
+// computes them once per column and row. Outside [0, 1] -> -1 (the kernel
+// returns 0 there).
+__device__ __forceinline__ float rad_s(const float* __restrict__ plane, const float* __restrict__ vp, unsigned x,
+                                       unsigned w)
+{
+  const float u = vp[0] + vp[2] * ((float)x / (float)w);
+  const float s = (u - plane[0]) / plane[2] + 0.5f;
+  return (s < 0.0f || s > 1.0f) ? -1.0f : s;
+}
+
+__device__ __forceinline__ float rad_t(const float* __restrict__ plane, const float* __restrict__ vp, unsigned y,
+                                       unsigned h)
+{
+  const float v = vp[1] + vp[3] * ((float)y / (float)h);
+  const float t = 0.5f - (v - plane[1]) / plane[3];
+  return (t < 0.0f || t > 1.0f) ? -1.0f : t;
+}
+
+template <int MAXH>
 __global__ __launch_bounds__(kBlock) void post_tiles(BatchArgs a)
 {
-  __shared__ float buf0[kRegion * kRegion];
-  __shared__ float buf1[kRegion * kRegion];
-  __shared__ float sdrr[kTile * kTile];
+  constexpr int R = kPTile + 2 * MAXH;  // region row stride
+  __shared__ float buf0[R * R];
+  __shared__ float buf1[R * R];
+  __shared__ float sdrr[kPTile * kPTile];
+  __shared__ float s_col[R], s_row[R];
   __shared__ double lds_red[4 * 9];
   __shared__ int s_tile;
   const BatchHeader& hd = *a.hdr;
-  const int total = hd.total_tiles;
+  const int total = hd.post.total;
   const int debug = hd.debug;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   int part = 0;
   for (;;) {
     const int t = next_tile(a.counters->post_next, total, &s_tile, part);
     if (t < 0) break;
-    int ii, tx, ty;
-    if (!decode_tile(hd, a.items, t, ii, tx, ty)) continue;
+    int ii, ptx, pty;
+    if (!decode_tile<true>(hd, a.items, t, ii, ptx, pty)) continue;
     const ItemDesc& it = a.items[ii];
     const CamDev& cam = a.cams[it.cam];
-    const int local = ty * it.tiles_x + tx;
-    const int x0 = tx * kTile, y0 = ty * kTile;
+    const int local = pty * it.ptiles_x + ptx;
+    const int x0 = ptx * kPTile, y0 = pty * kPTile;
     const int w = it.w, h = it.h;
 
     // --- DRR chain: stage 0 = DRR scratch ---
     {
       const ChainDev& ch = cam.chain[0];
-      const int H0 = ch.halo, side = kTile + 2 * H0;
+      const int H0 = ch.halo, side = kPTile + 2 * H0;
       const int ox = x0 - H0, oy = y0 - H0;
       const float* src = a.drr + it.drr_off;
-      for (int idx = threadIdx.x; idx < side * side; idx += kBlock) {
-        const int px = ox + idx % side, py = oy + idx / side;
-        if (px < 0 || py < 0 || px >= w || py >= h) continue;
-        buf0[(py - oy) * kRegion + (px - ox)] = src[(long long)py * w + px];
+      for (int yy = ty; yy < side; yy += 8) {
+        const int py = oy + yy;
+        if (py < 0 || py >= h) continue;
+        for (int xx = tx; xx < side; xx += 32) {
+          const int px = ox + xx;
+          if (px >= 0 && px < w) buf0[yy * R + xx] = src[(long long)py * w + px];
+        }
       }
       __syncthreads();
-      const float* outb = run_chain(ch, a.weights, buf0, buf1, x0, y0, w, h);
-      const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-      sdrr[threadIdx.x] = outb[(ly + H0) * kRegion + (lx + H0)];
+      const float* outb = run_chain<R>(ch, a.weights, buf0, buf1, x0, y0, w, h);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int ly = ty + 8 * k;
+        sdrr[ly * kPTile + tx] = outb[(ly + H0) * R + (tx + H0)];
+      }
       __syncthreads();
     }
     // --- radiograph chain: stage 0 = rad resample (RadRenderer.cl) ---
     const float* radb;
-    int rH0;
+    const int rH0 = cam.chain[1].halo;
     {
       const ChainDev& ch = cam.chain[1];
-      rH0 = ch.halo;
-      const int side = kTile + 2 * rH0;
+      const int side = kPTile + 2 * rH0;
       const int ox = x0 - rH0, oy = y0 - rH0;
-      for (int idx = threadIdx.x; idx < side * side; idx += kBlock) {
-        const int px = ox + idx % side, py = oy + idx / side;
-        if (px < 0 || py < 0 || px >= w || py >= h) continue;
-        buf0[(py - oy) * kRegion + (px - ox)] =
-          dev::rad_pixel(cam.rad, cam.plane, it.vp, px, py, w, h, false, 0.0f);
+      for (int k = threadIdx.x; k < 2 * side; k += kBlock) {
+        if (k < side) {
+          const int px = ox + k;
+          if (px >= 0 && px < w) s_col[k] = rad_s(cam.plane, it.vp, (unsigned)px, (unsigned)w);
+        } else {
+          const int py = oy + (k - side);
+          if (py >= 0 && py < h) s_row[k - side] = rad_t(cam.plane, it.vp, (unsigned)py, (unsigned)h);
+        }
       }
       __syncthreads();
-      radb = run_chain(ch, a.weights, buf0, buf1, x0, y0, w, h);
+      for (int yy = ty; yy < side; yy += 8) {
+        const int py = oy + yy;
+        if (py < 0 || py >= h) continue;
+        const float tt = s_row[yy];
+        for (int xx = tx; xx < side; xx += 32) {
+          const int px = ox + xx;
+          if (px < 0 || px >= w) continue;
+          const float ss = s_col[xx];
+          buf0[yy * R + xx] = (ss < 0.0f || tt < 0.0f) ? 0.0f : 1.0f - dev::img_sample(cam.rad, ss, tt);
+        }
+      }
+      __syncthreads();
+      radb = run_chain<R>(ch, a.weights, buf0, buf1, x0, y0, w, h);
     }
     // --- mask, multiplies, moments (Tracker.cpp:744-752, Ncc.cpp) ---
     dev::Moments acc;
-    {
-      const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-      const int px = x0 + lx, py = y0 + ly;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int ly = ty + 8 * k;
+      const int px = x0 + tx, py = y0 + ly;
       if (px < w && py < h) {
-        float d = sdrr[threadIdx.x];
-        float g = radb[(ly + rH0) * kRegion + (lx + rH0)];
+        float d = sdrr[ly * kPTile + tx];
+        float g = radb[(ly + rH0) * R + (tx + rH0)];
         float m = (d != 0.0f) ? 1.0f : 0.0f;
         if (cam.has_bg) {
-          const float bgv = dev::rad_pixel(cam.bg, cam.plane, it.vp, px, py, w, h, true, cam.bg_threshold);
+          // BackgroundRenderer.cl: same sampler coordinates as the radiograph
+          const float ss = s_col[tx + rH0], tt = s_row[ly + rH0];
+          const float bgv =
+            (ss < 0.0f || tt < 0.0f) ? 0.0f : (cam.bg_threshold <= dev::img_sample(cam.bg, ss, tt) ? 1.0f : 0.0f);
           m = bgv * m;
         }
         g = g * m;
@@ -339,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void finalize_items(BatchArgs a)
   const int n_items = a.hdr->n_items;
   for (int ii = blockIdx.x; ii < n_items; ii += gridDim.x) {
     const ItemDesc& it = a.items[ii];
-    const int nt = it.tiles_x * it.tiles_y;
+    const int nt = it.ptiles_x * it.ptiles_y;
     double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int k = threadIdx.x; k < nt; k += kBlock) {
       const double* p = a.partials + (size_t)(it.part_off + k) * 9;
@@ -439,7 +509,9 @@ struct asc_scene_s {
   std::vector<SceneVolume> vols;
   std::vector<SceneCamera> cams;
   bool dirty = true;
-  int grid_persistent = 0;
+  int grid_persistent = 0;  // ray-march grid (blocks)
+  int grid_post = 0;        // post-processing grid (blocks)
+  int post_small = 1;       // every chain's halo <= kSmallHalo: smaller-LDS post kernel
   // device
   DevBuf<CamDev> d_cams;
   DevBuf<VolParams> d_vols;
@@ -475,6 +547,8 @@ struct asc_scene_s {
 };
 
 namespace {
+
+void destroy_graph(asc_scene s);
 
 int scene_upload_config(asc_scene s)
 {
@@ -517,6 +591,14 @@ int scene_upload_config(asc_scene s)
       }
       if (ch.halo > kMaxHalo) return fail(ASC_ERR_UNSUPPORTED, "filter chain halo exceeds 12 pixels");
     }
+  }
+  int max_halo = 0;
+  for (const CamDev& cd : cams) max_halo = std::max(max_halo, std::max(cd.chain[0].halo, cd.chain[1].halo));
+  const int small = max_halo <= kSmallHalo ? 1 : 0;
+  if (small != s->post_small) {
+    s->post_small = small;
+    s->grid_post = 0;
+    destroy_graph(s);
   }
   std::vector<VolParams> vols(s->vols.size());
   for (size_t i = 0; i < s->vols.size(); i++) {
@@ -579,7 +661,10 @@ int issue_batch(asc_scene s, size_t batch_bytes)
     default: drr_tiles<2><<<s->grid_persistent, kBlock, 0, st>>>(a); break;
   }
   if (timed) ASC_HIP(hipEventRecord(s->ev[1], st));
-  post_tiles<<<s->grid_persistent, kBlock, 0, st>>>(a);
+  if (s->post_small)
+    post_tiles<kSmallHalo><<<s->grid_post, kBlock, 0, st>>>(a);
+  else
+    post_tiles<kMaxHalo><<<s->grid_post, kBlock, 0, st>>>(a);
   if (timed) ASC_HIP(hipEventRecord(s->ev[2], st));
   finalize_items<<<std::min(n_items, 4096), kBlock, 0, st>>>(a);
   if (timed) ASC_HIP(hipEventRecord(s->ev[3], st));
@@ -604,7 +689,7 @@ int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, doub
   s->items.resize(n_items);
   s->in_view.assign(n_items, 0);
   const size_t plane = (size_t)s->RW * s->RH;
-  const int max_tiles = ((s->RW + kTile - 1) / kTile) * ((s->RH + kTile - 1) / kTile);
+  const int max_ptiles = ((s->RW + kPTile - 1) / kPTile) * ((s->RH + kPTile - 1) / kPTile);
   int tile_base = 0;
   for (int i = 0; i < n; i++) {
     const int vi = volumes ? volumes[i] : 0;
@@ -621,7 +706,7 @@ int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, doub
       it.vol = vi;
       it.cam = c;
       it.drr_off = (long long)ii * (long long)plane;
-      it.part_off = ii * max_tiles;
+      it.part_off = ii * max_ptiles;
       it.tile_base = tile_base;
       if (inview) {
         if (sz[0] > (unsigned)s->RW || sz[1] > (unsigned)s->RH)
@@ -631,6 +716,8 @@ int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, doub
         it.h = (int)sz[1];
         it.tiles_x = (it.w + kTile - 1) / kTile;
         it.tiles_y = (it.h + kTile - 1) / kTile;
+        it.ptiles_x = (it.w + kPTile - 1) / kPTile;
+        it.ptiles_y = (it.h + kPTile - 1) / kPTile;
         tile_base += it.tiles_x * it.tiles_y;
         s->in_view[ii] = 1;
       }
@@ -643,7 +730,7 @@ int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, doub
   if ((rc = s->d_batch.ensure(batch_bytes))) return rc;
   if ((rc = s->h_batch.ensure(batch_bytes))) return rc;
   if ((rc = s->d_drr.ensure(plane * n_items))) return rc;
-  if ((rc = s->d_partials.ensure((size_t)max_tiles * n_items * 9))) return rc;
+  if ((rc = s->d_partials.ensure((size_t)max_ptiles * n_items * 9))) return rc;
   if ((rc = s->d_results.ensure((size_t)n_items * 10))) return rc;
   if ((rc = s->h_results.ensure((size_t)n_items * 10))) return rc;
   if ((rc = s->d_samples.ensure((size_t)n_items))) return rc;
@@ -655,13 +742,22 @@ int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, doub
   if (s->debug && (rc = s->d_dbg.ensure(plane * n_items * 3))) return rc;
   const void* after[6] = {s->d_batch.p, s->d_drr.p, s->d_partials.p, s->d_results.p, s->d_samples.p, s->d_dbg.p};
   if (std::memcmp(before, after, sizeof before) != 0) destroy_graph(s);
-  if (s->grid_persistent == 0) {
+  if (s->grid_persistent == 0 || s->grid_post == 0) {
     hipDeviceProp_t prop;
     ASC_HIP(hipGetDeviceProperties(&prop, s->ctx->device));
-    int per_cu = 0;
-    ASC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, drr_tiles<1>, kBlock, 0));
-    if (per_cu < 1) per_cu = 1;
-    s->grid_persistent = prop.multiProcessorCount * per_cu;
+    if (s->grid_persistent == 0) {
+      int per_cu = 0;
+      ASC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, drr_tiles<1>, kBlock, 0));
+      s->grid_persistent = prop.multiProcessorCount * std::max(per_cu, 1);
+    }
+    if (s->grid_post == 0) {
+      int per_cu = 0;
+      if (s->post_small)
+        ASC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, post_tiles<kSmallHalo>, kBlock, 0));
+      else
+        ASC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, post_tiles<kMaxHalo>, kBlock, 0));
+      s->grid_post = prop.multiProcessorCount * std::max(per_cu, 1);
+    }
   }
   BatchHeader hdr;
   std::memset(&hdr, 0, sizeof hdr);
@@ -669,21 +765,23 @@ int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, doub
   hdr.debug = s->debug;
   hdr.n_poses = n;
   hdr.n_cams = C;
-  {
+  for (int which = 0; which < 2; which++) {
+    TileEnum& en = which ? hdr.post : hdr.drr;
     long long base = 0;
     for (int c = 0; c < C; c++) {
       int tx = 0, ty = 0;
       for (int i = 0; i < n; i++) {
-        tx = std::max(tx, s->items[(size_t)i * C + c].tiles_x);
-        ty = std::max(ty, s->items[(size_t)i * C + c].tiles_y);
+        const ItemDesc& it = s->items[(size_t)i * C + c];
+        tx = std::max(tx, which ? it.ptiles_x : it.tiles_x);
+        ty = std::max(ty, which ? it.ptiles_y : it.tiles_y);
       }
-      hdr.cam_base[c] = (int)base;
-      hdr.tiles_x[c] = std::max(tx, 1);
-      hdr.tiles_y[c] = ty;
+      en.cam_base[c] = (int)base;
+      en.tiles_x[c] = std::max(tx, 1);
+      en.tiles_y[c] = ty;
       base += (long long)tx * ty * n;
     }
     ASC_CHECK(base < (1LL << 31) - (1LL << 20), ASC_ERR_TOO_LARGE, "batch has too many tiles");
-    hdr.total_tiles = (int)base;
+    en.total = (int)base;
   }
   std::memcpy(s->h_batch.p, &hdr, sizeof hdr);
   std::memcpy(s->h_batch.p + sizeof hdr, s->items.data(), sizeof(ItemDesc) * n_items);

@@ -314,13 +314,12 @@ __device__ __forceinline__ float drr_finish(float acc_raw, float step_vscale, fl
 // ---------------------------------------------------------------------------
 // 2-D image sampling, RadRenderer.cl / BackgroundRenderer.cl (exact order).
 // ---------------------------------------------------------------------------
+// Images hold texels already normalised at upload (raw / (2^bits - 1),
+// correctly rounded): the sampler's UNORM read is a plain load here.
 __device__ __forceinline__ float img_texel(const ImgDev& im, int i, int j)
 {
   if (i < 0 || j < 0 || i >= im.W || j >= im.H) return 0.0f;
-  const size_t idx = (size_t)j * im.W + i;
-  if (im.bps == 8) return (float)((const uint8_t*)im.data)[idx] / 255.0f;
-  if (im.bps == 16) return (float)((const uint16_t*)im.data)[idx] / 65535.0f;
-  return ((const float*)im.data)[idx];
+  return ((const float*)im.data)[(size_t)j * im.W + i];
 }
 
 __device__ __forceinline__ int floor_idx(float x, float& frac)

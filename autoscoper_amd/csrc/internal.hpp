@@ -42,6 +42,8 @@ int hip_fail(hipError_t e, const char* what);
 // Layouts (ASC_LAYOUT_*): LINEAR stores one texel per element; QUAD stores
 // the 2x2 (x, y) corner block of each element (4 texels), OCTET the 2x2x2
 // block (8 texels), so a trilinear sample reads 4, 2 or 1 aligned vectors.
+// BRICK stores OCTET cells grouped in 2x2x2 bricks (pW and pS then count
+// the cells of one brick row and one brick layer; see layout_shape()).
 struct VolDev {
   const void* data;
   int W, H, D;
@@ -54,9 +56,9 @@ struct VolDev {
 };
 
 struct ImgDev {
-  const void* data;
+  const void* data;  // float texels, already normalised (UNORM -> [0, 1])
   int W, H;
-  int bps;  // 8, 16 or 32
+  int bps;           // 32
 };
 
 }  // namespace asc
@@ -89,9 +91,10 @@ struct asc_volume_s {
 
 struct asc_image_s {
   asc_context ctx;
-  asc::ImgDev dev;
+  asc::ImgDev dev;  // always 32-bit float texels on the device
   void* dptr;
-  size_t bytes;
+  size_t bytes;     // device bytes (4 per texel)
+  int src_bps;      // bit depth of the caller's data (8, 16 or 32)
 };
 
 namespace asc {

@@ -486,6 +486,24 @@ ASC_API int asc_volume_destroy(asc_volume vol)
   return ASC_OK;
 }
 
+// Texels as the sampler reads them (UNORM8/UNORM16 -> raw / (2^bits - 1),
+// correctly rounded, as the device division would give; float as is),
+// converted once here instead of at every sample.
+static std::vector<float> normalised_texels(const void* data, size_t n, int bps)
+{
+  std::vector<float> out(n);
+  if (bps == 8) {
+    const uint8_t* p = (const uint8_t*)data;
+    for (size_t i = 0; i < n; i++) out[i] = (float)p[i] / 255.0f;
+  } else if (bps == 16) {
+    const uint16_t* p = (const uint16_t*)data;
+    for (size_t i = 0; i < n; i++) out[i] = (float)p[i] / 65535.0f;
+  } else {
+    std::memcpy(out.data(), data, n * sizeof(float));
+  }
+  return out;
+}
+
 ASC_API int asc_image_create(asc_context ctx, const void* data, int width, int height, int bps, asc_image* out)
 {
   ASC_CHECK(valid_ctx(ctx) && out, ASC_ERR_INVALID, "bad argument");
@@ -493,7 +511,9 @@ ASC_API int asc_image_create(asc_context ctx, const void* data, int width, int h
   ASC_CHECK(bps == 8 || bps == 16 || bps == 32, ASC_ERR_UNSUPPORTED, "unsupported image bit depth (8, 16, 32)");
   asc_image im = new asc_image_s();
   im->ctx = ctx;
-  im->bytes = (size_t)width * height * (bps / 8);
+  im->src_bps = bps;
+  const size_t n = (size_t)width * height;
+  im->bytes = n * sizeof(float);
   hipError_t e = hipMalloc(&im->dptr, im->bytes);
   if (e != hipSuccess) {
     delete im;
@@ -502,9 +522,10 @@ ASC_API int asc_image_create(asc_context ctx, const void* data, int width, int h
   im->dev.data = im->dptr;
   im->dev.W = width;
   im->dev.H = height;
-  im->dev.bps = bps;
+  im->dev.bps = 32;
   if (data) {
-    e = hipMemcpy(im->dptr, data, im->bytes, hipMemcpyHostToDevice);
+    const std::vector<float> host = normalised_texels(data, n, bps);
+    e = hipMemcpy(im->dptr, host.data(), im->bytes, hipMemcpyHostToDevice);
   } else {
     e = hipMemset(im->dptr, 0, im->bytes);
   }
@@ -520,8 +541,9 @@ ASC_API int asc_image_create(asc_context ctx, const void* data, int width, int h
 ASC_API int asc_image_update(asc_image img, const void* data)
 {
   ASC_CHECK(img && data, ASC_ERR_INVALID, "bad argument");
+  const std::vector<float> host = normalised_texels(data, img->bytes / sizeof(float), img->src_bps);
   ASC_HIP(hipStreamSynchronize(img->ctx->stream));
-  ASC_HIP(hipMemcpy(img->dptr, data, img->bytes, hipMemcpyHostToDevice));
+  ASC_HIP(hipMemcpy(img->dptr, host.data(), img->bytes, hipMemcpyHostToDevice));
   return ASC_OK;
 }
 

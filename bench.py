@@ -99,6 +99,8 @@ def main(argv=None) -> int:
     ap.add_argument("--layout", default="auto", choices=["auto"] + sorted(LAYOUTS),
                     help="device volume layout (auto: the densest that fits, BRICK for the workloads here)")
     ap.add_argument("--graph", action="store_true", help="replay the fused launches from a hipGraph")
+    ap.add_argument("--unroll", type=int, default=0, help="ray-march unroll (tuning; 0 = library default)")
+    ap.add_argument("--grid", type=int, default=0, help="persistent grid in blocks (tuning; 0 = library default)")
     ap.add_argument("--plain-evals", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -131,6 +133,10 @@ def main(argv=None) -> int:
     nb = args.warmup + args.steps
     batches = [gt + np.concatenate([rng.uniform(-3, 3, (P, 3)), rng.uniform(-3, 3, (P, 3))], 1) for _ in range(nb)]
 
+    if args.unroll:
+        scene.set_tuning(scene.TUNE_UNROLL, args.unroll)
+    if args.grid:
+        scene.set_tuning(scene.TUNE_GRID, args.grid)
     scene.set_timing(not args.graph)
     scene.use_graph(args.graph)
     log(f"[bench] warmup {args.warmup} x {P} poses")
