@@ -204,3 +204,45 @@ def workload(name: str, frames: int | None = None) -> Workload:
     start = base + np.concatenate([rng.uniform(-2, 2, 3), rng.uniform(-5, 5, 3)])
     gt = random_walk(start, nf, 0.5, 1.0, SEED_POSES + 1)
     return Workload(name, vol, vox, cams, size, gt, dflt, rflt)
+
+
+@dataclass
+class MultiWorkload:
+    """Several bones tracked in the same cameras (one volume each)."""
+    name: str
+    volumes: list      # (D, H, W) u16 each
+    voxel_mm: list     # isotropic voxel size of each volume
+    cameras: list      # CameraSpec
+    render: tuple
+    gt_poses: np.ndarray  # (frames, bones, 6)
+    drr_filters: list = field(default_factory=list)
+    rad_filters: list = field(default_factory=list)
+
+
+def multi_workload(name: str, frames: int | None = None) -> MultiWorkload:
+    """Multi-bone workloads: 'tiny3' (three test-size bones, 2 x 96x80) and
+    'cfg3' (96^3 @0.5 mm, 160x160x220 @0.4 mm, 256x256x320 @0.3 mm, 2 x
+    1024^2). Bones are stacked along the vertical axis, as at a joint, so
+    they project to different detector regions of both cameras."""
+    dflt, rflt = default_filters()
+    if name == "tiny3":
+        specs = [((14, 18, 18), 1.0, "ellipsoid", -17.0), ((20, 22, 20), 0.9, "longbone", 0.0),
+                 ((16, 20, 24), 1.0, "ellipsoid", 17.0)]
+        size, sod, pix, nf = (96, 80), 400.0, 0.9, 2
+    elif name == "cfg3":
+        specs = [((96, 96, 96), 0.5, "ellipsoid", -80.0), ((220, 160, 160), 0.4, "longbone", -10.0),
+                 ((320, 256, 256), 0.3, "longbone", 65.0)]
+        size, sod, pix, nf = (1024, 1024), 800.0, 0.3, 50
+    else:
+        raise ValueError(name)
+    nf = frames or nf
+    vols, voxs, gts = [], [], []
+    for b, (shape, vox, kind, z) in enumerate(specs):
+        vols.append(phantom(shape, vox, kind, seed=SEED_PHANTOM + 10 * b, noise_seed=SEED_NOISE + 10 * b))
+        voxs.append(vox)
+        base = volume_centre_pose(shape, vox) + np.array([0.0, 0.0, z, 0.0, 0.0, 0.0])
+        rng = np.random.default_rng(SEED_POSES + 10 * b)
+        start = base + np.concatenate([rng.uniform(-2, 2, 3), rng.uniform(-5, 5, 3)])
+        gts.append(random_walk(start, nf, 0.5, 1.0, SEED_POSES + 10 * b + 1))
+    cams = biplane(sod_mm=sod, size=size, pixel_mm=pix, sid_mm=sod * 1.375)
+    return MultiWorkload(name, vols, voxs, cams, size, np.stack(gts, 1), dflt, rflt)

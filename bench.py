@@ -64,6 +64,48 @@ def build_workload(A, synth, name: str, storage: str, ctx, layout: str = "auto")
     return wl, vol, cams, rads, scene, gt
 
 
+MULTI = ("cfg3", "tiny3")
+
+
+def build_multi(A, synth, name: str, storage: str, ctx, layout: str = "auto"):
+    """Several bones in one scene; targets are the merged full-frame DRRs of
+    all bones (View::renderDrr with gpu::merge)."""
+    t0 = time.time()
+    wl = synth.multi_workload(name, frames=2)
+    log(f"[bench] {len(wl.volumes)} phantoms {[v.shape for v in wl.volumes]} generated in {time.time() - t0:.1f}s")
+    vols = []
+    for v, vox in zip(wl.volumes, wl.voxel_mm):
+        vol = A.Volume(ctx, v, scale=(vox,) * 3, storage=STORAGES[storage])
+        if layout != "auto":
+            vol.set_layout(LAYOUTS[layout])
+        vols.append(vol)
+    RW, RH = wl.render
+    cams = [A.geometry.camera_from_mayacam2(*c.mayacam2()) for c in wl.cameras]
+    gts = wl.gt_poses[0]
+    rads = []
+    full, acc = ctx.alloc(4 * RW * RH), ctx.alloc(4 * RW * RH)
+    for i, cam in enumerate(cams):
+        fvp = synth.full_frame_viewport(list(cam.viewport), RW, RH)
+        acc.upload(np.zeros((RH, RW), np.float32))
+        for b, vol in enumerate(vols):
+            inview, imv, _, _ = A.geometry.view_geometry(cam, gts[b], vol.desc, RW, RH)
+            assert inview
+            ctx.drr_render(vol, imv, fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, full, RW, RH)
+            ctx.merge(acc, full, acc, RW, RH)
+        rads.append(synth.target_from_drr(acc.download((RH, RW)), 500 + i))
+    full.free()
+    acc.free()
+    scene = A.Scene(ctx, RW, RH)
+    for vol in vols:
+        pivot = A.geometry.frame_from_xyzypr(list(vol.desc.trans_center) + [0, 0, 0])
+        scene.add_volume(vol, synth.DRR_STEP, synth.DRR_INTENSITY, synth.DRR_CUTOFF, pivot)
+    for i, cam in enumerate(cams):
+        c = scene.add_camera(cam, A.Image(ctx, rads[i][1]))
+        scene.set_filters(c, 0, wl.drr_filters)
+        scene.set_filters(c, 1, wl.rad_filters)
+    return wl, vols, cams, rads, scene, gts
+
+
 def cpu_baseline(wl, cams, rads, gt, budget_s: float):
     """Oracle trackFrame (float-faithful restatement, OpenMP over rows) on the
     host cores: a bounded sample of the same workload's evaluations."""
@@ -86,6 +128,16 @@ def cpu_baseline(wl, cams, rads, gt, budget_s: float):
             break
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return n / el, n, el, threads
+
+
+def workload_name(args, wl, cams, P, multi) -> str:
+    RW, RH = wl.render
+    if multi:
+        shapes = ", ".join(f"{v.shape[2]}x{v.shape[1]}x{v.shape[0]}@{vx}mm" for v, vx in zip(wl.volumes, wl.voxel_mm))
+        return (f"{args.workload}: {len(wl.volumes)} bones ({shapes}) {args.storage}, {len(cams)} cams {RW}x{RH}, "
+                f"{args.particles} poses per bone, {P} per batch")
+    return (f"{args.workload}: {wl.volume.shape[2]}x{wl.volume.shape[1]}x{wl.volume.shape[0]} "
+            f"{args.storage} volume @{wl.voxel_mm}mm, {len(cams)} cams {RW}x{RH}, {P} poses per batch")
 
 
 def main(argv=None) -> int:
@@ -126,12 +178,20 @@ def main(argv=None) -> int:
         ctx.synchronize()
         log(f"[bench] copied {nbytes} bytes x 5")
         return 0
-    wl, vol, cams, rads, scene, gt = build_workload(A, synth, args.workload, args.storage, ctx, args.layout)
+    multi = args.workload in MULTI
+    if multi:
+        wl, vols, cams, rads, scene, gts = build_multi(A, synth, args.workload, args.storage, ctx, args.layout)
+    else:
+        wl, vol, cams, rads, scene, gt = build_workload(A, synth, args.workload, args.storage, ctx, args.layout)
+        vols, gts = [vol], gt[None]
     RW, RH = wl.render
-    P = args.particles
+    B = len(vols)
+    P = args.particles * B  # poses per fused launch: `particles` per bone
+    bone_of = np.repeat(np.arange(B, dtype=np.int32), args.particles)
     rng = np.random.default_rng(synth.SEED_START)
     nb = args.warmup + args.steps
-    batches = [gt + np.concatenate([rng.uniform(-3, 3, (P, 3)), rng.uniform(-3, 3, (P, 3))], 1) for _ in range(nb)]
+    batches = [gts[bone_of] + np.concatenate([rng.uniform(-3, 3, (P, 3)), rng.uniform(-3, 3, (P, 3))], 1)
+               for _ in range(nb)]
 
     if args.unroll:
         scene.set_tuning(scene.TUNE_UNROLL, args.unroll)
@@ -141,14 +201,14 @@ def main(argv=None) -> int:
     scene.use_graph(args.graph)
     log(f"[bench] warmup {args.warmup} x {P} poses")
     for k in range(args.warmup):
-        scene.eval_poses(batches[k])
+        scene.eval_poses(batches[k], volumes=bone_of)
     ctx.synchronize()
     kern_ms = np.zeros(3)
     samples = 0
     log(f"[bench] timing {args.steps} steps")
     t0 = time.perf_counter()
     for k in range(args.warmup, nb):
-        scene.eval_poses(batches[k])
+        scene.eval_poses(batches[k], volumes=bone_of)
         if not args.graph:
             kern_ms += scene.last_timing()
             samples += scene.last_info(P * len(cams))[1]
@@ -162,7 +222,7 @@ def main(argv=None) -> int:
         scene.use_graph(False)
         scene.set_timing(True)
         for k in range(args.warmup, nb):
-            scene.eval_poses(batches[k])
+            scene.eval_poses(batches[k], volumes=bone_of)
             kern_ms += scene.last_timing()
             samples += scene.last_info(P * len(cams))[1]
     drr_ms = kern_ms[0] / args.steps
@@ -175,12 +235,12 @@ def main(argv=None) -> int:
     # plain one-eval-at-a-time form
     t0 = time.perf_counter()
     for k in range(args.plain_evals):
-        scene.track_frame_plain(batches[args.warmup][k % P])
+        scene.track_frame_plain(batches[args.warmup][k % P], volume=int(bone_of[k % P]))
     plain_s = args.plain_evals / (time.perf_counter() - t0)
     log(f"[bench] plain form {plain_s:.1f} evals/s")
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not multi:
         log(f"[bench] CPU oracle baseline (~{args.cpu_seconds:.0f}s)")
         v, n, el_c, threads = cpu_baseline(wl, cams, rads, gt, args.cpu_seconds)
         cpu = {"value": v, "unit": "pose-evals/s", "cores": threads, "kind": "port",
@@ -209,9 +269,7 @@ def main(argv=None) -> int:
         "dtype": "f32",
         "data": "synthetic (seeded phantom, rendered targets)",
         "config": {
-            "workload": f"{args.workload}: {wl.volume.shape[2]}x{wl.volume.shape[1]}x{wl.volume.shape[0]} "
-                        f"{args.storage} volume @{wl.voxel_mm}mm, {len(cams)} cams {RW}x{RH}, "
-                        f"{P} poses per batch",
+            "workload": workload_name(args, wl, cams, P, multi),
             "volume_storage": args.storage,
             "volume_layout": args.layout,
             "particles_per_step": P,
@@ -240,7 +298,8 @@ def main(argv=None) -> int:
     }
     print(json.dumps(out), flush=True)
     scene.close()
-    vol.close()
+    for v in vols:
+        v.close()
     ctx.close()
     return 0
 
