@@ -1,0 +1,150 @@
+"""Volume storage precision (U16 exact, F32, F16, BF16) against the FP64
+oracle: DRR pixels, per-camera NCC cost, and same-seed final poses.
+
+Bars:
+  * DRR pixels, storage alone: a DRR pixel is a non-negative weighted sum of
+    stored texels, so against the exact U16 path (same rays, samples and
+    weights) its relative error is at most the storage's unit roundoff u
+    (F32 2^-24; F16 rounds to nearest, u = 2^-11; BF16 to nearest even,
+    u = 2^-8) plus float accumulation (held to 4e-6 here);
+  * DRR pixels against the all-FP64 oracle (geometry, sample count and sums
+    in double): max / rms relative error U16 and F32 1e-4 / 1e-5, F16
+    5e-4 / 1e-4, BF16 4e-3 / 1e-3 (measured: 3.3e-5 / 2.1e-6,
+    1.6e-4 / 3.4e-5, 2.3e-3 / 5.0e-4);
+  * 1 - NCC per camera: U16/F32 within 1e-5 (the parity bar); F16 within
+    5e-4 and BF16 within 5e-3 (measured errors are several times smaller);
+  * final pose after PSO (synchronous, batched) + simplex with the same
+    seed: F16 and U16 within 0.05 mm / 0.05 deg of the FP64 oracle's.
+"""
+import numpy as np
+import pytest
+
+from autoscoper_amd import synth
+
+from test_gpu_scene import oracle_scene
+from test_gpu_optimize import START, pose_gap
+
+pytestmark = pytest.mark.gpu
+
+U16, F16, BF16, F32 = 0, 1, 2, 3
+UNIT_ROUNDOFF = {U16: 0.0, F32: 2.0 ** -24, F16: 2.0 ** -11, BF16: 2.0 ** -8}
+NCC_BAR = {U16: 1e-5, F32: 1e-5, F16: 5e-4, BF16: 5e-3}
+# all-FP64 oracle DRR: FP32 geometry and sums add up to ~3e-5 at grazing
+# rays; the storage's rounding (u) adds to that
+FP64_RMS_BAR = {U16: 1e-5, F32: 1e-5, F16: 1e-4, BF16: 1e-3}
+FP64_MAX_BAR = {U16: 1e-4, F32: 1e-4, F16: 5e-4, BF16: 4e-3}
+
+
+def scene_with(asc, ctx, tiny, storage):
+    wl = tiny["wl"]
+    vol = asc.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3, storage=storage)
+    RW, RH = tiny["render"]
+    sc = asc.Scene(ctx, RW, RH)
+    pivot = asc.geometry.frame_from_xyzypr(list(vol.desc.trans_center) + [0, 0, 0])
+    sc.add_volume(vol, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, pivot)
+    for i, cam in enumerate(tiny["cams"]):
+        c = sc.add_camera(asc.AscCamera.from_array(cam), asc.Image(ctx, tiny["rads"][i][1]))
+        sc.set_filters(c, 0, wl.drr_filters)
+        sc.set_filters(c, 1, wl.rad_filters)
+    return sc, vol, pivot
+
+
+def render_set(asc, ctx, orc, tiny, storage, poses):
+    wl, prep = tiny["wl"], tiny["prep"]
+    vol = asc.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3, storage=storage)
+    RW, RH = tiny["render"]
+    out = ctx.alloc(4 * RW * RH)
+    imgs = []
+    for cam in tiny["cams"]:
+        for pose in poses:
+            _, imv, _, _ = orc.view_geometry(cam, pose, prep["inv_scale"], prep["inv_trans"], RW, RH)
+            fvp = synth.full_frame_viewport(cam[24:28], RW, RH)
+            ctx.drr_render(vol, imv[:12], fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, out, RW, RH)
+            imgs.append(out.download((RH, RW)).astype(np.float64))
+    out.free()
+    vol.close()
+    return imgs
+
+
+def drr_poses(tiny):
+    rng = np.random.default_rng(17)
+    return [tiny["wl"].gt_poses[0] + np.concatenate([rng.uniform(-2, 2, 3), rng.uniform(-5, 5, 3)])
+            for _ in range(3)]
+
+
+@pytest.mark.parametrize("storage", [F32, F16, BF16])
+def test_storage_error_within_unit_roundoff(asc, ctx, orc, tiny, storage):
+    """Same rays, samples and weights as the exact U16 path: the storage's
+    rounding is the only difference."""
+    poses = drr_poses(tiny)
+    exact = render_set(asc, ctx, orc, tiny, U16, poses)
+    other = render_set(asc, ctx, orc, tiny, storage, poses)
+    worst = 0.0
+    for a, b in zip(exact, other):
+        live = (a > 1e-3) & (a < 1.0)
+        worst = max(worst, float((np.abs(b[live] - a[live]) / a[live]).max()))
+    print(f"storage {storage}: DRR rel difference to U16 max {worst:.3e} (unit roundoff {UNIT_ROUNDOFF[storage]:.3e})")
+    assert worst <= UNIT_ROUNDOFF[storage] * 1.001 + 4e-6
+
+
+@pytest.mark.parametrize("storage", [U16, F32, F16, BF16])
+def test_drr_error_vs_fp64_oracle(asc, ctx, orc, tiny, storage):
+    """FP32 ray integration against the oracle's all-FP64 DRR (geometry,
+    sample count and sums in double)."""
+    prep = tiny["prep"]
+    RW, RH = tiny["render"]
+    poses = drr_poses(tiny)
+    imgs = render_set(asc, ctx, orc, tiny, storage, poses)
+    rels = []
+    k = 0
+    for cam in tiny["cams"]:
+        for pose in poses:
+            _, imv, _, _ = orc.view_geometry(cam, pose, prep["inv_scale"], prep["inv_trans"], RW, RH)
+            fvp = synth.full_frame_viewport(cam[24:28], RW, RH)
+            ref, _ = orc.drr(prep["volume"], imv[:12], fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, RW, RH,
+                             fp64=True)
+            live = (ref > 1e-3) & (ref < 1.0)
+            rels.append(np.abs(imgs[k][live] - ref[live]) / ref[live])
+            k += 1
+    rel = np.concatenate(rels)
+    worst, rms, p999 = float(rel.max()), float(np.sqrt((rel ** 2).mean())), float(np.quantile(rel, 0.999))
+    print(f"storage {storage}: DRR rel error vs FP64 max {worst:.3e} rms {rms:.3e} p99.9 {p999:.3e} "
+          f"over {rel.size} pixels")
+    assert rel.size > 1000
+    assert rms <= FP64_RMS_BAR[storage] and worst <= FP64_MAX_BAR[storage]
+
+
+@pytest.mark.parametrize("storage", [U16, F32, F16, BF16])
+def test_ncc_cost_error(asc, ctx, orc, tiny, storage):
+    sc, vol, _ = scene_with(asc, ctx, tiny, storage)
+    S = oracle_scene(orc, asc, tiny)
+    rng = np.random.default_rng(23)
+    poses = tiny["wl"].gt_poses[0] + np.concatenate([rng.uniform(-2, 2, (10, 3)), rng.uniform(-4, 4, (10, 3))], 1)
+    _, per = sc.eval_poses(poses)
+    err = 0.0
+    for i, p in enumerate(poses):
+        r = S.track_frame(p, fp64=True)
+        ok = r["info"][:, 0] == 1
+        err = max(err, float(np.abs(per[i][ok] - r["corr"][ok]).max()))
+    print(f"storage {storage}: max |d(1-NCC)| vs FP64 oracle {err:.3e}")
+    assert err <= NCC_BAR[storage]
+    sc.close()
+    vol.close()
+
+
+@pytest.mark.parametrize("storage", [U16, F16])
+def test_final_pose_vs_fp64_oracle(asc, ctx, orc, tiny, storage):
+    sc, vol, pivot = scene_with(asc, ctx, tiny, storage)
+    S = oracle_scene(orc, asc, tiny)
+    store0 = asc.geometry.pose_store_set(tiny["wl"].gt_poses[0] + START)
+    gaps = []
+    for seed in (3, 11):
+        opts = dict(method=0, max_iter=6, max_stall=5, seed=seed)
+        s_gpu, _ = sc.optimize_frame(store0, synchronous=True, **opts)
+        s64, _, _ = S.optimize_frame(store0, pivot, nparticles=100, synchronous=True, fp64=True, **opts)
+        gaps.append(pose_gap(orc, asc.geometry.pose_store_get(s_gpu), orc.pose_store_get(s64)))
+    print(f"storage {storage}: final pose gaps vs FP64 oracle (mm, deg) {gaps}")
+    for dt, dr in gaps:
+        assert dt <= 0.05 and dr <= 0.05
+    sc.close()
+    vol.close()
