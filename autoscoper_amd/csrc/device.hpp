@@ -121,6 +121,32 @@ __device__ __forceinline__ void unpack2(uint32_t p, float& lo, float& hi)
   }
 }
 
+// 16-bit storages: the 4 dwords holding the x-pairs {t000|t100, t010|t110,
+// t001|t101, t011|t111} (low half = x0, high half = x1) of element voff.
+template <int LY>
+__device__ __forceinline__ void gather_pairs16(const VolRsrc& r, int voff, uint32_t* q)
+{
+  if constexpr (LY == ASC_LAYOUT_LINEAR) {
+    const int so[4] = {0, r.sy, r.sz, r.sz + r.sy};
+#pragma unroll
+    for (int e = 0; e < 4; e++) q[e] = __builtin_amdgcn_raw_buffer_load_b32(r.rsrc, voff, so[e], 0);
+  } else if constexpr (LY == ASC_LAYOUT_QUAD) {
+    const auto p0 = __builtin_amdgcn_raw_buffer_load_b64(r.rsrc, voff, 0, 0);
+    const auto p1 = __builtin_amdgcn_raw_buffer_load_b64(r.rsrc, voff, r.sz, 0);
+    q[0] = p0[0];
+    q[1] = p0[1];
+    q[2] = p1[0];
+    q[3] = p1[1];
+  } else {
+    const auto p = __builtin_amdgcn_raw_buffer_load_b128(r.rsrc, voff, 0, 0);
+#pragma unroll
+    for (int e = 0; e < 4; e++) q[e] = p[e];
+  }
+}
+
+__device__ __forceinline__ float f16_lo(uint32_t p) { return __half2float(__ushort_as_half((unsigned short)(p & 0xFFFFu))); }
+__device__ __forceinline__ float f16_hi(uint32_t p) { return __half2float(__ushort_as_half((unsigned short)(p >> 16))); }
+
 // The 8 corner texels {t000, t100, t010, t110, t001, t101, t011, t111} of
 // element voff (byte offset of corner (i0, j0, k0)) in layout LY.
 template <int ST, int LY>
@@ -201,6 +227,22 @@ __device__ __forceinline__ float trilinear(const VolDev& v, const VolRsrc& r, fl
     voff = mad_i24(i >> 1, 6 * E, mad_i24(j >> 1, r.sy - 4 * E, mad_i24(k >> 1, r.sz - 8 * E, base)));
   } else {
     voff = mad_i24(k, r.sz, mad_i24(j, r.sy, (i << elem_shift<ST, LY>()) + r.c0));
+  }
+  if constexpr (ST == ASC_STORAGE_F16) {
+    // F16 volumes: the x-lerps read the half-precision texels directly in
+    // FP32 mixed-precision FMAs (v_fma_mix_f32), as (1-a)*T0 + a*T1 with one
+    // rounding per FMA; y and z lerps as below. Zero weights and zero
+    // texels still give exact zeros.
+    uint32_t q[4];
+    gather_pairs16<LY>(r, voff, q);
+    const float oma = 1.0f - a;
+    const float c00 = fmaf(a, f16_hi(q[0]), fmaf(oma, f16_lo(q[0]), 0.0f));
+    const float c10 = fmaf(a, f16_hi(q[1]), fmaf(oma, f16_lo(q[1]), 0.0f));
+    const float c01 = fmaf(a, f16_hi(q[2]), fmaf(oma, f16_lo(q[2]), 0.0f));
+    const float c11 = fmaf(a, f16_hi(q[3]), fmaf(oma, f16_lo(q[3]), 0.0f));
+    const float c0 = fmaf(b, c10 - c00, c00);
+    const float c1 = fmaf(b, c11 - c01, c01);
+    return fmaf(c, c1 - c0, c0);
   }
   float t[8];
   gather8<ST, LY>(r, voff, t);
