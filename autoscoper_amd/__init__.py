@@ -151,6 +151,13 @@ _SIGS = {
     "asc_scene_set_filters": [_vp, _i, _i, _P(AscFilter), _i],
     "asc_scene_eval_poses": [_vp, _P(_i), _P(_d), _i, _P(_d), _P(_d)],
     "asc_scene_eval_manips": [_vp, _P(_i), _P(_f), _P(_d), _i, _P(_d), _P(_d)],
+    "asc_scene_remove_volume": [_vp, _i],
+    "asc_scene_remove_camera": [_vp, _i],
+    "asc_scene_add_camera_set": [_vp, _P(_i), _i, _i, _i, _P(_i)],
+    "asc_scene_remove_camera_set": [_vp, _i],
+    "asc_scene_add_frame": [_vp, _i, _P(_vp), _P(_i)],
+    "asc_scene_remove_frame": [_vp, _i],
+    "asc_scene_eval_requests": [_vp, _i, _P(_i), _P(_i), _P(_i), _P(_d), _P(_d), _P(_d)],
     "asc_scene_use_graph": [_vp, _i],
     "asc_scene_set_debug": [_vp, _i],
     "asc_scene_last_info": [_vp, _P(_i), _i, _P(C.c_int64)],
@@ -599,12 +606,57 @@ class Scene:
         _check(lib().asc_scene_volume_pivot(self.handle, volume, _ptr(out, _d)))
         return out
 
-    def add_camera(self, cam: AscCamera, rad: Image) -> int:
+    def add_camera(self, cam: AscCamera, rad: "Image | None") -> int:
         idx = C.c_int()
-        _check(lib().asc_scene_add_camera(self.handle, C.byref(cam), rad.handle, C.byref(idx)))
-        self._keep.append(rad)
+        _check(lib().asc_scene_add_camera(self.handle, C.byref(cam), rad.handle if rad else None, C.byref(idx)))
+        if rad:
+            self._keep.append(rad)
         self.ncam += 1
         return idx.value
+
+    def remove_volume(self, volume: int):
+        _check(lib().asc_scene_remove_volume(self.handle, volume))
+        self.nvol -= 1
+
+    def remove_camera(self, camera: int):
+        _check(lib().asc_scene_remove_camera(self.handle, camera))
+        self.ncam -= 1
+
+    def add_camera_set(self, cameras, render_w: int, render_h: int) -> int:
+        """One job's views (scene camera indices) and its render size."""
+        cams = np.ascontiguousarray(cameras, np.int32)
+        out = C.c_int()
+        _check(lib().asc_scene_add_camera_set(self.handle, _ptr(cams, _i), len(cams), render_w, render_h,
+                                              C.byref(out)))
+        return out.value
+
+    def remove_camera_set(self, camera_set: int):
+        _check(lib().asc_scene_remove_camera_set(self.handle, camera_set))
+
+    def add_frame(self, camera_set: int, rads) -> int:
+        """One radiograph per camera of the set (Image objects, set order)."""
+        arr = (C.c_void_p * len(rads))(*[r.handle for r in rads])
+        out = C.c_int()
+        _check(lib().asc_scene_add_frame(self.handle, camera_set, arr, C.byref(out)))
+        self._keep.extend(rads)
+        return out.value
+
+    def remove_frame(self, frame: int):
+        _check(lib().asc_scene_remove_frame(self.handle, frame))
+
+    def eval_requests(self, volumes, sets, frames, poses):
+        """n requests (volume, camera set, frame or -1, xyzypr) in one fused
+        launch -> (costs[n], per_camera[n, 8])."""
+        p = _f64(poses).reshape(-1, 6)
+        n = p.shape[0]
+        v = np.ascontiguousarray(volumes, np.int32)
+        st = np.ascontiguousarray(sets, np.int32)
+        fr = np.ascontiguousarray(frames if frames is not None else np.full(n, -1), np.int32)
+        costs = np.zeros(n)
+        per = np.zeros((n, 8))
+        _check(lib().asc_scene_eval_requests(self.handle, n, _ptr(v, _i), _ptr(st, _i), _ptr(fr, _i), _ptr(p, _d),
+                                             _ptr(costs, _d), _ptr(per, _d)))
+        return costs, per
 
     def set_rad(self, camera: int, rad: Image):
         _check(lib().asc_scene_set_rad(self.handle, camera, rad.handle))

@@ -53,7 +53,6 @@ struct ChainDev {
 };
 
 struct CamDev {
-  ImgDev rad;
   ImgDev bg;
   int has_bg;
   float bg_threshold;
@@ -69,6 +68,7 @@ struct VolParams {
 struct ItemDesc {
   float imv[12];
   float vp[4];
+  ImgDev rad;  // the radiograph of this item's frame and camera
   int w, h;
   int vol, cam;
   int tiles_x, tiles_y;    // ray-march tiles (kTile)
@@ -78,14 +78,18 @@ struct ItemDesc {
   long long drr_off;
 };
 
-constexpr int kMaxCams = 8;
-constexpr int kParts = 8;  // XCDs per MI355X
+constexpr int kMaxCams = 64;    // camera slots per scene (all jobs of a server)
+constexpr int kMaxSetCams = 8;  // cameras per camera set (one job's views)
+constexpr int kParts = 8;       // XCDs per MI355X
 
-// A (camera, tile row, tile column, pose) enumeration of the batch's tiles.
+// A (camera, tile row, tile column, item) enumeration of the batch's tiles,
+// over the cameras that have items in the batch ("active" cameras). Items
+// are stored camera-major: the items of active camera a are
+// item_base[a] .. item_base[a] + cam_items[a] - 1.
 struct TileEnum {
   int total;
   int pad;
-  int cam_base[kMaxCams];  // first enumeration index of each camera
+  int cam_base[kMaxCams];  // first enumeration index of each active camera
   int tiles_x[kMaxCams];   // widest item of the camera, in tiles
   int tiles_y[kMaxCams];
 };
@@ -93,8 +97,10 @@ struct TileEnum {
 struct BatchHeader {
   int n_items;
   int debug;
-  int n_poses;
-  int n_cams;
+  int n_acams;                // active cameras
+  int pad;
+  int cam_items[kMaxCams];    // items per active camera
+  int item_base[kMaxCams];    // first item of each active camera
   TileEnum drr;   // kTile tiles
   TileEnum post;  // kPTile tiles
 };
@@ -146,14 +152,23 @@ __device__ __forceinline__ bool decode_tile(const BatchHeader& hd, const ItemDes
                                             int& ii, int& tx, int& ty)
 {
   const TileEnum& en = POST ? hd.post : hd.drr;
-  int c = 0;
-  while (c + 1 < hd.n_cams && en.cam_base[c + 1] <= t) ++c;
+  // last active camera whose range starts at or before t
+  int lo = 0, hi = hd.n_acams - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (en.cam_base[mid] <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const int c = lo;
   const int r = t - en.cam_base[c];
-  const int e = r % hd.n_poses;
-  const int q = r / hd.n_poses;
+  const int n_c = hd.cam_items[c];
+  const int e = r % n_c;
+  const int q = r / n_c;
   tx = q % en.tiles_x[c];
   ty = q / en.tiles_x[c];
-  ii = e * hd.n_cams + c;
+  ii = hd.item_base[c] + e;
   const ItemDesc& it = items[ii];
   return POST ? (tx < it.ptiles_x && ty < it.ptiles_y) : (tx < it.tiles_x && ty < it.tiles_y);
 }
@@ -358,7 +373,7 @@ __global__ __launch_bounds__(kBlock) void post_tiles(BatchArgs a)
           const int px = ox + xx;
           if (px < 0 || px >= w) continue;
           const float ss = s_col[xx];
-          buf0[yy * R + xx] = (ss < 0.0f || tt < 0.0f) ? 0.0f : 1.0f - dev::img_sample(cam.rad, ss, tt);
+          buf0[yy * R + xx] = (ss < 0.0f || tt < 0.0f) ? 0.0f : 1.0f - dev::img_sample(it.rad, ss, tt);
         }
       }
       __syncthreads();
@@ -489,25 +504,44 @@ struct PinBuf {
 using namespace asc;
 
 struct SceneVolume {
-  asc_volume vol;
+  bool in_use = false;
+  asc_volume vol = nullptr;
   int generation_seen = -1;
-  float step, intensity, cutoff;
+  float step = 0, intensity = 1, cutoff = 0;
   Frame pivot;
 };
 
 struct SceneCamera {
+  bool in_use = false;
   asc_camera cam;
-  asc_image rad = nullptr;
+  asc_image rad = nullptr;  // used by items without a frame
   asc_image bg = nullptr;
   float bg_threshold = -1.0f;
   std::vector<asc_filter> chain[2];
 };
 
+// The views of one job: cameras (scene slots) and their render size
+// (Tracker's render_width/height).
+struct CameraSet {
+  bool in_use = false;
+  std::vector<int> cams;
+  int RW = 0, RH = 0;
+};
+
+// One time point of a camera set: a radiograph per camera of the set.
+struct SceneFrame {
+  bool in_use = false;
+  int set = -1;
+  std::vector<asc_image> rads;
+};
+
 struct asc_scene_s {
   asc_context ctx;
-  int RW, RH;
+  int RW, RH;  // render size of the implicit all-camera set
   std::vector<SceneVolume> vols;
   std::vector<SceneCamera> cams;
+  std::vector<CameraSet> sets;
+  std::vector<SceneFrame> frames;
   bool dirty = true;
   int grid_persistent = 0;  // ray-march grid (blocks)
   int grid_post = 0;        // post-processing grid (blocks)
@@ -527,21 +561,25 @@ struct asc_scene_s {
   PinBuf<unsigned char> h_batch;
   PinBuf<double> h_results;
   PinBuf<unsigned long long> h_samples;
+  // items of the last batch in request order (request i, k-th camera of its
+  // set at req_base[i] + k); kernel order is camera-major: korder[logical]
   std::vector<ItemDesc> items;
   std::vector<int> in_view;
+  std::vector<int> korder;
+  std::vector<int> req_base, req_ncam;
   int n_items = 0;
+  size_t drr_total = 0;  // floats of DRR scratch used by the last batch
   int debug = 0;
   // graph
   bool use_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_items = -1;
-  const void* graph_key[8] = {};
   // plain-path scratch (one eval at a time)
   DevBuf<float> p_bufs;
   // per-kernel timing of the last eager batch (HIP events on the scene stream)
   bool timing = false;
-  int unroll = 2;  // ray samples per loop iteration (1, 2 or 4)
+  int unroll = 2;  // ray samples per loop iteration (1 or 2)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float last_ms[3] = {0, 0, 0};
 };
@@ -550,19 +588,28 @@ namespace {
 
 void destroy_graph(asc_scene s);
 
+// Cameras in use, in slot order: the implicit set of eval_poses/eval_manips.
+std::vector<int> all_cameras(const asc_scene s)
+{
+  std::vector<int> c;
+  for (size_t i = 0; i < s->cams.size(); i++)
+    if (s->cams[i].in_use) c.push_back((int)i);
+  return c;
+}
+
 int scene_upload_config(asc_scene s)
 {
   for (const SceneVolume& sv : s->vols)
-    if (sv.generation_seen != sv.vol->generation) s->dirty = true;
+    if (sv.in_use && sv.generation_seen != sv.vol->generation) s->dirty = true;
   if (!s->dirty) return ASC_OK;
-  std::vector<CamDev> cams(s->cams.size());
+  std::vector<CamDev> cams(std::max<size_t>(s->cams.size(), 1));
   std::vector<float> pool;
+  int max_halo = 0;
   for (size_t c = 0; c < s->cams.size(); c++) {
     const SceneCamera& sc = s->cams[c];
     CamDev& cd = cams[c];
     std::memset(&cd, 0, sizeof cd);
-    if (!sc.rad) return fail(ASC_ERR_INVALID, "camera has no radiograph");
-    cd.rad = sc.rad->dev;
+    if (!sc.in_use) continue;
     cd.has_bg = (sc.bg && sc.bg_threshold >= 0.0f) ? 1 : 0;  // View.cpp:173-174
     if (sc.bg) cd.bg = sc.bg->dev;
     cd.bg_threshold = sc.bg_threshold;
@@ -590,19 +637,20 @@ int scene_upload_config(asc_scene s)
         ch.halo += ch.radius[k];
       }
       if (ch.halo > kMaxHalo) return fail(ASC_ERR_UNSUPPORTED, "filter chain halo exceeds 12 pixels");
+      max_halo = std::max(max_halo, ch.halo);
     }
   }
-  int max_halo = 0;
-  for (const CamDev& cd : cams) max_halo = std::max(max_halo, std::max(cd.chain[0].halo, cd.chain[1].halo));
   const int small = max_halo <= kSmallHalo ? 1 : 0;
   if (small != s->post_small) {
     s->post_small = small;
     s->grid_post = 0;
     destroy_graph(s);
   }
-  std::vector<VolParams> vols(s->vols.size());
+  std::vector<VolParams> vols(std::max<size_t>(s->vols.size(), 1));
   for (size_t i = 0; i < s->vols.size(); i++) {
     SceneVolume& sv = s->vols[i];
+    std::memset(&vols[i], 0, sizeof(VolParams));
+    if (!sv.in_use) continue;
     sv.generation_seen = sv.vol->generation;
     vols[i].v = sv.vol->dev;
     vols[i].step = sv.step;
@@ -611,10 +659,13 @@ int scene_upload_config(asc_scene s)
     vols[i].step_vscale = sv.step * sv.vol->dev.vscale;
   }
   if (pool.empty()) pool.push_back(0.0f);
+  const void* before[3] = {s->d_cams.p, s->d_vols.p, s->d_weights.p};
   int rc;
   if ((rc = s->d_cams.ensure(cams.size()))) return rc;
   if ((rc = s->d_vols.ensure(vols.size()))) return rc;
   if ((rc = s->d_weights.ensure(pool.size()))) return rc;
+  const void* after[3] = {s->d_cams.p, s->d_vols.p, s->d_weights.p};
+  if (std::memcmp(before, after, sizeof before) != 0) destroy_graph(s);
   ASC_HIP(hipStreamSynchronize(s->ctx->stream));
   ASC_HIP(hipMemcpy(s->d_cams.p, cams.data(), sizeof(CamDev) * cams.size(), hipMemcpyHostToDevice));
   ASC_HIP(hipMemcpy(s->d_vols.p, vols.data(), sizeof(VolParams) * vols.size(), hipMemcpyHostToDevice));
@@ -648,10 +699,10 @@ int issue_batch(asc_scene s, size_t batch_bytes)
   a.results = s->d_results.p;
   a.samples = s->d_samples.p;
   a.counters = s->d_counters.p;
-  const size_t plane = (size_t)s->RW * s->RH;
+  const size_t plane = s->drr_total;
   a.dbg_drr = s->debug ? s->d_dbg.p : nullptr;
-  a.dbg_rad = s->debug ? s->d_dbg.p + plane * n_items : nullptr;
-  a.dbg_mask = s->debug ? s->d_dbg.p + 2 * plane * n_items : nullptr;
+  a.dbg_rad = s->debug ? s->d_dbg.p + plane : nullptr;
+  a.dbg_mask = s->debug ? s->d_dbg.p + 2 * plane : nullptr;
   ASC_HIP(hipMemcpyAsync(s->d_batch.p, s->h_batch.p, batch_bytes, hipMemcpyHostToDevice, st));
   zero_u64<<<(n_items + 255) / 256, 256, 0, st>>>(s->d_samples.p, n_items);
   const bool timed = s->timing && s->ev[0];
@@ -675,53 +726,138 @@ int issue_batch(asc_scene s, size_t batch_bytes)
   return ASC_OK;
 }
 
-// Geometry for every (pose, camera) item, then the fused launches.
-int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, double* costs, double* per_camera)
+// One request of a batch: a pose of a volume seen by a camera set (-1: all
+// cameras in use) at a frame (-1: each camera's own radiograph).
+struct Request {
+  int vol, set, frame;
+  const double* pose;
+};
+
+// Geometry for every (request, camera) item, then the fused launches.
+// per_camera: n x stride values (camera k of request i at i*stride + k).
+int eval_requests(asc_scene s, const std::vector<Request>& reqs, double* costs, double* per_camera, int stride)
 {
   int rc = scene_upload_config(s);
   if (rc) return rc;
-  const int C = (int)s->cams.size();
-  ASC_CHECK(C > 0, ASC_ERR_INVALID, "scene has no cameras");
-  ASC_CHECK(C <= kMaxCams, ASC_ERR_UNSUPPORTED, "at most 8 cameras per scene");
-  ASC_CHECK(!s->vols.empty(), ASC_ERR_INVALID, "scene has no volumes");
-  const int n_items = n * C;
+  const int n = (int)reqs.size();
+  const std::vector<int> all = all_cameras(s);
+  // items in request order
+  s->req_base.assign(n, 0);
+  s->req_ncam.assign(n, 0);
+  int n_items = 0;
+  for (int i = 0; i < n; i++) {
+    const Request& q = reqs[i];
+    ASC_CHECK(q.vol >= 0 && q.vol < (int)s->vols.size() && s->vols[q.vol].in_use, ASC_ERR_INVALID,
+              "volume index out of range");
+    int nc;
+    if (q.set < 0) {
+      nc = (int)all.size();
+    } else {
+      ASC_CHECK(q.set < (int)s->sets.size() && s->sets[q.set].in_use, ASC_ERR_INVALID, "bad camera set");
+      nc = (int)s->sets[q.set].cams.size();
+    }
+    ASC_CHECK(nc > 0, ASC_ERR_INVALID, "scene has no cameras");
+    ASC_CHECK(nc <= stride, ASC_ERR_INVALID, "more cameras than the per-camera output holds");
+    if (q.frame >= 0)
+      ASC_CHECK(q.frame < (int)s->frames.size() && s->frames[q.frame].in_use && s->frames[q.frame].set == q.set,
+                ASC_ERR_INVALID, "frame does not belong to the request's camera set");
+    s->req_base[i] = n_items;
+    s->req_ncam[i] = nc;
+    n_items += nc;
+  }
   s->n_items = n_items;
   s->items.resize(n_items);
   s->in_view.assign(n_items, 0);
-  const size_t plane = (size_t)s->RW * s->RH;
-  const int max_ptiles = ((s->RW + kPTile - 1) / kPTile) * ((s->RH + kPTile - 1) / kPTile);
-  int tile_base = 0;
   for (int i = 0; i < n; i++) {
-    const int vi = volumes ? volumes[i] : 0;
-    ASC_CHECK(vi >= 0 && vi < (int)s->vols.size(), ASC_ERR_INVALID, "volume index out of range");
-    const asc_volume_desc& d = s->vols[vi].vol->desc;
-    for (int c = 0; c < C; c++) {
-      const int ii = i * C + c;
+    const Request& q = reqs[i];
+    const asc_volume_desc& d = s->vols[q.vol].vol->desc;
+    const std::vector<int>& cl = q.set < 0 ? all : s->sets[q.set].cams;
+    const int RW = q.set < 0 ? s->RW : s->sets[q.set].RW, RH = q.set < 0 ? s->RH : s->sets[q.set].RH;
+    for (int k = 0; k < s->req_ncam[i]; k++) {
+      const int ii = s->req_base[i] + k;
+      const int c = cl[k];
       ItemDesc& it = s->items[ii];
       std::memset(&it, 0, sizeof it);
+      asc_image rad = q.frame >= 0 ? s->frames[q.frame].rads[k] : s->cams[c].rad;
+      ASC_CHECK(rad != nullptr, ASC_ERR_INVALID, "camera has no radiograph");
+      it.rad = rad->dev;
       double vp[4];
       unsigned sz[2] = {0, 0};
-      const int inview = view_geometry(s->cams[c].cam, poses + 6 * i, d.inv_scale, d.inv_trans, s->RW, s->RH,
-                                       it.imv, vp, sz);
-      it.vol = vi;
+      const int inview =
+        view_geometry(s->cams[c].cam, q.pose, d.inv_scale, d.inv_trans, RW, RH, it.imv, vp, sz);
+      it.vol = q.vol;
       it.cam = c;
-      it.drr_off = (long long)ii * (long long)plane;
-      it.part_off = ii * max_ptiles;
-      it.tile_base = tile_base;
       if (inview) {
-        if (sz[0] > (unsigned)s->RW || sz[1] > (unsigned)s->RH)
+        if (sz[0] > (unsigned)RW || sz[1] > (unsigned)RH)
           return fail(ASC_ERR_TOO_LARGE, "Rendered image is larger than the viewport buffer");
-        for (int k = 0; k < 4; k++) it.vp[k] = (float)vp[k];
+        for (int m = 0; m < 4; m++) it.vp[m] = (float)vp[m];
         it.w = (int)sz[0];
         it.h = (int)sz[1];
         it.tiles_x = (it.w + kTile - 1) / kTile;
         it.tiles_y = (it.h + kTile - 1) / kTile;
         it.ptiles_x = (it.w + kPTile - 1) / kPTile;
         it.ptiles_y = (it.h + kPTile - 1) / kPTile;
-        tile_base += it.tiles_x * it.tiles_y;
         s->in_view[ii] = 1;
       }
     }
+  }
+  // camera-major kernel order; compact DRR scratch and partial offsets
+  std::vector<int> acams;
+  {
+    std::vector<int> count(s->cams.size(), 0);
+    for (int ii = 0; ii < n_items; ii++) count[s->items[ii].cam]++;
+    for (size_t c = 0; c < count.size(); c++)
+      if (count[c]) acams.push_back((int)c);
+  }
+  ASC_CHECK(acams.size() <= (size_t)kMaxCams, ASC_ERR_UNSUPPORTED, "more than 64 cameras in one batch");
+  std::vector<int> slot(s->cams.size(), -1);
+  for (size_t a = 0; a < acams.size(); a++) slot[acams[a]] = (int)a;
+  BatchHeader hdr;
+  std::memset(&hdr, 0, sizeof hdr);
+  hdr.n_items = n_items;
+  hdr.debug = s->debug;
+  hdr.n_acams = (int)acams.size();
+  for (int ii = 0; ii < n_items; ii++) hdr.cam_items[slot[s->items[ii].cam]]++;
+  for (int a = 1; a < hdr.n_acams; a++) hdr.item_base[a] = hdr.item_base[a - 1] + hdr.cam_items[a - 1];
+  s->korder.assign(n_items, 0);
+  {
+    std::vector<int> fill(hdr.n_acams, 0);
+    for (int ii = 0; ii < n_items; ii++) {
+      const int a = slot[s->items[ii].cam];
+      s->korder[ii] = hdr.item_base[a] + fill[a]++;
+    }
+  }
+  std::vector<ItemDesc> kitems(n_items);
+  size_t drr_total = 0, part_total = 0;
+  for (int ii = 0; ii < n_items; ii++) {
+    ItemDesc it = s->items[ii];
+    it.drr_off = (long long)drr_total;
+    it.part_off = (int)part_total;
+    drr_total += (size_t)it.w * it.h;
+    part_total += (size_t)it.ptiles_x * it.ptiles_y;
+    it.tile_base = 0;
+    s->items[ii].drr_off = it.drr_off;
+    kitems[s->korder[ii]] = it;
+  }
+  ASC_CHECK(part_total < (size_t)1 << 31, ASC_ERR_TOO_LARGE, "batch has too many tiles");
+  s->drr_total = std::max<size_t>(drr_total, 1);
+  for (int which = 0; which < 2; which++) {
+    TileEnum& en = which ? hdr.post : hdr.drr;
+    long long base = 0;
+    for (int a = 0; a < hdr.n_acams; a++) {
+      int tx = 0, ty = 0;
+      for (int e = 0; e < hdr.cam_items[a]; e++) {
+        const ItemDesc& it = kitems[hdr.item_base[a] + e];
+        tx = std::max(tx, which ? it.ptiles_x : it.tiles_x);
+        ty = std::max(ty, which ? it.ptiles_y : it.tiles_y);
+      }
+      en.cam_base[a] = (int)base;
+      en.tiles_x[a] = std::max(tx, 1);
+      en.tiles_y[a] = ty;
+      base += (long long)tx * ty * hdr.cam_items[a];
+    }
+    ASC_CHECK(base < (1LL << 31) - (1LL << 20), ASC_ERR_TOO_LARGE, "batch has too many tiles");
+    en.total = (int)base;
   }
   // buffers
   const size_t batch_bytes = sizeof(BatchHeader) + sizeof(ItemDesc) * (size_t)n_items;
@@ -729,8 +865,8 @@ int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, doub
                            s->d_dbg.p};
   if ((rc = s->d_batch.ensure(batch_bytes))) return rc;
   if ((rc = s->h_batch.ensure(batch_bytes))) return rc;
-  if ((rc = s->d_drr.ensure(plane * n_items))) return rc;
-  if ((rc = s->d_partials.ensure((size_t)max_ptiles * n_items * 9))) return rc;
+  if ((rc = s->d_drr.ensure(s->drr_total))) return rc;
+  if ((rc = s->d_partials.ensure(std::max<size_t>(part_total, 1) * 9))) return rc;
   if ((rc = s->d_results.ensure((size_t)n_items * 10))) return rc;
   if ((rc = s->h_results.ensure((size_t)n_items * 10))) return rc;
   if ((rc = s->d_samples.ensure((size_t)n_items))) return rc;
@@ -739,7 +875,7 @@ int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, doub
     if ((rc = s->d_counters.ensure(1))) return rc;
     ASC_HIP(hipMemset(s->d_counters.p, 0, sizeof(Counters)));
   }
-  if (s->debug && (rc = s->d_dbg.ensure(plane * n_items * 3))) return rc;
+  if (s->debug && (rc = s->d_dbg.ensure(s->drr_total * 3))) return rc;
   const void* after[6] = {s->d_batch.p, s->d_drr.p, s->d_partials.p, s->d_results.p, s->d_samples.p, s->d_dbg.p};
   if (std::memcmp(before, after, sizeof before) != 0) destroy_graph(s);
   if (s->grid_persistent == 0 || s->grid_post == 0) {
@@ -759,32 +895,8 @@ int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, doub
       s->grid_post = prop.multiProcessorCount * std::max(per_cu, 1);
     }
   }
-  BatchHeader hdr;
-  std::memset(&hdr, 0, sizeof hdr);
-  hdr.n_items = n_items;
-  hdr.debug = s->debug;
-  hdr.n_poses = n;
-  hdr.n_cams = C;
-  for (int which = 0; which < 2; which++) {
-    TileEnum& en = which ? hdr.post : hdr.drr;
-    long long base = 0;
-    for (int c = 0; c < C; c++) {
-      int tx = 0, ty = 0;
-      for (int i = 0; i < n; i++) {
-        const ItemDesc& it = s->items[(size_t)i * C + c];
-        tx = std::max(tx, which ? it.ptiles_x : it.tiles_x);
-        ty = std::max(ty, which ? it.ptiles_y : it.tiles_y);
-      }
-      en.cam_base[c] = (int)base;
-      en.tiles_x[c] = std::max(tx, 1);
-      en.tiles_y[c] = ty;
-      base += (long long)tx * ty * n;
-    }
-    ASC_CHECK(base < (1LL << 31) - (1LL << 20), ASC_ERR_TOO_LARGE, "batch has too many tiles");
-    en.total = (int)base;
-  }
   std::memcpy(s->h_batch.p, &hdr, sizeof hdr);
-  std::memcpy(s->h_batch.p + sizeof hdr, s->items.data(), sizeof(ItemDesc) * n_items);
+  std::memcpy(s->h_batch.p + sizeof hdr, kitems.data(), sizeof(ItemDesc) * n_items);
 
   hipStream_t st = s->ctx->stream;
   if (s->use_graph) {
@@ -812,16 +924,40 @@ int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, doub
   // :814-826 (sum over cameras; negative totals become 9999).
   for (int i = 0; i < n; i++) {
     double total = 0.0;
-    for (int c = 0; c < C; c++) {
-      const int ii = i * C + c;
-      const double corr = s->in_view[ii] ? 1.0 - s->h_results.p[(size_t)ii * 10 + 9] : 9999.0;
-      if (per_camera) per_camera[ii] = corr;
-      total = c == 0 ? corr : total + corr;
+    for (int k = 0; k < s->req_ncam[i]; k++) {
+      const int ii = s->req_base[i] + k;
+      const int ki = s->korder[ii];
+      const double corr = s->in_view[ii] ? 1.0 - s->h_results.p[(size_t)ki * 10 + 9] : 9999.0;
+      if (per_camera) per_camera[(size_t)i * stride + k] = corr;
+      total = k == 0 ? corr : total + corr;
     }
     if (total < 0) total = 9999;
     if (costs) costs[i] = total;
   }
   return ASC_OK;
+}
+
+// The all-camera form used by eval_poses / eval_manips.
+int eval_batch(asc_scene s, const int* volumes, const double* poses, int n, double* costs, double* per_camera)
+{
+  std::vector<Request> reqs(n);
+  for (int i = 0; i < n; i++) reqs[i] = Request{volumes ? volumes[i] : 0, -1, -1, poses + 6 * i};
+  const int C = (int)all_cameras(s).size();
+  ASC_CHECK(C > 0, ASC_ERR_INVALID, "scene has no cameras");
+  return eval_requests(s, reqs, costs, per_camera, C);
+}
+
+// First free slot of a scene table (reused after a remove), or a new one.
+template <class T>
+int take_slot(std::vector<T>& v)
+{
+  for (size_t i = 0; i < v.size(); i++)
+    if (!v[i].in_use) {
+      v[i] = T();
+      return (int)i;
+    }
+  v.emplace_back();
+  return (int)v.size() - 1;
 }
 
 }  // namespace
@@ -869,21 +1005,32 @@ ASC_API int asc_scene_add_volume(asc_scene s, asc_volume vol, float step, float 
                                  const double pivot[12], int* index)
 {
   ASC_CHECK(s && vol, ASC_ERR_INVALID, "bad argument");
-  SceneVolume sv;
+  const int i = take_slot(s->vols);
+  SceneVolume& sv = s->vols[i];
+  sv.in_use = true;
   sv.vol = vol;
   sv.step = step;
   sv.intensity = intensity;
   sv.cutoff = cutoff;
   sv.pivot = pivot ? frame_load(pivot) : frame_identity();
-  s->vols.push_back(sv);
-  if (index) *index = (int)s->vols.size() - 1;
+  if (index) *index = i;
+  s->dirty = true;
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_remove_volume(asc_scene s, int volume)
+{
+  ASC_CHECK(s && volume >= 0 && volume < (int)s->vols.size() && s->vols[volume].in_use, ASC_ERR_INVALID,
+            "bad volume index");
+  s->vols[volume] = SceneVolume();
   s->dirty = true;
   return ASC_OK;
 }
 
 ASC_API int asc_scene_volume_pivot(asc_scene s, int volume, double pivot[12])
 {
-  ASC_CHECK(s && pivot && volume >= 0 && volume < (int)s->vols.size(), ASC_ERR_INVALID, "bad volume index");
+  ASC_CHECK(s && pivot && volume >= 0 && volume < (int)s->vols.size() && s->vols[volume].in_use, ASC_ERR_INVALID,
+            "bad volume index");
   frame_store(s->vols[volume].pivot, pivot);
   return ASC_OK;
 }
@@ -901,18 +1048,84 @@ ASC_API int asc_scene_set_drr_params(asc_scene s, int volume, float step, float 
 ASC_API int asc_scene_add_camera(asc_scene s, const asc_camera* cam, asc_image rad, int* index)
 {
   ASC_CHECK(s && cam, ASC_ERR_INVALID, "bad argument");
-  SceneCamera sc;
+  const int i = take_slot(s->cams);
+  ASC_CHECK(i < kMaxCams, ASC_ERR_UNSUPPORTED, "at most 64 cameras per scene");
+  SceneCamera& sc = s->cams[i];
+  sc.in_use = true;
   sc.cam = *cam;
   sc.rad = rad;
-  s->cams.push_back(sc);
-  if (index) *index = (int)s->cams.size() - 1;
+  if (index) *index = i;
   s->dirty = true;
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_remove_camera(asc_scene s, int camera)
+{
+  ASC_CHECK(s && camera >= 0 && camera < (int)s->cams.size() && s->cams[camera].in_use, ASC_ERR_INVALID,
+            "bad camera index");
+  for (const CameraSet& cs : s->sets)
+    if (cs.in_use)
+      for (int c : cs.cams) ASC_CHECK(c != camera, ASC_ERR_INVALID, "camera is used by a camera set");
+  s->cams[camera] = SceneCamera();
+  s->dirty = true;
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_add_camera_set(asc_scene s, const int* cameras, int count, int render_width,
+                                     int render_height, int* set)
+{
+  ASC_CHECK(s && cameras && set, ASC_ERR_INVALID, "bad argument");
+  ASC_CHECK(count > 0 && count <= kMaxSetCams, ASC_ERR_INVALID, "a camera set has 1 to 8 cameras");
+  ASC_CHECK(render_width > 0 && render_height > 0, ASC_ERR_INVALID, "bad render size");
+  for (int k = 0; k < count; k++)
+    ASC_CHECK(cameras[k] >= 0 && cameras[k] < (int)s->cams.size() && s->cams[cameras[k]].in_use, ASC_ERR_INVALID,
+              "bad camera index");
+  const int i = take_slot(s->sets);
+  CameraSet& cs = s->sets[i];
+  cs.in_use = true;
+  cs.cams.assign(cameras, cameras + count);
+  cs.RW = render_width;
+  cs.RH = render_height;
+  *set = i;
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_remove_camera_set(asc_scene s, int set)
+{
+  ASC_CHECK(s && set >= 0 && set < (int)s->sets.size() && s->sets[set].in_use, ASC_ERR_INVALID, "bad camera set");
+  for (SceneFrame& f : s->frames)
+    if (f.in_use && f.set == set) f = SceneFrame();
+  s->sets[set] = CameraSet();
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_add_frame(asc_scene s, int set, const asc_image* rads, int* frame)
+{
+  ASC_CHECK(s && rads && frame, ASC_ERR_INVALID, "bad argument");
+  ASC_CHECK(set >= 0 && set < (int)s->sets.size() && s->sets[set].in_use, ASC_ERR_INVALID, "bad camera set");
+  const size_t nc = s->sets[set].cams.size();
+  for (size_t k = 0; k < nc; k++) ASC_CHECK(rads[k] != nullptr, ASC_ERR_INVALID, "null radiograph");
+  const int i = take_slot(s->frames);
+  SceneFrame& f = s->frames[i];
+  f.in_use = true;
+  f.set = set;
+  f.rads.assign(rads, rads + nc);
+  *frame = i;
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_remove_frame(asc_scene s, int frame)
+{
+  ASC_CHECK(s && frame >= 0 && frame < (int)s->frames.size() && s->frames[frame].in_use, ASC_ERR_INVALID,
+            "bad frame");
+  s->frames[frame] = SceneFrame();
   return ASC_OK;
 }
 
 ASC_API int asc_scene_set_rad(asc_scene s, int camera, asc_image rad)
 {
-  ASC_CHECK(s && camera >= 0 && camera < (int)s->cams.size() && rad, ASC_ERR_INVALID, "bad argument");
+  ASC_CHECK(s && camera >= 0 && camera < (int)s->cams.size() && s->cams[camera].in_use && rad, ASC_ERR_INVALID,
+            "bad argument");
   s->cams[camera].rad = rad;
   s->dirty = true;
   return ASC_OK;
@@ -920,8 +1133,9 @@ ASC_API int asc_scene_set_rad(asc_scene s, int camera, asc_image rad)
 
 ASC_API int asc_scene_set_background(asc_scene s, int camera, asc_image bg, float threshold)
 {
-  ASC_CHECK(s && camera >= 0 && camera < (int)s->cams.size(), ASC_ERR_INVALID, "bad argument");
-  ASC_CHECK(!bg || bg->dev.bps == 32, ASC_ERR_UNSUPPORTED, "background image must be float32 (CL_FLOAT)");
+  ASC_CHECK(s && camera >= 0 && camera < (int)s->cams.size() && s->cams[camera].in_use, ASC_ERR_INVALID,
+            "bad argument");
+  ASC_CHECK(!bg || bg->src_bps == 32, ASC_ERR_UNSUPPORTED, "background image must be float32 (CL_FLOAT)");
   s->cams[camera].bg = bg;
   s->cams[camera].bg_threshold = threshold;
   s->dirty = true;
@@ -930,8 +1144,8 @@ ASC_API int asc_scene_set_background(asc_scene s, int camera, asc_image bg, floa
 
 ASC_API int asc_scene_set_filters(asc_scene s, int camera, int which, const asc_filter* filters, int count)
 {
-  ASC_CHECK(s && camera >= 0 && camera < (int)s->cams.size() && (which == 0 || which == 1), ASC_ERR_INVALID,
-            "bad argument");
+  ASC_CHECK(s && camera >= 0 && camera < (int)s->cams.size() && s->cams[camera].in_use && (which == 0 || which == 1),
+            ASC_ERR_INVALID, "bad argument");
   ASC_CHECK(count >= 0 && (count == 0 || filters), ASC_ERR_INVALID, "bad filter list");
   s->cams[camera].chain[which].assign(filters, filters + count);
   s->dirty = true;
@@ -945,6 +1159,15 @@ ASC_API int asc_scene_eval_poses(asc_scene s, const int* volumes, const double* 
   return eval_batch(s, volumes, xyzypr, n, costs, per_camera);
 }
 
+ASC_API int asc_scene_eval_requests(asc_scene s, int n, const int* volumes, const int* sets, const int* frames,
+                                    const double* xyzypr, double* costs, double* per_camera)
+{
+  ASC_CHECK(s && volumes && sets && xyzypr && n > 0, ASC_ERR_INVALID, "bad argument");
+  std::vector<Request> reqs(n);
+  for (int i = 0; i < n; i++) reqs[i] = Request{volumes[i], sets[i], frames ? frames[i] : -1, xyzypr + 6 * i};
+  return eval_requests(s, reqs, costs, per_camera, kMaxSetCams);
+}
+
 ASC_API int asc_scene_eval_manips(asc_scene s, const int* volumes, const float* stores, const double* manips, int n,
                                   double* costs, double* per_camera)
 {
@@ -952,7 +1175,8 @@ ASC_API int asc_scene_eval_manips(asc_scene s, const int* volumes, const float* 
   std::vector<double> poses((size_t)n * 6);
   for (int i = 0; i < n; i++) {
     const int vi = volumes ? volumes[i] : 0;
-    ASC_CHECK(vi >= 0 && vi < (int)s->vols.size(), ASC_ERR_INVALID, "volume index out of range");
+    ASC_CHECK(vi >= 0 && vi < (int)s->vols.size() && s->vols[vi].in_use, ASC_ERR_INVALID,
+              "volume index out of range");
     PoseStore ps;
     std::memcpy(&ps, stores + 7 * i, sizeof ps);
     compose_pose(ps, s->vols[vi].pivot, manips + 6 * i, poses.data() + 6 * i);
@@ -1016,16 +1240,17 @@ ASC_API int asc_scene_last_info(asc_scene s, int* info, int capacity, int64_t* s
   ASC_CHECK(s, ASC_ERR_INVALID, "null scene");
   int64_t total = 0;
   for (int ii = 0; ii < s->n_items; ii++) {
-    total += (int64_t)s->h_samples.p[ii];
+    total += (int64_t)s->h_samples.p[s->korder[ii]];
     if (info && (ii + 1) * 8 <= capacity) {
       int* o = info + 8 * ii;
       const ItemDesc& it = s->items[ii];
+      const int ki = s->korder[ii];
       o[0] = s->in_view[ii];
       o[1] = it.w;
       o[2] = it.h;
-      o[3] = (int)s->h_results.p[(size_t)ii * 10 + 3];
+      o[3] = (int)s->h_results.p[(size_t)ki * 10 + 3];
       o[4] = it.tiles_x * it.tiles_y;
-      o[5] = (int)s->h_samples.p[ii];
+      o[5] = (int)s->h_samples.p[ki];
       o[6] = o[7] = 0;
     }
   }
@@ -1037,25 +1262,24 @@ ASC_API int asc_scene_last_moments(asc_scene s, double* moments, int capacity)
 {
   ASC_CHECK(s && moments, ASC_ERR_INVALID, "bad argument");
   for (int ii = 0; ii < s->n_items && (ii + 1) * 10 <= capacity; ii++)
-    for (int k = 0; k < 10; k++) moments[ii * 10 + k] = s->h_results.p[(size_t)ii * 10 + k];
+    for (int k = 0; k < 10; k++) moments[ii * 10 + k] = s->h_results.p[(size_t)s->korder[ii] * 10 + k];
   return ASC_OK;
 }
 
 ASC_API int asc_scene_last_images(asc_scene s, int pose, int camera, float* drr, float* rad, float* mask)
 {
   ASC_CHECK(s && s->debug, ASC_ERR_INVALID, "enable debug images with asc_scene_set_debug first");
-  const int C = (int)s->cams.size();
-  const int ii = pose * C + camera;
-  ASC_CHECK(pose >= 0 && camera >= 0 && camera < C && ii < s->n_items, ASC_ERR_INVALID, "bad item");
+  ASC_CHECK(pose >= 0 && pose < (int)s->req_base.size() && camera >= 0 && camera < s->req_ncam[pose], ASC_ERR_INVALID,
+            "bad item");
+  const int ii = s->req_base[pose] + camera;
   const ItemDesc& it = s->items[ii];
   const size_t n = (size_t)it.w * it.h;
-  const size_t plane = (size_t)s->RW * s->RH;
+  const size_t plane = s->drr_total;
   if (n == 0) return ASC_OK;
   float* bases[3] = {drr, rad, mask};
   for (int k = 0; k < 3; k++)
     if (bases[k])
-      ASC_HIP(hipMemcpy(bases[k], s->d_dbg.p + k * plane * s->n_items + it.drr_off, n * sizeof(float),
-                        hipMemcpyDeviceToHost));
+      ASC_HIP(hipMemcpy(bases[k], s->d_dbg.p + k * plane + it.drr_off, n * sizeof(float), hipMemcpyDeviceToHost));
   return ASC_OK;
 }
 
@@ -1076,9 +1300,12 @@ ASC_API int asc_scene_track_frame_plain(asc_scene s, int volume, const double xy
   float* bgm = mask + plane;
   float* scratch = bgm + plane;
   const SceneVolume& sv = s->vols[volume];
+  ASC_CHECK(sv.in_use, ASC_ERR_INVALID, "bad volume index");
+  const std::vector<int> cl = all_cameras(s);
   double total = 0.0;
-  for (size_t c = 0; c < s->cams.size(); c++) {
-    const SceneCamera& sc = s->cams[c];
+  for (size_t c = 0; c < cl.size(); c++) {
+    const SceneCamera& sc = s->cams[cl[c]];
+    ASC_CHECK(sc.rad != nullptr, ASC_ERR_INVALID, "camera has no radiograph");
     float imv[12];
     double vp[4];
     unsigned sz[2];

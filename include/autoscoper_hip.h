@@ -255,6 +255,27 @@ ASC_API int asc_scene_eval_poses(asc_scene scene, const int* volumes, const doub
  * {x, y, z, qw, qx, qy, qz}, manips n*6 doubles. */
 ASC_API int asc_scene_eval_manips(asc_scene scene, const int* volumes, const float* stores, const double* manips,
                                   int n, double* costs, double* per_camera);
+
+/* Several jobs in one scene (a serving loop): volumes and cameras are
+ * reusable slots; a camera set is one job's views with its render size
+ * (Tracker's render_width/height); a frame is one radiograph per camera of
+ * a set (Video frame, Tracker.cpp:380-385). These have no counterpart in the
+ * reference, which holds one trial per process; they let requests of
+ * different jobs share one fused launch. */
+ASC_API int asc_scene_remove_volume(asc_scene scene, int volume);
+ASC_API int asc_scene_remove_camera(asc_scene scene, int camera);
+ASC_API int asc_scene_add_camera_set(asc_scene scene, const int* cameras, int count, int render_width,
+                                     int render_height, int* set);
+ASC_API int asc_scene_remove_camera_set(asc_scene scene, int set);
+ASC_API int asc_scene_add_frame(asc_scene scene, int set, const asc_image* rads, int* frame);
+ASC_API int asc_scene_remove_frame(asc_scene scene, int frame);
+/* n requests, each a pose of volumes[i] seen by camera set sets[i] at frame
+ * frames[i] (frames may be NULL or -1: each camera's own radiograph), all in
+ * one fused launch. per_camera (optional) is n x 8: camera k of request i at
+ * i*8 + k. costs[i] = sum over the set's cameras of 1 - NCC (9999 rules of
+ * Tracker.cpp:614-618, 814-826). */
+ASC_API int asc_scene_eval_requests(asc_scene scene, int n, const int* volumes, const int* sets, const int* frames,
+                                    const double* xyzypr, double* costs, double* per_camera);
 /* Record the fused launches of an n-pose batch in a hipGraph and replay it
  * for later batches of the same size (1 = on, 0 = off). */
 ASC_API int asc_scene_use_graph(asc_scene scene, int enable);
