@@ -81,6 +81,32 @@ class AscOptimizeStats(C.Structure):
     ]
 
 
+class AscJobDesc(C.Structure):
+    _fields_ = [
+        ("n_volumes", C.c_int), ("volumes", C.POINTER(C.c_void_p)), ("volume_dims", C.POINTER(C.c_int)),
+        ("voxel_mm", C.POINTER(C.c_double)), ("initial_poses", C.POINTER(C.c_double)), ("storage", C.c_int),
+        ("drr_step", C.c_float), ("drr_intensity", C.c_float), ("drr_cutoff", C.c_float),
+        ("n_cameras", C.c_int), ("cameras", C.POINTER(AscCamera)), ("render_width", C.c_int),
+        ("render_height", C.c_int), ("drr_filters", C.POINTER(AscFilter)), ("n_drr_filters", C.c_int),
+        ("rad_filters", C.POINTER(AscFilter)), ("n_rad_filters", C.c_int), ("n_frames", C.c_int),
+        ("frames", C.POINTER(C.c_void_p)), ("frame_dims", C.POINTER(C.c_int)), ("frame_bps", C.c_int),
+        ("options", AscOptimizeOptions), ("guess", C.c_int),
+    ]
+
+
+class AscJobStats(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("total_ms", C.c_double), ("evaluations", C.c_int64), ("frames", C.c_int)]
+
+
+class AscServeStats(C.Structure):
+    _fields_ = [
+        ("jobs_done", C.c_int64), ("frames_done", C.c_int64), ("batches", C.c_int64), ("requests", C.c_int64),
+        ("evaluations", C.c_int64), ("busy_ms", C.c_double), ("elapsed_ms", C.c_double),
+        ("latency_p50_ms", C.c_double), ("latency_p99_ms", C.c_double), ("latency_max_ms", C.c_double),
+        ("active_jobs", C.c_int), ("waiting_jobs", C.c_int), ("device_bytes_in_use", C.c_size_t),
+    ]
+
+
 _vp = C.c_void_p
 _i = C.c_int
 _u = C.c_uint
@@ -173,6 +199,13 @@ _SIGS = {
     "asc_optimize_frame": [_vp, _P(_f), _P(AscOptimizeOptions), _P(AscOptimizeStats)],
     "asc_pso": [_vp, _vp, _u, _f, _f, _u, _u, _i, _i, _P(_f), _P(_f), _P(_i)],
     "asc_simplex": [_vp, _vp, _P(_d), _P(_d), _i, _d, _P(_i)],
+    "asc_optimize_frame_fn": [_vp, _vp, _P(_f), _P(_d), _P(AscOptimizeOptions), _P(AscOptimizeStats)],
+    "asc_frame_linear_extrap": [_P(_d), _P(_d), _P(_d)],
+    "asc_server_create": [_vp, _sz, _i, _P(_vp)],
+    "asc_server_submit": [_vp, _P(AscJobDesc), _P(_i)],
+    "asc_server_wait": [_vp, _i, _P(_d), _P(_d), _P(AscJobStats)],
+    "asc_server_stats": [_vp, _P(AscServeStats)],
+    "asc_server_destroy": [_vp],
 }
 _RES = {"asc_last_error": C.c_char_p, "asc_context_stream": _vp}
 
@@ -262,6 +295,13 @@ class geometry:  # namespace
         a = _f64(a)
         out = np.zeros(12)
         _check(lib().asc_frame_inverse(_ptr(a, _d), _ptr(out, _d)))
+        return out
+
+    @staticmethod
+    def frame_linear_extrap(a, b) -> np.ndarray:
+        a, b = _f64(a), _f64(b)
+        out = np.zeros(12)
+        _check(lib().asc_frame_linear_extrap(_ptr(a, _d), _ptr(b, _d), _ptr(out, _d)))
         return out
 
     @staticmethod
@@ -366,6 +406,118 @@ class optimizers:  # namespace: the reference's optimisers on any cost function
         it = C.c_int()
         _check(lib().asc_simplex(C.cast(cb, _vp), user, _ptr(P, _d), _ptr(Y, _d), ndim, ftol, C.byref(it)))
         return P, Y, it.value
+
+
+def optimize_options(method=0, max_iter=100, max_stall=25, min_limit=-3.0, max_limit=3.0, offsets=(0.1,) * 6,
+                     seed=1, particles=100, synchronous=False, volume=0) -> "AscOptimizeOptions":
+    o = AscOptimizeOptions()
+    o.method, o.max_iter, o.max_stall = method, max_iter, max_stall
+    o.min_limit, o.max_limit = min_limit, max_limit
+    for k in range(6):
+        o.offsets[k] = offsets[k]
+    o.seed, o.particles, o.synchronous, o.volume = seed, particles, int(synchronous), volume
+    return o
+
+
+def optimize_frame_fn(cost, store, pivot, **options):
+    """Tracker::optimize for one frame on a Python cost of manipulator
+    offsets: cost(manips (n, 6), store) -> n costs, where `store` is the
+    current float pose store (updated in place by the driver).
+    Returns (final store, AscOptimizeStats)."""
+    s = _f32(store).copy()
+    pv = _f64(pivot)
+
+    def cb(x, n, out, _user):
+        try:
+            pts = np.ctypeslib.as_array(x, (n * 6,)).reshape(n, 6).copy()
+            np.ctypeslib.as_array(out, (n,))[:] = np.asarray(cost(pts, s.copy()), np.float64).reshape(n)
+            return 0
+        except Exception:  # noqa: BLE001 - reported to the driver as a status
+            return ERR_INVALID
+
+    fn = COST_FN(cb)
+    st = AscOptimizeStats()
+    o = optimize_options(**options)
+    _check(lib().asc_optimize_frame_fn(C.cast(fn, _vp), None, _ptr(s, _f), _ptr(pv, _d), C.byref(o), C.byref(st)))
+    return s, st
+
+
+class Server:
+    """Registration server (asc_server_*): continuous batching of tracking
+    jobs on one GPU, admitted by a device-memory budget."""
+
+    def __init__(self, ctx: "Context", budget_bytes: int, max_active_jobs: int = 8):
+        h = C.c_void_p()
+        _check(lib().asc_server_create(ctx.handle, budget_bytes, max_active_jobs, C.byref(h)))
+        self.handle = h.value
+        self._jobs = {}
+
+    def submit(self, volumes, voxel_mm, initial_poses, cameras, frames, render, drr_filters=(), rad_filters=(),
+               drr=(0.5, 1.0, 0.0), storage=STORAGE_U16, guess=1, **options) -> int:
+        """volumes: list of (D, H, W) u16 arrays; voxel_mm: per volume (float
+        or 3-tuple); initial_poses: (n_volumes, 6); cameras: AscCamera list;
+        frames: list over frames of per-camera 2-D arrays (u8, u16 or f32);
+        render: (width, height); options: optimize_options keywords."""
+        vols = [np.ascontiguousarray(v, np.uint16) for v in volumes]
+        dims = np.array([[v.shape[2], v.shape[1], v.shape[0]] for v in vols], np.int32)
+        vox = np.array([[x] * 3 if np.isscalar(x) else list(x) for x in voxel_mm], np.float64)
+        init = np.ascontiguousarray(initial_poses, np.float64).reshape(len(vols), 6)
+        cams = (AscCamera * len(cameras))(*cameras)
+        first = np.asarray(frames[0][0])
+        dt = {np.dtype(np.uint8): (np.uint8, 8), np.dtype(np.uint16): (np.uint16, 16),
+              np.dtype(np.float32): (np.float32, 32)}[first.dtype]
+        fr = [np.ascontiguousarray(im, dt[0]) for frame in frames for im in frame]
+        fdims = np.array([[np.asarray(im).shape[1], np.asarray(im).shape[0]] for im in frames[0]], np.int32)
+        dfl = (AscFilter * max(len(drr_filters), 1))(*drr_filters)
+        rfl = (AscFilter * max(len(rad_filters), 1))(*rad_filters)
+        d = AscJobDesc()
+        d.n_volumes = len(vols)
+        d.volumes = (C.c_void_p * len(vols))(*[v.ctypes.data for v in vols])
+        d.volume_dims = _ptr(dims, _i)
+        d.voxel_mm = _ptr(vox, _d)
+        d.initial_poses = _ptr(init, _d)
+        d.storage = storage
+        d.drr_step, d.drr_intensity, d.drr_cutoff = drr
+        d.n_cameras = len(cameras)
+        d.cameras = cams
+        d.render_width, d.render_height = render
+        d.drr_filters, d.n_drr_filters = dfl, len(drr_filters)
+        d.rad_filters, d.n_rad_filters = rfl, len(rad_filters)
+        d.n_frames = len(frames)
+        d.frames = (C.c_void_p * len(fr))(*[f.ctypes.data for f in fr])
+        d.frame_dims = _ptr(fdims, _i)
+        d.frame_bps = dt[1]
+        d.options = optimize_options(**options)
+        d.guess = guess
+        jid = C.c_int()
+        _check(lib().asc_server_submit(self.handle, C.byref(d), C.byref(jid)))
+        self._jobs[jid.value] = (len(frames), len(vols))
+        return jid.value
+
+    def wait(self, job_id: int):
+        """-> (poses (frames, volumes, 6), per-frame latency ms, AscJobStats)"""
+        nf, nv = self._jobs.pop(job_id)
+        poses = np.zeros((nf, nv, 6))
+        lat = np.zeros(nf)
+        st = AscJobStats()
+        _check(lib().asc_server_wait(self.handle, job_id, _ptr(poses, _d), _ptr(lat, _d), C.byref(st)))
+        return poses, lat, st
+
+    def stats(self) -> dict:
+        st = AscServeStats()
+        _check(lib().asc_server_stats(self.handle, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in AscServeStats._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().asc_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
 
 
 class filters:  # namespace: filter records

@@ -261,20 +261,15 @@ void apply_manip(const double* xyzypr_in, const Frame& pivot, const Frame& manip
 
 }  // namespace
 
-extern "C" ASC_API int asc_scene_volume_pivot(asc_scene scene, int volume, double pivot[12]);
-
-extern "C" ASC_API int asc_optimize_frame(asc_scene scene, float store[7], const asc_optimize_options* opt,
-                                          asc_optimize_stats* stats)
+// Tracker::optimize for one frame on any cost of manipulator offsets
+// relative to `store` (which the cost reads when it is called, and which is
+// updated in place: after the PSO, Tracker.cpp:476-489, and at the end,
+// :563-570).
+static int optimize_frame_cb(asc_cost_fn fn, void* user, float* store, const Frame& pivot,
+                             const asc_optimize_options* opt, asc_optimize_stats* stats)
 {
-  if (!scene || !store || !opt) return asc::fail(ASC_ERR_INVALID, "bad argument");
-  if (opt->particles < 1) return asc::fail(ASC_ERR_INVALID, "particles must be >= 1");
-  double pv[12];
-  int rc = asc_scene_volume_pivot(scene, opt->volume, pv);
-  if (rc) return rc;
-  const Frame pivot = asc::frame_load(pv);
   GlibcRand rng(opt->seed);  // initializeRandom, PSO.cpp:131-149
-  SceneCost sc{scene, store, opt->volume};
-  Objective obj{scene_cost, &sc};
+  Objective obj{fn, user};
   double xyzypr[6];
   asc_pose_store_get(store, xyzypr);
   const double zero[6] = {0, 0, 0, 0, 0, 0};
@@ -327,6 +322,28 @@ extern "C" ASC_API int asc_optimize_frame(asc_scene scene, float store[7], const
   st.evaluations = obj.evals;
   if (stats) *stats = st;
   return ASC_OK;
+}
+
+extern "C" ASC_API int asc_scene_volume_pivot(asc_scene scene, int volume, double pivot[12]);
+
+extern "C" ASC_API int asc_optimize_frame(asc_scene scene, float store[7], const asc_optimize_options* opt,
+                                          asc_optimize_stats* stats)
+{
+  if (!scene || !store || !opt) return asc::fail(ASC_ERR_INVALID, "bad argument");
+  if (opt->particles < 1) return asc::fail(ASC_ERR_INVALID, "particles must be >= 1");
+  double pv[12];
+  int rc = asc_scene_volume_pivot(scene, opt->volume, pv);
+  if (rc) return rc;
+  SceneCost sc{scene, store, opt->volume};
+  return optimize_frame_cb(scene_cost, &sc, store, asc::frame_load(pv), opt, stats);
+}
+
+extern "C" ASC_API int asc_optimize_frame_fn(asc_cost_fn fn, void* user, float store[7], const double pivot[12],
+                                             const asc_optimize_options* opt, asc_optimize_stats* stats)
+{
+  if (!fn || !store || !pivot || !opt) return asc::fail(ASC_ERR_INVALID, "bad argument");
+  if (opt->particles < 1) return asc::fail(ASC_ERR_INVALID, "particles must be >= 1");
+  return optimize_frame_cb(fn, user, store, asc::frame_load(pivot), opt, stats);
 }
 
 extern "C" ASC_API int asc_pso(asc_cost_fn fn, void* user, unsigned seed, float min_limit, float max_limit,

@@ -593,6 +593,13 @@ ASC_API int asc_frame_inverse(const double a[12], double out[12])
   return ASC_OK;
 }
 
+ASC_API int asc_frame_linear_extrap(const double a[12], const double b[12], double out[12])
+{
+  ASC_CHECK(a && b && out, ASC_ERR_INVALID, "null pointer");
+  frame_store(frame_linear_extrap(frame_load(a), frame_load(b)), out);
+  return ASC_OK;
+}
+
 // Camera::loadMayaCam2 after parsing (Camera.cpp:300-362): undo the file's
 // y-axis convention, invert the extrinsics, then calculateViewport and
 // calculateImagePlane (Camera.cpp:424-484).

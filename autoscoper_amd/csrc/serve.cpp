@@ -1,0 +1,549 @@
+// Registration server: continuous batching of many tracking jobs on one GPU.
+//
+// A job is a trial: one or more bones (volumes), a camera set with its
+// render size and filter chains, a stack of radiograph frames, optimiser
+// options and a guess heuristic. Each bone of an admitted job runs the
+// reference's frame loop (Tracker::optimize per frame, Tracker.cpp:337-580,
+// with guess 1 = previous frame and guess 2 = linear extrapolation,
+// :380-427) in its own host thread; every cost evaluation the optimiser
+// asks for becomes a request to the batcher. The batcher thread is the only
+// one that touches the GPU: it admits queued jobs when their device memory
+// fits the budget, uploads their volumes and frames into one shared scene,
+// and evaluates the pending requests of all running bones in one fused
+// launch (asc_scene_eval_requests) per tick. A request's cost does not
+// depend on what else is in the launch, so every job's trajectory is the
+// same alone and in the mix.
+//
+// Per-frame latency of a job: from the moment the frame could start (job
+// arrival for the first frame, completion of the previous frame by all
+// bones for the next ones) to its completion by all bones.
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "geometry.hpp"
+
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+double ms_since(Clock::time_point t0, Clock::time_point t) { return std::chrono::duration<double, std::milli>(t - t0).count(); }
+
+struct Job;
+
+// One cost evaluation batch of one bone: n manipulator offsets relative to
+// the bone's current float pose store.
+struct Request {
+  Job* job;
+  int bone;
+  int frame;
+  const double* manips;
+  int n;
+  const float* store;
+  double* costs;
+  bool done = false;
+  int status = ASC_OK;
+};
+
+struct Job {
+  int id = 0;
+  // host copy of the description
+  int n_bones = 0;
+  std::vector<std::vector<uint16_t>> vol_data;
+  std::vector<std::array<int, 3>> vol_dims;
+  std::vector<std::array<double, 3>> vol_scale;
+  int storage = ASC_STORAGE_U16;
+  std::vector<std::array<double, 6>> init_pose;
+  float step = 0.5f, intensity = 1.0f, cutoff = 0.0f;
+  std::vector<asc_camera> cams;
+  int RW = 0, RH = 0;
+  std::vector<asc_filter> drr_chain, rad_chain;
+  int n_frames = 0;
+  std::vector<std::vector<unsigned char>> frames;  // [f * ncam + c]
+  std::vector<std::array<int, 2>> frame_dims;      // per camera
+  int bps = 16;
+  asc_optimize_options opt{};
+  int guess = 1;
+  size_t device_bytes = 0;
+  // timing
+  Clock::time_point arrival;
+  double admit_ms = -1.0;
+  // device state (batcher thread only)
+  std::vector<asc_volume> dvols;
+  std::vector<int> svols, scams;
+  int set = -1;
+  std::vector<asc_image> images;
+  std::vector<int> sframes;
+  std::vector<std::array<double, 12>> pivots;
+  // results
+  std::vector<double> poses;                     // [f][b][6]
+  std::vector<std::vector<double>> bone_done_ms;  // [b][f], since arrival
+  std::vector<double> frame_latency_ms;          // [f]
+  std::vector<std::thread> threads;
+  int bones_left = 0;
+  int64_t evaluations = 0;
+  int status = ASC_OK;
+  bool admitted = false, finished = false, released = false;
+};
+
+}  // namespace
+
+struct asc_server_s {
+  asc_context ctx;
+  asc_scene scene = nullptr;
+  size_t budget = 0, used = 0;
+  int max_active = 1, active = 0;
+  std::mutex mu;
+  std::condition_variable cv_batcher, cv_req, cv_done;
+  std::deque<Job*> waiting;                  // submitted, not admitted
+  std::map<int, std::unique_ptr<Job>> jobs;  // all not yet collected by asc_server_wait
+  std::vector<Request*> pending;
+  int running_bones = 0;
+  int next_id = 1;
+  bool stop = false;
+  std::thread batcher;
+  Clock::time_point t0;
+  // statistics
+  int64_t batches = 0, requests = 0, evaluations = 0, frames_done = 0, jobs_done = 0;
+  double busy_ms = 0.0;
+  std::vector<double> latencies_ms;
+};
+
+namespace {
+
+int bone_cost(const double* manips, int n, double* costs, void* user);
+
+struct BoneCtx {
+  asc_server s;
+  Job* job;
+  int bone;
+  int frame;
+  float* store;
+};
+
+int bone_cost(const double* manips, int n, double* costs, void* user)
+{
+  BoneCtx* b = (BoneCtx*)user;
+  asc_server s = b->s;
+  Request r;
+  r.job = b->job;
+  r.bone = b->bone;
+  r.frame = b->frame;
+  r.manips = manips;
+  r.n = n;
+  r.store = b->store;
+  r.costs = costs;
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->pending.push_back(&r);
+  s->cv_batcher.notify_one();
+  s->cv_req.wait(lk, [&] { return r.done; });
+  return r.status;
+}
+
+// Tracker::optimize's initial guess for frame f (Tracker.cpp:380-427).
+void initial_guess(const Job& j, int b, int f, double* out)
+{
+  const int B = j.n_bones;
+  auto pose = [&](int fr) { return &j.poses[((size_t)fr * B + b) * 6]; };
+  if (f == 0) {
+    std::memcpy(out, j.init_pose[b].data(), 6 * sizeof(double));
+  } else if (j.guess == 2 && f >= 2) {
+    // float curves hold the stored poses; extrapolate frames f-2, f-1
+    float s1[7], s2[7];
+    double p1[6], p2[6];
+    asc_pose_store_set(s1, pose(f - 2));
+    asc_pose_store_get(s1, p1);
+    asc_pose_store_set(s2, pose(f - 1));
+    asc_pose_store_get(s2, p2);
+    const asc::Frame x = asc::frame_linear_extrap(asc::frame_from_xyzypr(p1), asc::frame_from_xyzypr(p2));
+    asc::frame_to_xyzypr(x, out);
+  } else {
+    std::memcpy(out, pose(f - 1), 6 * sizeof(double));
+  }
+}
+
+void bone_main(asc_server s, Job* j, int b)
+{
+  const int F = j->n_frames, B = j->n_bones;
+  int status = ASC_OK;
+  for (int f = 0; f < F && status == ASC_OK; f++) {
+    double guess[6];
+    initial_guess(*j, b, f, guess);
+    float store[7];
+    asc_pose_store_set(store, guess);
+    BoneCtx ctx{s, j, b, f, store};
+    asc_optimize_options opt = j->opt;
+    asc_optimize_stats st;
+    status = asc_optimize_frame_fn(bone_cost, &ctx, store, j->pivots[b].data(), &opt, &st);
+    double result[6];
+    asc_pose_store_get(store, result);
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::memcpy(&j->poses[((size_t)f * B + b) * 6], result, sizeof result);
+    j->bone_done_ms[b][f] = ms_since(j->arrival, Clock::now());
+    j->evaluations += st.evaluations;
+    if (status != ASC_OK) j->status = status;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->running_bones--;
+  j->bones_left--;
+  s->cv_batcher.notify_one();
+}
+
+// Device upload of an admitted job into the shared scene (batcher thread).
+int upload_job(asc_server s, Job& j)
+{
+  int rc;
+  for (int b = 0; b < j.n_bones; b++) {
+    asc_volume v = nullptr;
+    asc_volume_desc d;
+    const int flip[3] = {0, 0, 0};
+    const std::array<int, 3>& dm = j.vol_dims[b];
+    if ((rc = asc_volume_create(s->ctx, j.vol_data[b].data(), dm[0], dm[1], dm[2], 16, flip, j.vol_scale[b].data(),
+                                j.storage, &v, &d)))
+      return rc;
+    j.dvols.push_back(v);
+    // Tracker.cpp:281-290: the pivot is the volume centre
+    const double ypr0[6] = {d.trans_center[0], d.trans_center[1], d.trans_center[2], 0, 0, 0};
+    std::array<double, 12> pv;
+    asc_frame_from_xyzypr(ypr0, pv.data());
+    j.pivots.push_back(pv);
+    int idx;
+    if ((rc = asc_scene_add_volume(s->scene, v, j.step, j.intensity, j.cutoff, pv.data(), &idx))) return rc;
+    j.svols.push_back(idx);
+  }
+  const int C = (int)j.cams.size();
+  for (int c = 0; c < C; c++) {
+    int idx;
+    if ((rc = asc_scene_add_camera(s->scene, &j.cams[c], nullptr, &idx))) return rc;
+    j.scams.push_back(idx);
+    if ((rc = asc_scene_set_filters(s->scene, idx, 0, j.drr_chain.data(), (int)j.drr_chain.size()))) return rc;
+    if ((rc = asc_scene_set_filters(s->scene, idx, 1, j.rad_chain.data(), (int)j.rad_chain.size()))) return rc;
+  }
+  if ((rc = asc_scene_add_camera_set(s->scene, j.scams.data(), C, j.RW, j.RH, &j.set))) return rc;
+  for (int f = 0; f < j.n_frames; f++) {
+    std::vector<asc_image> imgs(C);
+    for (int c = 0; c < C; c++) {
+      if ((rc = asc_image_create(s->ctx, j.frames[(size_t)f * C + c].data(), j.frame_dims[c][0], j.frame_dims[c][1],
+                                 j.bps, &imgs[c])))
+        return rc;
+      j.images.push_back(imgs[c]);
+    }
+    int fid;
+    if ((rc = asc_scene_add_frame(s->scene, j.set, imgs.data(), &fid))) return rc;
+    j.sframes.push_back(fid);
+  }
+  return ASC_OK;
+}
+
+void release_job(asc_server s, Job& j)
+{
+  for (int f : j.sframes) asc_scene_remove_frame(s->scene, f);
+  if (j.set >= 0) asc_scene_remove_camera_set(s->scene, j.set);
+  for (int c : j.scams) asc_scene_remove_camera(s->scene, c);
+  for (int v : j.svols) asc_scene_remove_volume(s->scene, v);
+  for (asc_image im : j.images) asc_image_destroy(im);
+  for (asc_volume v : j.dvols) asc_volume_destroy(v);
+  j.sframes.clear();
+  j.scams.clear();
+  j.svols.clear();
+  j.images.clear();
+  j.dvols.clear();
+  j.set = -1;
+  j.released = true;
+}
+
+// Called with the lock held: admit waiting jobs that fit.
+void admit(asc_server s, std::unique_lock<std::mutex>& lk)
+{
+  while (!s->waiting.empty() && s->active < s->max_active && s->used + s->waiting.front()->device_bytes <= s->budget) {
+    Job* j = s->waiting.front();
+    s->waiting.pop_front();
+    s->active++;
+    s->used += j->device_bytes;
+    j->admit_ms = ms_since(j->arrival, Clock::now());
+    lk.unlock();
+    const int rc = upload_job(s, *j);
+    lk.lock();
+    j->admitted = true;
+    if (rc != ASC_OK) {
+      j->status = rc;
+      j->bones_left = 0;
+      continue;
+    }
+    j->bones_left = j->n_bones;
+    s->running_bones += j->n_bones;
+    for (int b = 0; b < j->n_bones; b++) j->threads.emplace_back(bone_main, s, j, b);
+  }
+}
+
+// Called with the lock held: finish jobs whose bones are all done.
+void reap(asc_server s, std::unique_lock<std::mutex>& lk)
+{
+  for (auto& kv : s->jobs) {
+    Job& j = *kv.second;
+    if (!j.admitted || j.finished || j.bones_left > 0) continue;
+    std::vector<std::thread> th;
+    th.swap(j.threads);
+    lk.unlock();
+    for (std::thread& t : th) t.join();
+    release_job(s, j);
+    lk.lock();
+    // frame latency: completion by all bones, from when the frame could start
+    double prev = 0.0;
+    for (int f = 0; f < j.n_frames; f++) {
+      double done = 0.0;
+      for (int b = 0; b < j.n_bones; b++) done = std::max(done, j.bone_done_ms[b][f]);
+      j.frame_latency_ms[f] = done - prev;
+      prev = done;
+      if (j.status == ASC_OK) s->latencies_ms.push_back(j.frame_latency_ms[f]);
+    }
+    s->frames_done += j.status == ASC_OK ? j.n_frames : 0;
+    s->jobs_done++;
+    s->active--;
+    s->used -= j.device_bytes;
+    j.finished = true;
+    s->cv_done.notify_all();
+  }
+}
+
+void batcher_main(asc_server s)
+{
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    reap(s, lk);
+    admit(s, lk);
+    // a tick: wait until every running bone has a request pending (they do
+    // between two optimiser steps only host work), or briefly for stragglers
+    auto ready = [&] {
+      return s->stop || (!s->pending.empty() && (int)s->pending.size() >= s->running_bones) ||
+             (!s->waiting.empty() && s->active < s->max_active &&
+              s->used + s->waiting.front()->device_bytes <= s->budget) ||
+             std::any_of(s->jobs.begin(), s->jobs.end(), [](const auto& kv) {
+               return kv.second->admitted && !kv.second->finished && kv.second->bones_left == 0;
+             });
+    };
+    if (s->pending.empty())
+      s->cv_batcher.wait(lk, ready);
+    else
+      s->cv_batcher.wait_for(lk, std::chrono::microseconds(500), ready);
+    if (s->stop && s->pending.empty() && s->running_bones == 0) break;
+    if (s->pending.empty()) continue;
+    std::vector<Request*> reqs;
+    reqs.swap(s->pending);
+    lk.unlock();
+    // one fused launch for every pending request
+    size_t n = 0;
+    for (Request* r : reqs) n += r->n;
+    std::vector<int> vols(n), sets(n), frames(n);
+    std::vector<double> poses(n * 6), costs(n);
+    size_t k = 0;
+    for (Request* r : reqs) {
+      Job& j = *r->job;
+      for (int i = 0; i < r->n; i++, k++) {
+        vols[k] = j.svols[r->bone];
+        sets[k] = j.set;
+        frames[k] = j.sframes[r->frame];
+        asc_compose_pose(r->store, j.pivots[r->bone].data(), r->manips + 6 * i, &poses[k * 6]);
+      }
+    }
+    const Clock::time_point t = Clock::now();
+    const int rc = asc_scene_eval_requests(s->scene, (int)n, vols.data(), sets.data(), frames.data(), poses.data(),
+                                           costs.data(), nullptr);
+    const double busy = ms_since(t, Clock::now());
+    lk.lock();
+    s->batches++;
+    s->requests += (int64_t)reqs.size();
+    s->evaluations += (int64_t)n;
+    s->busy_ms += busy;
+    k = 0;
+    for (Request* r : reqs) {
+      for (int i = 0; i < r->n; i++, k++) r->costs[i] = costs[k];
+      r->status = rc;
+      r->done = true;
+    }
+    s->cv_req.notify_all();
+  }
+}
+
+template <class T>
+std::vector<unsigned char> bytes_of(const T* p, size_t n)
+{
+  return std::vector<unsigned char>((const unsigned char*)p, (const unsigned char*)p + n * sizeof(T));
+}
+
+}  // namespace
+
+extern "C" {
+
+ASC_API int asc_server_create(asc_context ctx, size_t hbm_budget, int max_active_jobs, asc_server* out)
+{
+  if (!ctx || !out || max_active_jobs < 1) return asc::fail(ASC_ERR_INVALID, "bad argument");
+  asc_server s = new asc_server_s();
+  s->ctx = ctx;
+  s->budget = hbm_budget;
+  s->max_active = max_active_jobs;
+  s->t0 = Clock::now();
+  int rc = asc_scene_create(ctx, 1, 1, &s->scene);
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  s->batcher = std::thread(batcher_main, s);
+  *out = s;
+  return ASC_OK;
+}
+
+ASC_API int asc_server_submit(asc_server s, const asc_job_desc* d, int* job_id)
+{
+  if (!s || !d || !job_id) return asc::fail(ASC_ERR_INVALID, "bad argument");
+  if (d->n_volumes < 1 || d->n_cameras < 1 || d->n_cameras > 8 || d->n_frames < 1)
+    return asc::fail(ASC_ERR_INVALID, "a job needs 1+ volumes, 1 to 8 cameras and 1+ frames");
+  if (!d->volumes || !d->volume_dims || !d->voxel_mm || !d->initial_poses || !d->cameras || !d->frames ||
+      !d->frame_dims)
+    return asc::fail(ASC_ERR_INVALID, "null job field");
+  if (d->frame_bps != 8 && d->frame_bps != 16 && d->frame_bps != 32)
+    return asc::fail(ASC_ERR_UNSUPPORTED, "frame bit depth must be 8, 16 or 32");
+  if (d->render_width < 1 || d->render_height < 1) return asc::fail(ASC_ERR_INVALID, "bad render size");
+  if (d->options.particles < 1) return asc::fail(ASC_ERR_INVALID, "particles must be >= 1");
+  std::unique_ptr<Job> j(new Job());
+  j->n_bones = d->n_volumes;
+  size_t dev = 0;
+  for (int b = 0; b < d->n_volumes; b++) {
+    const int* dm = d->volume_dims + 3 * b;
+    if (dm[0] < 1 || dm[1] < 1 || dm[2] < 1) return asc::fail(ASC_ERR_INVALID, "bad volume shape");
+    const size_t nv = (size_t)dm[0] * dm[1] * dm[2];
+    j->vol_data.emplace_back((const uint16_t*)d->volumes[b], (const uint16_t*)d->volumes[b] + nv);
+    j->vol_dims.push_back({dm[0], dm[1], dm[2]});
+    j->vol_scale.push_back({d->voxel_mm[3 * b], d->voxel_mm[3 * b + 1], d->voxel_mm[3 * b + 2]});
+    std::array<double, 6> p;
+    std::memcpy(p.data(), d->initial_poses + 6 * b, sizeof(double) * 6);
+    j->init_pose.push_back(p);
+    // linear copy plus the 8-corner copy of the march layout
+    const size_t tex = d->storage == ASC_STORAGE_F32 ? 4 : 2;
+    dev += (size_t)(dm[0] + 3) * (dm[1] + 3) * (dm[2] + 3) * tex * 9;
+  }
+  j->storage = d->storage;
+  j->step = d->drr_step;
+  j->intensity = d->drr_intensity;
+  j->cutoff = d->drr_cutoff;
+  j->cams.assign(d->cameras, d->cameras + d->n_cameras);
+  j->RW = d->render_width;
+  j->RH = d->render_height;
+  if (d->n_drr_filters > 0) j->drr_chain.assign(d->drr_filters, d->drr_filters + d->n_drr_filters);
+  if (d->n_rad_filters > 0) j->rad_chain.assign(d->rad_filters, d->rad_filters + d->n_rad_filters);
+  j->n_frames = d->n_frames;
+  j->bps = d->frame_bps;
+  for (int c = 0; c < d->n_cameras; c++) j->frame_dims.push_back({d->frame_dims[2 * c], d->frame_dims[2 * c + 1]});
+  for (int f = 0; f < d->n_frames; f++)
+    for (int c = 0; c < d->n_cameras; c++) {
+      const size_t n = (size_t)j->frame_dims[c][0] * j->frame_dims[c][1];
+      const void* src = d->frames[(size_t)f * d->n_cameras + c];
+      if (!src) return asc::fail(ASC_ERR_INVALID, "null frame");
+      j->frames.push_back(d->frame_bps == 8    ? bytes_of((const uint8_t*)src, n)
+                          : d->frame_bps == 16 ? bytes_of((const uint16_t*)src, n)
+                                               : bytes_of((const float*)src, n));
+      dev += n * 4;
+    }
+  // the job's share of the batch scratch: one PSO epoch of DRR images
+  dev += (size_t)d->options.particles * d->n_volumes * d->n_cameras * d->render_width * d->render_height * 4;
+  j->device_bytes = dev;
+  j->opt = d->options;
+  j->guess = d->guess;
+  j->poses.assign((size_t)d->n_frames * d->n_volumes * 6, 0.0);
+  j->bone_done_ms.assign(d->n_volumes, std::vector<double>(d->n_frames, 0.0));
+  j->frame_latency_ms.assign(d->n_frames, 0.0);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (dev > s->budget) return asc::fail(ASC_ERR_NOMEM, "job does not fit the server's device-memory budget");
+  j->id = s->next_id++;
+  j->arrival = Clock::now();
+  *job_id = j->id;
+  s->waiting.push_back(j.get());
+  s->jobs[j->id] = std::move(j);
+  s->cv_batcher.notify_one();
+  return ASC_OK;
+}
+
+ASC_API int asc_server_wait(asc_server s, int job_id, double* poses, double* frame_latency_ms, asc_job_stats* stats)
+{
+  if (!s) return asc::fail(ASC_ERR_INVALID, "null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(job_id);
+  if (it == s->jobs.end()) return asc::fail(ASC_ERR_INVALID, "unknown job");
+  Job* j = it->second.get();
+  s->cv_done.wait(lk, [&] { return j->finished; });
+  const int status = j->status;
+  if (poses) std::memcpy(poses, j->poses.data(), j->poses.size() * sizeof(double));
+  if (frame_latency_ms) std::memcpy(frame_latency_ms, j->frame_latency_ms.data(), j->n_frames * sizeof(double));
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->queue_ms = j->admit_ms;
+    double last = 0.0;
+    for (const auto& bd : j->bone_done_ms) last = std::max(last, bd.back());
+    stats->total_ms = last;
+    stats->evaluations = j->evaluations;
+    stats->frames = j->n_frames;
+  }
+  s->jobs.erase(it);
+  if (status != ASC_OK) return asc::fail(status, "job failed");
+  return ASC_OK;
+}
+
+ASC_API int asc_server_stats(asc_server s, asc_serve_stats* out)
+{
+  if (!s || !out) return asc::fail(ASC_ERR_INVALID, "bad argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::memset(out, 0, sizeof *out);
+  out->jobs_done = s->jobs_done;
+  out->frames_done = s->frames_done;
+  out->batches = s->batches;
+  out->requests = s->requests;
+  out->evaluations = s->evaluations;
+  out->busy_ms = s->busy_ms;
+  out->elapsed_ms = ms_since(s->t0, Clock::now());
+  std::vector<double> l = s->latencies_ms;
+  std::sort(l.begin(), l.end());
+  auto q = [&](double p) { return l.empty() ? 0.0 : l[std::min(l.size() - 1, (size_t)(p * (l.size() - 1) + 0.5))]; };
+  out->latency_p50_ms = q(0.50);
+  out->latency_p99_ms = q(0.99);
+  out->latency_max_ms = l.empty() ? 0.0 : l.back();
+  out->active_jobs = s->active;
+  out->waiting_jobs = (int)s->waiting.size();
+  out->device_bytes_in_use = s->used;
+  return ASC_OK;
+}
+
+ASC_API int asc_server_destroy(asc_server s)
+{
+  if (!s) return ASC_OK;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    // jobs still queued are dropped; running ones finish first
+    for (Job* j : s->waiting) {
+      j->admitted = true;
+      j->finished = true;
+      j->status = ASC_ERR_INVALID;
+    }
+    s->waiting.clear();
+    s->cv_done.wait(lk, [&] {
+      for (auto& kv : s->jobs)
+        if (kv.second->admitted && !kv.second->finished) return false;
+      return true;
+    });
+    s->stop = true;
+    s->cv_batcher.notify_one();
+  }
+  s->batcher.join();
+  asc_scene_destroy(s->scene);
+  delete s;
+  return ASC_OK;
+}
+
+}  // extern "C"

@@ -41,6 +41,7 @@ typedef struct asc_context_s* asc_context;
 typedef struct asc_volume_s* asc_volume;
 typedef struct asc_image_s* asc_image;
 typedef struct asc_scene_s* asc_scene;
+typedef struct asc_server_s* asc_server;
 
 /* Volume storage on the device. U16 is exact (the reference uploads
  * CL_UNORM_INT16, VolumeDescription.cpp:287-303); F16/BF16 store the same
@@ -150,6 +151,8 @@ ASC_API int asc_frame_from_xyz_axis_angle(const double p[6], double frame[12]); 
 ASC_API int asc_frame_to_xyzypr(const double frame[12], double xyzypr[6]);        /* CoordFrame.cpp:177-189 */
 ASC_API int asc_frame_mul(const double a[12], const double b[12], double out[12]); /* CoordFrame.cpp:478-527 */
 ASC_API int asc_frame_inverse(const double a[12], double out[12]);                 /* CoordFrame.cpp:335-353 */
+/* a.linear_extrap(b): Tracker::optimize's guess 2 (CoordFrame.cpp:419-476). */
+ASC_API int asc_frame_linear_extrap(const double a[12], const double b[12], double out[12]);
 /* Camera::loadMayaCam2 arithmetic (Camera.cpp:229-362) from the parsed K, R
  * (file rows, row-major 3x3) and t, plus image size. */
 ASC_API int asc_camera_from_mayacam2(const double K[9], const double R[9], const double t[3], const double size[2],
@@ -364,6 +367,75 @@ ASC_API int asc_pso(asc_cost_fn fn, void* user, unsigned seed, float min_limit, 
  * ndim <= 20 as the reference's MAT. */
 ASC_API int asc_simplex(asc_cost_fn fn, void* user, double* P, double* Y, int ndim, double ftol,
                         int* iterations);
+
+/* Tracker::optimize for one frame (Tracker.cpp:337-580, guess handled by
+ * the caller) on any cost of manipulator offsets: fn(x, n, costs, user)
+ * evaluates n offsets relative to the pose in `store`, which is updated in
+ * place after the PSO and at the end, as asc_optimize_frame does it on a
+ * scene. pivot = Trial::getVolumeMatrix. opt->volume is not used. */
+ASC_API int asc_optimize_frame_fn(asc_cost_fn fn, void* user, float store[7], const double pivot[12],
+                                  const asc_optimize_options* opt, asc_optimize_stats* stats);
+
+/* ---- serving: continuous batching of tracking jobs ------------------------ */
+/* No counterpart in the reference (one trial per GUI process, one blocking
+ * frame at a time). A job is a trial: bones (16-bit volumes), a camera set
+ * with render size and filter chains, a stack of radiograph frames, and
+ * Tracker::optimize options; each bone runs the frame loop with the given
+ * guess (1: previous frame, 2: linear extrapolation of the two previous,
+ * Tracker.cpp:380-427; frame 0 starts at initial_poses). The server admits
+ * jobs first-come first-served while they fit the device-memory budget and
+ * the number of active jobs, and evaluates the pending cost requests of all
+ * running bones in one fused launch per tick. Results do not depend on what
+ * else runs. */
+typedef struct {
+  int n_volumes;
+  const void* const* volumes; /* n_volumes pointers to W*H*D u16 voxels, x fastest */
+  const int* volume_dims;     /* n_volumes x {W, H, D} */
+  const double* voxel_mm;     /* n_volumes x 3 */
+  const double* initial_poses; /* n_volumes x 6: xyzypr of frame 0 */
+  int storage;                /* ASC_STORAGE_* */
+  float drr_step, drr_intensity, drr_cutoff;
+  int n_cameras;              /* 1 .. 8 */
+  const asc_camera* cameras;
+  int render_width, render_height;
+  const asc_filter* drr_filters;
+  int n_drr_filters;
+  const asc_filter* rad_filters;
+  int n_rad_filters;
+  int n_frames;
+  const void* const* frames;  /* n_frames * n_cameras pointers, frame-major */
+  const int* frame_dims;      /* n_cameras x {width, height} */
+  int frame_bps;              /* 8, 16 or 32 */
+  asc_optimize_options options; /* options.volume is ignored */
+  int guess;
+} asc_job_desc;
+
+typedef struct {
+  double queue_ms;  /* arrival -> admission */
+  double total_ms;  /* arrival -> last frame done */
+  int64_t evaluations;
+  int frames;
+} asc_job_stats;
+
+typedef struct {
+  int64_t jobs_done, frames_done, batches, requests, evaluations;
+  double busy_ms;     /* time inside fused launches */
+  double elapsed_ms;  /* since the server started */
+  double latency_p50_ms, latency_p99_ms, latency_max_ms; /* per-frame latency of finished jobs */
+  int active_jobs, waiting_jobs;
+  size_t device_bytes_in_use;
+} asc_serve_stats;
+
+ASC_API int asc_server_create(asc_context ctx, size_t hbm_budget, int max_active_jobs, asc_server* out);
+/* Copies the job's host data; returns at once. */
+ASC_API int asc_server_submit(asc_server server, const asc_job_desc* job, int* job_id);
+/* Blocks until the job is done: poses n_frames x n_volumes x 6, per-frame
+ * latency n_frames (ms). A job can be waited for once. */
+ASC_API int asc_server_wait(asc_server server, int job_id, double* poses, double* frame_latency_ms,
+                            asc_job_stats* stats);
+ASC_API int asc_server_stats(asc_server server, asc_serve_stats* out);
+/* Drops queued jobs, lets running ones finish, then frees everything. */
+ASC_API int asc_server_destroy(asc_server server);
 
 #ifdef __cplusplus
 }

@@ -65,3 +65,28 @@ def test_simplex_matches_oracle(asc, name, ndim, ftol):
     Po, Yo, ito = O.amoeba_fn(name, CENTRE, P0, ftol)
     assert it == ito
     assert (bits(P) == bits(Po)).all() and (bits(Y) == bits(Yo)).all()
+
+
+@pytest.mark.parametrize("method,max_iter,particles,synchronous", [(1, 100, 100, False), (0, 3, 24, False),
+                                                                   (0, 3, 24, True)])
+def test_frame_driver_matches_oracle(asc, orc, tiny, method, max_iter, particles, synchronous):
+    """asc_optimize_frame_fn (the product's Tracker::optimize frame driver)
+    fed the oracle's own minimizationFunc reproduces the oracle's
+    optimize_frame bit for bit: PSO, store update, simplex, final pose."""
+    from autoscoper_amd import synth
+
+    wl = tiny["wl"]
+    recs = [(asc.filters.to_records(wl.drr_filters), asc.filters.to_records(wl.rad_filters)) for _ in tiny["cams"]]
+    S = orc.Scene(tiny["prep"], (synth.DRR_STEP, synth.DRR_INTENSITY, 0.0), tiny["cams"],
+                  [r[1] for r in tiny["rads"]], recs, tiny["render"])
+    pivot = orc.frame_from_xyzypr(list(tiny["prep"]["trans_center"]) + [0, 0, 0])
+    store0 = orc.pose_store_set(wl.gt_poses[0] + np.array([1.5, -1.2, 0.8, 2.0, -1.5, 1.0]))
+
+    def cost(manips, store):
+        return [S.min_func(store, pivot, m) for m in manips]
+
+    opts = dict(method=method, max_iter=max_iter, max_stall=3, seed=5)
+    s_prod, st = asc.optimize_frame_fn(cost, store0, pivot, particles=particles, synchronous=synchronous, **opts)
+    s_orc, ost, n = S.optimize_frame(store0, pivot, nparticles=particles, synchronous=synchronous, **opts)
+    assert st.evaluations == n
+    assert (bits(s_prod) == bits(s_orc)).all()

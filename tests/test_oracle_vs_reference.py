@@ -74,6 +74,7 @@ def test_frame_math_matches_reference(asc):
             continue
         x_ref = ref_call("ref_frame_linear_extrap", f_ref, a_ref)
         assert (bits(O.frame_linear_extrap(f_ref, a_ref)) == bits(x_ref)).all()
+        assert (bits(g.frame_linear_extrap(f_ref, a_ref)) == bits(x_ref)).all()
 
 
 def test_axis_angle_zero_rotation_matches_reference():
