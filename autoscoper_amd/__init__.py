@@ -71,6 +71,7 @@ class AscOptimizeOptions(C.Structure):
         ("method", C.c_int), ("max_iter", C.c_uint), ("max_stall", C.c_uint),
         ("min_limit", C.c_double), ("max_limit", C.c_double), ("offsets", C.c_double * 6),
         ("seed", C.c_uint), ("particles", C.c_int), ("synchronous", C.c_int), ("volume", C.c_int),
+        ("speculative", C.c_int),
     ]
 
 
@@ -409,13 +410,14 @@ class optimizers:  # namespace: the reference's optimisers on any cost function
 
 
 def optimize_options(method=0, max_iter=100, max_stall=25, min_limit=-3.0, max_limit=3.0, offsets=(0.1,) * 6,
-                     seed=1, particles=100, synchronous=False, volume=0) -> "AscOptimizeOptions":
+                     seed=1, particles=100, synchronous=False, volume=0, speculative=False) -> "AscOptimizeOptions":
     o = AscOptimizeOptions()
     o.method, o.max_iter, o.max_stall = method, max_iter, max_stall
     o.min_limit, o.max_limit = min_limit, max_limit
     for k in range(6):
         o.offsets[k] = offsets[k]
     o.seed, o.particles, o.synchronous, o.volume = seed, particles, int(synchronous), volume
+    o.speculative = int(speculative)
     return o
 
 

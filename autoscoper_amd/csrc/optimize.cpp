@@ -162,7 +162,13 @@ int pso(Objective& obj, GlibcRand& rng, float lo, float hi, unsigned max_epochs,
 
 // Nelder-Mead with the centroid-distance stop, DownhillSimplex.cpp:39-172.
 // P holds ndim+1 rows of ndim values; Y the costs of the rows.
-int amoeba(Objective& obj, std::vector<double>& P, std::vector<double>& Y, int ndim, double ftol)
+// speculative: evaluate each iteration's candidate points together (the
+// reflection, the expansion, the contraction of the worst vertex and of the
+// reflection) and a shrink's vertices together. Costs do not depend on how
+// points are grouped, so every decision, and the trajectory, is the one of
+// the one-at-a-time order; only the number of evaluations grows.
+int amoeba(Objective& obj, std::vector<double>& P, std::vector<double>& Y, int ndim, double ftol,
+           bool speculative = false)
 {
   const int itmax = 1000;
   const double alpha = 1.0, beta = 0.5, gamma = 2.0;
@@ -209,6 +215,57 @@ int amoeba(Objective& obj, std::vector<double>& P, std::vector<double>& Y, int n
     for (int j = 0; j < ndim; j++) {
       pbar[j] /= 1.0 * ndim;
       pr[j] = (1.0 + alpha) * pbar[j] - alpha * at(ihi, j);
+    }
+    if (speculative) {
+      // candidates: 0 reflection, 1 expansion, 2 contraction of the worst
+      // vertex, 3 contraction of the reflection (the worst vertex when the
+      // reflection replaced it)
+      std::vector<double> cand((size_t)4 * ndim), yc(4);
+      for (int j = 0; j < ndim; j++) {
+        cand[j] = pr[j];
+        cand[ndim + j] = gamma * pr[j] + (1.0 - gamma) * pbar[j];
+        cand[2 * ndim + j] = beta * at(ihi, j) + (1.0 - beta) * pbar[j];
+        cand[3 * ndim + j] = beta * pr[j] + (1.0 - beta) * pbar[j];
+      }
+      obj.eval(cand.data(), 4, yc.data());
+      if (obj.status) return -1;
+      const double ypr = yc[0];
+      if (ypr <= Y[ilo]) {
+        const int k = yc[1] < Y[ilo] ? 1 : 0;
+        for (int j = 0; j < ndim; j++) at(ihi, j) = cand[k * ndim + j];
+        Y[ihi] = yc[k];
+      } else if (ypr >= Y[inhi]) {
+        int k = 2;
+        if (ypr < Y[ihi]) {
+          for (int j = 0; j < ndim; j++) at(ihi, j) = pr[j];
+          Y[ihi] = ypr;
+          k = 3;
+        }
+        if (yc[k] < Y[ihi]) {
+          for (int j = 0; j < ndim; j++) at(ihi, j) = cand[k * ndim + j];
+          Y[ihi] = yc[k];
+        } else {
+          std::vector<double> pts((size_t)ndim * ndim), ys(ndim);
+          int m = 0;
+          for (int i = 0; i < np; i++)
+            if (i != ilo) {
+              for (int j = 0; j < ndim; j++) {
+                at(i, j) = 0.5 * (at(i, j) + at(ilo, j));
+                pts[(size_t)m * ndim + j] = at(i, j);
+              }
+              m++;
+            }
+          obj.eval(pts.data(), m, ys.data());
+          m = 0;
+          for (int i = 0; i < np; i++)
+            if (i != ilo) Y[i] = ys[m++];
+        }
+      } else {
+        for (int j = 0; j < ndim; j++) at(ihi, j) = pr[j];
+        Y[ihi] = ypr;
+      }
+      if (obj.status) return -1;
+      continue;
     }
     const double ypr = obj.one(pr.data());
     if (obj.status) return -1;
@@ -277,10 +334,14 @@ static int optimize_frame_cb(asc_cost_fn fn, void* user, float* store, const Fra
   if (obj.status) return obj.status;
   Frame manip = asc::frame_from_xyz_axis_angle(zero);
   std::vector<double> P(7 * 6), Y(7);
+  const bool spec = opt->speculative != 0;
   auto simplex_start = [&]() {
     for (int i = 0; i < 7; i++)
       for (int j = 0; j < 6; j++) P[(size_t)i * 6 + j] = (i == j + 1) ? opt->offsets[j] : 0.0;
-    for (int i = 0; i < 7; i++) Y[i] = obj.one(&P[(size_t)i * 6]);
+    if (spec)
+      obj.eval(P.data(), 7, Y.data());
+    else
+      for (int i = 0; i < 7; i++) Y[i] = obj.one(&P[(size_t)i * 6]);
   };
   asc_optimize_stats st;
   std::memset(&st, 0, sizeof st);
@@ -300,7 +361,7 @@ static int optimize_frame_cb(asc_cost_fn fn, void* user, float* store, const Fra
     st.pso_best_cost = g.ncc;
     simplex_start();
     if (obj.status) return obj.status;
-    const int it = amoeba(obj, P, Y, 6, 1e-3);
+    const int it = amoeba(obj, P, Y, 6, 1e-3, spec);
     if (obj.status) return obj.status;
     const double final_cost = obj.one(&P[0]);
     if (obj.status) return obj.status;
@@ -310,7 +371,7 @@ static int optimize_frame_cb(asc_cost_fn fn, void* user, float* store, const Fra
   } else {
     simplex_start();
     if (obj.status) return obj.status;
-    const int it = amoeba(obj, P, Y, 6, 1e-3);
+    const int it = amoeba(obj, P, Y, 6, 1e-3, spec);
     if (obj.status) return obj.status;
     st.simplex_iterations = it;
     st.final_cost = obj.one(&P[0]);

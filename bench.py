@@ -130,6 +130,118 @@ def cpu_baseline(wl, cams, rads, gt, budget_s: float):
     return n / el, n, el, threads
 
 
+SERVE_RATES = (1.0, 4.0, 16.0)  # job arrivals per second
+# trial templates of the serving stream: cameras, detector side, bones (phantom pool indices), frames
+SERVE_TEMPLATES = [
+    dict(ncam=1, det=512, bones=(0,), frames=8),
+    dict(ncam=2, det=1024, bones=(2,), frames=5),
+    dict(ncam=4, det=768, bones=(0, 1), frames=4),
+    dict(ncam=3, det=512, bones=(1,), frames=6),
+    dict(ncam=2, det=768, bones=(0, 1, 2), frames=4),
+    dict(ncam=2, det=512, bones=(1, 0), frames=6),
+]
+SERVE_OPTIONS = dict(method=0, max_iter=6, max_stall=3, synchronous=True, speculative=True)
+
+
+def serve_trials(A, synth, ctx):
+    """Seeded trials of mixed shapes (1-4 cameras, 512^2-1024^2 detectors,
+    1-3 bones from 64^3 to 256x256x320 u16) with rendered target frames."""
+    t0 = time.time()
+    pool = [((64, 64, 64), 0.5, "ellipsoid", -60.0), ((160, 128, 128), 0.4, "longbone", 0.0),
+            ((320, 256, 256), 0.3, "longbone", 70.0)]
+    vols = [synth.phantom(shape, vox, kind, seed=synth.SEED_PHANTOM + 20 * i, noise_seed=synth.SEED_NOISE + 20 * i)
+            for i, (shape, vox, kind, _) in enumerate(pool)]
+    dvols = [A.Volume(ctx, v, scale=(p[1],) * 3) for v, p in zip(vols, pool)]
+    trials = []
+    for ti, tp in enumerate(SERVE_TEMPLATES):
+        det = tp["det"]
+        specs = synth.biplane(sod_mm=800.0, size=(det, det), pixel_mm=307.2 / det, sid_mm=1100.0, n=tp["ncam"])
+        cams = [A.geometry.camera_from_mayacam2(*c.mayacam2()) for c in specs]
+        gts = []
+        for k, b in enumerate(tp["bones"]):
+            shape, vox, _, z = pool[b]
+            base = synth.volume_centre_pose(shape, vox) + np.array([0.0, 0.0, z, 0.0, 0.0, 0.0])
+            gts.append(synth.random_walk(base, tp["frames"], 0.5, 1.0, 5000 + 10 * ti + k))
+        gts = np.stack(gts, 1)  # frames x bones x 6
+        full, acc = ctx.alloc(4 * det * det), ctx.alloc(4 * det * det)
+        frames = []
+        for f in range(tp["frames"]):
+            per = []
+            for c, cam in enumerate(cams):
+                fvp = synth.full_frame_viewport(list(cam.viewport), det, det)
+                acc.upload(np.zeros((det, det), np.float32))
+                for k, b in enumerate(tp["bones"]):
+                    inview, imv, _, _ = A.geometry.view_geometry(cam, gts[f, k], dvols[b].desc, det, det)
+                    if inview:
+                        ctx.drr_render(dvols[b], imv, fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, full, det, det)
+                        ctx.merge(acc, full, acc, det, det)
+                per.append(synth.target_from_drr(acc.download((det, det)), 7000 + 100 * ti + 10 * f + c)[1])
+            frames.append(per)
+        full.free()
+        acc.free()
+        rng = np.random.default_rng(8000 + ti)
+        start = gts[0] + np.concatenate([rng.uniform(-2, 2, (len(tp["bones"]), 3)),
+                                         rng.uniform(-2, 2, (len(tp["bones"]), 3))], 1)
+        trials.append(dict(vols=[vols[b] for b in tp["bones"]], vox=[pool[b][1] for b in tp["bones"]], start=start,
+                           cams=cams, frames=frames, render=(det, det)))
+    for v in dvols:
+        v.close()
+    log(f"[serve] {len(trials)} trial templates ready in {time.time() - t0:.1f}s")
+    return trials
+
+
+def serve_bench(A, synth, ctx, trials, rates, arrival_s: float, seed: int = 9000):
+    """Poisson job arrivals at each rate for `arrival_s` seconds, served by
+    one server (continuous batching); frames/s and per-frame latency."""
+    import threading
+
+    dflt, rflt = synth.default_filters()
+    out = []
+    for ri, rate in enumerate(rates):
+        rng = np.random.default_rng(seed + ri)
+        t, arrivals = 0.0, []
+        while True:
+            t += rng.exponential(1.0 / rate)
+            if t > arrival_s:
+                break
+            arrivals.append(t)
+        jobs = [(a, int(rng.integers(len(trials))), int(rng.choice([50, 100, 200])), int(rng.integers(1 << 20)))
+                for a in arrivals]
+        srv = A.Server(ctx, 64 << 30, 16)
+        ids = [None] * len(jobs)
+        t0 = time.perf_counter()
+
+        def submitter():
+            for k, (a, ti, particles, s) in enumerate(jobs):
+                delay = t0 + a - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                tr = trials[ti]
+                ids[k] = srv.submit(tr["vols"], tr["vox"], tr["start"], tr["cams"], tr["frames"], tr["render"],
+                                    drr_filters=dflt, rad_filters=rflt,
+                                    drr=(synth.DRR_STEP, synth.DRR_INTENSITY, synth.DRR_CUTOFF), guess=1,
+                                    particles=particles, seed=s, **SERVE_OPTIONS)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        frames = 0
+        for jid in ids:
+            _, lat, st = srv.wait(jid)
+            frames += st.frames
+        el = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        res = {"rate_jobs_per_s": rate, "jobs": len(jobs), "frames": frames, "seconds": el,
+               "frames_per_s": frames / el, "latency_p50_ms": st["latency_p50_ms"],
+               "latency_p99_ms": st["latency_p99_ms"], "launches": st["batches"],
+               "requests_per_launch": st["requests"] / max(st["batches"], 1),
+               "gpu_busy_frac": st["busy_ms"] / (1e3 * el)}
+        log(f"[serve] {res}")
+        out.append(res)
+    return out
+
+
 def workload_name(args, wl, cams, P, multi) -> str:
     RW, RH = wl.render
     if multi:
@@ -156,6 +268,9 @@ def main(argv=None) -> int:
     ap.add_argument("--plain-evals", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=6.0,
+                    help="serving stream: seconds of Poisson job arrivals per rate (0 = skip)")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving stream")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run the device copy kernel over MIB MiB 5 times (counter calibration)")
     args = ap.parse_args(argv)
@@ -177,6 +292,12 @@ def main(argv=None) -> int:
             ctx.copy(dst, src, nbytes)
         ctx.synchronize()
         log(f"[bench] copied {nbytes} bytes x 5")
+        return 0
+    if args.serve_only:
+        trials = serve_trials(A, synth, ctx)
+        print(json.dumps({"serving": serve_bench(A, synth, ctx, trials, SERVE_RATES, args.serve_seconds)}),
+              flush=True)
+        ctx.close()
         return 0
     multi = args.workload in MULTI
     if multi:
@@ -256,7 +377,7 @@ def main(argv=None) -> int:
             traffic = json.load(fh)["traffic_bytes_per_launch"]
     tiles = info[:, 4].sum()
     out = {
-        "metric": "pose evals/sec (DRR+filters+NCC, 2 cams 1024^2)",
+        "metric": "pose evals/sec (DRR+filters+NCC, 2 cams 1024\u00b2); served frames/s + p99 latency",
         "value": evals_s,
         "unit": "pose-evals/s",
         "n_gpus": 1,
@@ -296,6 +417,19 @@ def main(argv=None) -> int:
         "per_eval": {"bbox_pixels": int((info[:, 1] * info[:, 2]).sum() / P),
                      "ray_samples": float(last_samples) / P, "tiles": int(tiles / P)},
     }
+    if args.serve_seconds > 0:
+        scene.close()
+        for v in vols:
+            v.close()
+        vols = []
+        trials = serve_trials(A, synth, ctx)
+        out["serving"] = {
+            "workload": "cfg5-style stream: Poisson job arrivals; jobs of 1-4 cameras (512^2-1024^2), 1-3 bones "
+                        "(64^3 .. 256x256x320 u16), 4-8 frames, PSO 50-200 particles (synchronous, <= 7 epochs) "
+                        "+ simplex per frame; targets rendered from hidden random-walk poses",
+            "per_frame_latency": "frame start (job arrival or previous frame done) to frame done",
+            "rates": serve_bench(A, synth, ctx, trials, SERVE_RATES, args.serve_seconds),
+        }
     print(json.dumps(out), flush=True)
     scene.close()
     for v in vols:

@@ -330,6 +330,10 @@ typedef struct {
   int synchronous;     /* 0: reference asynchronous PSO, one eval per call;
                           1: synchronous PSO, all particles of an epoch in one fused batch */
   int volume;          /* volume index (Trial::current_volume) */
+  int speculative;     /* 0: one simplex evaluation per call, in the reference's order;
+                          1: each simplex iteration evaluates its reflection, expansion and
+                          both contraction candidates together, and a shrink all its new
+                          vertices together: the same trajectory in fewer, larger batches */
 } asc_optimize_options;
 
 typedef struct {

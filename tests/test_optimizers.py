@@ -90,3 +90,32 @@ def test_frame_driver_matches_oracle(asc, orc, tiny, method, max_iter, particles
     s_orc, ost, n = S.optimize_frame(store0, pivot, nparticles=particles, synchronous=synchronous, **opts)
     assert st.evaluations == n
     assert (bits(s_prod) == bits(s_orc)).all()
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_speculative_simplex_keeps_the_trajectory(asc, orc, tiny, method):
+    """Evaluating each simplex iteration's candidates (and a shrink's
+    vertices) together gives the same final pose as the reference's
+    one-at-a-time order, with more evaluations."""
+    from autoscoper_amd import synth
+
+    wl = tiny["wl"]
+    recs = [(asc.filters.to_records(wl.drr_filters), asc.filters.to_records(wl.rad_filters)) for _ in tiny["cams"]]
+    S = orc.Scene(tiny["prep"], (synth.DRR_STEP, synth.DRR_INTENSITY, 0.0), tiny["cams"],
+                  [r[1] for r in tiny["rads"]], recs, tiny["render"])
+    pivot = orc.frame_from_xyzypr(list(tiny["prep"]["trans_center"]) + [0, 0, 0])
+    store0 = orc.pose_store_set(wl.gt_poses[0] + np.array([1.5, -1.2, 0.8, 2.0, -1.5, 1.0]))
+    calls = []
+
+    def cost(manips, store):
+        calls.append(len(manips))
+        return [S.min_func(store, pivot, m) for m in manips]
+
+    opts = dict(method=method, max_iter=3, max_stall=3, seed=5, particles=24, synchronous=True)
+    s_seq, st_seq = asc.optimize_frame_fn(cost, store0, pivot, **opts)
+    n_seq = len(calls)
+    s_spec, st_spec = asc.optimize_frame_fn(cost, store0, pivot, speculative=True, **opts)
+    n_spec = len(calls) - n_seq
+    assert (bits(s_spec) == bits(s_seq)).all()
+    assert st_spec.simplex_iterations == st_seq.simplex_iterations
+    assert st_spec.evaluations > st_seq.evaluations and n_spec < n_seq  # more points, fewer calls
