@@ -544,6 +544,7 @@ struct asc_scene_s {
   std::vector<SceneFrame> frames;
   bool dirty = true;
   int grid_persistent = 0;  // ray-march grid (blocks)
+  bool grid_user = false;   // grid_persistent set through asc_scene_set_tuning
   int grid_post = 0;        // post-processing grid (blocks)
   int post_small = 1;       // every chain's halo <= kSmallHalo: smaller-LDS post kernel
   // device
@@ -579,7 +580,7 @@ struct asc_scene_s {
   DevBuf<float> p_bufs;
   // per-kernel timing of the last eager batch (HIP events on the scene stream)
   bool timing = false;
-  int unroll = 2;  // ray samples per loop iteration (1 or 2)
+  int unroll = 4;  // ray samples per loop iteration (1, 2 or 4)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float last_ms[3] = {0, 0, 0};
 };
@@ -709,6 +710,7 @@ int issue_batch(asc_scene s, size_t batch_bytes)
   if (timed) ASC_HIP(hipEventRecord(s->ev[0], st));
   switch (s->unroll) {
     case 1: drr_tiles<1><<<s->grid_persistent, kBlock, 0, st>>>(a); break;
+    case 4: drr_tiles<4><<<s->grid_persistent, kBlock, 0, st>>>(a); break;
     default: drr_tiles<2><<<s->grid_persistent, kBlock, 0, st>>>(a); break;
   }
   if (timed) ASC_HIP(hipEventRecord(s->ev[1], st));
@@ -883,7 +885,12 @@ int eval_requests(asc_scene s, const std::vector<Request>& reqs, double* costs, 
     ASC_HIP(hipGetDeviceProperties(&prop, s->ctx->device));
     if (s->grid_persistent == 0) {
       int per_cu = 0;
-      ASC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, drr_tiles<1>, kBlock, 0));
+      if (s->unroll == 4)
+        ASC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, drr_tiles<4>, kBlock, 0));
+      else if (s->unroll == 2)
+        ASC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, drr_tiles<2>, kBlock, 0));
+      else
+        ASC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, drr_tiles<1>, kBlock, 0));
       s->grid_persistent = prop.multiProcessorCount * std::max(per_cu, 1);
     }
     if (s->grid_post == 0) {
@@ -1205,12 +1212,14 @@ ASC_API int asc_scene_set_tuning(asc_scene s, int key, int value)
   ASC_CHECK(s, ASC_ERR_INVALID, "null scene");
   switch (key) {
     case ASC_TUNE_UNROLL:
-      ASC_CHECK(value == 1 || value == 2, ASC_ERR_INVALID, "unroll must be 1 or 2");
+      ASC_CHECK(value == 1 || value == 2 || value == 4, ASC_ERR_INVALID, "unroll must be 1, 2 or 4");
       s->unroll = value;
+      if (!s->grid_user) s->grid_persistent = 0;  // re-derived from this variant's occupancy
       break;
     case ASC_TUNE_GRID:
       ASC_CHECK(value >= 0, ASC_ERR_INVALID, "grid must be >= 0 (0 = one wave of blocks per CU slot)");
       s->grid_persistent = value - value % 8;
+      s->grid_user = s->grid_persistent > 0;
       break;
     default: return fail(ASC_ERR_INVALID, "unknown tuning key");
   }

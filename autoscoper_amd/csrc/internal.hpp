@@ -44,6 +44,10 @@ int hip_fail(hipError_t e, const char* what);
 // block (8 texels), so a trilinear sample reads 4, 2 or 1 aligned vectors.
 // BRICK stores OCTET cells grouped in 2x2x2 bricks (pW and pS then count
 // the cells of one brick row and one brick layer; see layout_shape()).
+// pair_diff (u16 volumes in QUAD, OCTET or BRICK whose stored values are all
+// <= 32767): every (x0, x1) texel pair of an element is stored as x0 and the
+// signed 16-bit difference x1 - x0, which is exactly the integer the x-lerp
+// needs.
 struct VolDev {
   const void* data;
   int W, H, D;
@@ -53,6 +57,7 @@ struct VolDev {
   int layout;    // ASC_LAYOUT_*
   int esz;       // bytes per element of the layout
   int nbytes;    // bytes of the layout's array (the buffer's extent)
+  int pair_diff; // 1: x1 texels of the pairs hold x1 - x0 (u16 paired layouts)
 };
 
 struct ImgDev {
@@ -87,6 +92,7 @@ struct asc_volume_s {
   void* packed = nullptr;  // QUAD / OCTET copy when selected
   size_t packed_bytes = 0;
   int generation = 0;      // bumped whenever `dev` changes
+  int u16_max = 0;         // largest stored texel of a u16 volume
 };
 
 struct asc_image_s {
@@ -112,7 +118,7 @@ int launch_mask_multiply(asc_context ctx, float* drr, float* rad, const float* b
 int launch_composite(asc_context ctx, const float* s1, const float* s2, const float* s3, float* out, size_t n);
 int launch_fill_f32(asc_context ctx, float* dst, float v, size_t n);
 int launch_copy(asc_context ctx, void* dst, const void* src, size_t bytes);
-int launch_pack_volume(asc_context ctx, const VolDev& linear, void* dst, int layout);
+int launch_pack_volume(asc_context ctx, const VolDev& linear, void* dst, int layout, int pair_diff);
 int ncc_moments(asc_context ctx, const float* f, const float* g, const float* m, unsigned n, double* out9,
                 double* ncc);
 

@@ -206,22 +206,33 @@ __global__ __launch_bounds__(kBlock) void ncc_final_kernel(const double* __restr
   }
 }
 
+// The value stored for corner q of an element: with pair_diff (u16 only),
+// odd corners (x1) hold x1 - x0 as a signed 16-bit integer.
+template <typename T>
+__device__ __forceinline__ T pair_value(T v, T prev, int q, int diff)
+{
+  if (diff && (q & 1)) return (T)((int)v - (int)prev);
+  return v;
+}
+
 // QUAD / OCTET packing: element (i, j, k) of the padded grid receives the
 // texels (i+di, j+dj, k+dk), di, dj (, dk) in {0, 1}, in the order
 // gather8() expects; texels beyond the padded grid are 0.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void pack_kernel(const T* __restrict__ src, T* __restrict__ dst, int pW, int pH,
-                                                      int pD, int per)
+                                                      int pD, int per, int diff)
 {
   const size_t n = (size_t)pW * pH * pD;
   for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (size_t)gridDim.x * kBlock) {
     const int i = (int)(e % pW), j = (int)((e / pW) % pH), k = (int)(e / ((size_t)pW * pH));
+    T prev = T(0);
     for (int q = 0; q < per; q++) {
       const int di = q & 1, dj = (q >> 1) & 1, dk = (q >> 2) & 1;
       const int ii = i + di, jj = j + dj, kk = k + dk;
       T v = T(0);
       if (ii < pW && jj < pH && kk < pD) v = src[((size_t)kk * pH + jj) * pW + ii];
-      dst[e * per + q] = v;
+      dst[e * per + q] = pair_value(v, prev, q, diff);
+      prev = v;
     }
   }
 }
@@ -232,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void pack_kernel(const T* __restrict__ src,
 // texels outside the padded grid are 0.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void pack_brick_kernel(const T* __restrict__ src, T* __restrict__ dst, int pW,
-                                                            int pH, int pD, int BX, int BY, int BZ)
+                                                            int pH, int pD, int BX, int BY, int BZ, int diff)
 {
   const size_t n = (size_t)BX * BY * BZ * 8;
   for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (size_t)gridDim.x * kBlock) {
@@ -240,12 +251,14 @@ __global__ __launch_bounds__(kBlock) void pack_brick_kernel(const T* __restrict_
     const int c = (int)(e & 7);
     const int bx = (int)(b % BX), by = (int)((b / BX) % BY), bz = (int)(b / ((size_t)BX * BY));
     const int ci = 2 * bx + (c & 1) - 2, cj = 2 * by + ((c >> 1) & 1) - 2, ck = 2 * bz + (c >> 2) - 2;
+    T prev = T(0);
     for (int q = 0; q < 8; q++) {
       // padded coordinates of corner q
       const int ii = ci + (q & 1) + 1, jj = cj + ((q >> 1) & 1) + 1, kk = ck + (q >> 2) + 1;
       T v = T(0);
       if (ii >= 0 && jj >= 0 && kk >= 0 && ii < pW && jj < pH && kk < pD) v = src[((size_t)kk * pH + jj) * pW + ii];
-      dst[e * 8 + q] = v;
+      dst[e * 8 + q] = pair_value(v, prev, q, diff);
+      prev = v;
     }
   }
 }
@@ -361,8 +374,10 @@ int launch_composite(asc_context ctx, const float* s1, const float* s2, const fl
   return ASC_OK;
 }
 
-int launch_pack_volume(asc_context ctx, const VolDev& v, void* dst, int layout)
+int launch_pack_volume(asc_context ctx, const VolDev& v, void* dst, int layout, int pair_diff)
 {
+  if (pair_diff && (v.storage != ASC_STORAGE_U16 || layout == ASC_LAYOUT_LINEAR))
+    return fail(ASC_ERR_INVALID, "pair differences need a u16 volume in a paired layout");
   const int per = layout == ASC_LAYOUT_QUAD ? 4 : 8;
   const int pH = v.pS / v.pW, pD = v.D + 2;
   const unsigned grid = 256 * 8;
@@ -370,19 +385,19 @@ int launch_pack_volume(asc_context ctx, const VolDev& v, void* dst, int layout)
     const int BX = (v.W + 3) / 2, BY = (v.H + 3) / 2, BZ = (v.D + 3) / 2;
     if (v.storage == ASC_STORAGE_F32)
       pack_brick_kernel<uint32_t><<<grid, kBlock, 0, ctx->stream>>>((const uint32_t*)v.data, (uint32_t*)dst, v.pW,
-                                                                    pH, pD, BX, BY, BZ);
+                                                                    pH, pD, BX, BY, BZ, 0);
     else
       pack_brick_kernel<uint16_t><<<grid, kBlock, 0, ctx->stream>>>((const uint16_t*)v.data, (uint16_t*)dst, v.pW,
-                                                                    pH, pD, BX, BY, BZ);
+                                                                    pH, pD, BX, BY, BZ, pair_diff);
     ASC_HIP(hipGetLastError());
     return ASC_OK;
   }
   if (v.storage == ASC_STORAGE_F32)
     pack_kernel<uint32_t><<<grid, kBlock, 0, ctx->stream>>>((const uint32_t*)v.data, (uint32_t*)dst, v.pW, pH, pD,
-                                                            per);
+                                                            per, 0);
   else
     pack_kernel<uint16_t><<<grid, kBlock, 0, ctx->stream>>>((const uint16_t*)v.data, (uint16_t*)dst, v.pW, pH, pD,
-                                                            per);
+                                                            per, pair_diff);
   ASC_HIP(hipGetLastError());
   return ASC_OK;
 }

@@ -123,8 +123,7 @@ int upload_volume(asc_context ctx, const Cropped<T>& c, float full, int storage,
   const size_t n = pW * pH * pD;
   size_t esize = storage == ASC_STORAGE_F32 ? 4 : 2;
   std::vector<uint8_t> host(n * esize, 0);
-  const float inv_full = 1.0f / full;
-  (void)inv_full;
+  int u16_max = 0;
   for (int k = 0; k < D; k++)
     for (int j = 0; j < H; j++)
       for (int i = 0; i < W; i++) {
@@ -137,6 +136,7 @@ int upload_volume(asc_context ctx, const Cropped<T>& c, float full, int storage,
             // 8-bit volumes are widened exactly: v/255 == 257v/65535
             const uint16_t u = sizeof(T) == 1 ? (uint16_t)(raw * 257) : (uint16_t)raw;
             std::memcpy(&host[di * 2], &u, 2);
+            u16_max = std::max<int>(u16_max, u);
             break;
           }
           case ASC_STORAGE_F16: {
@@ -166,6 +166,8 @@ int upload_volume(asc_context ctx, const Cropped<T>& c, float full, int storage,
   v->dev.layout = ASC_LAYOUT_LINEAR;
   v->dev.esz = (int)esize;
   v->dev.nbytes = (int)std::min<size_t>(host.size(), 0x7fffffff);
+  v->dev.pair_diff = 0;
+  v->u16_max = u16_max;
   return ASC_OK;
 }
 
@@ -450,6 +452,7 @@ ASC_API int asc_volume_set_layout(asc_volume vol, int layout)
   lin.pW = (int)lin_shape.pW;
   lin.pS = (int)lin_shape.pS;
   lin.nbytes = (int)lin_shape.bytes;
+  lin.pair_diff = 0;
   if (layout == ASC_LAYOUT_LINEAR) {
     vol->dev = lin;
     return ASC_OK;
@@ -461,7 +464,10 @@ ASC_API int asc_volume_set_layout(asc_volume vol, int layout)
     vol->packed = nullptr;
     return fail(ASC_ERR_NOMEM, "packed volume allocation failed");
   }
-  int rc = launch_pack_volume(ctx, lin, vol->packed, layout);
+  // x1 - x0 of two u16 texels fits a signed 16-bit integer when every
+  // texel (and the zero border) lies in [0, 32767]
+  const int pair_diff = lin.storage == ASC_STORAGE_U16 && vol->u16_max <= 32767;
+  int rc = launch_pack_volume(ctx, lin, vol->packed, layout, pair_diff);
   if (rc) return rc;
   ASC_HIP(hipStreamSynchronize(ctx->stream));
   vol->packed_bytes = bytes;
@@ -472,6 +478,7 @@ ASC_API int asc_volume_set_layout(asc_volume vol, int layout)
   vol->dev.pW = (int)shape.pW;
   vol->dev.pS = (int)shape.pS;
   vol->dev.nbytes = (int)shape.bytes;
+  vol->dev.pair_diff = pair_diff;
   return ASC_OK;
 }
 

@@ -242,6 +242,57 @@ def serve_bench(A, synth, ctx, trials, rates, arrival_s: float, seed: int = 9000
     return out
 
 
+def time_fused(ctx, scene, batches, bone_of, ncams, args):
+    """Warm up, then time args.steps fused batches (pose upload, the three
+    launches, result readback). Returns (pose-evals/s, ms per step, summed
+    per-kernel ms over the timed steps, summed ray samples)."""
+    P = len(bone_of)
+    nb = args.warmup + args.steps
+    scene.set_timing(not args.graph)
+    scene.use_graph(args.graph)
+    log(f"[bench] warmup {args.warmup} x {P} poses")
+    for k in range(args.warmup):
+        scene.eval_poses(batches[k], volumes=bone_of)
+    ctx.synchronize()
+    kern_ms = np.zeros(3)
+    samples = 0
+    log(f"[bench] timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for k in range(args.warmup, nb):
+        scene.eval_poses(batches[k], volumes=bone_of)
+        if not args.graph:
+            kern_ms += scene.last_timing()
+            samples += scene.last_info(P * ncams)[1]
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    if args.graph:
+        # per-kernel time from an eager pass over the same batches
+        scene.use_graph(False)
+        scene.set_timing(True)
+        for k in range(args.warmup, nb):
+            scene.eval_poses(batches[k], volumes=bone_of)
+            kern_ms += scene.last_timing()
+            samples += scene.last_info(P * ncams)[1]
+    return P * args.steps / el, 1e3 * el / args.steps, kern_ms, samples
+
+
+def storage_variant(A, synth, ctx, args, batches, bone_of, storage):
+    """The same workload and poses with another volume storage: pose-evals/s
+    and the ray-march kernel's time (its error against the all-double oracle
+    is bounded by tests/test_gpu_precision.py)."""
+    wl, vol, cams, rads, scene, gt = build_workload(A, synth, args.workload, storage, ctx, args.layout)
+    if args.unroll:
+        scene.set_tuning(scene.TUNE_UNROLL, args.unroll)
+    if args.grid:
+        scene.set_tuning(scene.TUNE_GRID, args.grid)
+    evals_s, ms_step, kern_ms, _ = time_fused(ctx, scene, batches, bone_of, len(cams), args)
+    log(f"[bench] {storage} volume: {evals_s:.1f} evals/s, kernels ms {kern_ms / args.steps}")
+    scene.close()
+    vol.close()
+    return {"value": evals_s, "unit": "pose-evals/s", "ms_per_step": ms_step,
+            "kernel_ms": {"drr_tiles": kern_ms[0] / args.steps, "post_tiles": kern_ms[1] / args.steps}}
+
+
 def workload_name(args, wl, cams, P, multi) -> str:
     RW, RH = wl.render
     if multi:
@@ -271,6 +322,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-seconds", type=float, default=6.0,
                     help="serving stream: seconds of Poisson job arrivals per rate (0 = skip)")
     ap.add_argument("--serve-only", action="store_true", help="only the serving stream")
+    ap.add_argument("--variants", type=int, default=1,
+                    help="also time the F16-volume variant of a u16 workload (0 = skip)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run the device copy kernel over MIB MiB 5 times (counter calibration)")
     args = ap.parse_args(argv)
@@ -318,34 +371,8 @@ def main(argv=None) -> int:
         scene.set_tuning(scene.TUNE_UNROLL, args.unroll)
     if args.grid:
         scene.set_tuning(scene.TUNE_GRID, args.grid)
-    scene.set_timing(not args.graph)
-    scene.use_graph(args.graph)
-    log(f"[bench] warmup {args.warmup} x {P} poses")
-    for k in range(args.warmup):
-        scene.eval_poses(batches[k], volumes=bone_of)
-    ctx.synchronize()
-    kern_ms = np.zeros(3)
-    samples = 0
-    log(f"[bench] timing {args.steps} steps")
-    t0 = time.perf_counter()
-    for k in range(args.warmup, nb):
-        scene.eval_poses(batches[k], volumes=bone_of)
-        if not args.graph:
-            kern_ms += scene.last_timing()
-            samples += scene.last_info(P * len(cams))[1]
-    ctx.synchronize()
-    el = time.perf_counter() - t0
-    ms_step = 1e3 * el / args.steps
-    evals_s = P * args.steps / el
+    evals_s, ms_step, kern_ms, samples = time_fused(ctx, scene, batches, bone_of, len(cams), args)
     info, last_samples = scene.last_info(P * len(cams))
-    if args.graph:
-        # per-kernel time from an eager pass over the same batches
-        scene.use_graph(False)
-        scene.set_timing(True)
-        for k in range(args.warmup, nb):
-            scene.eval_poses(batches[k], volumes=bone_of)
-            kern_ms += scene.last_timing()
-            samples += scene.last_info(P * len(cams))[1]
     drr_ms = kern_ms[0] / args.steps
     bpv = BYTES_PER_VOXEL[args.storage]
     bytes_per_launch = 8 * bpv * samples / args.steps  # 8 texels per trilinear sample
@@ -417,6 +444,8 @@ def main(argv=None) -> int:
         "per_eval": {"bbox_pixels": int((info[:, 1] * info[:, 2]).sum() / P),
                      "ray_samples": float(last_samples) / P, "tiles": int(tiles / P)},
     }
+    if args.variants and not multi and args.storage == "u16":
+        out["storage_variants"] = {st: storage_variant(A, synth, ctx, args, batches, bone_of, st) for st in ("f16",)}
     if args.serve_seconds > 0:
         scene.close()
         for v in vols:

@@ -289,7 +289,7 @@ ASC_API int asc_scene_last_info(asc_scene scene, int* info, int capacity, int64_
  * following batches for asc_scene_last_images (costs extra stores). */
 ASC_API int asc_scene_set_debug(asc_scene scene, int enable);
 /* Launch-shape knobs of the fused path (an autotuner sets them):
- * ASC_TUNE_UNROLL: ray samples per loop iteration (1 or 2; default 2);
+ * ASC_TUNE_UNROLL: ray samples per loop iteration (1, 2 or 4; default 4);
  * ASC_TUNE_GRID: persistent grid size in blocks, multiple of 8 (0 = CUs x
  * occupancy). */
 enum { ASC_TUNE_UNROLL = 0, ASC_TUNE_GRID = 1 };

@@ -73,11 +73,16 @@ def test_drr_matches_oracle(asc, ctx, orc, tiny, storage):
     v.close()
 
 
-@pytest.mark.parametrize("storage", [0, 1, 2, 3])
-def test_layouts_are_bit_identical(asc, ctx, orc, tiny, storage):
-    """QUAD/OCTET/BRICK corner-replicated layouts change only the access pattern."""
+@pytest.mark.parametrize("storage,gain", [(0, 1), (0, 16), (1, 1), (2, 1), (3, 1)])
+def test_layouts_are_bit_identical(asc, ctx, orc, tiny, storage, gain):
+    """QUAD/OCTET/BRICK corner-replicated layouts change only the access
+    pattern. u16 volumes with every value <= 32767 store their x-pairs as
+    (x0, x1 - x0); gain 16 lifts the phantom's values past 32767, so the
+    pairs are stored plain: both encodings equal the LINEAR layout."""
     wl, prep = tiny["wl"], tiny["prep"]
-    v = asc.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3, storage=storage)
+    data = np.minimum(wl.volume.astype(np.uint32) * gain, 65535).astype(np.uint16)
+    assert (data.max() > 32767) == (gain > 1)
+    v = asc.Volume(ctx, data, scale=(wl.voxel_mm,) * 3, storage=storage)
     RW, RH = tiny["render"]
     cam = tiny["cams"][0]
     _, imv, _, _ = orc.view_geometry(cam, wl.gt_poses[0], prep["inv_scale"], prep["inv_trans"], RW, RH)

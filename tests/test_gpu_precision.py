@@ -13,8 +13,14 @@ Bars:
     1.6e-4 / 3.4e-5, 2.3e-3 / 5.0e-4);
   * 1 - NCC per camera: U16/F32 within 1e-5 (the parity bar); F16 within
     5e-4 and BF16 within 5e-3 (measured errors are several times smaller);
-  * final pose after PSO (synchronous, batched) + simplex with the same
-    seed: F16 and U16 within 0.05 mm / 0.05 deg of the FP64 oracle's.
+  * final pose of a converged optimisation (simplex alone from the start
+    pose; synchronous batched PSO run until it stalls, then the simplex)
+    with the same seed: F16 and U16 within 0.05 mm / 0.05 deg of the FP64
+    oracle's. A PSO cut short after a few epochs hands the simplex whichever
+    particle leads at that moment, and two near-equal leaders can swap
+    under any cost change (u16 costs, 7e-7 from FP64, already move such a
+    run by 0.04 mm), so only converged runs measure the storage's effect
+    on the pose.
 """
 import numpy as np
 import pytest
@@ -138,8 +144,9 @@ def test_final_pose_vs_fp64_oracle(asc, ctx, orc, tiny, storage):
     S = oracle_scene(orc, asc, tiny)
     store0 = asc.geometry.pose_store_set(tiny["wl"].gt_poses[0] + START)
     gaps = []
-    for seed in (3, 11):
-        opts = dict(method=0, max_iter=6, max_stall=5, seed=seed)
+    runs = [dict(method=1, max_iter=100, max_stall=25, seed=3)]
+    runs += [dict(method=0, max_iter=30, max_stall=10, seed=seed) for seed in (3, 11)]
+    for opts in runs:
         s_gpu, _ = sc.optimize_frame(store0, synchronous=True, **opts)
         s64, _, _ = S.optimize_frame(store0, pivot, nparticles=100, synchronous=True, fp64=True, **opts)
         gaps.append(pose_gap(orc, asc.geometry.pose_store_get(s_gpu), orc.pose_store_get(s64)))
