@@ -397,11 +397,21 @@ def main(argv=None) -> int:
 
     # HBM traffic per launch of the ray-march kernel, from the committed
     # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated with a copy kernel)
-    traffic = None
+    traffic, valu = None, None
     prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "drr_tiles_cfg2_traffic.json")
     if args.workload == "cfg2" and args.storage == "u16" and P == 100 and os.path.exists(prof):
         with open(prof) as fh:
-            traffic = json.load(fh)["traffic_bytes_per_launch"]
+            pj = json.load(fh)
+        traffic = pj["traffic_bytes_per_launch"]
+        # the issue-bound view: VALU lane-instructions of the launch at one
+        # instruction per lane per cycle on every SIMD lane (256 CUs x 64
+        # lanes x 2.4 GHz); gfx950 also dual-issues part of the time, which
+        # the measured busy fraction accounts for
+        ipws = pj["valu_instr_per_wave_sample"]
+        t_valu = ipws * (samples / args.steps) / (256 * 64 * 2.4e9)
+        valu = {"instr_per_wave_sample": ipws, "single_issue_ms": 1e3 * t_valu,
+                "frac_single_issue": 1e3 * t_valu / drr_ms, "busy_frac": pj.get("valu_busy_frac"),
+                "ta_busy_frac": pj.get("ta_busy_frac"), "source": "profiles/drr_tiles_cfg2_traffic.json"}
     tiles = info[:, 4].sum()
     out = {
         "metric": "pose evals/sec (DRR+filters+NCC, 2 cams 1024\u00b2); served frames/s + p99 latency",
@@ -437,6 +447,7 @@ def main(argv=None) -> int:
             "ray_samples_per_launch": samples / args.steps,
             "kernel_ms": {"drr_tiles": drr_ms, "post_tiles": kern_ms[1] / args.steps,
                           "finalize_items": kern_ms[2] / args.steps},
+            "valu": valu,
         },
         "cpu_baseline": cpu,
         "plain_per_eval": {"value": plain_s, "unit": "pose-evals/s",
