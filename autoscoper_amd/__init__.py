@@ -19,7 +19,7 @@ import numpy as np
 
 __all__ = [
     "lib", "AscFilter", "AscCamera", "AscVolumeDesc", "AscOptimizeOptions", "AscOptimizeStats",
-    "Context", "Volume", "Image", "Scene", "DeviceArray", "filters", "geometry",
+    "Context", "Volume", "Image", "Scene", "DeviceArray", "filters", "geometry", "io", "Trial", "Server",
     "STORAGE_U16", "STORAGE_F16", "STORAGE_BF16", "STORAGE_F32",
     "FILTER_CONTRAST", "FILTER_SOBEL", "FILTER_MEDIAN", "FILTER_GAUSSIAN", "FILTER_SHARPEN",
     "AscError",
@@ -105,6 +105,23 @@ class AscServeStats(C.Structure):
         ("evaluations", C.c_int64), ("busy_ms", C.c_double), ("elapsed_ms", C.c_double),
         ("latency_p50_ms", C.c_double), ("latency_p99_ms", C.c_double), ("latency_max_ms", C.c_double),
         ("active_jobs", C.c_int), ("waiting_jobs", C.c_int), ("device_bytes_in_use", C.c_size_t),
+    ]
+
+
+class AscFilterSettings(C.Structure):
+    _fields_ = [
+        ("has_drr_renderer", C.c_int), ("has_drr_filters", C.c_int), ("has_rad_filters", C.c_int),
+        ("n_drr_filters", C.c_int), ("n_rad_filters", C.c_int),
+        ("drr_filters", AscFilter * 16), ("rad_filters", AscFilter * 16),
+        ("sample_distance", C.c_double), ("ray_intensity", C.c_double), ("cutoff_fraction", C.c_double),
+        ("drr_step", C.c_float), ("drr_intensity", C.c_float),
+    ]
+
+
+class AscTrialInfo(C.Structure):
+    _fields_ = [
+        ("version", C.c_int * 2), ("n_cameras", C.c_int), ("n_volumes", C.c_int), ("n_frames", C.c_int),
+        ("render_width", C.c_int), ("render_height", C.c_int), ("offsets", C.c_double * 6),
     ]
 
 
@@ -207,6 +224,25 @@ _SIGS = {
     "asc_server_wait": [_vp, _i, _P(_d), _P(_d), _P(AscJobStats)],
     "asc_server_stats": [_vp, _P(AscServeStats)],
     "asc_server_destroy": [_vp],
+    "asc_tiff_info": [C.c_char_p, _P(_i), _P(_i), _P(_i), _P(_i)],
+    "asc_tiff_read": [C.c_char_p, _vp, _sz],
+    "asc_read_camera": [C.c_char_p, _P(AscCamera)],
+    "asc_camera_from_mayacam1": [_P(_d), _P(AscCamera)],
+    "asc_camera_from_vtkcam": [_P(_d), _P(_d), _P(_d), _d, _P(_d), _P(AscCamera)],
+    "asc_read_filter_settings": [C.c_char_p, _P(AscFilterSettings)],
+    "asc_drr_cutoff": [_d, _P(AscVolumeDesc), _P(_f)],
+    "asc_trial_open": [C.c_char_p, _P(_vp)],
+    "asc_trial_get_info": [_vp, _P(AscTrialInfo)],
+    "asc_trial_camera": [_vp, _i, _P(AscCamera)],
+    "asc_trial_volume_info": [_vp, _i, _P(_i), _P(_i), _P(_d), _P(_i)],
+    "asc_trial_volume_read": [_vp, _i, _vp, _sz],
+    "asc_trial_frame_info": [_vp, _i, _i, _P(_i), _P(_i)],
+    "asc_trial_frame_read": [_vp, _i, _i, _vp, _sz],
+    "asc_trial_frame_path": [_vp, _i, _i, C.c_char_p, _sz],
+    "asc_trial_background": [_vp, _i, _P(_f), _sz],
+    "asc_trial_close": [_vp],
+    "asc_write_tra": [C.c_char_p, _P(_d), _i, _i, _i],
+    "asc_read_tra": [C.c_char_p, _i, _i, _i, _P(_d), _P(_i)],
 }
 _RES = {"asc_last_error": C.c_char_p, "asc_context_stream": _vp}
 
@@ -519,6 +555,144 @@ class Server:
         try:
             self.close()
         except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+
+# ---------------------------------------------------------------------------
+# on-disk inputs and outputs (host only)
+# ---------------------------------------------------------------------------
+_UINT = {8: np.uint8, 16: np.uint16, 32: np.uint32}
+
+
+def _path(p) -> bytes:
+    return os.fsencode(str(p))
+
+
+class io:  # namespace: the trial's files (TIFF, MayaCam/VTKCam, .vie, .tra)
+    TRA_MATRIX, TRA_ROWS, TRA_COMMAS, TRA_CM, TRA_RADIANS = 1, 2, 4, 8, 16
+
+    @staticmethod
+    def tiff_info(path):
+        w, h, n, b = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(lib().asc_tiff_info(_path(path), C.byref(w), C.byref(h), C.byref(n), C.byref(b)))
+        return w.value, h.value, n.value, b.value
+
+    @staticmethod
+    def read_tiff(path) -> np.ndarray:
+        """All pages of a TIFF file as (pages, height, width) unsigned ints."""
+        w, h, n, b = io.tiff_info(path)
+        out = np.empty((n, h, w), _UINT[b])
+        _check(lib().asc_tiff_read(_path(path), out.ctypes.data_as(_vp), out.nbytes))
+        return out
+
+    @staticmethod
+    def read_camera(path) -> AscCamera:
+        c = AscCamera()
+        _check(lib().asc_read_camera(_path(path), C.byref(c)))
+        return c
+
+    @staticmethod
+    def camera_from_mayacam1(values) -> AscCamera:
+        v = _f64(values).reshape(15)
+        c = AscCamera()
+        _check(lib().asc_camera_from_mayacam1(_ptr(v, _d), C.byref(c)))
+        return c
+
+    @staticmethod
+    def camera_from_vtkcam(position, focal_point, view_up, view_angle_deg, size) -> AscCamera:
+        p, f, u, s = _f64(position), _f64(focal_point), _f64(view_up), _f64(size)
+        c = AscCamera()
+        _check(lib().asc_camera_from_vtkcam(_ptr(p, _d), _ptr(f, _d), _ptr(u, _d), float(view_angle_deg),
+                                            _ptr(s, _d), C.byref(c)))
+        return c
+
+    @staticmethod
+    def read_filter_settings(path) -> dict:
+        s = AscFilterSettings()
+        _check(lib().asc_read_filter_settings(_path(path), C.byref(s)))
+        return {"drr_filters": [s.drr_filters[k] for k in range(s.n_drr_filters)],
+                "rad_filters": [s.rad_filters[k] for k in range(s.n_rad_filters)],
+                "sample_distance": s.sample_distance, "ray_intensity": s.ray_intensity,
+                "cutoff_fraction": s.cutoff_fraction, "drr_step": s.drr_step, "drr_intensity": s.drr_intensity,
+                "sections": (bool(s.has_drr_renderer), bool(s.has_drr_filters), bool(s.has_rad_filters))}
+
+    @staticmethod
+    def drr_cutoff(fraction, desc) -> float:
+        out = C.c_float()
+        _check(lib().asc_drr_cutoff(float(fraction), C.byref(desc), C.byref(out)))
+        return out.value
+
+    @staticmethod
+    def write_tra(path, poses, flags=4) -> None:
+        """poses (frames, volumes, 6) xyzypr; default flags = comma separated
+        x y z yaw pitch roll."""
+        p = _f64(poses)
+        _check(lib().asc_write_tra(_path(path), _ptr(p, _d), p.shape[0], p.shape[1], int(flags)))
+
+    @staticmethod
+    def read_tra(path, n_frames, n_volumes, flags=4) -> np.ndarray:
+        out = np.empty((n_frames, n_volumes, 6))
+        got = C.c_int()
+        _check(lib().asc_read_tra(_path(path), n_frames, n_volumes, int(flags), _ptr(out, _d), C.byref(got)))
+        return out
+
+
+class Trial:
+    """A trial configuration (.cfg): cameras, volumes and radiograph frames
+    (xromm::Trial)."""
+
+    def __init__(self, cfg_path):
+        h = _vp()
+        _check(lib().asc_trial_open(_path(cfg_path), C.byref(h)))
+        self.h = h
+        info = AscTrialInfo()
+        _check(lib().asc_trial_get_info(h, C.byref(info)))
+        self.version = tuple(info.version)
+        self.n_cameras, self.n_volumes, self.n_frames = info.n_cameras, info.n_volumes, info.n_frames
+        self.render = (info.render_width, info.render_height)
+        self.offsets = np.array(info.offsets[:])
+        self.cameras = []
+        for i in range(self.n_cameras):
+            c = AscCamera()
+            _check(lib().asc_trial_camera(h, i, C.byref(c)))
+            self.cameras.append(c)
+
+    def volume(self, i) -> dict:
+        """{"data": (D, H, W) unsigned ints, "voxel_mm": (3,), "flip": (3,)}"""
+        dims, bps = (C.c_int * 3)(), C.c_int()
+        vox, flip = (C.c_double * 3)(), (C.c_int * 3)()
+        _check(lib().asc_trial_volume_info(self.h, i, dims, C.byref(bps), vox, flip))
+        out = np.empty((dims[2], dims[1], dims[0]), _UINT[bps.value])
+        _check(lib().asc_trial_volume_read(self.h, i, out.ctypes.data_as(_vp), out.nbytes))
+        return {"data": out, "voxel_mm": np.array(vox[:]), "flip": tuple(flip[:])}
+
+    def frame(self, camera, frame) -> np.ndarray:
+        dims, bps = (C.c_int * 2)(), C.c_int()
+        _check(lib().asc_trial_frame_info(self.h, camera, frame, dims, C.byref(bps)))
+        out = np.empty((dims[1], dims[0]), _UINT[bps.value])
+        _check(lib().asc_trial_frame_read(self.h, camera, frame, out.ctypes.data_as(_vp), out.nbytes))
+        return out
+
+    def frame_path(self, camera, frame) -> str:
+        buf = C.create_string_buffer(4096)
+        _check(lib().asc_trial_frame_path(self.h, camera, frame, buf, len(buf)))
+        return os.fsdecode(buf.value)
+
+    def background(self, camera) -> np.ndarray:
+        h, w = self.frame(camera, 0).shape
+        out = np.empty((h, w), np.float32)
+        _check(lib().asc_trial_background(self.h, camera, _ptr(out, _f), out.size))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().asc_trial_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
             pass
 
 

@@ -316,4 +316,32 @@ int gauss_weights(float radius, std::vector<float>& w)
   return n;
 }
 
+// Camera::calculateViewport and calculateImagePlane (Camera.cpp:424-484)
+// for principal point (cx, cy), focal lengths (fx, fy) and film distance z.
+int camera_finish(asc_camera& c, double cx, double cy, double fx, double fy, double z)
+{
+  if (fx == 0 || fy == 0) return fail(ASC_ERR_INVALID, "Invalid camera parameters (fx or fy is zero)");
+  c.viewport[0] = -(2.0f * cx - c.size[0]) / fx;
+  c.viewport[1] = -(2.0f * cy - c.size[1]) / fy;
+  c.viewport[2] = 2.0f * c.size[0] / fx;
+  c.viewport[3] = 2.0f * c.size[1] / fy;
+  const Frame cf = frame_load(c.rotation);  // rotation[9] then translation[3]
+  const double* tn = c.translation;
+  const double distance = sqrt(tn[0] * tn[0] + tn[1] * tn[1] + tn[2] * tn[2]);
+  const double scale = -1.5 * distance / z;
+  const double ipt[3] = {scale * (c.size[0] / 2.0 - cx), scale * (c.size[1] / 2.0 - cy), scale * z};
+  double ctr[3];
+  frame_point(cf, ipt, ctr);
+  const double hw = scale * c.size[0] / 2.0, hh = scale * c.size[1] / 2.0;
+  const double* right = cf.r;
+  const double* up = cf.r + 3;
+  for (int i = 0; i < 3; i++) {
+    c.image_plane[0 + i] = ctr[i] - hw * right[i] + hh * up[i];
+    c.image_plane[3 + i] = ctr[i] - hw * right[i] - hh * up[i];
+    c.image_plane[6 + i] = ctr[i] + hw * right[i] - hh * up[i];
+    c.image_plane[9 + i] = ctr[i] + hw * right[i] + hh * up[i];
+  }
+  return ASC_OK;
+}
+
 }  // namespace asc

@@ -49,6 +49,10 @@ void pose_store_set(PoseStore& s, const double* xyzypr);
 void pose_store_get(const PoseStore& s, double* xyzypr);
 void compose_pose(const PoseStore& s, const Frame& pivot, const double* manip, double* out_xyzypr);
 
+// Camera::calculateViewport and calculateImagePlane (Camera.cpp:424-484) of
+// a camera whose rotation, translation and size are set.
+int camera_finish(asc_camera& c, double cx, double cy, double fx, double fy, double z);
+
 // Gaussian/sharpen weights (GaussianFilter.cpp:75-127): returns N.
 int gauss_weights(float radius, std::vector<float>& w);
 

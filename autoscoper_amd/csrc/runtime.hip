@@ -641,28 +641,10 @@ ASC_API int asc_camera_from_mayacam2(const double Kf[9], const double Rf[9], con
   }
   std::memcpy(c.rotation, &rot[0][0], sizeof c.rotation);
   std::memcpy(c.translation, tn, sizeof c.translation);
-  const double cx = K[2][0], cy = K[2][1], fx = K[0][0], fy = K[1][1];
-  if (fx == 0 || fy == 0) return fail(ASC_ERR_INVALID, "Invalid camera parameters (fx or fy is zero)");
-  c.viewport[0] = -(2.0f * cx - c.size[0]) / fx;
-  c.viewport[1] = -(2.0f * cy - c.size[1]) / fy;
-  c.viewport[2] = 2.0f * c.size[0] / fx;
-  c.viewport[3] = 2.0f * c.size[1] / fy;
-  const double z = -0.5 * (fx + fy);
-  const Frame cf = frame_load(c.rotation);  // rotation[9] then translation[3]
-  const double distance = sqrt(tn[0] * tn[0] + tn[1] * tn[1] + tn[2] * tn[2]);
-  const double scale = -1.5 * distance / z;
-  const double ipt[3] = {scale * (c.size[0] / 2.0 - cx), scale * (c.size[1] / 2.0 - cy), scale * z};
-  double ctr[3];
-  frame_point(cf, ipt, ctr);
-  const double hw = scale * c.size[0] / 2.0, hh = scale * c.size[1] / 2.0;
-  const double* right = cf.r;
-  const double* up = cf.r + 3;
-  for (int i = 0; i < 3; i++) {
-    c.image_plane[0 + i] = ctr[i] - hw * right[i] + hh * up[i];
-    c.image_plane[3 + i] = ctr[i] - hw * right[i] - hh * up[i];
-    c.image_plane[6 + i] = ctr[i] + hw * right[i] - hh * up[i];
-    c.image_plane[9 + i] = ctr[i] + hw * right[i] + hh * up[i];
-  }
+  const double fx = K[0][0], fy = K[1][1];
+  // average focal length, negated as MayaCam 1.0's z (Camera.cpp:358-361)
+  const int rc = camera_finish(c, K[2][0], K[2][1], fx, fy, -0.5 * (fx + fy));
+  if (rc) return rc;
   *out = c;
   return ASC_OK;
 }

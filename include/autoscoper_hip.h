@@ -35,6 +35,7 @@ extern "C" {
 #define ASC_ERR_TOO_LARGE (-5)    /* View.cpp:448-455, Tracker.cpp:625-629 (throw there) */
 #define ASC_ERR_UNSUPPORTED (-6)  /* e.g. bit depth, RadRenderer.cpp:93-96 */
 #define ASC_ERR_NO_DEVICE (-7)
+#define ASC_ERR_IO (-8)           /* a file is missing or malformed (the reference throws runtime_error) */
 
 /* ---- handles and records ------------------------------------------------- */
 typedef struct asc_context_s* asc_context;
@@ -440,6 +441,89 @@ ASC_API int asc_server_wait(asc_server server, int job_id, double* poses, double
 ASC_API int asc_server_stats(asc_server server, asc_serve_stats* out);
 /* Drops queued jobs, lets running ones finish, then frees everything. */
 ASC_API int asc_server_destroy(asc_server server);
+
+/* ---- on-disk inputs and outputs (host only, no device) --------------------- */
+
+/* TIFF images as libautoscoper reads them through libtiff (TiffImage.cpp,
+ * Volume.cpp:51-140, Video.cpp:66-239): strips, uncompressed, LZW or
+ * PackBits, predictor 1 or 2, 8/16/32 bits per sample. A volume file holds
+ * one slice per page (all pages alike, one sample per pixel, unsigned). The
+ * pixels come out pages x height x width, x fastest, in host byte order. */
+ASC_API int asc_tiff_info(const char* path, int* width, int* height, int* pages, int* bits_per_sample);
+ASC_API int asc_tiff_read(const char* path, void* dst, size_t dst_bytes);
+
+/* xromm::Camera(filename) (Camera.cpp:111-140): a ".json" file is a VTKCam
+ * 1.0 camera, a file whose first line is "image size" a MayaCam 2.0 camera,
+ * any other a MayaCam 1.0 camera. */
+ASC_API int asc_read_camera(const char* path, asc_camera* out);
+/* Camera::loadMayaCam1 after parsing (Camera.cpp:141-227): the file's five
+ * lines of three values, row after row. */
+ASC_API int asc_camera_from_mayacam1(const double values[15], asc_camera* out);
+/* Camera::loadVTKCamera after parsing (Camera.cpp:364-422). */
+ASC_API int asc_camera_from_vtkcam(const double position[3], const double focal_point[3], const double view_up[3],
+                                   double view_angle_deg, const double size[2], asc_camera* out);
+
+/* A filter settings file (.vie) as the GUI applies it to a camera
+ * (FilterTreeWidget::loadFilterSettings, FilterTreeWidget.cpp:290-329):
+ * every value passes the GUI's spin box (rounded to 2 decimals, then bounded
+ * to its range), the Sobel keys are swapped as the GUI reads them (`Scale`
+ * sets blend, `Blend` sets scale; FilterTreeWidgetItem.cpp:158-165,298-299),
+ * and the DRR renderer's values map to step = exp(7 SD - 5) and intensity =
+ * exp(15 RI - 5) (ModelViewTreeWidgetItem.cpp:321-338). The cutoff is a
+ * fraction of the volume's value range: asc_drr_cutoff maps it. */
+#define ASC_MAX_FILTERS 16
+typedef struct {
+  int has_drr_renderer, has_drr_filters, has_rad_filters; /* sections present in the file */
+  int n_drr_filters, n_rad_filters;
+  asc_filter drr_filters[ASC_MAX_FILTERS];
+  asc_filter rad_filters[ASC_MAX_FILTERS];
+  double sample_distance, ray_intensity, cutoff_fraction; /* spin box values (defaults 0.62, 0.49, 0) */
+  float drr_step, drr_intensity;                          /* RayCaster::setSampleDistance, setRayIntensity */
+} asc_filter_settings;
+ASC_API int asc_read_filter_settings(const char* path, asc_filter_settings* out);
+/* RayCaster::setCutoff(fraction * (max - min) + min) over the volume's
+ * normalised value range (ModelViewTreeWidgetItem.cpp:336-338). */
+ASC_API int asc_drr_cutoff(double cutoff_fraction, const asc_volume_desc* desc, float* cutoff);
+
+/* A trial configuration (.cfg v1.0 to 1.2; xromm::Trial(filename),
+ * Trial.cpp:74-257): cameras are read at open, volumes and frames on
+ * demand. Paths of version 1.1 and later are relative to the file. Frames
+ * of a camera are the sorted *.tif* files of its directory; a frame index
+ * past a camera's last frame reads its last frame (Video::set_frame). */
+typedef struct asc_trial_s* asc_trial;
+typedef struct {
+  int version[2];
+  int n_cameras, n_volumes;
+  int n_frames;                     /* the longest camera's frame count */
+  int render_width, render_height;  /* RenderResolution, default 512 x 512 */
+  double offsets[6];                /* OptimizationOffsets, default 0.1 */
+} asc_trial_info;
+ASC_API int asc_trial_open(const char* cfg_path, asc_trial* out);
+ASC_API int asc_trial_get_info(asc_trial trial, asc_trial_info* out);
+ASC_API int asc_trial_camera(asc_trial trial, int camera, asc_camera* out);
+/* dims = {W, H, D} of the stack as stored (before VolumeFlip, which
+ * asc_volume_create applies); voxel_mm = VoxelSize; flip = VolumeFlip. */
+ASC_API int asc_trial_volume_info(asc_trial trial, int volume, int dims[3], int* bits_per_sample, double voxel_mm[3],
+                                  int flip[3]);
+ASC_API int asc_trial_volume_read(asc_trial trial, int volume, void* dst, size_t dst_bytes);
+ASC_API int asc_trial_frame_info(asc_trial trial, int camera, int frame, int dims[2], int* bits_per_sample);
+ASC_API int asc_trial_frame_read(asc_trial trial, int camera, int frame, void* dst, size_t dst_bytes);
+ASC_API int asc_trial_frame_path(asc_trial trial, int camera, int frame, char* dst, size_t dst_bytes);
+/* Video::create_background_image (Video.cpp:147-218): per pixel, the
+ * largest normalised value over all of the camera's frames (count = W*H). */
+ASC_API int asc_trial_background(asc_trial trial, int camera, float* dst, size_t count);
+ASC_API int asc_trial_close(asc_trial trial);
+
+/* Tracking results (.tra; AutoscoperMainWindow::save_tracking_results and
+ * load_tracking_results, AutoscoperMainWindow.cpp:736-830,983-1092): one line
+ * per frame, per volume 6 values x y z yaw pitch roll or the 16 of the 4x4
+ * matrix (column order, or row order with ASC_TRA_ROWS), printed "%.16f";
+ * a pose with any NaN component (an untracked frame) is written as NaNs.
+ * poses = n_frames x n_volumes x 6 (mm, degrees). Reading fills the poses
+ * of the lines present and leaves the others NaN. */
+enum { ASC_TRA_MATRIX = 1, ASC_TRA_ROWS = 2, ASC_TRA_COMMAS = 4, ASC_TRA_CM = 8, ASC_TRA_RADIANS = 16 };
+ASC_API int asc_write_tra(const char* path, const double* poses, int n_frames, int n_volumes, int flags);
+ASC_API int asc_read_tra(const char* path, int n_frames, int n_volumes, int flags, double* poses, int* frames_read);
 
 #ifdef __cplusplus
 }
