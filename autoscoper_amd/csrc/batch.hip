@@ -684,8 +684,10 @@ void destroy_graph(asc_scene s)
   s->graph_items = -1;
 }
 
-// Issue the batch (upload, three kernels, download) on the context stream.
-int issue_batch(asc_scene s, size_t batch_bytes)
+// Issue the batch (upload, three kernels, download) on the context stream,
+// with grid_drr and grid_post blocks for the two persistent kernels (any
+// grid of at least one block drains the whole tile list).
+int issue_batch(asc_scene s, size_t batch_bytes, int grid_drr, int grid_post)
 {
   hipStream_t st = s->ctx->stream;
   const int n_items = s->n_items;
@@ -709,15 +711,15 @@ int issue_batch(asc_scene s, size_t batch_bytes)
   const bool timed = s->timing && s->ev[0];
   if (timed) ASC_HIP(hipEventRecord(s->ev[0], st));
   switch (s->unroll) {
-    case 1: drr_tiles<1><<<s->grid_persistent, kBlock, 0, st>>>(a); break;
-    case 4: drr_tiles<4><<<s->grid_persistent, kBlock, 0, st>>>(a); break;
-    default: drr_tiles<2><<<s->grid_persistent, kBlock, 0, st>>>(a); break;
+    case 1: drr_tiles<1><<<grid_drr, kBlock, 0, st>>>(a); break;
+    case 4: drr_tiles<4><<<grid_drr, kBlock, 0, st>>>(a); break;
+    default: drr_tiles<2><<<grid_drr, kBlock, 0, st>>>(a); break;
   }
   if (timed) ASC_HIP(hipEventRecord(s->ev[1], st));
   if (s->post_small)
-    post_tiles<kSmallHalo><<<s->grid_post, kBlock, 0, st>>>(a);
+    post_tiles<kSmallHalo><<<grid_post, kBlock, 0, st>>>(a);
   else
-    post_tiles<kMaxHalo><<<s->grid_post, kBlock, 0, st>>>(a);
+    post_tiles<kMaxHalo><<<grid_post, kBlock, 0, st>>>(a);
   if (timed) ASC_HIP(hipEventRecord(s->ev[2], st));
   finalize_items<<<std::min(n_items, 4096), kBlock, 0, st>>>(a);
   if (timed) ASC_HIP(hipEventRecord(s->ev[3], st));
@@ -910,7 +912,8 @@ int eval_requests(asc_scene s, const std::vector<Request>& reqs, double* costs, 
     if (!s->graph_exec || s->graph_items != n_items) {
       destroy_graph(s);
       ASC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      rc = issue_batch(s, batch_bytes);
+      // a recorded graph replays batches of other tile counts: full grids
+      rc = issue_batch(s, batch_bytes, s->grid_persistent, s->grid_post);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(st, &g);
       if (rc) return rc;
@@ -921,7 +924,11 @@ int eval_requests(asc_scene s, const std::vector<Request>& reqs, double* costs, 
     }
     ASC_HIP(hipGraphLaunch(s->graph_exec, st));
   } else {
-    if ((rc = issue_batch(s, batch_bytes))) return rc;
+    // a small batch (a simplex step) launches only as many blocks as it has
+    // tiles, rounded up to the 8 partitions of the tile list
+    const int gd = (int)std::min<long long>(s->grid_persistent, ((long long)hdr.drr.total + 7) / 8 * 8);
+    const int gp = (int)std::min<long long>(s->grid_post, ((long long)hdr.post.total + 7) / 8 * 8);
+    if ((rc = issue_batch(s, batch_bytes, std::max(gd, 8), std::max(gp, 8)))) return rc;
   }
   ASC_HIP(hipStreamSynchronize(st));
   if (s->timing && s->ev[0] && !s->use_graph)
