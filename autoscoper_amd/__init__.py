@@ -118,6 +118,11 @@ class AscFilterSettings(C.Structure):
     ]
 
 
+class AscTuneResult(C.Structure):
+    _fields_ = [("unroll", C.c_int), ("wave_patch", C.c_int), ("grid", C.c_int), ("best_ms", C.c_double),
+                ("default_ms", C.c_double), ("candidates", C.c_int)]
+
+
 class AscTrialInfo(C.Structure):
     _fields_ = [
         ("version", C.c_int * 2), ("n_cameras", C.c_int), ("n_volumes", C.c_int), ("n_frames", C.c_int),
@@ -224,6 +229,7 @@ _SIGS = {
     "asc_server_wait": [_vp, _i, _P(_d), _P(_d), _P(AscJobStats)],
     "asc_server_stats": [_vp, _P(AscServeStats)],
     "asc_server_destroy": [_vp],
+    "asc_scene_autotune": [_vp, _i, _P(_i), _P(_d), _i, _P(AscTuneResult)],
     "asc_tiff_info": [C.c_char_p, _P(_i), _P(_i), _P(_i), _P(_i)],
     "asc_tiff_read": [C.c_char_p, _vp, _sz],
     "asc_read_camera": [C.c_char_p, _P(AscCamera)],
@@ -1029,7 +1035,18 @@ class Scene:
                                            _ptr(costs, _d), _ptr(per, _d)))
         return costs, per
 
-    TUNE_UNROLL, TUNE_GRID = 0, 1
+    TUNE_UNROLL, TUNE_GRID, TUNE_WAVE_PATCH = 0, 1, 2
+
+    def autotune(self, poses, volumes=None, reps=3) -> dict:
+        """Time the launch-shape variants on this batch of poses and keep the
+        fastest (asc_scene_autotune)."""
+        p = _f64(poses).reshape(-1, 6)
+        n = p.shape[0]
+        vols = np.zeros(n, np.int32) if volumes is None else np.ascontiguousarray(volumes, np.int32)
+        r = AscTuneResult()
+        _check(lib().asc_scene_autotune(self.handle, n, _ptr(vols, _i), _ptr(p, _d), int(reps), C.byref(r)))
+        return {"unroll": r.unroll, "wave_patch": r.wave_patch, "grid": r.grid, "best_ms": r.best_ms,
+                "default_ms": r.default_ms, "candidates": r.candidates}
 
     def set_tuning(self, key: int, value: int):
         _check(lib().asc_scene_set_tuning(self.handle, key, value))

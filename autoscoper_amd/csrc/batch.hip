@@ -124,6 +124,7 @@ struct BatchArgs {
   float* dbg_drr;
   float* dbg_rad;
   float* dbg_mask;
+  int wave_shift;  // log2 of the width of one wave's ray patch in a 16x16 tile: 3 (8x8), 4 (16x4) or 2 (4x16)
 };
 
 // Next enumeration index for the whole block, or -1 once every partition
@@ -197,8 +198,12 @@ __global__ __launch_bounds__(kBlock) void drr_tiles(BatchArgs a)
     int ii, tx, ty;
     if (!decode_tile<false>(hd, a.items, t, ii, tx, ty)) continue;
     const ItemDesc& it = a.items[ii];
-    const unsigned x = tx * kTile + (wave & 1) * 8 + (lane & 7);
-    const unsigned y = ty * kTile + (wave >> 1) * 8 + (lane >> 3);
+    // each of the tile's 4 waves marches a pw x (64 / pw) patch of rays,
+    // pw = 2^wave_shift; the patches cover the 16 x 16 tile row by row
+    const int wsh = a.wave_shift, pw = 1 << wsh, ph = 64 >> wsh;
+    const int wpr = kTile >> wsh;  // waves per tile row
+    const unsigned x = tx * kTile + (wave % wpr) * pw + (lane & (pw - 1));
+    const unsigned y = ty * kTile + (wave / wpr) * ph + (lane >> wsh);
     if (threadIdx.x == 0) s_samples = 0;
     __syncthreads();
     int nsamp = 0;
@@ -581,6 +586,7 @@ struct asc_scene_s {
   // per-kernel timing of the last eager batch (HIP events on the scene stream)
   bool timing = false;
   int unroll = 4;  // ray samples per loop iteration (1, 2 or 4)
+  int wave_shift = 3;  // a wave marches an 8 x 8 ray patch
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float last_ms[3] = {0, 0, 0};
 };
@@ -706,6 +712,7 @@ int issue_batch(asc_scene s, size_t batch_bytes, int grid_drr, int grid_post)
   a.dbg_drr = s->debug ? s->d_dbg.p : nullptr;
   a.dbg_rad = s->debug ? s->d_dbg.p + plane : nullptr;
   a.dbg_mask = s->debug ? s->d_dbg.p + 2 * plane : nullptr;
+  a.wave_shift = s->wave_shift;
   ASC_HIP(hipMemcpyAsync(s->d_batch.p, s->h_batch.p, batch_bytes, hipMemcpyHostToDevice, st));
   zero_u64<<<(n_items + 255) / 256, 256, 0, st>>>(s->d_samples.p, n_items);
   const bool timed = s->timing && s->ev[0];
@@ -1173,6 +1180,75 @@ ASC_API int asc_scene_eval_poses(asc_scene s, const int* volumes, const double* 
   return eval_batch(s, volumes, xyzypr, n, costs, per_camera);
 }
 
+ASC_API int asc_scene_autotune(asc_scene s, int n, const int* volumes, const double* poses, int reps,
+                               asc_tune_result* result)
+{
+  ASC_CHECK(s && poses && n > 0 && reps > 0, ASC_ERR_INVALID, "bad argument");
+  const bool timing0 = s->timing, graph0 = s->use_graph;
+  if (!s->ev[0])
+    for (hipEvent_t& e : s->ev) ASC_HIP(hipEventCreate(&e));
+  s->timing = true;
+  s->use_graph = false;
+  destroy_graph(s);
+  struct Cand {
+    int unroll, shift, gmul;
+    double ms;
+  };
+  std::vector<Cand> cands;
+  for (int unroll : {4, 2})
+    for (int shift : {3, 4, 2})
+      for (int gmul : {1, 2}) cands.push_back(Cand{unroll, shift, gmul, 0.0});
+  int rc = ASC_OK;
+  int auto_grid = 0;
+  std::vector<double> ms;
+  for (Cand& c : cands) {
+    s->unroll = c.unroll;
+    s->wave_shift = c.shift;
+    s->grid_user = false;
+    s->grid_persistent = 0;  // this variant's occupancy
+    if ((rc = eval_batch(s, volumes, poses, n, nullptr, nullptr))) break;  // warm, derives the grid
+    auto_grid = s->grid_persistent;
+    if (c.gmul > 1) {
+      s->grid_persistent = auto_grid * c.gmul;
+      s->grid_user = true;
+    }
+    ms.clear();
+    for (int r = 0; r < reps && rc == ASC_OK; r++) {
+      rc = eval_batch(s, volumes, poses, n, nullptr, nullptr);
+      ms.push_back(s->last_ms[0]);
+    }
+    if (rc) break;
+    std::sort(ms.begin(), ms.end());
+    c.ms = ms[ms.size() / 2];
+  }
+  s->timing = timing0;
+  s->use_graph = graph0;
+  if (rc) return rc;
+  const Cand* best = &cands[0];
+  for (const Cand& c : cands)
+    if (c.ms < best->ms) best = &c;
+  s->unroll = best->unroll;
+  s->wave_shift = best->shift;
+  s->grid_user = false;
+  s->grid_persistent = 0;
+  if (best->gmul > 1) {
+    // the occupancy grid of the chosen variant, times gmul
+    if ((rc = eval_batch(s, volumes, poses, n, nullptr, nullptr))) return rc;
+    s->grid_persistent *= best->gmul;
+    s->grid_user = true;
+  }
+  destroy_graph(s);
+  if (result) {
+    result->unroll = best->unroll;
+    result->wave_patch = 1 << best->shift;
+    result->grid = best->gmul > 1 ? s->grid_persistent : 0;
+    result->best_ms = best->ms;
+    result->default_ms = cands[0].ms;  // unroll 4, 8 x 8 patch, occupancy grid
+    result->candidates = (int)cands.size();
+  }
+  return ASC_OK;
+}
+
 ASC_API int asc_scene_eval_requests(asc_scene s, int n, const int* volumes, const int* sets, const int* frames,
                                     const double* xyzypr, double* costs, double* per_camera)
 {
@@ -1227,6 +1303,10 @@ ASC_API int asc_scene_set_tuning(asc_scene s, int key, int value)
       ASC_CHECK(value >= 0, ASC_ERR_INVALID, "grid must be >= 0 (0 = one wave of blocks per CU slot)");
       s->grid_persistent = value - value % 8;
       s->grid_user = s->grid_persistent > 0;
+      break;
+    case ASC_TUNE_WAVE_PATCH:
+      ASC_CHECK(value == 4 || value == 8 || value == 16, ASC_ERR_INVALID, "wave patch width must be 4, 8 or 16");
+      s->wave_shift = value == 4 ? 2 : value == 8 ? 3 : 4;
       break;
     default: return fail(ASC_ERR_INVALID, "unknown tuning key");
   }

@@ -281,6 +281,10 @@ def storage_variant(A, synth, ctx, args, batches, bone_of, storage):
     and the ray-march kernel's time (its error against the all-double oracle
     is bounded by tests/test_gpu_precision.py)."""
     wl, vol, cams, rads, scene, gt = build_workload(A, synth, args.workload, storage, ctx, args.layout)
+    tuning = None
+    if args.autotune and not (args.unroll or args.grid):
+        tuning = scene.autotune(batches[0], volumes=bone_of, reps=3)
+        log(f"[bench] {storage} autotune: {tuning}")
     if args.unroll:
         scene.set_tuning(scene.TUNE_UNROLL, args.unroll)
     if args.grid:
@@ -289,7 +293,7 @@ def storage_variant(A, synth, ctx, args, batches, bone_of, storage):
     log(f"[bench] {storage} volume: {evals_s:.1f} evals/s, kernels ms {kern_ms / args.steps}")
     scene.close()
     vol.close()
-    return {"value": evals_s, "unit": "pose-evals/s", "ms_per_step": ms_step,
+    return {"value": evals_s, "unit": "pose-evals/s", "ms_per_step": ms_step, "launch_shape": tuning,
             "kernel_ms": {"drr_tiles": kern_ms[0] / args.steps, "post_tiles": kern_ms[1] / args.steps}}
 
 
@@ -322,6 +326,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-seconds", type=float, default=6.0,
                     help="serving stream: seconds of Poisson job arrivals per rate (0 = skip)")
     ap.add_argument("--serve-only", action="store_true", help="only the serving stream")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="choose the ray march's launch shape with asc_scene_autotune before timing (0 = defaults)")
     ap.add_argument("--variants", type=int, default=1,
                     help="also time the F16-volume variant of a u16 workload (0 = skip)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -367,6 +373,12 @@ def main(argv=None) -> int:
     batches = [gts[bone_of] + np.concatenate([rng.uniform(-3, 3, (P, 3)), rng.uniform(-3, 3, (P, 3))], 1)
                for _ in range(nb)]
 
+    tuning = None
+    if args.autotune and not (args.unroll or args.grid):
+        # launch shape chosen by the autotuner on the first warm-up batch
+        # (outside the timed region; no variant changes a result)
+        tuning = scene.autotune(batches[0], volumes=bone_of, reps=3)
+        log(f"[bench] autotune: {tuning}")
     if args.unroll:
         scene.set_tuning(scene.TUNE_UNROLL, args.unroll)
     if args.grid:
@@ -434,6 +446,7 @@ def main(argv=None) -> int:
             "cameras": len(cams),
             "render": [RW, RH],
             "graph": bool(args.graph),
+            "launch_shape": tuning,
         },
         "roofline": {
             "bound": "hbm",

@@ -289,12 +289,29 @@ ASC_API int asc_scene_last_info(asc_scene scene, int* info, int capacity, int64_
 /* Keep the filtered, masked DRR/radiograph and mask of every item of the
  * following batches for asc_scene_last_images (costs extra stores). */
 ASC_API int asc_scene_set_debug(asc_scene scene, int enable);
-/* Launch-shape knobs of the fused path (an autotuner sets them):
+/* Launch-shape knobs of the fused path (asc_scene_autotune sets them):
  * ASC_TUNE_UNROLL: ray samples per loop iteration (1, 2 or 4; default 4);
  * ASC_TUNE_GRID: persistent grid size in blocks, multiple of 8 (0 = CUs x
- * occupancy). */
-enum { ASC_TUNE_UNROLL = 0, ASC_TUNE_GRID = 1 };
+ * occupancy);
+ * ASC_TUNE_WAVE_PATCH: width of the ray patch one wave marches inside a
+ * 16 x 16 tile: 8 (8 x 8, default), 16 (16 x 4) or 4 (4 x 16).
+ * None of them changes a result. */
+enum { ASC_TUNE_UNROLL = 0, ASC_TUNE_GRID = 1, ASC_TUNE_WAVE_PATCH = 2 };
 ASC_API int asc_scene_set_tuning(asc_scene scene, int key, int value);
+
+/* Autotuner: times every combination of unroll {2, 4}, wave patch {8, 16,
+ * 4} and grid {1, 2} x (CUs x occupancy) on a representative batch (n poses
+ * of `volumes`, all cameras), each `reps` times with HIP events around the
+ * ray-march kernel, keeps the fastest by median and sets it on the scene.
+ * result = {unroll, wave patch, grid, best ms, default ms} (the default is
+ * unroll 4, patch 8, grid 1x). */
+typedef struct {
+  int unroll, wave_patch, grid;
+  double best_ms, default_ms;
+  int candidates;
+} asc_tune_result;
+ASC_API int asc_scene_autotune(asc_scene scene, int n, const int* volumes, const double* poses, int reps,
+                               asc_tune_result* result);
 /* Time the fused kernels of eager (non-graph) batches with HIP events on
  * the scene's stream; ms = {ray march, filters+moments, finalize} of the
  * last batch. */

@@ -165,3 +165,33 @@ def test_manip_eval_matches_oracle_min_func(asc, ctx, orc, tiny):
         assert abs(costs[i] - S.min_func(store, pivot, m)) <= 2e-5
     sc.close()
     vol.close()
+
+
+def test_launch_shapes_and_autotune_do_not_change_results(asc, ctx, tiny):
+    """Unroll, wave patch and grid change only how the ray march is
+    scheduled: every variant gives the same bits, and the autotuner keeps
+    one of them."""
+    sc, vol, _ = build_scene(asc, ctx, tiny)
+    rng = np.random.default_rng(31)
+    poses = tiny["wl"].gt_poses[0] + np.concatenate([rng.uniform(-3, 3, (12, 3)), rng.uniform(-6, 6, (12, 3))], 1)
+    ref_c, ref_p = sc.eval_poses(poses)
+    ref_s = sc.last_info(24)[1]
+    for unroll in (1, 2, 4):
+        for patch in (4, 8, 16):
+            for grid in (0, 64):
+                sc.set_tuning(sc.TUNE_UNROLL, unroll)
+                sc.set_tuning(sc.TUNE_WAVE_PATCH, patch)
+                sc.set_tuning(sc.TUNE_GRID, grid)
+                c, p = sc.eval_poses(poses)
+                assert np.array_equal(c, ref_c) and np.array_equal(p, ref_p), (unroll, patch, grid)
+                assert sc.last_info(24)[1] == ref_s
+    with pytest.raises(asc.AscError):
+        sc.set_tuning(sc.TUNE_WAVE_PATCH, 32)
+    r = sc.autotune(poses, reps=2)
+    print("autotune:", r)
+    assert r["unroll"] in (2, 4) and r["wave_patch"] in (4, 8, 16) and r["candidates"] == 12
+    assert 0 < r["best_ms"] <= r["default_ms"]
+    c, p = sc.eval_poses(poses)
+    assert np.array_equal(c, ref_c) and np.array_equal(p, ref_p)
+    sc.close()
+    vol.close()
