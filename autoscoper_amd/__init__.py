@@ -19,7 +19,7 @@ import numpy as np
 
 __all__ = [
     "lib", "AscFilter", "AscCamera", "AscVolumeDesc", "AscOptimizeOptions", "AscOptimizeStats",
-    "Context", "Volume", "Image", "Scene", "DeviceArray", "filters", "geometry", "io", "Trial", "Server",
+    "Context", "Volume", "Image", "Scene", "DeviceArray", "filters", "geometry", "io", "Trial", "Server", "Remote",
     "STORAGE_U16", "STORAGE_F16", "STORAGE_BF16", "STORAGE_F32",
     "FILTER_CONTRAST", "FILTER_SOBEL", "FILTER_MEDIAN", "FILTER_GAUSSIAN", "FILTER_SHARPEN",
     "AscError",
@@ -230,6 +230,11 @@ _SIGS = {
     "asc_server_stats": [_vp, _P(AscServeStats)],
     "asc_server_destroy": [_vp],
     "asc_scene_autotune": [_vp, _i, _P(_i), _P(_d), _i, _P(AscTuneResult)],
+    "asc_remote_create": [_vp, _P(_vp)],
+    "asc_remote_listen": [_vp, _i],
+    "asc_remote_port": [_vp, _P(_i)],
+    "asc_remote_handle": [_vp, _vp, _sz, _vp, _sz, _P(_sz)],
+    "asc_remote_destroy": [_vp],
     "asc_tiff_info": [C.c_char_p, _P(_i), _P(_i), _P(_i), _P(_i)],
     "asc_tiff_read": [C.c_char_p, _vp, _sz],
     "asc_read_camera": [C.c_char_p, _P(AscCamera)],
@@ -693,6 +698,42 @@ class Trial:
     def close(self):
         if self.h:
             lib().asc_trial_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Remote:
+    """A session answering the reference's socket protocol (version 2):
+    handle() one message directly, or listen() on 127.0.0.1."""
+
+    def __init__(self, ctx: "Context | None" = None):
+        h = _vp()
+        _check(lib().asc_remote_create(ctx.handle if ctx else None, C.byref(h)))
+        self.h = h
+        self._ctx = ctx
+
+    def handle(self, message: bytes) -> bytes:
+        n = C.c_size_t()
+        buf = C.create_string_buffer(1 << 22)
+        rc = lib().asc_remote_handle(self.h, message, len(message), buf, len(buf), C.byref(n))
+        _check(rc)
+        self.closed = rc == 1  # message 13 asks to close the connection
+        return buf.raw[:n.value]
+
+    def listen(self, port: int = 0) -> int:
+        _check(lib().asc_remote_listen(self.h, port))
+        p = C.c_int()
+        _check(lib().asc_remote_port(self.h, C.byref(p)))
+        return p.value
+
+    def close(self):
+        if self.h:
+            lib().asc_remote_destroy(self.h)
             self.h = None
 
     def __del__(self):

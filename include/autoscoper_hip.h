@@ -542,6 +542,29 @@ enum { ASC_TRA_MATRIX = 1, ASC_TRA_ROWS = 2, ASC_TRA_COMMAS = 4, ASC_TRA_CM = 8,
 ASC_API int asc_write_tra(const char* path, const double* poses, int n_frames, int n_volumes, int flags);
 ASC_API int asc_read_tra(const char* path, int n_frames, int n_volumes, int flags, double* poses, int* frames_read);
 
+/* ---- remote control: the reference's socket protocol, version 2 ----------- */
+/* The server side of autoscoper/src/net/Socket.cpp:53-444 (the protocol
+ * PyAutoscoper speaks, scripts/python/PyAutoscoper/connect.py): a session
+ * holding a trial, its cameras, volumes, frames, filter settings,
+ * background threshold and per-volume pose curves (KeyCurve.hpp: cubic
+ * Hermite x, y, z with smooth tangents, slerped orientation), answering
+ * the 18 message types on 127.0.0.1. Costs and optimisation (messages 8
+ * and 11) run on this library's fused scene, on `ctx` (NULL: a session
+ * without a device, whose device requests fail). Only the NCC cost is
+ * built (message 11 with cf_model 1, SAD, fails). */
+typedef struct asc_remote_s* asc_remote;
+ASC_API int asc_remote_create(asc_context ctx, asc_remote* out);
+/* Serves on 127.0.0.1:port from a thread of its own (port 0: any free
+ * port; the reference's client uses 30007). */
+ASC_API int asc_remote_listen(asc_remote remote, int port);
+ASC_API int asc_remote_port(asc_remote remote, int* port);
+/* Answers one message without a socket: reply_length gets the answer's
+ * size (the first `capacity` bytes are copied); returns 1 for message 13
+ * (close the connection). */
+ASC_API int asc_remote_handle(asc_remote remote, const void* message, size_t length, void* reply, size_t capacity,
+                              size_t* reply_length);
+ASC_API int asc_remote_destroy(asc_remote remote);
+
 #ifdef __cplusplus
 }
 #endif

@@ -140,3 +140,31 @@ int ref_amoeba(ref_cost_fn fn, void* user, double* P, double* Y, int ndim, doubl
 long ref_calls() { return g_calls; }
 
 }  // extern "C"
+
+// The reference's pose curves (KeyCurve.hpp, header only): keys at `times`
+// inserted in the given order, then the curve evaluated at `at`.
+#include "KeyCurve.hpp"
+
+extern "C" {
+
+void ref_float_curve(int n, const int* times, const float* values, int m, const float* at, float* out)
+{
+  KeyCurve<float> c;
+  for (int i = 0; i < n; i++) c.insert(times[i], values[i]);
+  for (int i = 0; i < m; i++) out[i] = c(at[i]);
+}
+
+void ref_quat_curve(int n, const int* times, const float* wxyz, int m, const float* at, float* out)
+{
+  KeyCurve<Quatf> c;
+  for (int i = 0; i < n; i++) c.insert(times[i], Quatf(wxyz[4 * i], wxyz[4 * i + 1], wxyz[4 * i + 2], wxyz[4 * i + 3]));
+  for (int i = 0; i < m; i++) {
+    const Quatf q = c(at[i]);
+    out[4 * i] = q.w;
+    out[4 * i + 1] = q.x;
+    out[4 * i + 2] = q.y;
+    out[4 * i + 3] = q.z;
+  }
+}
+
+}  // extern "C"
