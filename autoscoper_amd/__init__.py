@@ -145,6 +145,7 @@ _SIGS = {
     "asc_context_create": [_i, _P(_vp)],
     "asc_context_destroy": [_vp],
     "asc_context_set_stream": [_vp, _vp],
+    "asc_context_set_blocking": [_vp, C.c_int],
     "asc_context_stream": [_vp],
     "asc_synchronize": [_vp],
     "asc_malloc": [_vp, _sz, _P(_vp)],
@@ -216,6 +217,7 @@ _SIGS = {
     "asc_scene_last_moments": [_vp, _P(_d), _i],
     "asc_scene_last_images": [_vp, _i, _i, _P(_f), _P(_f), _P(_f)],
     "asc_scene_track_frame_plain": [_vp, _i, _P(_d), _P(_d), _P(_d)],
+    "asc_scene_render_view": [_vp, _i, _P(_d), _i, _vp, _sz, _P(_u), _P(_u)],
     "asc_pose_store_set": [_P(_f), _P(_d)],
     "asc_pose_store_get": [_P(_f), _P(_d)],
     "asc_compose_pose": [_P(_f), _P(_d), _P(_d), _P(_d)],
@@ -235,6 +237,7 @@ _SIGS = {
     "asc_remote_port": [_vp, _P(_i)],
     "asc_remote_handle": [_vp, _vp, _sz, _vp, _sz, _P(_sz)],
     "asc_remote_destroy": [_vp],
+    "asc_remote_set_output_dir": [_vp, C.c_char_p],
     "asc_tiff_info": [C.c_char_p, _P(_i), _P(_i), _P(_i), _P(_i)],
     "asc_tiff_read": [C.c_char_p, _vp, _sz],
     "asc_read_camera": [C.c_char_p, _P(AscCamera)],
@@ -253,6 +256,9 @@ _SIGS = {
     "asc_trial_background": [_vp, _i, _P(_f), _sz],
     "asc_trial_close": [_vp],
     "asc_write_tra": [C.c_char_p, _P(_d), _i, _i, _i],
+    "asc_write_png": [C.c_char_p, _vp, _i, _i, _i],
+    "asc_write_view_png": [C.c_char_p, _P(_f), _i, _i],
+    "asc_write_full_drr_pgm": [C.c_char_p, _P(_f), _i, _i],
     "asc_read_tra": [C.c_char_p, _i, _i, _i, _P(_d), _P(_i)],
 }
 _RES = {"asc_last_error": C.c_char_p, "asc_context_stream": _vp}
@@ -647,6 +653,26 @@ class io:  # namespace: the trial's files (TIFF, MayaCam/VTKCam, .vie, .tra)
         _check(lib().asc_read_tra(_path(path), n_frames, n_volumes, int(flags), _ptr(out, _d), C.byref(got)))
         return out
 
+    @staticmethod
+    def write_png(path, pixels) -> None:
+        """8-bit (H, W) greyscale or (H, W, 3) RGB, top row first."""
+        p = np.ascontiguousarray(pixels, np.uint8)
+        ch = 1 if p.ndim == 2 else p.shape[2]
+        _check(lib().asc_write_png(_path(path), p.ctypes.data_as(_vp), p.shape[1], p.shape[0], ch))
+
+    @staticmethod
+    def write_view_png(path, rgb) -> None:
+        """A Scene.render_view composite (H, W, 3) float, row 0 at the bottom
+        as GL draws it: clamped, rounded to 8 bits, written top row first."""
+        p = _f32(rgb)
+        _check(lib().asc_write_view_png(_path(path), _ptr(p, _f), p.shape[1], p.shape[0]))
+
+    @staticmethod
+    def write_full_drr_pgm(path, image) -> None:
+        """save_full_drr's ASCII PGM of an (H, W) float image."""
+        p = _f32(image)
+        _check(lib().asc_write_full_drr_pgm(_path(path), _ptr(p, _f), p.shape[1], p.shape[0]))
+
 
 class Trial:
     """A trial configuration (.cfg): cameras, volumes and radiograph frames
@@ -730,6 +756,10 @@ class Remote:
         p = C.c_int()
         _check(lib().asc_remote_port(self.h, C.byref(p)))
         return p.value
+
+    def set_output_dir(self, path):
+        """Where message 12 (saveFullDRR) writes image_camNN.pgm."""
+        _check(lib().asc_remote_set_output_dir(self.h, _path(path)))
 
     def close(self):
         if self.h:
@@ -827,6 +857,11 @@ class Context:
     @property
     def stream(self) -> int:
         return lib().asc_context_stream(self.handle) or 0
+
+    def set_blocking(self, enable: bool):
+        """Make every per-operator call wait for its launch before returning
+        (the reference's OpenCL Kernel::launch finishes each kernel)."""
+        _check(lib().asc_context_set_blocking(self.handle, int(bool(enable))))
 
     def synchronize(self):
         _check(lib().asc_synchronize(self.handle))
@@ -1118,6 +1153,23 @@ class Scene:
         m = np.zeros((h, w), np.float32)
         _check(lib().asc_scene_last_images(self.handle, pose, camera, _ptr(d, _f), _ptr(r, _f), _ptr(m, _f)))
         return d, r, m
+
+    def render_view(self, camera: int, poses) -> np.ndarray:
+        """View::render of one camera over its whole image plane with every
+        volume at its pose (poses: one xyzypr per volume slot): the
+        Compositor's RGB, (H, W, 3) float32, row 0 as rendered."""
+        p = _f64(np.atleast_2d(poses))
+        w, h = C.c_uint(), C.c_uint()
+        _check(lib().asc_scene_render_view(self.handle, camera, _ptr(p, _d), p.shape[0], None, 0, C.byref(w),
+                                           C.byref(h)))
+        n = 3 * w.value * h.value
+        buf = self.ctx.alloc(4 * n)
+        try:
+            _check(lib().asc_scene_render_view(self.handle, camera, _ptr(p, _d), p.shape[0], buf.ptr, n, C.byref(w),
+                                               C.byref(h)))
+            return buf.download((h.value, w.value, 3))
+        finally:
+            buf.free()
 
     def track_frame_plain(self, pose, volume: int = 0):
         p = _f64(pose)

@@ -1445,4 +1445,78 @@ ASC_API int asc_scene_track_frame_plain(asc_scene s, int volume, const double xy
   return ASC_OK;
 }
 
+// View::render (View.cpp:355-394) over the camera's whole image plane, the
+// viewport Tracker::getFullDRR uses (Tracker.cpp:1056-1064): each volume's
+// DRR merged into one (Merger, min(a + b, 1)), the DRR filters, the
+// radiograph and its filters, the background mask (1 without one), the DRR
+// mask, then the Compositor's RGB (R = DRR, G = (DRR + rad) / 2 where the
+// background mask is >= 0.5, B = rad).
+ASC_API int asc_scene_render_view(asc_scene s, int camera, const double* poses, int n_poses, float* rgb,
+                                  size_t rgb_capacity, unsigned* width, unsigned* height)
+{
+  ASC_CHECK(s && (poses || n_poses == 0) && n_poses >= 0, ASC_ERR_INVALID, "bad argument");
+  for (size_t v = (size_t)n_poses; v < s->vols.size(); v++)
+    ASC_CHECK(!s->vols[v].in_use, ASC_ERR_INVALID, "a pose is needed for every volume slot in use");
+  ASC_CHECK(camera >= 0 && camera < (int)s->cams.size() && s->cams[camera].in_use, ASC_ERR_INVALID,
+            "bad camera index");
+  const SceneCamera& sc = s->cams[camera];
+  ASC_CHECK(sc.rad != nullptr, ASC_ERR_INVALID, "camera has no radiograph");
+  const asc_camera& cam = sc.cam;
+  const double vpd[4] = {-cam.viewport[2] / 2, -cam.viewport[3] / 2, cam.viewport[2], cam.viewport[3]};
+  const unsigned w = (unsigned)(vpd[2] * s->RW / cam.viewport[2]);
+  const unsigned h = (unsigned)(vpd[3] * s->RH / cam.viewport[3]);
+  if (width) *width = w;
+  if (height) *height = h;
+  if (!rgb) return ASC_OK;
+  const size_t n = (size_t)w * h;
+  ASC_CHECK(rgb_capacity >= 3 * n, ASC_ERR_TOO_LARGE, "rgb buffer smaller than 3 * width * height floats");
+  asc_context ctx = s->ctx;
+  int rc;
+  // drr, merged, drr filtered, rad, rad filtered, background, drr mask, scratch
+  if ((rc = s->p_bufs.ensure(n * 8))) return rc;
+  float* drr = s->p_bufs.p;
+  float* merged = drr + n;
+  float* drr_f = merged + n;
+  float* rad = drr_f + n;
+  float* rad_f = rad + n;
+  float* bgm = rad_f + n;
+  float* mask = bgm + n;
+  float* scratch = mask + n;
+  const float vp[4] = {(float)vpd[0], (float)vpd[1], (float)vpd[2], (float)vpd[3]};
+  const float plane[4] = {(float)cam.viewport[0], (float)cam.viewport[1], (float)cam.viewport[2],
+                          (float)cam.viewport[3]};
+  Frame cf;
+  std::memcpy(cf.r, cam.rotation, sizeof cf.r);
+  std::memcpy(cf.t, cam.translation, sizeof cf.t);
+  const Frame cf_inv = frame_inverse(cf);
+  if ((rc = launch_fill_f32(ctx, merged, 0.0f, n))) return rc;
+  for (size_t v = 0; v < s->vols.size(); v++) {
+    const SceneVolume& sv = s->vols[v];
+    if (!sv.in_use) continue;
+    // modelview = camera^-1 * pose; the ray caster takes its inverse (a
+    // volume outside the view leaves its DRR all zero: the rays miss it)
+    const Frame mv = frame_mul(cf_inv, frame_from_xyzypr(poses + 6 * v));
+    double m16[16];
+    frame_row_order(frame_inverse(mv), m16);
+    float imv[16];
+    inv_model_view(m16, sv.vol->desc.inv_scale, sv.vol->desc.inv_trans, imv);
+    if ((rc = launch_drr(ctx, sv.vol->dev, imv, vp, sv.step, sv.intensity, sv.cutoff, drr, w, h))) return rc;
+    if ((rc = launch_binary(ctx, BIN_MERGE, merged, drr, merged, n))) return rc;
+  }
+  if ((rc = asc_filter_chain(ctx, sc.chain[0].data(), (int)sc.chain[0].size(), merged, drr_f, scratch, (int)w,
+                             (int)h)))
+    return rc;
+  if ((rc = launch_rad(ctx, sc.rad->dev, plane, vp, rad, w, h, false, 0.0f))) return rc;
+  if ((rc = asc_filter_chain(ctx, sc.chain[1].data(), (int)sc.chain[1].size(), rad, rad_f, scratch, (int)w,
+                             (int)h)))
+    return rc;
+  if (sc.bg && sc.bg_threshold >= 0.0f)
+    rc = launch_rad(ctx, sc.bg->dev, plane, vp, bgm, w, h, true, sc.bg_threshold);
+  else
+    rc = launch_fill_f32(ctx, bgm, 1.0f, n);
+  if (rc) return rc;
+  if ((rc = launch_binary(ctx, BIN_DRR_MASK, drr_f, drr_f, mask, n))) return rc;
+  return launch_composite(ctx, drr_f, rad_f, bgm, rgb, n);
+}
+
 }  // extern "C"

@@ -251,7 +251,8 @@ void Tracker::load(const std::vector<VolumeSpec>& volumes, std::vector<Camera>& 
   render_height_ = render_height;
   // Tracker.cpp:281-290: describe each volume and centre its pivot
   for (const VolumeSpec& vs : volumes) {
-    volumeDescription_.emplace_back(new gpu::VolumeDescription(*vs.volume));
+    volumeDescription_.emplace_back(new gpu::VolumeDescription(*vs.volume, vs.storage));
+    if (vs.layout >= 0) check(asc_volume_set_layout(volumeDescription_.back()->image(), vs.layout));
     CoordFrame pivot = vs.pivot;
     pivot.translate(volumeDescription_.back()->transCenter());
     pivots_.push_back(pivot);

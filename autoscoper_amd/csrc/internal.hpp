@@ -81,6 +81,10 @@ struct asc_context_s {
   double* ncc_result = nullptr;  // 10 doubles
   double* ncc_host = nullptr;    // pinned
   float* conv_wts = nullptr;     // large Gaussian/sharpen weight sets
+  // When set, every per-operator entry point waits for its work before
+  // returning, as the reference's OpenCL Kernel::launch does (clFinish after
+  // each launch, OpenCL.cpp:978). Off by default: the calls stay asynchronous.
+  bool blocking = false;
 };
 
 struct asc_volume_s {

@@ -1246,3 +1246,139 @@ ASC_API int asc_read_tra(const char* path, int n_frames, int n_volumes, int flag
   if (frames_read) *frames_read = i;
   return ASC_OK;
 }
+
+// ---- display output ----------------------------------------------------------
+
+namespace {
+
+// CRC-32 (ISO 3309 / ITU-T V.42, reflected polynomial 0xEDB88320), as the
+// PNG specification's annex D computes it, table driven.
+uint32_t png_crc(const uint8_t* p, size_t n, uint32_t crc = 0xFFFFFFFFu)
+{
+  static uint32_t table[256];
+  static bool ready = false;
+  if (!ready) {
+    for (uint32_t i = 0; i < 256; i++) {
+      uint32_t c = i;
+      for (int k = 0; k < 8; k++) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+      table[i] = c;
+    }
+    ready = true;
+  }
+  for (size_t i = 0; i < n; i++) crc = table[(crc ^ p[i]) & 0xFF] ^ (crc >> 8);
+  return crc;
+}
+
+void put_be32(std::vector<uint8_t>& o, uint32_t v)
+{
+  for (int s = 24; s >= 0; s -= 8) o.push_back((uint8_t)(v >> s));
+}
+
+void png_chunk(std::vector<uint8_t>& o, const char* type, const std::vector<uint8_t>& data)
+{
+  put_be32(o, (uint32_t)data.size());
+  const size_t at = o.size();
+  o.insert(o.end(), type, type + 4);
+  o.insert(o.end(), data.begin(), data.end());
+  put_be32(o, png_crc(o.data() + at, o.size() - at) ^ 0xFFFFFFFFu);
+}
+
+// An 8-bit greyscale (channels 1) or RGB (channels 3) PNG whose image data
+// is a zlib stream (RFC 1950) of stored deflate blocks (RFC 1951 section
+// 3.2.4), each scanline with filter type 0.
+int write_png(const char* path, const uint8_t* pix, int w, int h, int channels)
+{
+  if (!path || !pix || w <= 0 || h <= 0 || (channels != 1 && channels != 3))
+    return fail(ASC_ERR_INVALID, "bad argument");
+  const size_t row = (size_t)w * channels;
+  std::vector<uint8_t> raw;
+  raw.reserve((row + 1) * h);
+  for (int y = 0; y < h; y++) {
+    raw.push_back(0);
+    raw.insert(raw.end(), pix + row * y, pix + row * (y + 1));
+  }
+  std::vector<uint8_t> z = {0x78, 0x01};
+  size_t at = 0;
+  do {
+    const size_t len = std::min<size_t>(65535, raw.size() - at);
+    z.push_back(at + len == raw.size() ? 1 : 0);
+    z.push_back((uint8_t)(len & 0xFF));
+    z.push_back((uint8_t)(len >> 8));
+    z.push_back((uint8_t)(~len & 0xFF));
+    z.push_back((uint8_t)((~len >> 8) & 0xFF));
+    z.insert(z.end(), raw.begin() + at, raw.begin() + at + len);
+    at += len;
+  } while (at < raw.size());
+  uint32_t a = 1, b = 0;  // Adler-32
+  for (uint8_t c : raw) {
+    a = (a + c) % 65521u;
+    b = (b + a) % 65521u;
+  }
+  put_be32(z, (b << 16) | a);
+  std::vector<uint8_t> out = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
+  std::vector<uint8_t> ihdr;
+  put_be32(ihdr, (uint32_t)w);
+  put_be32(ihdr, (uint32_t)h);
+  ihdr.insert(ihdr.end(), {8, (uint8_t)(channels == 3 ? 2 : 0), 0, 0, 0});
+  png_chunk(out, "IHDR", ihdr);
+  png_chunk(out, "IDAT", z);
+  png_chunk(out, "IEND", {});
+  std::FILE* f = std::fopen(path, "wb");
+  if (!f) return io_fail(std::string("cannot write ") + path);
+  const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+  return (std::fclose(f) == 0 && ok) ? ASC_OK : io_fail(std::string("write failed: ") + path);
+}
+
+}  // namespace
+
+ASC_API int asc_write_png(const char* path, const uint8_t* pixels, int width, int height, int channels)
+{
+  return write_png(path, pixels, width, height, channels);
+}
+
+// The composite as glDrawPixels(GL_RGB, GL_FLOAT) shows it: row 0 at the
+// bottom, each component clamped to [0, 1] and rounded to 8 bits.
+ASC_API int asc_write_view_png(const char* path, const float* rgb, int width, int height)
+{
+  if (!rgb || width <= 0 || height <= 0) return fail(ASC_ERR_INVALID, "bad argument");
+  const size_t row = 3 * (size_t)width;
+  std::vector<uint8_t> pix(row * height);
+  for (int y = 0; y < height; y++) {
+    const float* src = rgb + row * (height - 1 - y);
+    for (size_t i = 0; i < row; i++) {
+      const float v = src[i];
+      pix[row * y + i] = v > 0.0f ? (uint8_t)std::floor(std::min(v, 1.0f) * 255.0f + 0.5f) : 0;
+    }
+  }
+  return write_png(path, pix.data(), width, height, 3);
+}
+
+// save_full_drr (Tracker.cpp:155-204): the image stretched to 0..255 over
+// its own minimum and maximum (the maximum starting from the smallest
+// positive float), (int) truncation, stored inverted (255 - value, air
+// white) as an ASCII PGM, each value followed by a space, rows in memory
+// order. An image with no range (maximum == minimum) writes 255 everywhere.
+ASC_API int asc_write_full_drr_pgm(const char* path, const float* image, int width, int height)
+{
+  if (!path || !image || width <= 0 || height <= 0) return fail(ASC_ERR_INVALID, "bad argument");
+  const size_t n = (size_t)width * height;
+  float lo = std::numeric_limits<float>::max();
+  float hi = std::numeric_limits<float>::min();
+  for (size_t i = 0; i < n; i++) {
+    if (image[i] > hi) hi = image[i];
+    if (image[i] < lo) lo = image[i];
+  }
+  const float range = hi - lo;
+  std::string text = "P2\n" + std::to_string(width) + " " + std::to_string(height) + "\n255\n";
+  text.reserve(text.size() + 4 * n);
+  for (size_t i = 0; i < n; i++) {
+    unsigned char u = 0;
+    if (range > 0.0f) u = (unsigned char)(int)(255.0f * (image[i] - lo) / range);
+    text += std::to_string(255 - (int)u);
+    text += ' ';
+  }
+  std::FILE* f = std::fopen(path, "w");
+  if (!f) return io_fail(std::string("cannot write ") + path);
+  const bool ok = std::fwrite(text.data(), 1, text.size(), f) == text.size();
+  return (std::fclose(f) == 0 && ok) ? ASC_OK : io_fail(std::string("write failed: ") + path);
+}

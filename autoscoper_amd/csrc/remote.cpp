@@ -242,6 +242,9 @@ struct Session {
   double bg_threshold = -1.0;
   asc_scene scene = nullptr;
   bool scene_dirty = true;
+  int current_volume = 0;       // Trial::current_volume, set by optimizeFrame
+  std::string full_drr_dir = ".";  // where message 12 writes its images
+  int full_drr_count = 0;        // save_full_drr's running file number
 
   ~Session() { close_trial(); }
 
@@ -399,6 +402,7 @@ int optimize(Session& S, int volume, int frame, int dframe, int repeats, int met
   if (!S.trial || volume < 0 || volume >= (int)S.bones.size())
     return fail(ASC_ERR_INVALID, "optimize: bad volume");
   if (frame < 0 || frame >= S.info.n_frames) return fail(ASC_ERR_INVALID, "Tracker::optimize(): Invalid frame.");
+  S.current_volume = volume;  // AutoscoperMainWindow::optimizeFrame sets it first
   if (cf_model != 0) return fail(ASC_ERR_UNSUPPORTED, "only the NCC cost (cf_model 0) is built");
   S.frame = frame;
   int rc;
@@ -442,8 +446,12 @@ int optimize(Session& S, int volume, int frame, int dframe, int repeats, int met
 
 // Tracker::getImageData (Tracker.cpp:865-921): the whole camera view at
 // the trial's render size, as 8-bit radiograph, DRR and mask images.
-int image_data(Session& S, int volume, int camera, const double* pose, unsigned& w, unsigned& h,
-               std::vector<uint8_t>& out)
+// The masked DRR, radiograph and mask of one volume over the camera's whole
+// image plane, as Tracker::getImageData (Tracker.cpp:829-863) and
+// Tracker::getFullDRR (Tracker.cpp:1035-1086) render them: host planes
+// {rad, drr, mask}, each w x h.
+int render_full_view(Session& S, int volume, int camera, const double* pose, unsigned& w, unsigned& h,
+                     std::vector<float>& host)
 {
   int rc;
   if ((rc = ensure_scene(S))) return rc;
@@ -491,16 +499,48 @@ int image_data(Session& S, int volume, int camera, const double* pose, unsigned&
   if (!rc && use_bg)
     rc = asc_background_render(S.ctx, cs.bg, plane, vp, (float)S.bg_threshold, bgm, w, h);
   if (!rc) rc = asc_mask_multiply(S.ctx, drr, rad, use_bg ? bgm : nullptr, mask, w, h);
-  std::vector<float> host(3 * n);
+  host.assign(3 * n, 0.0f);
   if (!rc) rc = asc_synchronize(S.ctx);
   if (!rc) rc = asc_memcpy_d2h(S.ctx, host.data(), rad, n * sizeof(float));
   if (!rc) rc = asc_memcpy_d2h(S.ctx, host.data() + n, drr, n * sizeof(float));
   if (!rc) rc = asc_memcpy_d2h(S.ctx, host.data() + 2 * n, mask, n * sizeof(float));
   asc_free(S.ctx, buf);
+  return rc;
+}
+
+int image_data(Session& S, int volume, int camera, const double* pose, unsigned& w, unsigned& h,
+               std::vector<uint8_t>& out)
+{
+  std::vector<float> host;
+  const int rc = render_full_view(S, volume, camera, pose, w, h, host);
   if (rc) return rc;
   // get_image (Tracker.cpp:844-863): (unsigned char)(255 * value)
-  out.resize(3 * n);
-  for (size_t i = 0; i < 3 * n; i++) out[i] = (uint8_t)(int)(255 * host[i]);
+  out.resize(host.size());
+  for (size_t i = 0; i < host.size(); i++) out[i] = (uint8_t)(int)(255 * host[i]);
+  return ASC_OK;
+}
+
+// Tracker::getFullDRR (Tracker.cpp:1035-1086) of the current volume at its
+// pose curves' value at the current frame: per camera, the masked DRR over
+// the whole image plane, written by save_full_drr as image_camNN.pgm, NN
+// counting up over the session.
+int save_full_drr(Session& S)
+{
+  if (!S.trial) return fail(ASC_ERR_INVALID, "saveFullDRR: no trial");
+  const int v = S.current_volume;
+  if (v < 0 || v >= (int)S.bones.size()) return fail(ASC_ERR_INVALID, "saveFullDRR: bad volume");
+  double pose[6];
+  S.bones[v].curves.pose_at(S.frame, pose);
+  for (int c = 0; c < (int)S.cams.size(); c++) {
+    unsigned w, h;
+    std::vector<float> host;
+    int rc = render_full_view(S, v, c, pose, w, h, host);
+    if (rc) return rc;
+    char name[32];
+    std::snprintf(name, sizeof name, "image_cam%02d.pgm", S.full_drr_count++);
+    const std::string path = S.full_drr_dir + "/" + name;
+    if ((rc = asc_write_full_drr_pgm(path.c_str(), host.data() + (size_t)w * h, (int)w, (int)h))) return rc;
+  }
   return ASC_OK;
 }
 
@@ -742,7 +782,10 @@ bool handle(Session& S, const uint8_t* msg, size_t len, std::vector<uint8_t>& ou
             ASC_OK);
       break;
     }
-    case 12: reply(true); break;  // save the full DRR: display output, nothing to do here
+    case 12:  // save the full DRR images; the reference answers 12 whatever happened
+      save_full_drr(S);
+      reply(true);
+      break;
     case 13: return false;         // close the connection
     case 14: {
       reply(true);
@@ -889,6 +932,14 @@ ASC_API int asc_remote_handle(asc_remote r, const void* message, size_t length, 
   *reply_length = out.size();
   if (reply && capacity) std::memcpy(reply, out.data(), std::min(capacity, out.size()));
   return keep ? ASC_OK : 1;
+}
+
+ASC_API int asc_remote_set_output_dir(asc_remote r, const char* dir)
+{
+  if (!r || !dir || !*dir) return fail(ASC_ERR_INVALID, "bad argument");
+  std::lock_guard<std::mutex> lk(r->mu);
+  r->session.full_drr_dir = dir;
+  return ASC_OK;
 }
 
 ASC_API int asc_remote_destroy(asc_remote r)

@@ -236,6 +236,13 @@ ASC_API int asc_context_set_stream(asc_context ctx, void* s)
 
 ASC_API void* asc_context_stream(asc_context ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+ASC_API int asc_context_set_blocking(asc_context ctx, int enable)
+{
+  ASC_CHECK(valid_ctx(ctx), ASC_ERR_INVALID, "null context");
+  ctx->blocking = enable != 0;
+  return ASC_OK;
+}
+
 ASC_API int asc_synchronize(asc_context ctx)
 {
   ASC_CHECK(valid_ctx(ctx), ASC_ERR_INVALID, "null context");
@@ -675,18 +682,26 @@ ASC_API int asc_view_geometry(const asc_camera* cam, const double xyzypr[6], con
 
 // ---- per-operator entry points --------------------------------------------
 
+// A blocking context waits for each launch before the call returns.
+static int finish(asc_context ctx, int rc)
+{
+  if (rc != ASC_OK || !ctx->blocking) return rc;
+  ASC_HIP(hipStreamSynchronize(ctx->stream));
+  return ASC_OK;
+}
+
 ASC_API int asc_drr_render(asc_context ctx, asc_volume vol, const float imv[12], const float viewport[4],
                            float step, float intensity, float cutoff, float* out, unsigned width, unsigned height)
 {
   ASC_CHECK(valid_ctx(ctx) && vol && imv && viewport && out, ASC_ERR_INVALID, "bad argument");
-  return launch_drr(ctx, vol->dev, imv, viewport, step, intensity, cutoff, out, width, height);
+  return finish(ctx, launch_drr(ctx, vol->dev, imv, viewport, step, intensity, cutoff, out, width, height));
 }
 
 ASC_API int asc_rad_render(asc_context ctx, asc_image img, const float image_plane[4], const float viewport[4],
                            float* out, unsigned width, unsigned height)
 {
   ASC_CHECK(valid_ctx(ctx) && img && image_plane && viewport && out, ASC_ERR_INVALID, "bad argument");
-  return launch_rad(ctx, img->dev, image_plane, viewport, out, width, height, false, 0.0f);
+  return finish(ctx, launch_rad(ctx, img->dev, image_plane, viewport, out, width, height, false, 0.0f));
 }
 
 ASC_API int asc_background_render(asc_context ctx, asc_image img, const float image_plane[4],
@@ -694,7 +709,7 @@ ASC_API int asc_background_render(asc_context ctx, asc_image img, const float im
                                   unsigned height)
 {
   ASC_CHECK(valid_ctx(ctx) && img && image_plane && viewport && out, ASC_ERR_INVALID, "bad argument");
-  return launch_rad(ctx, img->dev, image_plane, viewport, out, width, height, true, threshold);
+  return finish(ctx, launch_rad(ctx, img->dev, image_plane, viewport, out, width, height, true, threshold));
 }
 
 ASC_API int asc_sobel(asc_context ctx, const float* in, float* out, int width, int height, float scale,
@@ -702,7 +717,7 @@ ASC_API int asc_sobel(asc_context ctx, const float* in, float* out, int width, i
 {
   ASC_CHECK(valid_ctx(ctx) && in && out, ASC_ERR_INVALID, "bad argument");
   asc_filter f{ASC_FILTER_SOBEL, 1, {scale, blend, 0, 0}};
-  return launch_filter(ctx, f, in, out, width, height);
+  return finish(ctx, launch_filter(ctx, f, in, out, width, height));
 }
 
 ASC_API int asc_contrast(asc_context ctx, const float* in, float* out, int width, int height, float alpha,
@@ -710,7 +725,7 @@ ASC_API int asc_contrast(asc_context ctx, const float* in, float* out, int width
 {
   ASC_CHECK(valid_ctx(ctx) && in && out, ASC_ERR_INVALID, "bad argument");
   asc_filter f{ASC_FILTER_CONTRAST, 1, {alpha, beta, (float)size, 0}};
-  return launch_filter(ctx, f, in, out, width, height);
+  return finish(ctx, launch_filter(ctx, f, in, out, width, height));
 }
 
 ASC_API int asc_gaussian_weights(float radius, float* out, int capacity, int* size)
@@ -727,7 +742,7 @@ ASC_API int asc_gaussian(asc_context ctx, const float* in, float* out, int width
 {
   ASC_CHECK(valid_ctx(ctx) && in && out, ASC_ERR_INVALID, "bad argument");
   asc_filter f{ASC_FILTER_GAUSSIAN, 1, {radius, 0, 0, 0}};
-  return launch_filter(ctx, f, in, out, width, height);
+  return finish(ctx, launch_filter(ctx, f, in, out, width, height));
 }
 
 ASC_API int asc_sharpen(asc_context ctx, const float* in, float* out, int width, int height, float radius,
@@ -735,14 +750,14 @@ ASC_API int asc_sharpen(asc_context ctx, const float* in, float* out, int width,
 {
   ASC_CHECK(valid_ctx(ctx) && in && out, ASC_ERR_INVALID, "bad argument");
   asc_filter f{ASC_FILTER_SHARPEN, 1, {radius, contrast, threshold, 0}};
-  return launch_filter(ctx, f, in, out, width, height);
+  return finish(ctx, launch_filter(ctx, f, in, out, width, height));
 }
 
 ASC_API int asc_filter_apply(asc_context ctx, const asc_filter* f, const float* in, float* out, int width,
                              int height)
 {
   ASC_CHECK(valid_ctx(ctx) && f && in && out, ASC_ERR_INVALID, "bad argument");
-  return launch_filter(ctx, *f, in, out, width, height);
+  return finish(ctx, launch_filter(ctx, *f, in, out, width, height));
 }
 
 // View::filter, View.cpp:520-554: with an odd number of filters the first
@@ -753,14 +768,14 @@ ASC_API int asc_filter_chain(asc_context ctx, const asc_filter* filters, int cou
 {
   ASC_CHECK(valid_ctx(ctx) && in && out && (count == 0 || filters), ASC_ERR_INVALID, "bad argument");
   const size_t bytes = sizeof(float) * (size_t)width * height;
-  if (count == 0) return launch_copy(ctx, out, in, bytes);
+  if (count == 0) return finish(ctx, launch_copy(ctx, out, in, bytes));
   ASC_CHECK(count == 1 || scratch, ASC_ERR_INVALID, "scratch buffer required for chains longer than 1");
   float* b1 = (count % 2) ? out : scratch;
   float* b2 = (count % 2) ? scratch : out;
-  int rc = launch_filter(ctx, filters[0], in, b1, width, height);
+  int rc = finish(ctx, launch_filter(ctx, filters[0], in, b1, width, height));
   if (rc) return rc;
   for (int k = 1; k < count; k++) {
-    rc = launch_filter(ctx, filters[k], b1, b2, width, height);
+    rc = finish(ctx, launch_filter(ctx, filters[k], b1, b2, width, height));
     if (rc) return rc;
     std::swap(b1, b2);
   }
@@ -770,28 +785,28 @@ ASC_API int asc_filter_chain(asc_context ctx, const asc_filter* filters, int cou
 ASC_API int asc_drr_background(asc_context ctx, const float* src, float* dest, unsigned width, unsigned height)
 {
   ASC_CHECK(valid_ctx(ctx) && src && dest, ASC_ERR_INVALID, "bad argument");
-  return launch_binary(ctx, BIN_DRR_MASK, src, src, dest, (size_t)width * height);
+  return finish(ctx, launch_binary(ctx, BIN_DRR_MASK, src, src, dest, (size_t)width * height));
 }
 
 ASC_API int asc_multiply(asc_context ctx, const float* a, const float* b, float* dest, unsigned width,
                          unsigned height)
 {
   ASC_CHECK(valid_ctx(ctx) && a && b && dest, ASC_ERR_INVALID, "bad argument");
-  return launch_binary(ctx, BIN_MULTIPLY, a, b, dest, (size_t)width * height);
+  return finish(ctx, launch_binary(ctx, BIN_MULTIPLY, a, b, dest, (size_t)width * height));
 }
 
 ASC_API int asc_mask_multiply(asc_context ctx, float* drr, float* rad, const float* bg, float* mask,
                               unsigned width, unsigned height)
 {
   ASC_CHECK(valid_ctx(ctx) && drr && rad, ASC_ERR_INVALID, "bad argument");
-  return launch_mask_multiply(ctx, drr, rad, bg, mask, (size_t)width * height);
+  return finish(ctx, launch_mask_multiply(ctx, drr, rad, bg, mask, (size_t)width * height));
 }
 
 ASC_API int asc_ncc(asc_context ctx, const float* f, const float* g, const float* mask, unsigned n, double* ncc,
                     double moments[9])
 {
   ASC_CHECK(valid_ctx(ctx) && f && g && mask, ASC_ERR_INVALID, "bad argument");
-  return ncc_moments(ctx, f, g, mask, n, moments, ncc);
+  return ncc_moments(ctx, f, g, mask, n, moments, ncc);  // waits for its result already
 }
 
 ASC_API int asc_ncc_init(asc_context ctx, unsigned max_n)
@@ -812,7 +827,7 @@ ASC_API int asc_merge(asc_context ctx, const float* a, const float* b, float* de
                       unsigned height)
 {
   ASC_CHECK(valid_ctx(ctx) && a && b && dest, ASC_ERR_INVALID, "bad argument");
-  return launch_binary(ctx, BIN_MERGE, a, b, dest, (size_t)width * height);
+  return finish(ctx, launch_binary(ctx, BIN_MERGE, a, b, dest, (size_t)width * height));
 }
 
 ASC_API int asc_composite(asc_context ctx, const float* src1, const float* src2, const float* src3,
@@ -820,7 +835,7 @@ ASC_API int asc_composite(asc_context ctx, const float* src1, const float* src2,
 {
   ASC_CHECK(valid_ctx(ctx) && src1 && src2 && src3 && dest, ASC_ERR_INVALID, "bad argument");
   (void)src4;  // unused by the reference kernel as well (Compositor.cl:5)
-  return launch_composite(ctx, src1, src2, src3, dest, (size_t)width * height);
+  return finish(ctx, launch_composite(ctx, src1, src2, src3, dest, (size_t)width * height));
 }
 
 ASC_API int asc_pose_store_set(float store[7], const double xyzypr[6])

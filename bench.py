@@ -9,9 +9,10 @@ targets rendered at a hidden pose. One step = one PSO-sized batch of
 reference's PSO start range) x 2 cameras evaluated by the fused path, inputs
 resident in HBM. Prints one JSON line.
 
-Beside it: the plain one-eval-at-a-time GPU form (one launch per operator,
-blocking NCC per camera) and the CPU oracle on the host cores (bounded
-sample).
+Beside it: the plain one-eval-at-a-time GPU forms (the scene's, and
+xromm::Tracker::trackFrame through the C++ facade in the reference's launch
+order, asynchronous and with a finish after every launch) and the CPU oracle
+on the host cores (bounded sample).
 """
 from __future__ import annotations
 
@@ -297,6 +298,52 @@ def storage_variant(A, synth, ctx, args, batches, bone_of, storage):
             "kernel_ms": {"drr_tiles": kern_ms[0] / args.steps, "post_tiles": kern_ms[1] / args.steps}}
 
 
+def plain_facade(synth, wl, cams, rads, poses, evals: int):
+    """The reference's own form of the cost: xromm::Tracker::trackFrame of the
+    C++ facade (tests/cpp/facade_check.bin), one pose at a time, one launch
+    per operator in the reference's order (DRR, DRR filters, radiograph,
+    radiograph filters, background, DRR mask, three multiplies, NCC with its
+    blocking read-back). Timed as it runs by default and with every
+    per-operator call finishing its kernel first (the OpenCL backend's
+    clFinish after each launch), with the volume as the reference keeps it
+    (u16, here in the densest layout) and as a plain FP32 copy in the linear
+    layout. Returns None when the program is not built."""
+    import subprocess
+    import tempfile
+
+    root = os.path.dirname(os.path.abspath(__file__))
+    exe = os.path.join(root, "tests", "cpp", "facade_check.bin")
+    if not os.path.exists(exe):
+        return None
+    D, H, W = wl.volume.shape
+    RW, RH = wl.render
+    rh, rw = rads[0][1].shape
+    env = dict(os.environ)
+    env["LD_LIBRARY_PATH"] = os.path.join(root, "autoscoper_amd") + os.pathsep + env.get("LD_LIBRARY_PATH", "")
+    res = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        inp = os.path.join(tmp, "facade_in.bin")
+        with open(inp, "wb") as f:
+            np.array([W, H, D, len(cams), RW, RH, len(poses), rw, rh], np.int32).tofile(f)
+            np.array([wl.voxel_mm], np.float64).tofile(f)
+            np.ascontiguousarray(wl.volume, np.uint16).tofile(f)
+            np.asarray([c.to_array() for c in cams], np.float64).tofile(f)
+            for r in rads:
+                np.ascontiguousarray(r[1], np.uint16).tofile(f)
+            np.asarray(poses, np.float64).tofile(f)
+            np.array([synth.DRR_STEP, synth.DRR_INTENSITY, synth.DRR_CUTOFF], np.float32).tofile(f)
+        runs = {"u16": [], "u16_blocking": ["--blocking"],
+                "f32_linear": ["--storage", "f32", "--layout", "linear"],
+                "f32_linear_blocking": ["--storage", "f32", "--layout", "linear", "--blocking"]}
+        for key, extra in runs.items():
+            out = subprocess.run([exe, inp, "--time", str(evals), *extra], capture_output=True, text=True,
+                                 timeout=300, env=env)
+            if out.returncode != 0:
+                raise RuntimeError(f"facade_check failed: {out.stderr.strip()}")
+            res[key] = json.loads(out.stdout.strip().splitlines()[-1])["evals_per_s"]
+    return res
+
+
 def workload_name(args, wl, cams, P, multi) -> str:
     RW, RH = wl.render
     if multi:
@@ -321,6 +368,8 @@ def main(argv=None) -> int:
     ap.add_argument("--unroll", type=int, default=0, help="ray-march unroll (tuning; 0 = library default)")
     ap.add_argument("--grid", type=int, default=0, help="persistent grid in blocks (tuning; 0 = library default)")
     ap.add_argument("--plain-evals", type=int, default=10)
+    ap.add_argument("--facade-evals", type=int, default=300,
+                    help="pose evaluations timed through the C++ facade's trackFrame (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=6.0,
@@ -398,6 +447,10 @@ def main(argv=None) -> int:
         scene.track_frame_plain(batches[args.warmup][k % P], volume=int(bone_of[k % P]))
     plain_s = args.plain_evals / (time.perf_counter() - t0)
     log(f"[bench] plain form {plain_s:.1f} evals/s")
+    facade = None
+    if args.facade_evals > 0 and not multi:
+        facade = plain_facade(synth, wl, cams, rads, batches[args.warmup], args.facade_evals)
+        log(f"[bench] facade trackFrame {facade} evals/s")
 
     cpu = None
     if not args.no_cpu and not multi:
@@ -465,11 +518,21 @@ def main(argv=None) -> int:
         "cpu_baseline": cpu,
         "plain_per_eval": {"value": plain_s, "unit": "pose-evals/s",
                            "speedup_of_fused": evals_s / plain_s if plain_s > 0 else None},
+        "plain_facade": None if facade is None else {
+            "form": "xromm::Tracker::trackFrame through the C++ facade: one pose, one launch per operator in the "
+                    "reference's order, blocking NCC read-back per camera; 'blocking' also waits after every "
+                    "launch (the OpenCL backend's clFinish); f32_linear: FP32 volume copy, linear layout",
+            "value": facade["f32_linear_blocking"], "unit": "pose-evals/s", "evals": args.facade_evals,
+            "rates": facade,
+            "speedup_of_fused": evals_s / facade["f32_linear_blocking"]},
         "per_eval": {"bbox_pixels": int((info[:, 1] * info[:, 2]).sum() / P),
                      "ray_samples": float(last_samples) / P, "tiles": int(tiles / P)},
     }
     if args.variants and not multi and args.storage == "u16":
         out["storage_variants"] = {st: storage_variant(A, synth, ctx, args, batches, bone_of, st) for st in ("f16",)}
+        if facade is not None:
+            out["plain_facade"]["speedup_of_fused_f16"] = (out["storage_variants"]["f16"]["value"]
+                                                           / facade["f32_linear_blocking"])
     if args.serve_seconds > 0:
         scene.close()
         for v in vols:

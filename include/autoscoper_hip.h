@@ -99,6 +99,11 @@ ASC_API int asc_context_destroy(asc_context ctx);
 /* Install a caller-owned hipStream_t (NULL restores the context's own). */
 ASC_API int asc_context_set_stream(asc_context ctx, void* hip_stream);
 ASC_API void* asc_context_stream(asc_context ctx);
+/* Blocking mode: each per-operator call (asc_drr_render ... asc_composite)
+ * waits for its launch to finish before it returns, as the reference's
+ * OpenCL backend does after every kernel (Kernel::launch, clFinish,
+ * OpenCL.cpp:978). Off by default; scenes are not affected. */
+ASC_API int asc_context_set_blocking(asc_context ctx, int enable);
 ASC_API int asc_synchronize(asc_context ctx);
 
 ASC_API int asc_malloc(asc_context ctx, size_t bytes, void** dptr);
@@ -326,6 +331,16 @@ ASC_API int asc_scene_last_images(asc_scene scene, int pose, int camera, float* 
  * operator and a blocking NCC readback per camera, as trackFrame does. */
 ASC_API int asc_scene_track_frame_plain(asc_scene scene, int volume, const double xyzypr[6], double* per_camera,
                                         double* cost);
+/* Display composite of one scene camera over its whole image plane
+ * (View::render, View.cpp:355-394, with the viewport of Tracker::getFullDRR,
+ * Tracker.cpp:1056-1064): every volume's DRR at its pose (poses: n_poses x 6
+ * doubles, xyzypr, one per scene volume slot) merged, the DRR filters, the radiograph and
+ * its filters, the background mask, then the Compositor's 3 floats per
+ * pixel into device memory `rgb` (capacity in floats). The size is
+ * (unsigned)(viewport_w * render_w / viewport_w) by the same for h, returned
+ * in width/height; rgb == NULL only reports it. */
+ASC_API int asc_scene_render_view(asc_scene scene, int camera, const double* poses, int n_poses, float* rgb,
+                                  size_t rgb_capacity, unsigned* width, unsigned* height);
 
 /* ---- pose store helpers (KeyCurve<float> / Quatf, KeyCurve.hpp:330-356,
  * Quaternion.hpp:80-94,220-243) -------------------------------------------- */
@@ -542,6 +557,18 @@ enum { ASC_TRA_MATRIX = 1, ASC_TRA_ROWS = 2, ASC_TRA_COMMAS = 4, ASC_TRA_CM = 8,
 ASC_API int asc_write_tra(const char* path, const double* poses, int n_frames, int n_volumes, int flags);
 ASC_API int asc_read_tra(const char* path, int n_frames, int n_volumes, int flags, double* poses, int* frames_read);
 
+/* Display output, headless (the GUI shows the composite with
+ * glDrawPixels(GL_RGB, GL_FLOAT), GLView.cpp:473-527).
+ * asc_write_png: 8-bit greyscale (channels 1) or RGB (3) rows, top first.
+ * asc_write_view_png: an asc_scene_render_view composite copied to the
+ *   host (3 floats per pixel, row 0 at the bottom as GL draws it), clamped
+ *   to [0, 1] and rounded to 8 bits, written top row first.
+ * asc_write_full_drr_pgm: save_full_drr (Tracker.cpp:155-204), the image of
+ *   Tracker::getFullDRR: min-max stretched, inverted, ASCII PGM. */
+ASC_API int asc_write_png(const char* path, const uint8_t* pixels, int width, int height, int channels);
+ASC_API int asc_write_view_png(const char* path, const float* rgb, int width, int height);
+ASC_API int asc_write_full_drr_pgm(const char* path, const float* image, int width, int height);
+
 /* ---- remote control: the reference's socket protocol, version 2 ----------- */
 /* The server side of autoscoper/src/net/Socket.cpp:53-444 (the protocol
  * PyAutoscoper speaks, scripts/python/PyAutoscoper/connect.py): a session
@@ -563,6 +590,10 @@ ASC_API int asc_remote_port(asc_remote remote, int* port);
  * (close the connection). */
 ASC_API int asc_remote_handle(asc_remote remote, const void* message, size_t length, void* reply, size_t capacity,
                               size_t* reply_length);
+/* Directory for message 12 (saveFullDRR: Tracker::getFullDRR of the
+ * current volume, one image_camNN.pgm per camera, NN counting up over the
+ * session; default "."). The reference writes to a fixed path. */
+ASC_API int asc_remote_set_output_dir(asc_remote remote, const char* dir);
 ASC_API int asc_remote_destroy(asc_remote remote);
 
 #ifdef __cplusplus

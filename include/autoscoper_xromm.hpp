@@ -116,6 +116,10 @@ class XROMM_API Tracker {
   struct VolumeSpec {
     const Volume* volume;
     CoordFrame pivot;  // Trial::getVolumeMatrix before the transCenter shift
+    // device copy of the volume (an extension: the reference always keeps
+    // the file's texel type): ASC_STORAGE_* and ASC_LAYOUT_* (-1 = densest)
+    int storage = ASC_STORAGE_U16;
+    int layout = -1;
   };
   Tracker();
   ~Tracker();

@@ -3,7 +3,17 @@
 // the one-pose-at-a-time path (View/RayCaster/filters/ncc) and of the fused
 // batch path as JSON lines.
 //   facade_check <input.bin>
+// With --time N it instead times N trackFrame calls (cycling over the input
+// poses, after two untimed ones) and prints the rate; --blocking makes every
+// per-operator call wait for its kernel, as the reference's OpenCL backend
+// does after each launch.
+// --storage and --layout choose the device copy of the volume (default u16,
+// the densest layout).
+//   facade_check <input.bin> --time N [--blocking] [--storage u16|f16|bf16|f32]
+//                [--layout linear|quad|octet|brick]
+#include <chrono>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <cstdio>
 #include <fstream>
@@ -20,6 +30,29 @@ static void rd(std::ifstream& f, T* p, size_t n)
 int main(int argc, char** argv)
 {
   if (argc < 2) return 2;
+  int time_n = 0;
+  bool blocking = false;
+  int storage = ASC_STORAGE_U16, layout = -1;
+  static const char* const kStorages[] = {"u16", "f16", "bf16", "f32"};
+  static const char* const kLayouts[] = {"linear", "quad", "octet", "brick"};
+  auto pick = [](const char* s, const char* const* names, int n) {
+    for (int i = 0; i < n; i++)
+      if (!std::strcmp(s, names[i])) return i;
+    return -2;
+  };
+  for (int a = 2; a < argc; a++) {
+    if (!std::strcmp(argv[a], "--time") && a + 1 < argc)
+      time_n = std::atoi(argv[++a]);
+    else if (!std::strcmp(argv[a], "--blocking"))
+      blocking = true;
+    else if (!std::strcmp(argv[a], "--storage") && a + 1 < argc)
+      storage = pick(argv[++a], kStorages, 4);
+    else if (!std::strcmp(argv[a], "--layout") && a + 1 < argc)
+      layout = pick(argv[++a], kLayouts, 4);
+    else
+      return 2;
+    if (storage < 0 || layout < -1) return 2;
+  }
   std::ifstream f(argv[1], std::ios::binary);
   int32_t hdr[9];
   rd(f, hdr, 9);
@@ -53,7 +86,7 @@ int main(int argc, char** argv)
       rp.push_back(&rads[(size_t)c * rw * rh]);
       rs.emplace_back(rw, rh);
     }
-    xromm::Tracker::VolumeSpec vs{&volume, xromm::CoordFrame()};
+    xromm::Tracker::VolumeSpec vs{&volume, xromm::CoordFrame(), storage, layout};
     tracker.load({vs}, cameras, rp, rs, 16, RW, RH);
     tracker.setDrrParams(0, drr[0], drr[1], drr[2]);
     for (size_t i = 0; i < tracker.numViews(); i++) {
@@ -69,6 +102,18 @@ int main(int argc, char** argv)
       c2->set_beta(0.9f);
       tracker.view((int)i)->radFilters().push_back(s2);
       tracker.view((int)i)->radFilters().push_back(c2);
+    }
+    if (time_n > 0) {
+      if (blocking && asc_context_set_blocking(xromm::gpu::context(), 1) != 0)
+        throw xromm::gpu::Error(-1, asc_last_error());
+      double sink = 0;
+      for (int i = 0; i < 2; i++) sink += tracker.trackFrame(0, &poses[(size_t)(i % npose) * 6])[0];
+      const auto t0 = std::chrono::steady_clock::now();
+      for (int i = 0; i < time_n; i++) sink += tracker.trackFrame(0, &poses[(size_t)(i % npose) * 6])[0];
+      const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      std::printf("{\"evals\": %d, \"seconds\": %.9g, \"evals_per_s\": %.9g, \"blocking\": %s, \"storage\": \"%s\", \"layout\": %d, \"sink\": %.9g}\n",
+                  time_n, s, time_n / s, blocking ? "true" : "false", kStorages[storage], layout, sink);
+      return 0;
     }
     // fused batch of all poses (absolute poses through a zero manipulator)
     std::vector<double> fused((size_t)npose * ncam);

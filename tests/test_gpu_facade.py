@@ -71,3 +71,28 @@ def test_facade_tracker_matches_oracle(asc, ctx, orc, tiny, tmp_path):
                 assert plain[c] == 9999.0 and fused[c] == 9999.0
     comp = [r for r in rows if "composite_sum" in r]
     assert comp and np.isfinite(comp[0]["composite_sum"]) and comp[0]["composite_sum"] > 0
+
+
+def test_facade_timing_mode(tiny, tmp_path):
+    """The timing mode bench.py uses: trackFrame N times, asynchronous and
+    with every per-operator call finishing its kernel (blocking context),
+    with the u16 volume and with an FP32 copy in the linear layout (whose
+    texels are the normalised u/65535 rounded to FP32, so its costs move by
+    float roundings only)."""
+    if not BIN.exists():
+        pytest.fail(f"{BIN} is not built (run __graft_entry__.build())")
+    poses = tiny["wl"].gt_poses[:2]
+    inp = tmp_path / "facade_in.bin"
+    write_input(inp, tiny, poses)
+    env = dict(os.environ)
+    env["LD_LIBRARY_PATH"] = str(ROOT / "autoscoper_amd") + os.pathsep + env.get("LD_LIBRARY_PATH", "")
+    sinks = []
+    for extra in ([], ["--blocking"], ["--blocking", "--storage", "f32", "--layout", "linear"]):
+        out = subprocess.run([str(BIN), str(inp), "--time", "6", *extra], capture_output=True, text=True,
+                             timeout=100, env=env)
+        assert out.returncode == 0, out.stderr
+        row = json.loads(out.stdout.strip().splitlines()[-1])
+        assert row["evals"] == 6 and row["evals_per_s"] > 0 and row["blocking"] == bool(extra)
+        sinks.append(row["sink"])
+    assert sinks[0] == sinks[1]  # blocking changes timing only
+    assert abs(sinks[2] - sinks[0]) <= 1e-6 * abs(sinks[0])
