@@ -21,6 +21,7 @@ __all__ = [
     "lib", "AscFilter", "AscCamera", "AscVolumeDesc", "AscOptimizeOptions", "AscOptimizeStats",
     "Context", "Volume", "Image", "Scene", "DeviceArray", "filters", "geometry", "io", "Trial", "Server", "Remote",
     "STORAGE_U16", "STORAGE_F16", "STORAGE_BF16", "STORAGE_F32",
+    "NCC_MOMENTS", "NCC_TREE_CUDA", "NCC_TREE_OPENCL",
     "FILTER_CONTRAST", "FILTER_SOBEL", "FILTER_MEDIAN", "FILTER_GAUSSIAN", "FILTER_SHARPEN",
     "AscError",
 ]
@@ -29,6 +30,7 @@ _HERE = Path(__file__).resolve().parent
 LIB_PATH = _HERE / "libautoscoper_hip.so"
 
 STORAGE_U16, STORAGE_F16, STORAGE_BF16, STORAGE_F32 = 0, 1, 2, 3
+NCC_MOMENTS, NCC_TREE_CUDA, NCC_TREE_OPENCL = 0, 1, 2
 FILTER_CONTRAST, FILTER_SOBEL, FILTER_MEDIAN, FILTER_GAUSSIAN, FILTER_SHARPEN = 0, 1, 2, 3, 4
 
 
@@ -146,6 +148,7 @@ _SIGS = {
     "asc_context_destroy": [_vp],
     "asc_context_set_stream": [_vp, _vp],
     "asc_context_set_blocking": [_vp, C.c_int],
+    "asc_context_set_ncc_form": [_vp, C.c_int],
     "asc_context_stream": [_vp],
     "asc_synchronize": [_vp],
     "asc_malloc": [_vp, _sz, _P(_vp)],
@@ -857,6 +860,11 @@ class Context:
     @property
     def stream(self) -> int:
         return lib().asc_context_stream(self.handle) or 0
+
+    def set_ncc_form(self, form: int):
+        """NCC_MOMENTS (default), or the reference's float tree form
+        NCC_TREE_CUDA / NCC_TREE_OPENCL, for ncc() and the C++ facade."""
+        _check(lib().asc_context_set_ncc_form(self.handle, int(form)))
 
     def set_blocking(self, enable: bool):
         """Make every per-operator call wait for its launch before returning

@@ -231,50 +231,83 @@ struct LdsGet {
   __device__ __forceinline__ float operator()(int x, int y) const { return buf[(y - oy) * R + (x - ox)]; }
 };
 
+// Visit every in-image pixel of the side x side square at (rx0, ry0), one
+// flat index per thread (side <= 56: the float quotient k / side is exact
+// to far better than the 1 / (2 side) margin its floor needs).
+template <class F>
+__device__ __forceinline__ void for_region(int side, int rx0, int ry0, int w, int h, F&& fn)
+{
+  const int n = side * side;
+  const float inv = 1.0f / (float)side;
+  for (int k = threadIdx.x; k < n; k += kBlock) {
+    const int yy = (int)(((float)k + 0.5f) * inv);
+    const int xx = k - yy * side;
+    const int px = rx0 + xx, py = ry0 + yy;
+    if (px >= 0 && px < w && py >= 0 && py < h) fn(xx, yy, px, py);
+  }
+}
+
 // Run a filter chain over the region in LDS (row stride R). On entry `cur`
 // holds stage-0 values at every in-image position of the post tile grown by
 // the chain's halo; on exit the returned buffer holds the chain output for
-// the tile. Thread (tx, ty) = (tid % 32, tid / 32) walks the output
-// rectangle in 32 x 8 steps (no division).
+// the tile. Each stage's kind is uniform, so each has a loop of its own;
+// Sobel and the 3 x 3 Contrast read a pixel's neighbourhood straight from
+// LDS where it lies inside the image (the same values the clamped form
+// reads there).
 template <int R>
 __device__ float* run_chain(const ChainDev& ch, const float* __restrict__ weights, float* cur, float* nxt, int x0,
                             int y0, int w, int h)
 {
   const int H0 = ch.halo;
   const int ox = x0 - H0, oy = y0 - H0;  // region origin
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   int remaining = H0;
   for (int k = 0; k < ch.n; k++) {
     const asc_filter& f = ch.f[k];
-    const int r = ch.radius[k];
-    remaining -= r;
+    remaining -= ch.radius[k];
     // output rectangle: tile grown by `remaining`
     const int rx0 = x0 - remaining, ry0 = y0 - remaining;
     const int side = kPTile + 2 * remaining;
     const LdsGet<R> get{cur, ox, oy};
-    for (int yy = ty; yy < side; yy += 8) {
-      const int py = ry0 + yy;
-      if (py < 0 || py >= h) continue;
-      for (int xx = tx; xx < side; xx += 32) {
-        const int px = rx0 + xx;
-        if (px < 0 || px >= w) continue;
+    float* const out = nxt;
+    const float* const in = cur;
+    auto at = [&](int px, int py) { return (py - oy) * R + (px - ox); };
+    if (f.enabled && f.type == ASC_FILTER_SOBEL) {
+      const float scale = f.p[0], blend = f.p[1];
+      for_region(side, rx0, ry0, w, h, [&](int, int, int px, int py) {
+        const int c = at(px, py);
         float v;
-        if (!f.enabled) {
-          v = get(px, py);
-        } else if (f.type == ASC_FILTER_SOBEL) {
-          v = dev::sobel_px(get, px, py, w, h, f.p[0], f.p[1]);
-        } else if (f.type == ASC_FILTER_CONTRAST) {
-          v = dev::contrast_px(get, px, py, w, h, f.p[0], f.p[1], (int)f.p[2]);
-        } else if (ch.gn[k] <= 1) {
-          v = get(px, py);
-        } else {
-          const float c = dev::conv_px(get, px, py, w, h, weights + ch.woff[k], ch.gn[k]);
-          v = f.type == ASC_FILTER_SHARPEN
-                ? dev::sharpen_finish(get(px, py), c, f.p[1] < 1.0f ? 1.0f : f.p[1], f.p[2])
-                : dev::gaussian_finish(c);
-        }
-        nxt[(py - oy) * R + (px - ox)] = v;
-      }
+        if (px >= 1 && px <= w - 2 && py >= 1 && py <= h - 2)
+          v = dev::sobel_core(in[c - R - 1], in[c - R], in[c - R + 1], in[c - 1], in[c], in[c + 1], in[c + R - 1],
+                              in[c + R], in[c + R + 1], scale, blend);
+        else
+          v = dev::sobel_px(get, px, py, w, h, scale, blend);
+        out[c] = v;
+      });
+    } else if (f.enabled && f.type == ASC_FILTER_CONTRAST) {
+      const float alpha = f.p[0], beta = f.p[1];
+      const int size = (int)f.p[2];
+      for_region(side, rx0, ry0, w, h, [&](int, int, int px, int py) {
+        const int c = at(px, py);
+        float v;
+        if (size == 3 && px >= 1 && px <= w - 2 && py >= 1 && py <= h - 2)
+          v = dev::contrast3_core(in[c - R - 1], in[c - R], in[c - R + 1], in[c - 1], in[c], in[c + 1],
+                                  in[c + R - 1], in[c + R], in[c + R + 1], alpha, beta);
+        else
+          v = dev::contrast_px(get, px, py, w, h, alpha, beta, size);
+        out[c] = v;
+      });
+    } else if (f.enabled && ch.gn[k] > 1 && (f.type == ASC_FILTER_GAUSSIAN || f.type == ASC_FILTER_SHARPEN)) {
+      const float* wts = weights + ch.woff[k];
+      const int gn = ch.gn[k];
+      const bool sharpen = f.type == ASC_FILTER_SHARPEN;
+      const float contrast = f.p[1] < 1.0f ? 1.0f : f.p[1], threshold = f.p[2];
+      for_region(side, rx0, ry0, w, h, [&](int, int, int px, int py) {
+        const float c = dev::conv_px(get, px, py, w, h, wts, gn);
+        out[at(px, py)] = sharpen ? dev::sharpen_finish(get(px, py), c, contrast, threshold) : dev::gaussian_finish(c);
+      });
+    } else {
+      // disabled, or a Gaussian/Sharpen of size <= 1: a copy
+      for_region(side, rx0, ry0, w, h, [&](int, int, int px, int py) { out[at(px, py)] = in[at(px, py)]; });
     }
     __syncthreads();
     float* t = cur;
@@ -336,14 +369,8 @@ __global__ __launch_bounds__(kBlock) void post_tiles(BatchArgs a)
       const int H0 = ch.halo, side = kPTile + 2 * H0;
       const int ox = x0 - H0, oy = y0 - H0;
       const float* src = a.drr + it.drr_off;
-      for (int yy = ty; yy < side; yy += 8) {
-        const int py = oy + yy;
-        if (py < 0 || py >= h) continue;
-        for (int xx = tx; xx < side; xx += 32) {
-          const int px = ox + xx;
-          if (px >= 0 && px < w) buf0[yy * R + xx] = src[(long long)py * w + px];
-        }
-      }
+      for_region(side, ox, oy, w, h,
+                 [&](int xx, int yy, int px, int py) { buf0[yy * R + xx] = src[(long long)py * w + px]; });
       __syncthreads();
       const float* outb = run_chain<R>(ch, a.weights, buf0, buf1, x0, y0, w, h);
 #pragma unroll
@@ -370,22 +397,15 @@ __global__ __launch_bounds__(kBlock) void post_tiles(BatchArgs a)
         }
       }
       __syncthreads();
-      for (int yy = ty; yy < side; yy += 8) {
-        const int py = oy + yy;
-        if (py < 0 || py >= h) continue;
-        const float tt = s_row[yy];
-        for (int xx = tx; xx < side; xx += 32) {
-          const int px = ox + xx;
-          if (px < 0 || px >= w) continue;
-          const float ss = s_col[xx];
-          buf0[yy * R + xx] = (ss < 0.0f || tt < 0.0f) ? 0.0f : 1.0f - dev::img_sample(it.rad, ss, tt);
-        }
-      }
+      for_region(side, ox, oy, w, h, [&](int xx, int yy, int, int) {
+        const float ss = s_col[xx], tt = s_row[yy];
+        buf0[yy * R + xx] = (ss < 0.0f || tt < 0.0f) ? 0.0f : 1.0f - dev::img_sample(it.rad, ss, tt);
+      });
       __syncthreads();
       radb = run_chain<R>(ch, a.weights, buf0, buf1, x0, y0, w, h);
     }
     // --- mask, multiplies, moments (Tracker.cpp:744-752, Ncc.cpp) ---
-    dev::Moments acc;
+    dev::MaskedMoments acc;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const int ly = ty + 8 * k;
@@ -412,13 +432,15 @@ __global__ __launch_bounds__(kBlock) void post_tiles(BatchArgs a)
         }
       }
     }
-    double v[9];
-    acc.to_double(v);
-    dev::block_sum9(v, lds_red);
+    double v[6];
+    acc.to_double6(v);
+    dev::block_sum<6>(v, lds_red);
     if (threadIdx.x == 0) {
+      // {sum mask, sum f, sum g, count, S_f, S_g, S_ff, S_gg, S_fg}
       double* dst = a.partials + (size_t)(it.part_off + local) * 9;
+      const double o[9] = {v[0], v[1], v[2], v[0], v[1], v[2], v[3], v[4], v[5]};
 #pragma unroll
-      for (int k = 0; k < 9; k++) dst[k] = v[k];
+      for (int k = 0; k < 9; k++) dst[k] = o[k];
     }
   }
 }

@@ -483,15 +483,10 @@ __device__ __forceinline__ float rad_pixel(const ImgDev& im, const float* __rest
 // Filter pixel functions over an accessor get(x, y) that returns the input
 // at in-image coordinates. Same operation order as the reference kernels.
 // ---------------------------------------------------------------------------
-// SobelFilter.cl:10-47
-template <class G>
-__device__ __forceinline__ float sobel_px(const G& get, int x1, int y1, int w, int h, float scale, float blend)
+// SobelFilter.cl:10-47, from the 3 x 3 neighbourhood p[row][column].
+__device__ __forceinline__ float sobel_core(float p00, float p01, float p02, float p10, float p11, float p12,
+                                            float p20, float p21, float p22, float scale, float blend)
 {
-  const int x0 = x1 - 1 < 0 ? 0 : x1 - 1, y0 = y1 - 1 < 0 ? 0 : y1 - 1;
-  const int x2 = x1 + 1 > w - 1 ? w - 1 : x1 + 1, y2 = y1 + 1 > h - 1 ? h - 1 : y1 + 1;
-  const float p00 = get(x0, y0), p01 = get(x1, y0), p02 = get(x2, y0);
-  const float p10 = get(x0, y1), p11 = get(x1, y1), p12 = get(x2, y1);
-  const float p20 = get(x0, y2), p21 = get(x1, y2), p22 = get(x2, y2);
   const float horz = p02 + 2.0f * p12 + p22 - p00 - 2.0f * p10 - p20;
   const float vert = p00 + 2.0f * p01 + p02 - p20 - 2.0f * p21 - p22;
   const float grad = sqrtf(horz * horz + vert * vert);
@@ -505,6 +500,40 @@ __device__ __forceinline__ float sobel_px(const G& get, int x1, int y1, int w, i
   else if (sum > 1.0f)
     sum = 1.0f;
   return sum;
+}
+
+// The neighbourhood clamped to the image, as the reference kernel reads it.
+template <class G>
+__device__ __forceinline__ float sobel_px(const G& get, int x1, int y1, int w, int h, float scale, float blend)
+{
+  const int x0 = x1 - 1 < 0 ? 0 : x1 - 1, y0 = y1 - 1 < 0 ? 0 : y1 - 1;
+  const int x2 = x1 + 1 > w - 1 ? w - 1 : x1 + 1, y2 = y1 + 1 > h - 1 ? h - 1 : y1 + 1;
+  return sobel_core(get(x0, y0), get(x1, y0), get(x2, y0), get(x0, y1), get(x1, y1), get(x2, y1), get(x0, y2),
+                    get(x1, y2), get(x2, y2), scale, blend);
+}
+
+// x^y of the Contrast filter. For a positive normal x and y != 0 it is
+// 2^(y * log2 x) on the hardware base-2 logarithm and exponential, when the
+// result stays a normal number; everything else (zero, negative, subnormal
+// or non-finite x, y == 0, results outside the normal range) goes to powf.
+// The fast form is within a few units in the last place for the inputs the
+// filter sees (its error grows with |y * log2 x|: about 2^-24 * ln 2 of it,
+// plus an ulp from each hardware step), inside the OpenCL pow's 16-ulp
+// allowance that the reference kernel runs under.
+__device__ __forceinline__ float contrast_pow(float x, float y)
+{
+  if (x >= 1.17549435e-38f && x <= 3.40282347e+38f && y != 0.0f) {
+    const float e = y * __builtin_amdgcn_logf(x);
+    if (e > -126.0f && e < 128.0f) return __builtin_amdgcn_exp2f(e);
+  }
+  return powf(x, y);
+}
+
+__device__ __forceinline__ float contrast_finish(float fxy, float axy, float alpha, float beta)
+{
+  float gxy = 0.0f;
+  if (axy > 0.01f) gxy = contrast_pow(axy, alpha - beta) * contrast_pow(fxy, beta);
+  return gxy;
 }
 
 // ContrastFilter.cl:11-38
@@ -521,10 +550,25 @@ __device__ __forceinline__ float contrast_px(const G& get, int x, int y, int w, 
       n += 1.0f;
       sum += get(j, i);
     }
-  const float axy = sum / n;
-  float gxy = 0.0f;
-  if (axy > 0.01f) gxy = powf(axy, alpha - beta) * powf(fxy, beta);
-  return gxy;
+  return contrast_finish(fxy, sum / n, alpha, beta);
+}
+
+// The 3 x 3 window wholly inside the image: nine terms in the kernel's
+// order (rows outer, columns inner), n = 9.
+__device__ __forceinline__ float contrast3_core(float p00, float p01, float p02, float p10, float p11, float p12,
+                                                float p20, float p21, float p22, float alpha, float beta)
+{
+  float sum = 0.0f;
+  sum += p00;
+  sum += p01;
+  sum += p02;
+  sum += p10;
+  sum += p11;
+  sum += p12;
+  sum += p20;
+  sum += p21;
+  sum += p22;
+  return contrast_finish(p11, sum / 9.0f, alpha, beta);
 }
 
 // GaussianFilter.cl:15-27 / SharpenFilter.cl:18-30: zero-padded N x N
@@ -632,6 +676,32 @@ struct Moments {
   }
 };
 
+// Moments of a post-processing tile, where the mask is 0 or 1 and f and g
+// are already multiplied by it: a masked-out pixel adds zeros, which leave
+// a compensated sum as it was, so sum f = S_f, sum g = S_g and sum mask =
+// count, and six sums give the nine of Moments with the same bits.
+struct MaskedMoments {
+  Comp m, f, g, ff, gg, fg;
+  __device__ __forceinline__ void add(float fv, float gv, float mv)
+  {
+    m.add(mv);
+    f.add(fv);
+    g.add(gv);
+    ff.add(fv * fv);
+    gg.add(gv * gv);
+    fg.add(fv * gv);
+  }
+  __device__ __forceinline__ void to_double6(double* o) const
+  {
+    o[0] = m.value();
+    o[1] = f.value();
+    o[2] = g.value();
+    o[3] = ff.value();
+    o[4] = gg.value();
+    o[5] = fg.value();
+  }
+};
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -639,20 +709,23 @@ __device__ __forceinline__ double wave_sum(double v)
   return v;
 }
 
-// Block-wide sum of 9 doubles (blockDim.x == 256). Result valid in thread 0.
-__device__ __forceinline__ void block_sum9(double* v, double* lds /* 4*9 */)
+// Block-wide sum of N doubles (blockDim.x == 256). Result valid in thread 0.
+template <int N>
+__device__ __forceinline__ void block_sum(double* v, double* lds /* 4*N */)
 {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k < 9; k++) v[k] = wave_sum(v[k]);
+  for (int k = 0; k < N; k++) v[k] = wave_sum(v[k]);
   if (lane == 0)
 #pragma unroll
-    for (int k = 0; k < 9; k++) lds[wave * 9 + k] = v[k];
+    for (int k = 0; k < N; k++) lds[wave * N + k] = v[k];
   __syncthreads();
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < 9; k++) v[k] = (lds[k] + lds[9 + k]) + (lds[18 + k] + lds[27 + k]);
+    for (int k = 0; k < N; k++) v[k] = (lds[k] + lds[N + k]) + (lds[2 * N + k] + lds[3 * N + k]);
 }
+
+__device__ __forceinline__ void block_sum9(double* v, double* lds /* 4*9 */) { block_sum<9>(v, lds); }
 
 }  // namespace dev
 }  // namespace asc

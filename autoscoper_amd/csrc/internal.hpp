@@ -81,6 +81,11 @@ struct asc_context_s {
   double* ncc_result = nullptr;  // 10 doubles
   double* ncc_host = nullptr;    // pinned
   float* conv_wts = nullptr;     // large Gaussian/sharpen weight sets
+  // the reference-form NCC (ASC_NCC_TREE_*): per-pixel products and pass
+  // buffers, grown on demand
+  float* tree_buf = nullptr;
+  size_t tree_cap = 0, tree_part = 0;
+  int ncc_form = ASC_NCC_MOMENTS;
   // When set, every per-operator entry point waits for its work before
   // returning, as the reference's OpenCL Kernel::launch does (clFinish after
   // each launch, OpenCL.cpp:978). Off by default: the calls stay asynchronous.
@@ -123,6 +128,11 @@ int launch_composite(asc_context ctx, const float* s1, const float* s2, const fl
 int launch_fill_f32(asc_context ctx, float* dst, float v, size_t n);
 int launch_copy(asc_context ctx, void* dst, const void* src, size_t bytes);
 int launch_pack_volume(asc_context ctx, const VolDev& linear, void* dst, int layout, int pair_diff);
+// gpu::ncc as the reference computes it (float tree sums, Ncc.cpp:180-220):
+// sums6 = {mask, f, g, products, f dev^2, g dev^2} tree sums (products NaN
+// when the denominator is below 1e-5 and the reference skips that sum).
+int ncc_tree(asc_context ctx, const float* f, const float* g, const float* m, unsigned n, bool opencl,
+             float* sums6, float* ncc);
 int ncc_moments(asc_context ctx, const float* f, const float* g, const float* m, unsigned n, double* out9,
                 double* ncc);
 

@@ -1,6 +1,8 @@
 // Per-operator kernels: one launch per reference operator (the "plain"
 // form that gpu::View and Tracker::trackFrame drive one call at a time).
 #include <algorithm>
+#include <cmath>
+#include <limits>
 
 #include "device.hpp"
 
@@ -446,6 +448,132 @@ int ncc_moments(asc_context ctx, const float* f, const float* g, const float* m,
   if (out9)
     for (int k = 0; k < 9; k++) out9[k] = ctx->ncc_host[k];
   if (ncc) *ncc = ctx->ncc_host[9];
+  return ASC_OK;
+}
+
+// ---- the reference's float NCC, launch by launch (gpu::ncc, Ncc.cpp:180-220)
+
+// One pass of ncc_sum (NccSum.cl:2-31): block b of T threads loads element
+// i (CUDA indexing b*T + t, or the OpenCL kernel's t + b*B as written) or 0,
+// then halves `for s = T/2; s > 0; s >>= 1` in LDS (an odd width drops its
+// last element, as the reference's tree does) and writes its sum.
+__global__ __launch_bounds__(256) void tree_sum_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                      unsigned n, unsigned nblocks, int opencl)
+{
+  __shared__ float buf[256];
+  const unsigned t = threadIdx.x, b = blockIdx.x, T = blockDim.x;
+  const unsigned i = opencl ? t + b * nblocks : b * T + t;
+  buf[t] = i < n ? in[i] : 0.0f;
+  __syncthreads();
+  for (unsigned s = T / 2; s > 0; s >>= 1) {
+    if (t < s) buf[t] += buf[t + s];
+    __syncthreads();
+  }
+  if (t == 0) out[b] = buf[0];
+}
+
+// ncc_kernel (Ncc.cl:2-31): per-pixel products about the float means.
+__global__ __launch_bounds__(128) void ncc_products_kernel(const float* __restrict__ f, float mf,
+                                                          const float* __restrict__ g, float mg,
+                                                          const float* __restrict__ m, float* __restrict__ nums,
+                                                          float* __restrict__ d1s, float* __restrict__ d2s,
+                                                          unsigned n)
+{
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (m[i] > 0.5f) {
+    const float a = f[i] - mf, c = g[i] - mg;
+    nums[i] = a * c;
+    d1s[i] = a * a;
+    d2s[i] = c * c;
+  } else {
+    nums[i] = d1s[i] = d2s[i] = 0.0f;
+  }
+}
+
+namespace {
+
+// A blocking context waits after every launch (the reference's clFinish).
+int after_launch(asc_context ctx)
+{
+  ASC_HIP(hipGetLastError());
+  if (ctx->blocking) ASC_HIP(hipStreamSynchronize(ctx->stream));
+  return ASC_OK;
+}
+
+// ncc_sum (Ncc.cpp:103-146): passes of at most maxT-thread blocks until one
+// value is left, then a blocking read of it. Each pass writes a buffer of
+// its own (the reference's OpenCL passes run in place, a race).
+int tree_sum(asc_context ctx, const float* in, unsigned n, bool opencl, float* result)
+{
+  const unsigned maxT = opencl ? 128u : 256u;
+  if (n == 0) {
+    *result = 0.0f;
+    return ASC_OK;
+  }
+  const float* cur = in;
+  float* bufs[2] = {ctx->tree_buf + ctx->tree_cap - 2 * ctx->tree_part, ctx->tree_buf + ctx->tree_cap - ctx->tree_part};
+  int which = 0;
+  int rc;
+  while (n > 1) {
+    const unsigned T = n < maxT ? n : maxT;
+    const unsigned B = (n + T - 1) / T;
+    tree_sum_kernel<<<B, T, 0, ctx->stream>>>(cur, bufs[which], n, B, opencl ? 1 : 0);
+    if ((rc = after_launch(ctx))) return rc;
+    cur = bufs[which];
+    which ^= 1;
+    n = B;
+  }
+  float* host = reinterpret_cast<float*>(ctx->ncc_host);
+  ASC_HIP(hipMemcpyAsync(host, cur, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  ASC_HIP(hipStreamSynchronize(ctx->stream));
+  *result = host[0];
+  return ASC_OK;
+}
+
+}  // namespace
+
+int ncc_tree(asc_context ctx, const float* f, const float* g, const float* m, unsigned n, bool opencl,
+             float* sums6, float* ncc)
+{
+  // scratch: nums, den1s, den2s (n each) and two pass buffers
+  const size_t part = n / 128 + 2;
+  const size_t need = 3 * (size_t)n + 2 * part;
+  if (ctx->tree_cap < need) {
+    if (ctx->tree_buf) ASC_HIP(hipFree(ctx->tree_buf));
+    ctx->tree_buf = nullptr;
+    ctx->tree_cap = 0;
+    ASC_HIP(hipMalloc((void**)&ctx->tree_buf, sizeof(float) * need));
+    ctx->tree_cap = need;
+  }
+  ctx->tree_part = part;
+  if (!ctx->ncc_host) ASC_HIP(hipHostMalloc((void**)&ctx->ncc_host, sizeof(double) * 10, hipHostMallocDefault));
+  float* nums = ctx->tree_buf;
+  float* d1s = nums + n;
+  float* d2s = d1s + n;
+  int rc;
+  float nb, sf, sg, s1, s2, sn;
+  if ((rc = tree_sum(ctx, m, n, opencl, &nb))) return rc;
+  if ((rc = tree_sum(ctx, f, n, opencl, &sf))) return rc;
+  if ((rc = tree_sum(ctx, g, n, opencl, &sg))) return rc;
+  const float mf = sf / nb, mg = sg / nb;
+  if (n > 0) {
+    ncc_products_kernel<<<(n + 127) / 128, 128, 0, ctx->stream>>>(f, mf, g, mg, m, nums, d1s, d2s, n);
+    if ((rc = after_launch(ctx))) return rc;
+  }
+  if ((rc = tree_sum(ctx, d1s, n, opencl, &s1))) return rc;
+  if ((rc = tree_sum(ctx, d2s, n, opencl, &s2))) return rc;
+  const float den = std::sqrt(s1 * s2);
+  sn = std::numeric_limits<float>::quiet_NaN();
+  if (den < 1e-5) {  // (a double comparison, as written)
+    *ncc = 1e5f;
+  } else {
+    if ((rc = tree_sum(ctx, nums, n, opencl, &sn))) return rc;
+    *ncc = sn / den;
+  }
+  if (sums6) {
+    sums6[0] = nb, sums6[1] = sf, sums6[2] = sg, sums6[3] = sn, sums6[4] = s1, sums6[5] = s2;
+  }
   return ASC_OK;
 }
 

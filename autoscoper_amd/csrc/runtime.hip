@@ -222,6 +222,7 @@ ASC_API int asc_context_destroy(asc_context ctx)
   if (ctx->ncc_result) hipFree(ctx->ncc_result);
   if (ctx->ncc_host) hipHostFree(ctx->ncc_host);
   if (ctx->conv_wts) hipFree(ctx->conv_wts);
+  if (ctx->tree_buf) hipFree(ctx->tree_buf);
   hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return ASC_OK;
@@ -235,6 +236,14 @@ ASC_API int asc_context_set_stream(asc_context ctx, void* s)
 }
 
 ASC_API void* asc_context_stream(asc_context ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+ASC_API int asc_context_set_ncc_form(asc_context ctx, int form)
+{
+  ASC_CHECK(valid_ctx(ctx), ASC_ERR_INVALID, "null context");
+  ASC_CHECK(form >= ASC_NCC_MOMENTS && form <= ASC_NCC_TREE_OPENCL, ASC_ERR_INVALID, "bad NCC form");
+  ctx->ncc_form = form;
+  return ASC_OK;
+}
 
 ASC_API int asc_context_set_blocking(asc_context ctx, int enable)
 {
@@ -806,6 +815,17 @@ ASC_API int asc_ncc(asc_context ctx, const float* f, const float* g, const float
                     double moments[9])
 {
   ASC_CHECK(valid_ctx(ctx) && f && g && mask, ASC_ERR_INVALID, "bad argument");
+  if (ctx->ncc_form != ASC_NCC_MOMENTS) {
+    float sums[6], v;
+    const int rc = ncc_tree(ctx, f, g, mask, n, ctx->ncc_form == ASC_NCC_TREE_OPENCL, sums, &v);
+    if (rc) return rc;
+    if (ncc) *ncc = v;
+    if (moments) {
+      for (int k = 0; k < 6; k++) moments[k] = sums[k];
+      for (int k = 6; k < 9; k++) moments[k] = std::numeric_limits<double>::quiet_NaN();
+    }
+    return ASC_OK;
+  }
   return ncc_moments(ctx, f, g, mask, n, moments, ncc);  // waits for its result already
 }
 

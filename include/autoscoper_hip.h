@@ -222,7 +222,20 @@ ASC_API int asc_mask_multiply(asc_context ctx, float* drr, float* rad, const flo
 /* Intended semantics of the reference (exact sums): returns 1e5 when the
  * denominator is < 1e-5. moments (optional, 9 doubles): {sum mask, sum f,
  * sum g, count(mask>0.5), S_f, S_g, S_ff, S_gg, S_fg} with the S_ over
- * mask > 0.5. Blocking (returns a host value, like the reference). */
+ * mask > 0.5. Blocking (returns a host value, like the reference).
+ * asc_context_set_ncc_form selects how asc_ncc computes:
+ *   ASC_NCC_MOMENTS (default): compensated moments, FP64 combine;
+ *   ASC_NCC_TREE_CUDA, ASC_NCC_TREE_OPENCL: the reference's own float
+ *     computation, launch by launch: six tree sums (NccSum.cl / the CUDA
+ *     sum kernel; 256-thread blocks with contiguous indexing, or 128-thread
+ *     blocks with the OpenCL kernel's index t + b * blocks as written), each
+ *     ended by a blocking read, float means, the products kernel (Ncc.cl),
+ *     den = sqrt(float * float), 1e5 below 1e-5. The values are the
+ *     reference backend's; moments then holds {sum mask, sum f, sum g,
+ *     sum products, sum f dev^2, sum g dev^2} as tree-summed floats
+ *     (products NaN when the reference skips that sum) and NaN in 6..8. */
+enum { ASC_NCC_MOMENTS = 0, ASC_NCC_TREE_CUDA = 1, ASC_NCC_TREE_OPENCL = 2 };
+ASC_API int asc_context_set_ncc_form(asc_context ctx, int form);
 ASC_API int asc_ncc(asc_context ctx, const float* f, const float* g, const float* mask, unsigned n, double* ncc,
                     double moments[9]);
 /* ncc_init / ncc_deinit (Ncc.cpp:151-178): reserve scratch for n elements. */

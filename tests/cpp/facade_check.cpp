@@ -8,9 +8,10 @@
 // per-operator call wait for its kernel, as the reference's OpenCL backend
 // does after each launch.
 // --storage and --layout choose the device copy of the volume (default u16,
-// the densest layout).
+// the densest layout); --ncc-tree opencl|cuda computes gpu::ncc the
+// reference's way (float tree sums, a blocking read after each).
 //   facade_check <input.bin> --time N [--blocking] [--storage u16|f16|bf16|f32]
-//                [--layout linear|quad|octet|brick]
+//                [--layout linear|quad|octet|brick] [--ncc-tree opencl|cuda]
 #include <chrono>
 #include <cstdint>
 #include <cstdlib>
@@ -32,7 +33,7 @@ int main(int argc, char** argv)
   if (argc < 2) return 2;
   int time_n = 0;
   bool blocking = false;
-  int storage = ASC_STORAGE_U16, layout = -1;
+  int storage = ASC_STORAGE_U16, layout = -1, ncc_form = ASC_NCC_MOMENTS;
   static const char* const kStorages[] = {"u16", "f16", "bf16", "f32"};
   static const char* const kLayouts[] = {"linear", "quad", "octet", "brick"};
   auto pick = [](const char* s, const char* const* names, int n) {
@@ -49,6 +50,11 @@ int main(int argc, char** argv)
       storage = pick(argv[++a], kStorages, 4);
     else if (!std::strcmp(argv[a], "--layout") && a + 1 < argc)
       layout = pick(argv[++a], kLayouts, 4);
+    else if (!std::strcmp(argv[a], "--ncc-tree") && a + 1 < argc) {
+      const char* m = argv[++a];
+      ncc_form = !std::strcmp(m, "opencl") ? ASC_NCC_TREE_OPENCL : !std::strcmp(m, "cuda") ? ASC_NCC_TREE_CUDA : -1;
+      if (ncc_form < 0) return 2;
+    }
     else
       return 2;
     if (storage < 0 || layout < -1) return 2;
@@ -106,13 +112,15 @@ int main(int argc, char** argv)
     if (time_n > 0) {
       if (blocking && asc_context_set_blocking(xromm::gpu::context(), 1) != 0)
         throw xromm::gpu::Error(-1, asc_last_error());
+      if (asc_context_set_ncc_form(xromm::gpu::context(), ncc_form) != 0)
+        throw xromm::gpu::Error(-1, asc_last_error());
       double sink = 0;
       for (int i = 0; i < 2; i++) sink += tracker.trackFrame(0, &poses[(size_t)(i % npose) * 6])[0];
       const auto t0 = std::chrono::steady_clock::now();
       for (int i = 0; i < time_n; i++) sink += tracker.trackFrame(0, &poses[(size_t)(i % npose) * 6])[0];
       const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      std::printf("{\"evals\": %d, \"seconds\": %.9g, \"evals_per_s\": %.9g, \"blocking\": %s, \"storage\": \"%s\", \"layout\": %d, \"sink\": %.9g}\n",
-                  time_n, s, time_n / s, blocking ? "true" : "false", kStorages[storage], layout, sink);
+      std::printf("{\"evals\": %d, \"seconds\": %.9g, \"evals_per_s\": %.9g, \"blocking\": %s, \"storage\": \"%s\", \"layout\": %d, \"ncc_form\": %d, \"sink\": %.9g}\n",
+                  time_n, s, time_n / s, blocking ? "true" : "false", kStorages[storage], layout, ncc_form, sink);
       return 0;
     }
     // fused batch of all poses (absolute poses through a zero manipulator)

@@ -78,7 +78,8 @@ def test_facade_timing_mode(tiny, tmp_path):
     with every per-operator call finishing its kernel (blocking context),
     with the u16 volume and with an FP32 copy in the linear layout (whose
     texels are the normalised u/65535 rounded to FP32, so its costs move by
-    float roundings only)."""
+    float roundings only), and with gpu::ncc in the reference's float tree
+    form."""
     if not BIN.exists():
         pytest.fail(f"{BIN} is not built (run __graft_entry__.build())")
     poses = tiny["wl"].gt_poses[:2]
@@ -87,7 +88,8 @@ def test_facade_timing_mode(tiny, tmp_path):
     env = dict(os.environ)
     env["LD_LIBRARY_PATH"] = str(ROOT / "autoscoper_amd") + os.pathsep + env.get("LD_LIBRARY_PATH", "")
     sinks = []
-    for extra in ([], ["--blocking"], ["--blocking", "--storage", "f32", "--layout", "linear"]):
+    for extra in ([], ["--blocking"], ["--blocking", "--storage", "f32", "--layout", "linear"],
+                  ["--blocking", "--storage", "f32", "--layout", "linear", "--ncc-tree", "opencl"]):
         out = subprocess.run([str(BIN), str(inp), "--time", "6", *extra], capture_output=True, text=True,
                              timeout=100, env=env)
         assert out.returncode == 0, out.stderr
@@ -96,3 +98,4 @@ def test_facade_timing_mode(tiny, tmp_path):
         sinks.append(row["sink"])
     assert sinks[0] == sinks[1]  # blocking changes timing only
     assert abs(sinks[2] - sinks[0]) <= 1e-6 * abs(sinks[0])
+    assert np.isfinite(sinks[3])  # the reference's own float NCC: other values

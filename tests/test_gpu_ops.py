@@ -255,6 +255,42 @@ def test_ncc_degenerate(asc, ctx):
     assert abs(v - 1.0) <= 1e-6
 
 
+@pytest.mark.parametrize("n", [1, 10, 100, 1000, 16384, 65536, 163841])
+def test_ncc_reference_tree_forms_bit_exact(asc, ctx, orc, n):
+    """asc_ncc in the reference's float form (six tree sums, each read back)
+    equals the oracle's restatement of NccSum.cl / the CUDA sum kernel bit
+    for bit, for both index models; sizes include non-power-of-two groups
+    (n = 10, 100 drop elements in the LDS tree) and multi-pass sums."""
+    rng = np.random.default_rng(n + 7)
+    f = rng.random(n).astype(np.float32)
+    g = (0.3 * f + 0.7 * rng.random(n)).astype(np.float32)
+    m = (rng.random(n) < 0.8).astype(np.float32)
+    m[0] = 1.0
+    f, g = f * m, g * m
+    df, dg, dm = ctx.to_device(f), ctx.to_device(g), ctx.to_device(m)
+    try:
+        for form, mode in ((asc.NCC_TREE_CUDA, 0), (asc.NCC_TREE_OPENCL, 1)):
+            ctx.set_ncc_form(form)
+            v, mom = ctx.ncc(df, dg, dm, n)
+            ref = orc.ncc_tree(f, g, m, mode)
+            assert v == ref or (np.isnan(v) and np.isnan(ref)), (form, v, ref)
+            assert mom[0] == orc.tree_sum(m, mode) and mom[1] == orc.tree_sum(f, mode)
+            assert np.isnan(mom[6:]).all()
+        # a constant image under a full mask: whatever elements the tree
+        # keeps, the mean is 0.5 exactly, den = 0 < 1e-5, 1e5, and the
+        # products sum is skipped
+        ctx.set_ncc_form(asc.NCC_TREE_OPENCL)
+        c = ctx.to_device(np.full(n, 0.5, np.float32))
+        ones = np.ones(n, np.float32)
+        v, mom = ctx.ncc(c, dg, ctx.to_device(ones), n)
+        assert v == 1e5 == orc.ncc_tree(np.full(n, 0.5, np.float32), g, ones, 1) and np.isnan(mom[3])
+    finally:
+        ctx.set_ncc_form(asc.NCC_MOMENTS)
+    v, _ = ctx.ncc(df, dg, dm, n)
+    ref, _ = orc.ncc_exact(f.astype(np.float64), g.astype(np.float64), m.astype(np.float64))
+    assert abs(v - ref) <= 1e-5 or (ref == 1e5 and v == 1e5)  # the default form is back
+
+
 def test_wide_slices_fall_back_to_a_layout_that_fits(asc, ctx):
     """A 1100 x 1100 slice is too wide for BRICK/OCTET/QUAD addressing
     (>= 8 MiB per slice step): creation falls back to LINEAR and asking for
