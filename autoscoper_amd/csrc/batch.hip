@@ -232,18 +232,28 @@ struct LdsGet {
 };
 
 // Visit every in-image pixel of the side x side square at (rx0, ry0), one
-// flat index per thread (side <= 56: the float quotient k / side is exact
-// to far better than the 1 / (2 side) margin its floor needs).
+// flat index k per thread, stepping by the block: thread t starts at
+// (t % side, t / side) and each step adds kBlock = q rows + r columns (the
+// float quotients below are exact: side <= 56 leaves their floors a margin
+// of 1 / (2 side), far above the rounding error).
 template <class F>
 __device__ __forceinline__ void for_region(int side, int rx0, int ry0, int w, int h, F&& fn)
 {
   const int n = side * side;
   const float inv = 1.0f / (float)side;
+  int yy = (int)(((float)threadIdx.x + 0.5f) * inv);
+  int xx = (int)threadIdx.x - yy * side;
+  const int q = (int)(((float)kBlock + 0.5f) * inv);
+  const int r = kBlock - q * side;
   for (int k = threadIdx.x; k < n; k += kBlock) {
-    const int yy = (int)(((float)k + 0.5f) * inv);
-    const int xx = k - yy * side;
     const int px = rx0 + xx, py = ry0 + yy;
-    if (px >= 0 && px < w && py >= 0 && py < h) fn(xx, yy, px, py);
+    if ((unsigned)px < (unsigned)w && (unsigned)py < (unsigned)h) fn(xx, yy, px, py);
+    xx += r;
+    yy += q;
+    if (xx >= side) {
+      xx -= side;
+      ++yy;
+    }
   }
 }
 
@@ -255,7 +265,7 @@ __device__ __forceinline__ void for_region(int side, int rx0, int ry0, int w, in
 // LDS where it lies inside the image (the same values the clamped form
 // reads there).
 template <int R>
-__device__ float* run_chain(const ChainDev& ch, const float* __restrict__ weights, float* cur, float* nxt, int x0,
+__device__ __forceinline__ float* run_chain(const ChainDev& ch, const float* __restrict__ weights, float* cur, float* nxt, int x0,
                             int y0, int w, int h)
 {
   const int H0 = ch.halo;

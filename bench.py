@@ -292,9 +292,14 @@ def storage_variant(A, synth, ctx, args, batches, bone_of, storage):
         scene.set_tuning(scene.TUNE_GRID, args.grid)
     evals_s, ms_step, kern_ms, _ = time_fused(ctx, scene, batches, bone_of, len(cams), args)
     log(f"[bench] {storage} volume: {evals_s:.1f} evals/s, kernels ms {kern_ms / args.steps}")
+    g_s = None
+    if not args.graph:
+        g_s, _, _, _ = time_fused(ctx, scene, batches, bone_of, len(cams), argparse.Namespace(**{**vars(args), "graph": True}))
+        log(f"[bench] {storage} volume, hipGraph replay: {g_s:.1f} evals/s")
     scene.close()
     vol.close()
     return {"value": evals_s, "unit": "pose-evals/s", "ms_per_step": ms_step, "launch_shape": tuning,
+            "graph_replay_value": g_s,
             "kernel_ms": {"drr_tiles": kern_ms[0] / args.steps, "post_tiles": kern_ms[1] / args.steps}}
 
 
@@ -435,6 +440,16 @@ def main(argv=None) -> int:
     if args.grid:
         scene.set_tuning(scene.TUNE_GRID, args.grid)
     evals_s, ms_step, kern_ms, samples = time_fused(ctx, scene, batches, bone_of, len(cams), args)
+    graph_replay = None
+    if not args.graph:
+        # the same steps replayed from a hipGraph per batch (kernel times above
+        # come from the eager run, whose HIP events sit in its timed region)
+        g_args = argparse.Namespace(**{**vars(args), "graph": True})
+        g_s, g_ms, _, _ = time_fused(ctx, scene, batches, bone_of, len(cams), g_args)
+        scene.use_graph(False)
+        scene.set_timing(True)
+        graph_replay = {"value": g_s, "unit": "pose-evals/s", "ms_per_step": g_ms}
+        log(f"[bench] hipGraph replay {g_s:.1f} evals/s")
     info, last_samples = scene.last_info(P * len(cams))
     drr_ms = kern_ms[0] / args.steps
     bpv = BYTES_PER_VOXEL[args.storage]
@@ -518,6 +533,7 @@ def main(argv=None) -> int:
             "valu": valu,
         },
         "cpu_baseline": cpu,
+        "graph_replay": graph_replay,
         "plain_per_eval": {"value": plain_s, "unit": "pose-evals/s",
                            "speedup_of_fused": evals_s / plain_s if plain_s > 0 else None},
         "plain_facade": None if facade is None else {
@@ -535,8 +551,10 @@ def main(argv=None) -> int:
     if args.variants and not multi and args.storage == "u16":
         out["storage_variants"] = {st: storage_variant(A, synth, ctx, args, batches, bone_of, st) for st in ("f16",)}
         if facade is not None:
-            out["plain_facade"]["speedup_of_fused_f16"] = (out["storage_variants"]["f16"]["value"]
-                                                           / facade["reference_form"])
+            f16 = out["storage_variants"]["f16"]
+            out["plain_facade"]["speedup_of_fused_f16"] = f16["value"] / facade["reference_form"]
+            if f16["graph_replay_value"]:
+                out["plain_facade"]["speedup_of_fused_f16_graph"] = f16["graph_replay_value"] / facade["reference_form"]
     if args.serve_seconds > 0:
         scene.close()
         for v in vols:
