@@ -195,3 +195,43 @@ def test_launch_shapes_and_autotune_do_not_change_results(asc, ctx, tiny):
     assert np.array_equal(c, ref_c) and np.array_equal(p, ref_p)
     sc.close()
     vol.close()
+
+
+@pytest.mark.parametrize("chains", [
+    # Gaussian, a disabled stage and Sharpen on the DRR; a 5 x 5 Contrast on the radiograph
+    ("gauss-off-sharpen/contrast5", lambda F: ([F.gaussian(1.0), F.sobel(1.0, 0.5, enabled=False),
+                                                F.sharpen(1.0, 2.0, 0.0)], [F.contrast(1.3, 2.0, 5)])),
+    # a wide Gaussian: halo above 4, the larger-LDS kernel; Sobel with blend >= 0.5
+    ("sharpen/gauss3-sobel", lambda F: ([F.sharpen(2.0, 0.5, 0.05)], [F.gaussian(3.0), F.sobel(2.0, 0.7)])),
+    # no filters at all; an even-sized Contrast window
+    ("none/contrast4", lambda F: ([], [F.contrast(2.5, 0.9, 4)])),
+], ids=lambda c: c[0])
+def test_fused_filter_kinds_match_oracle_pipeline(asc, ctx, orc, tiny, chains):
+    """Every filter kind through the fused per-tile chain (LDS stages, the
+    in-image fast paths and the clamped borders) against the oracle's
+    pipeline: mask exact, DRR channel within 2e-4, radiograph within 1e-5."""
+    name, make = chains
+    drr_fs, rad_fs = make(asc.filters)
+    sc, vol, _ = build_scene(asc, ctx, tiny)
+    for c in range(len(tiny["cams"])):
+        sc.set_filters(c, 0, drr_fs)
+        sc.set_filters(c, 1, rad_fs)
+    sc.set_debug(True)
+    prep = tiny["prep"]
+    RW, RH = tiny["render"]
+    pose = tiny["wl"].gt_poses[0] + np.array([0.5, -0.3, 0.4, -2.0, 1.5, 2.5])
+    sc.eval_poses(pose[None])
+    for c, cam in enumerate(tiny["cams"]):
+        _, imv, vp, (w, h) = orc.view_geometry(cam, pose, prep["inv_scale"], prep["inv_trans"], RW, RH)
+        d, g, m = sc.last_images(0, c, w, h)
+        drr, _ = orc.drr(prep["volume"], imv[:12], vp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, w, h)
+        drr_f = orc.filter_chain(asc.filters.to_records(drr_fs), drr)
+        rad = orc.rad(tiny["rads"][c][1], cam[24:28], vp, w, h)
+        rad_f = orc.filter_chain(asc.filters.to_records(rad_fs), rad)
+        mask = (drr_f != 0).astype(np.float32)
+        assert np.array_equal(m, mask), name
+        assert np.allclose(d, drr_f * mask, rtol=2e-4, atol=1e-7), name
+        assert np.allclose(g, rad_f * mask, rtol=1e-5, atol=1e-7), name
+        assert w * h > 1000
+    sc.close()
+    vol.close()
