@@ -391,8 +391,8 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(jobs and particle batches would shard across GPUs without collectives; see DESIGN.md)",
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(jobs and particle batches would shard across GPUs without collectives)",
               file=sys.stderr)
         return 2
 

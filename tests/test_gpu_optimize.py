@@ -4,7 +4,7 @@ optimize_frame with the same seed, start pose and options.
 
 Bar: final poses within 0.05 mm (translation) and 0.05 deg (rotation angle
 between the two orientations). The reference-order (asynchronous) PSO is
-judged against the float-faithful oracle; the synchronous batched-epoch PSO
+checked against the float-faithful oracle; the synchronous batched-epoch PSO
 against the oracle's synchronous mode in float and in FP64.
 """
 import numpy as np

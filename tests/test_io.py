@@ -18,7 +18,7 @@ How each part is pinned:
     oracle's frame math), the film rectangle of the image plane equals the
     viewport, the optical axis passes through the principal point, and a
     VTKCam view angle of 2a spans a film width of 4 tan(a) on both axes.
-  * .vie: the effective values the survey derives from the GUI code for
+  * .vie: the effective values the GUI code (FilterTreeWidget*.cpp) gives for
     sample_data/xParameters/control_settings.vie and empty_filter.vie, and
     spin-box rounding and bounds. Rounding of an exact binary tie (e.g.
     0.125) follows Qt's fixed formatting (away from zero): parity unpinned

@@ -2,7 +2,7 @@
 oracle's restatement, bit-exact, on the oracle's exported test objectives.
 The reference order (asynchronous PSO) is also pinned against the
 reference's own compiled code in test_oracle_vs_reference.py; the
-synchronous (batched-epoch) PSO is this build's variant and is pinned here
+synchronous (batched-epoch) PSO is this library's variant and is pinned here
 against the oracle's synchronous mode and by its single-particle identity.
 """
 import ctypes as C
