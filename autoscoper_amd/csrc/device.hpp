@@ -220,15 +220,15 @@ __device__ __forceinline__ float trilinear(const VolDev& v, const VolRsrc& r, fl
   int voff;
   if constexpr (LY == ASC_LAYOUT_BRICK) {
     // cell (i, j, k) lives in brick (i/2+1, j/2+1, k/2+1) (floor division),
-    // at cell (i&1, j&1, k&1) of the brick; with E bytes per cell, a brick
-    // of 8E bytes, and row/layer strides SY = r.sy, SZ = r.sz:
-    //   E*(i&1) + 8E*((i>>1)+1) = E*i + 6E*(i>>1) + 8E, and likewise
-    //   2E*(j&1) + SY*((j>>1)+1) = 2E*j + (SY-4E)*(j>>1) + SY,
-    //   4E*(k&1) + SZ*((k>>1)+1) = 4E*k + (SZ-8E)*(k>>1) + SZ;
-    // the constants sum to r.c0
+    // at cell (j&1) + 2(k&1) + 4(i&1) of the brick; with E bytes per cell, a
+    // brick of 8E bytes, and row/layer strides SY = r.sy, SZ = r.sz:
+    //   4E*(i&1) + 8E*((i>>1)+1) = 4E*i + 8E,
+    //   E*(j&1) + SY*((j>>1)+1) = E*j + (SY-2E)*(j>>1) + SY,
+    //   2E*(k&1) + SZ*((k>>1)+1) = 2E*k + (SZ-4E)*(k>>1) + SZ;
+    // the constants sum to r.c0, and x needs no split at all
     constexpr int S = elem_shift<ST, LY>(), E = 1 << S;
-    const int base = (i << S) + (j << (S + 1)) + (k << (S + 2)) + r.c0;
-    voff = mad_i24(i >> 1, 6 * E, mad_i24(j >> 1, r.sy - 4 * E, mad_i24(k >> 1, r.sz - 8 * E, base)));
+    const int base = (i << (S + 2)) + (j << S) + (k << (S + 1)) + r.c0;
+    voff = mad_i24(j >> 1, r.sy - 2 * E, mad_i24(k >> 1, r.sz - 4 * E, base));
   } else {
     voff = mad_i24(k, r.sz, mad_i24(j, r.sy, (i << elem_shift<ST, LY>()) + r.c0));
   }

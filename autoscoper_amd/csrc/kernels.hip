@@ -240,9 +240,10 @@ __global__ __launch_bounds__(kBlock) void pack_kernel(const T* __restrict__ src,
 }
 
 // BRICK layout from the padded linear volume (pW = W+2, pH = H+2, pD = D+2,
-// voxel x at padded x+1): brick (bx, by, bz), cell c = px + 2py + 4pz holds
+// voxel x at padded x+1): brick (bx, by, bz), cell c = py + 2pz + 4px holds
 // the 2x2x2 corner texels of voxel cell (2bx+px-2, 2by+py-2, 2bz+pz-2);
-// texels outside the padded grid are 0.
+// texels outside the padded grid are 0. The x bit is the highest of the
+// cell index, so along x a cell's offset is simply 4 cells per step.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void pack_brick_kernel(const T* __restrict__ src, T* __restrict__ dst, int pW,
                                                             int pH, int pD, int BX, int BY, int BZ, int diff)
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void pack_brick_kernel(const T* __restrict_
     const size_t b = e >> 3;
     const int c = (int)(e & 7);
     const int bx = (int)(b % BX), by = (int)((b / BX) % BY), bz = (int)(b / ((size_t)BX * BY));
-    const int ci = 2 * bx + (c & 1) - 2, cj = 2 * by + ((c >> 1) & 1) - 2, ck = 2 * bz + (c >> 2) - 2;
+    const int ci = 2 * bx + (c >> 2) - 2, cj = 2 * by + (c & 1) - 2, ck = 2 * bz + ((c >> 1) & 1) - 2;
     T prev = T(0);
     for (int q = 0; q < 8; q++) {
       // padded coordinates of corner q
