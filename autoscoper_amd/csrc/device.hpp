@@ -276,11 +276,14 @@ __device__ __forceinline__ float trilinear(const VolDev& v, const VolRsrc& r, fl
 // t_near (the same float loop, so the same sample count). Returns the sum
 // of raw texel values above the raw cutoff.
 //
-// Exact volumes (u16, F32) form the sample position exactly as the
-// reference does it (p = o + t*d, then s*W - 0.5 for the sampler): the
-// origin lies far outside the unit box, so o + t*d cancels, and any other
-// association moves near-zero silhouette pixels by more than 1e-4 relative
-// and can flip an exact zero of the DRR (and so a mask count).
+// Exact volumes (u16, F32) form the sample position as the reference's
+// expressions compile in OpenCL C, where FP_CONTRACT is on by default and
+// clang fuses `ray.origin + t*ray.direction` and the sampler's `s*W - 0.5`
+// into one FMA each: p = fma(t, d, o), X = fma(p, W, -0.5). The origin lies
+// far outside the unit box, so o + t*d cancels, and a different association
+// (the entry-point form below) moves near-zero silhouette pixels by more
+// than 1e-4 relative and can flip an exact zero of the DRR (and so a mask
+// count).
 // Half-precision volumes (F16, BF16), whose storage rounding already
 // dominates their error against the all-double DRR, form positions from the
 // ray's entry point in voxel space when there is no cutoff:
@@ -309,13 +312,13 @@ template <int ST, int LY, bool CUT>
 __device__ __forceinline__ void accumulate(const VolDev& v, const VolRsrc& rs, const Ray& r, float t, float W,
                                            float H, float D, float cutoff_raw, float& acc)
 {
-  const float px = r.ox + t * r.dx;
-  const float py = r.oy + t * r.dy;
-  const float pz = r.oz + t * r.dz;
+  const float px = fmaf(t, r.dx, r.ox);
+  const float py = fmaf(t, r.dy, r.oy);
+  const float pz = fmaf(t, r.dz, r.oz);
   // sampler coordinates (p.x, 1 - p.y, -p.z) in voxel space
-  const float X = px * W - 0.5f;
-  const float Y = (1.0f - py) * H - 0.5f;
-  const float Z = (-pz) * D - 0.5f;
+  const float X = fmaf(px, W, -0.5f);
+  const float Y = fmaf(1.0f - py, H, -0.5f);
+  const float Z = fmaf(-pz, D, -0.5f);
   const float val = trilinear<ST, LY>(v, rs, X, Y, Z);
   if constexpr (CUT)
     acc += val > cutoff_raw ? val : 0.0f;

@@ -8,9 +8,12 @@
 //
 // Two arithmetic modes are provided through templates on the real type R:
 //   R = float  : "float-faithful". The same float operations, in the same
-//                order, as the reference kernels, with no contraction into
-//                FMA (built with -ffp-contract=off) and correctly rounded
-//                division and sqrt. The OpenCL 1.2 image sampler rules
+//                order, as the reference kernels, with no implicit
+//                contraction into FMA (built with -ffp-contract=off) and
+//                correctly rounded division and sqrt. Two expressions of
+//                the ray march are written as the FMA an OpenCL compiler
+//                forms by default (FP_CONTRACT on): the sample position
+//                o + t*d and the sampler's s*W - 0.5. The OpenCL 1.2 image sampler rules
 //                (normalised coordinates, CLK_ADDRESS_CLAMP border 0,
 //                CLK_FILTER_LINEAR, UNORM conversion) are written out.
 //   R = double : the same algorithm evaluated in double ("FP64 oracle").
@@ -135,12 +138,15 @@ inline R vtex(const uint16_t* v, int W, int H, int D, int i, int j, int k)
   return unorm16<R>(v[((size_t)k * H + j) * W + i]);
 }
 
+// The un-normalisation s*W - 0.5 as an OpenCL compiler emits it: FP_CONTRACT
+// is on by default in OpenCL C, so clang (PoCL's compiler) fuses it into one
+// FMA, one rounding.
 template <typename R>
 R sample3(const uint16_t* v, int W, int H, int D, R s, R t, R r)
 {
-  R x = s * R(W) - R(0.5);
-  R y = t * R(H) - R(0.5);
-  R z = r * R(D) - R(0.5);
+  R x = std::fma(s, R(W), R(-0.5));
+  R y = std::fma(t, R(H), R(-0.5));
+  R z = std::fma(r, R(D), R(-0.5));
   R a, b, c;
   int i0 = floor_index(x, a), j0 = floor_index(y, b), k0 = floor_index(z, c);
   int i1 = i0 + 1, j1 = j0 + 1, k1 = k0 + 1;
@@ -214,7 +220,8 @@ int64_t drr(const uint16_t* vol, int W, int H, int D, const float* imv, const fl
       R t = farT;
       R density = R(0);
       while (t > nearT) {
-        const R px = ox + t * dx, py = oy + t * dy, pz = oz + t * dz;
+        // ray.origin + t*ray.direction, contracted as above: fma(t, d, o)
+        const R px = std::fma(t, dx, ox), py = std::fma(t, dy, oy), pz = std::fma(t, dz, oz);
         const R val = sample3<R>(vol, W, H, D, px, R(1) - py, -pz);
         density += val > cutoff ? step * val : R(0);
         t -= R(1);
