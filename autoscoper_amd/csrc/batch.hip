@@ -1224,56 +1224,52 @@ ASC_API int asc_scene_autotune(asc_scene s, int n, const int* volumes, const dou
   destroy_graph(s);
   struct Cand {
     int unroll, shift, gmul;
-    double ms;
+    int grid = 0;  // persistent grid of this variant
+    double ms = 0.0;
+    std::vector<double> t;
   };
   std::vector<Cand> cands;
   for (int unroll : {4, 2})
     for (int shift : {3, 4, 2})
-      for (int gmul : {1, 2}) cands.push_back(Cand{unroll, shift, gmul, 0.0});
+      for (int gmul : {1, 2}) cands.push_back(Cand{unroll, shift, gmul});
   int rc = ASC_OK;
-  int auto_grid = 0;
-  std::vector<double> ms;
-  for (Cand& c : cands) {
+  auto use = [&](const Cand& c) {
     s->unroll = c.unroll;
     s->wave_shift = c.shift;
-    s->grid_user = false;
-    s->grid_persistent = 0;  // this variant's occupancy
-    if ((rc = eval_batch(s, volumes, poses, n, nullptr, nullptr))) break;  // warm, derives the grid
-    auto_grid = s->grid_persistent;
-    if (c.gmul > 1) {
-      s->grid_persistent = auto_grid * c.gmul;
-      s->grid_user = true;
-    }
-    ms.clear();
-    for (int r = 0; r < reps && rc == ASC_OK; r++) {
-      rc = eval_batch(s, volumes, poses, n, nullptr, nullptr);
-      ms.push_back(s->last_ms[0]);
-    }
-    if (rc) break;
-    std::sort(ms.begin(), ms.end());
-    c.ms = ms[ms.size() / 2];
+    s->grid_user = c.gmul > 1;
+    s->grid_persistent = c.grid;  // 0: this variant's occupancy grid, derived by the next batch
+  };
+  // a first round warms caches and clocks and derives each variant's grid;
+  // the timed rounds then visit every variant once per round, so a drift in
+  // clocks or cache state during tuning falls on all of them alike
+  for (Cand& c : cands) {
+    use(c);
+    if ((rc = eval_batch(s, volumes, poses, n, nullptr, nullptr))) break;
+    c.grid = s->grid_persistent * c.gmul;
   }
+  for (int r = 0; r < reps && rc == ASC_OK; r++)
+    for (Cand& c : cands) {
+      use(c);
+      if ((rc = eval_batch(s, volumes, poses, n, nullptr, nullptr))) break;
+      c.t.push_back(s->last_ms[0]);
+    }
+  if (rc == ASC_OK)
+    for (Cand& c : cands) {
+      std::sort(c.t.begin(), c.t.end());
+      c.ms = c.t[c.t.size() / 2];
+    }
   s->timing = timing0;
   s->use_graph = graph0;
   if (rc) return rc;
   const Cand* best = &cands[0];
   for (const Cand& c : cands)
     if (c.ms < best->ms) best = &c;
-  s->unroll = best->unroll;
-  s->wave_shift = best->shift;
-  s->grid_user = false;
-  s->grid_persistent = 0;
-  if (best->gmul > 1) {
-    // the occupancy grid of the chosen variant, times gmul
-    if ((rc = eval_batch(s, volumes, poses, n, nullptr, nullptr))) return rc;
-    s->grid_persistent *= best->gmul;
-    s->grid_user = true;
-  }
+  use(*best);
   destroy_graph(s);
   if (result) {
     result->unroll = best->unroll;
     result->wave_patch = 1 << best->shift;
-    result->grid = best->gmul > 1 ? s->grid_persistent : 0;
+    result->grid = best->gmul > 1 ? best->grid : 0;
     result->best_ms = best->ms;
     result->default_ms = cands[0].ms;  // unroll 4, 8 x 8 patch, occupancy grid
     result->candidates = (int)cands.size();

@@ -284,7 +284,7 @@ def storage_variant(A, synth, ctx, args, batches, bone_of, storage):
     wl, vol, cams, rads, scene, gt = build_workload(A, synth, args.workload, storage, ctx, args.layout)
     tuning = None
     if args.autotune and not (args.unroll or args.grid):
-        tuning = scene.autotune(batches[0], volumes=bone_of, reps=3)
+        tuning = scene.autotune(batches[0], volumes=bone_of, reps=5)
         log(f"[bench] {storage} autotune: {tuning}")
     if args.unroll:
         scene.set_tuning(scene.TUNE_UNROLL, args.unroll)
@@ -433,7 +433,7 @@ def main(argv=None) -> int:
     if args.autotune and not (args.unroll or args.grid):
         # launch shape chosen by the autotuner on the first warm-up batch
         # (outside the timed region; no variant changes a result)
-        tuning = scene.autotune(batches[0], volumes=bone_of, reps=3)
+        tuning = scene.autotune(batches[0], volumes=bone_of, reps=5)
         log(f"[bench] autotune: {tuning}")
     if args.unroll:
         scene.set_tuning(scene.TUNE_UNROLL, args.unroll)
