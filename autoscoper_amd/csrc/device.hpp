@@ -440,29 +440,32 @@ __device__ __forceinline__ float drr_finish(float acc_raw, float step_vscale, fl
 // ---------------------------------------------------------------------------
 // Images hold texels already normalised at upload (raw / (2^bits - 1),
 // correctly rounded): the sampler's UNORM read is a plain load here.
-__device__ __forceinline__ float img_texel(const ImgDev& im, int i, int j)
-{
-  if (i < 0 || j < 0 || i >= im.W || j >= im.H) return 0.0f;
-  return ((const float*)im.data)[(size_t)j * im.W + i];
-}
-
-__device__ __forceinline__ int floor_idx(float x, float& frac)
-{
-  const float f = floorf(x);
-  frac = x - f;
-  if (!(f > -4.0f)) return -4;
-  if (!(f < 1073741824.0f)) return 1073741824;
-  return (int)f;
-}
-
+// Bilinear sample at sampler coordinates (s, t) in [0, 1] (every caller
+// returns before sampling outside), OpenCL 1.2 section 8.2: x = s*W - 0.5,
+// i0 = floor(x), a = x - i0, texels outside the image read the border 0
+// (CLK_ADDRESS_CLAMP). The texels come through buffer loads, whose
+// out-of-range offsets read 0: a row outside [0, H) lands out of range by
+// itself (the image holds fewer than 2^29 texels, checked on the host), a
+// column outside [0, W) is sent there explicitly.
 __device__ __forceinline__ float img_sample(const ImgDev& im, float s, float t)
 {
-  const float x = s * (float)im.W - 0.5f;
-  const float y = t * (float)im.H - 0.5f;
-  float a, b;
-  const int i0 = floor_idx(x, a), j0 = floor_idx(y, b);
-  const float T00 = img_texel(im, i0, j0), T10 = img_texel(im, i0 + 1, j0);
-  const float T01 = img_texel(im, i0, j0 + 1), T11 = img_texel(im, i0 + 1, j0 + 1);
+  const int W = __builtin_amdgcn_readfirstlane(im.W), H = __builtin_amdgcn_readfirstlane(im.H);
+  const uint64_t base = (uint64_t)(uintptr_t)im.data;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+    (void*)(uintptr_t)(((uint64_t)hi << 32) | lo), 0, W * H * 4, 0x00020000);
+  const float x = s * (float)W - 0.5f;
+  const float y = t * (float)H - 0.5f;
+  const float fx = floorf(x), fy = floorf(y);
+  const float a = x - fx, b = y - fy;
+  const int i0 = (int)fx, j0 = (int)fy;
+  const int row0 = (j0 * W + i0) * 4, row1 = row0 + W * 4;
+  const bool c0 = (unsigned)i0 < (unsigned)W, c1 = (unsigned)(i0 + 1) < (unsigned)W;
+  const float T00 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, c0 ? row0 : -1, 0, 0));
+  const float T10 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, c1 ? row0 + 4 : -1, 0, 0));
+  const float T01 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, c0 ? row1 : -1, 0, 0));
+  const float T11 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, c1 ? row1 + 4 : -1, 0, 0));
   return (1.0f - a) * (1.0f - b) * T00 + a * (1.0f - b) * T10 + (1.0f - a) * b * T01 + a * b * T11;
 }
 
@@ -684,10 +687,11 @@ struct Moments {
 // a compensated sum as it was, so sum f = S_f, sum g = S_g and sum mask =
 // count, and six sums give the nine of Moments with the same bits.
 struct MaskedMoments {
-  Comp m, f, g, ff, gg, fg;
+  int m = 0;  // the mask is 0 or 1: its sum is a count
+  Comp f, g, ff, gg, fg;
   __device__ __forceinline__ void add(float fv, float gv, float mv)
   {
-    m.add(mv);
+    m += mv > 0.5f;
     f.add(fv);
     g.add(gv);
     ff.add(fv * fv);
@@ -696,7 +700,7 @@ struct MaskedMoments {
   }
   __device__ __forceinline__ void to_double6(double* o) const
   {
-    o[0] = m.value();
+    o[0] = (double)m;
     o[1] = f.value();
     o[2] = g.value();
     o[3] = ff.value();
