@@ -633,18 +633,19 @@ __host__ __device__ inline int filter_radius(const asc_filter& f, int gauss_n)
 }
 
 // ---------------------------------------------------------------------------
-// NCC moments: per-thread fp32 partials with Neumaier compensation, combined
+// NCC moments: per-thread fp32 partials with compensation (the running sum and
+// its exact rounding errors), combined
 // in double across lanes, waves and blocks.
 // ---------------------------------------------------------------------------
 struct Comp {
   float s = 0.0f, c = 0.0f;
   __device__ __forceinline__ void add(float x)
   {
+    // Knuth's TwoSum: the exact rounding error of s + x, without a branch
+    // (Neumaier's two cases give the same value)
     const float t = s + x;
-    if (fabsf(s) >= fabsf(x))
-      c += (s - t) + x;
-    else
-      c += (x - t) + s;
+    const float bp = t - s;
+    c += (s - (t - bp)) + (x - bp);
     s = t;
   }
   __device__ __forceinline__ double value() const { return (double)s + (double)c; }
