@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <dirent.h>
+#include <sys/stat.h>
 #include <fstream>
 #include <limits>
 #include <map>
@@ -32,14 +33,34 @@ namespace {
 
 int io_fail(const std::string& msg) { return fail(ASC_ERR_IO, msg); }
 
+// The entry points return a status whatever happens inside: a failed
+// allocation or a library exception becomes an error, never an abort.
+template <class F>
+int guarded(F&& f)
+{
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return fail(ASC_ERR_NOMEM, "out of host memory");
+  } catch (const std::exception& e) {
+    return fail(ASC_ERR_IO, e.what());
+  }
+}
+
 bool read_file(const std::string& path, std::vector<uint8_t>& out)
 {
+  struct stat st;
+  if (::stat(path.c_str(), &st) != 0 || !S_ISREG(st.st_mode)) return false;  // not a directory
   std::FILE* f = std::fopen(path.c_str(), "rb");
   if (!f) return false;
   std::fseek(f, 0, SEEK_END);
   const long n = std::ftell(f);
   std::fseek(f, 0, SEEK_SET);
-  out.resize(n > 0 ? (size_t)n : 0);
+  if (n < 0) {
+    std::fclose(f);
+    return false;
+  }
+  out.resize((size_t)n);
   const size_t got = n > 0 ? std::fread(out.data(), 1, out.size(), f) : 0;
   std::fclose(f);
   return got == out.size();
@@ -332,11 +353,24 @@ int tiff_read_all(const std::string& path, std::vector<uint8_t>& out, int dims[3
   if (volume && p0.sample_format != 1)
     return fail(ASC_ERR_UNSUPPORTED, "Unsupported TIFF image format. Invalid 'SampleFormat' value in " + path +
                                        ". Current value is " + std::to_string(p0.sample_format) + ", expected 1");
+  if (p0.width > (1u << 20) || p0.height > (1u << 20) || p0.spp > 4)
+    return io_fail("Unsupported TIFF image size in " + path);
   const size_t page_bytes = (size_t)p0.width * p0.height * (p0.bps / 8) * p0.spp;
   const size_t npages = volume ? t.pages.size() : 1;
+  // the pixels must fit in the file: uncompressed strips hold them as they
+  // are, and no PackBits or LZW strip expands by more than 2048 times
+  uint64_t stored = 0;
   for (size_t i = 0; i < npages; i++) {
     const TiffPage& p = t.pages[i];
-    if (p.width != p0.width || p.height != p0.height || p.bps != p0.bps)
+    uint64_t bytes = 0;
+    for (uint64_t b : p.strip_bytes) bytes += std::min<uint64_t>(b, t.bytes.size());
+    stored += p.compression == 1 ? std::min<uint64_t>(bytes ? bytes : t.bytes.size(), t.bytes.size())
+                                 : 2048 * std::min<uint64_t>(bytes, t.bytes.size());
+  }
+  if ((uint64_t)page_bytes * npages > stored) return io_fail("TIFF pixel data missing or truncated in " + path);
+  for (size_t i = 0; i < npages; i++) {
+    const TiffPage& p = t.pages[i];
+    if (p.width != p0.width || p.height != p0.height || p.bps != p0.bps || p.spp != p0.spp)
       return fail(ASC_ERR_UNSUPPORTED, "Unsupported TIFF image format. Non uniform volume slices in " + path +
                                          ". Slice " + std::to_string(i) + " has dimensions " +
                                          std::to_string(p.width) + "x" + std::to_string(p.height) + "x" +
@@ -357,6 +391,13 @@ int tiff_info(const std::string& path, int dims[3], int* bps, bool volume)
   TiffFile t;
   int rc = tiff_open(path, t);
   if (rc) return rc;
+  // what the reader can decode, so that the sizes reported describe it
+  const TiffPage& p = t.pages[0];
+  if (p.tiled) return fail(ASC_ERR_UNSUPPORTED, "Tiled TIFF images are not supported: " + path);
+  if (p.bps != 8 && p.bps != 16 && p.bps != 32)
+    return fail(ASC_ERR_UNSUPPORTED, "Unsupported TIFF bits per sample " + std::to_string(p.bps) + ": " + path);
+  if (p.width == 0 || p.height == 0 || p.width > (1u << 20) || p.height > (1u << 20))
+    return io_fail("Unsupported TIFF image size in " + path);
   dims[0] = (int)t.pages[0].width;
   dims[1] = (int)t.pages[0].height;
   dims[2] = volume ? (int)t.pages.size() : 1;
@@ -788,273 +829,297 @@ struct asc_trial_s {
 
 ASC_API int asc_tiff_info(const char* path, int* width, int* height, int* pages, int* bits_per_sample)
 {
-  if (!path || !width || !height || !pages || !bits_per_sample) return fail(ASC_ERR_INVALID, "null pointer");
-  int d[3], bps;
-  const int rc = tiff_info(path, d, &bps, true);
-  if (rc) return rc;
-  *width = d[0];
-  *height = d[1];
-  *pages = d[2];
-  *bits_per_sample = bps;
-  return ASC_OK;
+  return guarded([&]() -> int {
+    if (!path || !width || !height || !pages || !bits_per_sample) return fail(ASC_ERR_INVALID, "null pointer");
+    int d[3], bps;
+    const int rc = tiff_info(path, d, &bps, true);
+    if (rc) return rc;
+    *width = d[0];
+    *height = d[1];
+    *pages = d[2];
+    *bits_per_sample = bps;
+    return ASC_OK;
+  });
 }
 
 ASC_API int asc_tiff_read(const char* path, void* dst, size_t dst_bytes)
 {
-  if (!path || !dst) return fail(ASC_ERR_INVALID, "null pointer");
-  std::vector<uint8_t> data;
-  int d[3], bps;
-  const int rc = tiff_read_all(path, data, d, &bps, true);
-  if (rc) return rc;
-  if (dst_bytes != data.size())
-    return fail(ASC_ERR_INVALID, "asc_tiff_read: buffer of " + std::to_string(dst_bytes) + " bytes for " +
-                                     std::to_string(data.size()) + " bytes of pixels");
-  std::memcpy(dst, data.data(), data.size());
-  return ASC_OK;
+  return guarded([&]() -> int {
+    if (!path || !dst) return fail(ASC_ERR_INVALID, "null pointer");
+    std::vector<uint8_t> data;
+    int d[3], bps;
+    const int rc = tiff_read_all(path, data, d, &bps, true);
+    if (rc) return rc;
+    if (dst_bytes != data.size())
+      return fail(ASC_ERR_INVALID, "asc_tiff_read: buffer of " + std::to_string(dst_bytes) + " bytes for " +
+                                       std::to_string(data.size()) + " bytes of pixels");
+    std::memcpy(dst, data.data(), data.size());
+    return ASC_OK;
+  });
 }
 
 ASC_API int asc_camera_from_mayacam1(const double values[15], asc_camera* out)
 {
-  if (!values || !out) return fail(ASC_ERR_INVALID, "null pointer");
-  return camera_mayacam1(values, *out);
+  return guarded([&]() -> int {
+    if (!values || !out) return fail(ASC_ERR_INVALID, "null pointer");
+    return camera_mayacam1(values, *out);
+  });
 }
 
 ASC_API int asc_camera_from_vtkcam(const double position[3], const double focal_point[3], const double view_up[3],
                                    double view_angle_deg, const double size[2], asc_camera* out)
 {
-  if (!position || !focal_point || !view_up || !size || !out) return fail(ASC_ERR_INVALID, "null pointer");
-  return camera_vtk(position, focal_point, view_up, view_angle_deg, size, *out);
+  return guarded([&]() -> int {
+    if (!position || !focal_point || !view_up || !size || !out) return fail(ASC_ERR_INVALID, "null pointer");
+    return camera_vtk(position, focal_point, view_up, view_angle_deg, size, *out);
+  });
 }
 
 ASC_API int asc_read_camera(const char* path, asc_camera* out)
 {
-  if (!path || !out) return fail(ASC_ERR_INVALID, "null pointer");
-  return read_camera(path, *out);
+  return guarded([&]() -> int {
+    if (!path || !out) return fail(ASC_ERR_INVALID, "null pointer");
+    return read_camera(path, *out);
+  });
 }
 
 ASC_API int asc_read_filter_settings(const char* path, asc_filter_settings* out)
 {
-  if (!path || !out) return fail(ASC_ERR_INVALID, "null pointer");
-  std::ifstream f(path);
-  if (!f.is_open()) return io_fail(std::string("File not found: ") + path);
-  asc_filter_settings s;
-  std::memset(&s, 0, sizeof s);
-  // DRR renderer spinboxes: Sample Distance 0.62 in [0.01, 1], XRay
-  // Intensity 0.49 in [0, 1], XRay Cutoff 0 in [0, 1]
-  // (ModelViewTreeWidgetItem.cpp:88-90)
-  double sd = 0.62, ri = 0.49, co = 0.0;
-  std::vector<asc_filter> drr, rad;
-  std::string line;
-  while (get_line(f, line)) {
-    if (line == "DrrRenderer_begin") {
-      s.has_drr_renderer = 1;
-      while (get_line(f, line) && line != "DrrRenderer_end") {
-        std::istringstream ls(line);
-        std::string k;
-        float v = 0.0f;
-        ls >> k;
-        if (k == "SampleDistance" && (ls >> v)) sd = spinbox(v, 0.01, 1.0);
-        if (k == "RayIntensity" && (ls >> v)) ri = spinbox(v, 0.0, 1.0);
-        if (k == "Cutoff" && (ls >> v)) co = spinbox(v, 0.0, 1.0);
+  return guarded([&]() -> int {
+    if (!path || !out) return fail(ASC_ERR_INVALID, "null pointer");
+    std::ifstream f(path);
+    if (!f.is_open()) return io_fail(std::string("File not found: ") + path);
+    asc_filter_settings s;
+    std::memset(&s, 0, sizeof s);
+    // DRR renderer spinboxes: Sample Distance 0.62 in [0.01, 1], XRay
+    // Intensity 0.49 in [0, 1], XRay Cutoff 0 in [0, 1]
+    // (ModelViewTreeWidgetItem.cpp:88-90)
+    double sd = 0.62, ri = 0.49, co = 0.0;
+    std::vector<asc_filter> drr, rad;
+    std::string line;
+    while (get_line(f, line)) {
+      if (line == "DrrRenderer_begin") {
+        s.has_drr_renderer = 1;
+        while (get_line(f, line) && line != "DrrRenderer_end") {
+          std::istringstream ls(line);
+          std::string k;
+          float v = 0.0f;
+          ls >> k;
+          if (k == "SampleDistance" && (ls >> v)) sd = spinbox(v, 0.01, 1.0);
+          if (k == "RayIntensity" && (ls >> v)) ri = spinbox(v, 0.0, 1.0);
+          if (k == "Cutoff" && (ls >> v)) co = spinbox(v, 0.0, 1.0);
+        }
+      } else if (line == "DrrFilters_begin") {
+        s.has_drr_filters = 1;
+        drr.clear();
+        read_filters(f, "DrrFilters_end", drr);
+      } else if (line == "RadFilters_begin") {
+        s.has_rad_filters = 1;
+        rad.clear();
+        read_filters(f, "RadFilters_end", rad);
       }
-    } else if (line == "DrrFilters_begin") {
-      s.has_drr_filters = 1;
-      drr.clear();
-      read_filters(f, "DrrFilters_end", drr);
-    } else if (line == "RadFilters_begin") {
-      s.has_rad_filters = 1;
-      rad.clear();
-      read_filters(f, "RadFilters_end", rad);
     }
-  }
-  if ((int)drr.size() > ASC_MAX_FILTERS || (int)rad.size() > ASC_MAX_FILTERS)
-    return fail(ASC_ERR_UNSUPPORTED, std::string("more than ") + std::to_string(ASC_MAX_FILTERS) +
-                                         " filters in one chain: " + path);
-  s.n_drr_filters = (int)drr.size();
-  s.n_rad_filters = (int)rad.size();
-  std::copy(drr.begin(), drr.end(), s.drr_filters);
-  std::copy(rad.begin(), rad.end(), s.rad_filters);
-  s.sample_distance = sd;
-  s.ray_intensity = ri;
-  s.cutoff_fraction = co;
-  // ModelViewTreeWidgetItem::updateModelview (:330-338), through the
-  // RayCaster's float setters
-  s.drr_step = (float)std::exp(7 * sd - 5);
-  s.drr_intensity = (float)std::exp(15 * ri - 5);
-  *out = s;
-  return ASC_OK;
+    if ((int)drr.size() > ASC_MAX_FILTERS || (int)rad.size() > ASC_MAX_FILTERS)
+      return fail(ASC_ERR_UNSUPPORTED, std::string("more than ") + std::to_string(ASC_MAX_FILTERS) +
+                                           " filters in one chain: " + path);
+    s.n_drr_filters = (int)drr.size();
+    s.n_rad_filters = (int)rad.size();
+    std::copy(drr.begin(), drr.end(), s.drr_filters);
+    std::copy(rad.begin(), rad.end(), s.rad_filters);
+    s.sample_distance = sd;
+    s.ray_intensity = ri;
+    s.cutoff_fraction = co;
+    // ModelViewTreeWidgetItem::updateModelview (:330-338), through the
+    // RayCaster's float setters
+    s.drr_step = (float)std::exp(7 * sd - 5);
+    s.drr_intensity = (float)std::exp(15 * ri - 5);
+    *out = s;
+    return ASC_OK;
+  });
 }
 
 ASC_API int asc_drr_cutoff(double cutoff_fraction, const asc_volume_desc* desc, float* cutoff)
 {
-  if (!desc || !cutoff) return fail(ASC_ERR_INVALID, "null pointer");
-  // value * (max cutoff - min cutoff) + min cutoff, the volume's normalised
-  // value range (ModelViewTreeWidgetItem.cpp:336-338)
-  *cutoff = (float)(cutoff_fraction * ((double)desc->max_value - (double)desc->min_value) + (double)desc->min_value);
-  return ASC_OK;
+  return guarded([&]() -> int {
+    if (!desc || !cutoff) return fail(ASC_ERR_INVALID, "null pointer");
+    // value * (max cutoff - min cutoff) + min cutoff, the volume's normalised
+    // value range (ModelViewTreeWidgetItem.cpp:336-338)
+    *cutoff = (float)(cutoff_fraction * ((double)desc->max_value - (double)desc->min_value) + (double)desc->min_value);
+    return ASC_OK;
+  });
 }
 
 ASC_API int asc_trial_open(const char* cfg, asc_trial* out)
 {
-  if (!cfg || !out) return fail(ASC_ERR_INVALID, "null pointer");
-  const std::string path(cfg);
-  std::ifstream f(path);
-  if (!f.is_open()) return io_fail("File not found: " + path);
-  auto err = [&](const std::string& m) {
-    return io_fail("Invalid trial configuration file. " + m + " See " + path + ".");
-  };
-  int version[2] = {0, 0};
-  std::vector<std::string> cams, roots, vols, sizes, flips, res, offs, meshes;
-  std::string line;
-  while (get_line(f, line)) {
-    if (line.empty() || line[0] == '#') continue;
-    std::istringstream ls(line);
-    std::string key, value;
-    std::getline(ls, key, ' ');
-    std::getline(ls, value);
-    if (key == "mayaCam_csv") {
-      unix_slashes(value);
-      cams.push_back(value);
-    } else if (key == "CameraRootDir") {
-      unix_slashes(value);
-      roots.push_back(value);
-    } else if (key == "VolumeFile") {
-      unix_slashes(value);
-      vols.push_back(value);
-    } else if (key == "VolumeFlip") {
-      flips.push_back(value);
-    } else if (key == "VoxelSize") {
-      sizes.push_back(value);
-    } else if (key == "RenderResolution") {
-      res.push_back(value);
-    } else if (key == "OptimizationOffsets") {
-      offs.push_back(value);
-    } else if (key == "MeshFile") {
-      meshes.push_back(value);
-    } else if (key == "Version") {
-      std::istringstream vs(value);
-      for (int k = 0; k < 2; k++) {
-        std::string part;
-        std::getline(vs, part, '.');
-        version[k] = std::atoi(part.c_str());
+  return guarded([&]() -> int {
+    if (!cfg || !out) return fail(ASC_ERR_INVALID, "null pointer");
+    const std::string path(cfg);
+    std::ifstream f(path);
+    if (!f.is_open()) return io_fail("File not found: " + path);
+    auto err = [&](const std::string& m) {
+      return io_fail("Invalid trial configuration file. " + m + " See " + path + ".");
+    };
+    int version[2] = {0, 0};
+    std::vector<std::string> cams, roots, vols, sizes, flips, res, offs, meshes;
+    std::string line;
+    while (get_line(f, line)) {
+      if (line.empty() || line[0] == '#') continue;
+      std::istringstream ls(line);
+      std::string key, value;
+      std::getline(ls, key, ' ');
+      std::getline(ls, value);
+      if (key == "mayaCam_csv") {
+        unix_slashes(value);
+        cams.push_back(value);
+      } else if (key == "CameraRootDir") {
+        unix_slashes(value);
+        roots.push_back(value);
+      } else if (key == "VolumeFile") {
+        unix_slashes(value);
+        vols.push_back(value);
+      } else if (key == "VolumeFlip") {
+        flips.push_back(value);
+      } else if (key == "VoxelSize") {
+        sizes.push_back(value);
+      } else if (key == "RenderResolution") {
+        res.push_back(value);
+      } else if (key == "OptimizationOffsets") {
+        offs.push_back(value);
+      } else if (key == "MeshFile") {
+        meshes.push_back(value);
+      } else if (key == "Version") {
+        std::istringstream vs(value);
+        for (int k = 0; k < 2; k++) {
+          std::string part;
+          std::getline(vs, part, '.');
+          version[k] = std::atoi(part.c_str());
+        }
       }
     }
-  }
-  if (version[0] == 0 && version[1] == 0) {
-    version[0] = 1;
-    version[1] = 0;
-  }
-  if (version[0] == 1 && version[1] >= 1) {
-    const std::string base = parent_dir(path);
-    for (auto* list : {&cams, &roots, &vols, &meshes})
-      for (auto& p : *list) p = absolute(p, base);
-  }
-  if (cams.size() < 1) return err("There must be at least one mayacam files.");
-  if (cams.size() != roots.size())
-    return err("The number of cameras and videos must match. Found " + std::to_string(cams.size()) +
-               " cameras and " + std::to_string(roots.size()) + " videos.");
-  if (vols.size() < 1) return err("There must be at least one volume file.");
-  if (vols.size() != sizes.size())
-    return err("Each volume must be associated with its corresponding voxel sizes. Found " +
-               std::to_string(vols.size()) + " volumes and " + std::to_string(sizes.size()) + " voxel sizes.");
-  if (!meshes.empty() && meshes.size() != vols.size())
-    return err("You must specify a mesh file for each volume or none at all. Found " + std::to_string(vols.size()) +
-               " volumes and " + std::to_string(meshes.size()) + " mesh files.");
-  std::unique_ptr<asc_trial_s> t(new asc_trial_s());
-  std::memset(&t->info, 0, sizeof t->info);
-  t->info.version[0] = version[0];
-  t->info.version[1] = version[1];
-  int rc;
-  for (const std::string& c : cams) {
-    asc_camera cam;
-    if ((rc = read_camera(c, cam))) return rc;
-    t->cameras.push_back(cam);
-  }
-  for (size_t i = 0; i < vols.size(); i++) {
-    int d[3], bps;
-    if ((rc = tiff_info(vols[i], d, &bps, true))) return rc;
-    int fx = 0, fy = 0, fz = 0;
-    if (i < flips.size()) {
-      std::stringstream fs(flips[i]);
-      fs >> fx >> fy >> fz;
+    if (version[0] == 0 && version[1] == 0) {
+      version[0] = 1;
+      version[1] = 0;
     }
-    float sx = 0, sy = 0, sz = 0;
-    std::stringstream ss(sizes[i]);
-    ss >> sx >> sy >> sz;
-    t->volume_files.push_back(vols[i]);
-    t->voxel_mm.push_back({(double)sx, (double)sy, (double)sz});
-    t->flips.push_back({fx != 0, fy != 0, fz != 0});
-  }
-  int max_frames = 0;
-  for (const std::string& root : roots) {
-    DIR* dir = opendir(root.c_str());
-    if (!dir) return io_fail("Unable to open directory: " + root);
-    std::vector<std::string> names;
-    while (dirent* ent = readdir(dir)) {
-      const std::string n(ent->d_name);
-      if (n.compare(0, 1, ".") == 0) continue;
-      const size_t ext = n.find_last_of('.') + 1;
-      if (n.compare(ext, 3, "tif") != 0) continue;
-      names.push_back(root + "/" + n);
+    if (version[0] == 1 && version[1] >= 1) {
+      const std::string base = parent_dir(path);
+      for (auto* list : {&cams, &roots, &vols, &meshes})
+        for (auto& p : *list) p = absolute(p, base);
     }
-    closedir(dir);
-    if (names.empty()) return io_fail("No TIFF files found in directory: " + root);
-    std::sort(names.begin(), names.end());
-    max_frames = std::max(max_frames, (int)names.size());
-    t->frames.push_back(names);
-  }
-  t->info.n_cameras = (int)t->cameras.size();
-  t->info.n_volumes = (int)t->volume_files.size();
-  t->info.n_frames = max_frames;
-  for (int k = 0; k < 6; k++) t->info.offsets[k] = 0.1;
-  if (!offs.empty()) {
-    std::stringstream os(offs.back());
-    for (int k = 0; k < 6; k++) os >> t->info.offsets[k];
-  }
-  t->info.render_width = 512;
-  t->info.render_height = 512;
-  if (!res.empty()) {
-    std::stringstream rs(res.back());
-    rs >> t->info.render_width >> t->info.render_height;
-  }
-  *out = t.release();
-  return ASC_OK;
+    if (cams.size() < 1) return err("There must be at least one mayacam files.");
+    if (cams.size() != roots.size())
+      return err("The number of cameras and videos must match. Found " + std::to_string(cams.size()) +
+                 " cameras and " + std::to_string(roots.size()) + " videos.");
+    if (vols.size() < 1) return err("There must be at least one volume file.");
+    if (vols.size() != sizes.size())
+      return err("Each volume must be associated with its corresponding voxel sizes. Found " +
+                 std::to_string(vols.size()) + " volumes and " + std::to_string(sizes.size()) + " voxel sizes.");
+    if (!meshes.empty() && meshes.size() != vols.size())
+      return err("You must specify a mesh file for each volume or none at all. Found " + std::to_string(vols.size()) +
+                 " volumes and " + std::to_string(meshes.size()) + " mesh files.");
+    std::unique_ptr<asc_trial_s> t(new asc_trial_s());
+    std::memset(&t->info, 0, sizeof t->info);
+    t->info.version[0] = version[0];
+    t->info.version[1] = version[1];
+    int rc;
+    for (const std::string& c : cams) {
+      asc_camera cam;
+      if ((rc = read_camera(c, cam))) return rc;
+      t->cameras.push_back(cam);
+    }
+    for (size_t i = 0; i < vols.size(); i++) {
+      int d[3], bps;
+      if ((rc = tiff_info(vols[i], d, &bps, true))) return rc;
+      int fx = 0, fy = 0, fz = 0;
+      if (i < flips.size()) {
+        std::stringstream fs(flips[i]);
+        fs >> fx >> fy >> fz;
+      }
+      float sx = 0, sy = 0, sz = 0;
+      std::stringstream ss(sizes[i]);
+      ss >> sx >> sy >> sz;
+      t->volume_files.push_back(vols[i]);
+      t->voxel_mm.push_back({(double)sx, (double)sy, (double)sz});
+      t->flips.push_back({fx != 0, fy != 0, fz != 0});
+    }
+    int max_frames = 0;
+    for (const std::string& root : roots) {
+      DIR* dir = opendir(root.c_str());
+      if (!dir) return io_fail("Unable to open directory: " + root);
+      std::vector<std::string> names;
+      while (dirent* ent = readdir(dir)) {
+        const std::string n(ent->d_name);
+        if (n.compare(0, 1, ".") == 0) continue;
+        const size_t ext = n.find_last_of('.') + 1;
+        if (n.compare(ext, 3, "tif") != 0) continue;
+        names.push_back(root + "/" + n);
+      }
+      closedir(dir);
+      if (names.empty()) return io_fail("No TIFF files found in directory: " + root);
+      std::sort(names.begin(), names.end());
+      max_frames = std::max(max_frames, (int)names.size());
+      t->frames.push_back(names);
+    }
+    t->info.n_cameras = (int)t->cameras.size();
+    t->info.n_volumes = (int)t->volume_files.size();
+    t->info.n_frames = max_frames;
+    for (int k = 0; k < 6; k++) t->info.offsets[k] = 0.1;
+    if (!offs.empty()) {
+      std::stringstream os(offs.back());
+      for (int k = 0; k < 6; k++) os >> t->info.offsets[k];
+    }
+    t->info.render_width = 512;
+    t->info.render_height = 512;
+    if (!res.empty()) {
+      std::stringstream rs(res.back());
+      rs >> t->info.render_width >> t->info.render_height;
+    }
+    *out = t.release();
+    return ASC_OK;
+  });
 }
 
 ASC_API int asc_trial_get_info(asc_trial t, asc_trial_info* out)
 {
-  if (!t || !out) return fail(ASC_ERR_INVALID, "null pointer");
-  *out = t->info;
-  return ASC_OK;
+  return guarded([&]() -> int {
+    if (!t || !out) return fail(ASC_ERR_INVALID, "null pointer");
+    *out = t->info;
+    return ASC_OK;
+  });
 }
 
 ASC_API int asc_trial_camera(asc_trial t, int camera, asc_camera* out)
 {
-  if (!t || !out) return fail(ASC_ERR_INVALID, "null pointer");
-  if (camera < 0 || camera >= t->info.n_cameras) return fail(ASC_ERR_INVALID, "camera index out of range");
-  *out = t->cameras[camera];
-  return ASC_OK;
+  return guarded([&]() -> int {
+    if (!t || !out) return fail(ASC_ERR_INVALID, "null pointer");
+    if (camera < 0 || camera >= t->info.n_cameras) return fail(ASC_ERR_INVALID, "camera index out of range");
+    *out = t->cameras[camera];
+    return ASC_OK;
+  });
 }
 
 ASC_API int asc_trial_volume_info(asc_trial t, int volume, int dims[3], int* bits_per_sample, double voxel_mm[3],
                                   int flip[3])
 {
-  if (!t || !dims || !bits_per_sample || !voxel_mm || !flip) return fail(ASC_ERR_INVALID, "null pointer");
-  if (volume < 0 || volume >= t->info.n_volumes) return fail(ASC_ERR_INVALID, "volume index out of range");
-  const int rc = tiff_info(t->volume_files[volume], dims, bits_per_sample, true);
-  if (rc) return rc;
-  for (int a = 0; a < 3; a++) {
-    voxel_mm[a] = t->voxel_mm[volume][a];
-    flip[a] = t->flips[volume][a];
-  }
-  return ASC_OK;
+  return guarded([&]() -> int {
+    if (!t || !dims || !bits_per_sample || !voxel_mm || !flip) return fail(ASC_ERR_INVALID, "null pointer");
+    if (volume < 0 || volume >= t->info.n_volumes) return fail(ASC_ERR_INVALID, "volume index out of range");
+    const int rc = tiff_info(t->volume_files[volume], dims, bits_per_sample, true);
+    if (rc) return rc;
+    for (int a = 0; a < 3; a++) {
+      voxel_mm[a] = t->voxel_mm[volume][a];
+      flip[a] = t->flips[volume][a];
+    }
+    return ASC_OK;
+  });
 }
 
 ASC_API int asc_trial_volume_read(asc_trial t, int volume, void* dst, size_t dst_bytes)
 {
-  if (!t || !dst) return fail(ASC_ERR_INVALID, "null pointer");
-  if (volume < 0 || volume >= t->info.n_volumes) return fail(ASC_ERR_INVALID, "volume index out of range");
-  return asc_tiff_read(t->volume_files[volume].c_str(), dst, dst_bytes);
+  return guarded([&]() -> int {
+    if (!t || !dst) return fail(ASC_ERR_INVALID, "null pointer");
+    if (volume < 0 || volume >= t->info.n_volumes) return fail(ASC_ERR_INVALID, "volume index out of range");
+    return asc_tiff_read(t->volume_files[volume].c_str(), dst, dst_bytes);
+  });
 }
 
 // Video::set_frame clamps the index to the last frame (Video.cpp:221-225).
@@ -1067,184 +1132,198 @@ static const std::string* trial_frame(asc_trial t, int camera, int frame)
 
 ASC_API int asc_trial_frame_info(asc_trial t, int camera, int frame, int dims[2], int* bits_per_sample)
 {
-  if (!t || !dims || !bits_per_sample) return fail(ASC_ERR_INVALID, "null pointer");
-  const std::string* p = trial_frame(t, camera, frame);
-  if (!p) return fail(ASC_ERR_INVALID, "camera or frame index out of range");
-  int d[3];
-  const int rc = tiff_info(*p, d, bits_per_sample, false);
-  dims[0] = d[0];
-  dims[1] = d[1];
-  return rc;
+  return guarded([&]() -> int {
+    if (!t || !dims || !bits_per_sample) return fail(ASC_ERR_INVALID, "null pointer");
+    const std::string* p = trial_frame(t, camera, frame);
+    if (!p) return fail(ASC_ERR_INVALID, "camera or frame index out of range");
+    int d[3];
+    const int rc = tiff_info(*p, d, bits_per_sample, false);
+    dims[0] = d[0];
+    dims[1] = d[1];
+    return rc;
+  });
 }
 
 ASC_API int asc_trial_frame_read(asc_trial t, int camera, int frame, void* dst, size_t dst_bytes)
 {
-  if (!t || !dst) return fail(ASC_ERR_INVALID, "null pointer");
-  const std::string* p = trial_frame(t, camera, frame);
-  if (!p) return fail(ASC_ERR_INVALID, "camera or frame index out of range");
-  std::vector<uint8_t> data;
-  int d[3], bps;
-  const int rc = tiff_read_all(*p, data, d, &bps, false);
-  if (rc) return rc;
-  if (dst_bytes != data.size())
-    return fail(ASC_ERR_INVALID, "asc_trial_frame_read: buffer of " + std::to_string(dst_bytes) + " bytes for " +
-                                     std::to_string(data.size()) + " bytes of pixels");
-  std::memcpy(dst, data.data(), data.size());
-  return ASC_OK;
+  return guarded([&]() -> int {
+    if (!t || !dst) return fail(ASC_ERR_INVALID, "null pointer");
+    const std::string* p = trial_frame(t, camera, frame);
+    if (!p) return fail(ASC_ERR_INVALID, "camera or frame index out of range");
+    std::vector<uint8_t> data;
+    int d[3], bps;
+    const int rc = tiff_read_all(*p, data, d, &bps, false);
+    if (rc) return rc;
+    if (dst_bytes != data.size())
+      return fail(ASC_ERR_INVALID, "asc_trial_frame_read: buffer of " + std::to_string(dst_bytes) + " bytes for " +
+                                       std::to_string(data.size()) + " bytes of pixels");
+    std::memcpy(dst, data.data(), data.size());
+    return ASC_OK;
+  });
 }
 
 ASC_API int asc_trial_frame_path(asc_trial t, int camera, int frame, char* dst, size_t dst_bytes)
 {
-  if (!t || !dst || dst_bytes == 0) return fail(ASC_ERR_INVALID, "null pointer");
-  const std::string* p = trial_frame(t, camera, frame);
-  if (!p) return fail(ASC_ERR_INVALID, "camera or frame index out of range");
-  if (p->size() + 1 > dst_bytes) return fail(ASC_ERR_INVALID, "path buffer too small");
-  std::memcpy(dst, p->c_str(), p->size() + 1);
-  return ASC_OK;
+  return guarded([&]() -> int {
+    if (!t || !dst || dst_bytes == 0) return fail(ASC_ERR_INVALID, "null pointer");
+    const std::string* p = trial_frame(t, camera, frame);
+    if (!p) return fail(ASC_ERR_INVALID, "camera or frame index out of range");
+    if (p->size() + 1 > dst_bytes) return fail(ASC_ERR_INVALID, "path buffer too small");
+    std::memcpy(dst, p->c_str(), p->size() + 1);
+    return ASC_OK;
+  });
 }
 
 // Video::create_background_image (Video.cpp:147-218): per pixel, the
 // largest normalised value over all frames of the camera.
 ASC_API int asc_trial_background(asc_trial t, int camera, float* dst, size_t count)
 {
-  if (!t || !dst) return fail(ASC_ERR_INVALID, "null pointer");
-  if (camera < 0 || camera >= t->info.n_cameras) return fail(ASC_ERR_INVALID, "camera index out of range");
-  const auto& names = t->frames[camera];
-  if (names.size() < 2) return fail(ASC_ERR_INVALID, "Not enough images to create background image.");
-  std::fill(dst, dst + count, 0.0f);
-  for (const std::string& n : names) {
-    std::vector<uint8_t> data;
-    int d[3], bps;
-    const int rc = tiff_read_all(n, data, d, &bps, false);
-    if (rc) return rc;
-    const size_t px = (size_t)d[0] * d[1];
-    if (px != count)
-      return fail(ASC_ERR_INVALID, "asc_trial_background: " + std::to_string(count) + " floats for " +
-                                       std::to_string(px) + " pixels");
-    for (size_t i = 0; i < px; i++) {
-      float v;
-      if (bps == 8)
-        v = (float)data[i] / 255u;
-      else if (bps == 16)
-        v = (float)((const uint16_t*)data.data())[i] / 65535u;
-      else
-        v = (float)((const uint32_t*)data.data())[i] / 4294967295u;
-      if (v > dst[i]) dst[i] = v;
+  return guarded([&]() -> int {
+    if (!t || !dst) return fail(ASC_ERR_INVALID, "null pointer");
+    if (camera < 0 || camera >= t->info.n_cameras) return fail(ASC_ERR_INVALID, "camera index out of range");
+    const auto& names = t->frames[camera];
+    if (names.size() < 2) return fail(ASC_ERR_INVALID, "Not enough images to create background image.");
+    std::fill(dst, dst + count, 0.0f);
+    for (const std::string& n : names) {
+      std::vector<uint8_t> data;
+      int d[3], bps;
+      const int rc = tiff_read_all(n, data, d, &bps, false);
+      if (rc) return rc;
+      const size_t px = (size_t)d[0] * d[1];
+      if (px != count)
+        return fail(ASC_ERR_INVALID, "asc_trial_background: " + std::to_string(count) + " floats for " +
+                                         std::to_string(px) + " pixels");
+      for (size_t i = 0; i < px; i++) {
+        float v;
+        if (bps == 8)
+          v = (float)data[i] / 255u;
+        else if (bps == 16)
+          v = (float)((const uint16_t*)data.data())[i] / 65535u;
+        else
+          v = (float)((const uint32_t*)data.data())[i] / 4294967295u;
+        if (v > dst[i]) dst[i] = v;
+      }
     }
-  }
-  return ASC_OK;
+    return ASC_OK;
+  });
 }
 
 ASC_API int asc_trial_close(asc_trial t)
 {
-  delete t;
-  return ASC_OK;
+  return guarded([&]() -> int {
+    delete t;
+    return ASC_OK;
+  });
 }
 
 // ---- tracking results (.tra) ------------------------------------------------
 
 ASC_API int asc_write_tra(const char* path, const double* poses, int n_frames, int n_volumes, int flags)
 {
-  if (!path || (!poses && n_frames * n_volumes > 0) || n_frames < 0 || n_volumes < 1)
-    return fail(ASC_ERR_INVALID, "bad argument");
-  std::FILE* f = std::fopen(path, "w");
-  if (!f) return io_fail(std::string("Failed to save to file: ") + path);
-  const char* s = (flags & ASC_TRA_COMMAS) ? "," : " ";
-  const bool matrix = flags & ASC_TRA_MATRIX;
-  for (int i = 0; i < n_frames; i++) {
-    for (int j = 0; j < n_volumes; j++) {
-      const double* p = poses + ((size_t)i * n_volumes + j) * 6;
-      bool invalid = false;
-      for (int k = 0; k < 6; k++) invalid |= std::isnan(p[k]);
-      const int n = matrix ? 16 : 6;
-      double v[16];
-      if (!invalid && matrix) {
-        const Frame fr = frame_from_xyzypr(p);
-        // CoordFrame::to_matrix (CoordFrame.cpp:203-221): column-major 4x4
-        for (int c = 0; c < 3; c++) {
-          for (int r = 0; r < 3; r++) v[4 * c + r] = fr.r[3 * c + r];
-          v[4 * c + 3] = 0.0;
-          v[12 + c] = fr.t[c];
+  return guarded([&]() -> int {
+    if (!path || (!poses && n_frames * n_volumes > 0) || n_frames < 0 || n_volumes < 1)
+      return fail(ASC_ERR_INVALID, "bad argument");
+    std::FILE* f = std::fopen(path, "w");
+    if (!f) return io_fail(std::string("Failed to save to file: ") + path);
+    const char* s = (flags & ASC_TRA_COMMAS) ? "," : " ";
+    const bool matrix = flags & ASC_TRA_MATRIX;
+    for (int i = 0; i < n_frames; i++) {
+      for (int j = 0; j < n_volumes; j++) {
+        const double* p = poses + ((size_t)i * n_volumes + j) * 6;
+        bool invalid = false;
+        for (int k = 0; k < 6; k++) invalid |= std::isnan(p[k]);
+        const int n = matrix ? 16 : 6;
+        double v[16];
+        if (!invalid && matrix) {
+          const Frame fr = frame_from_xyzypr(p);
+          // CoordFrame::to_matrix (CoordFrame.cpp:203-221): column-major 4x4
+          for (int c = 0; c < 3; c++) {
+            for (int r = 0; r < 3; r++) v[4 * c + r] = fr.r[3 * c + r];
+            v[4 * c + 3] = 0.0;
+            v[12 + c] = fr.t[c];
+          }
+          v[15] = 1.0;
+          if (flags & ASC_TRA_CM)
+            for (int c = 12; c < 15; c++) v[c] /= 10.0;
+          if (flags & ASC_TRA_ROWS) {
+            double w[16];
+            for (int r = 0; r < 4; r++)
+              for (int c = 0; c < 4; c++) w[4 * r + c] = v[4 * c + r];
+            std::memcpy(v, w, sizeof v);
+          }
+        } else if (!invalid) {
+          std::memcpy(v, p, 6 * sizeof(double));
+          if (flags & ASC_TRA_CM)
+            for (int c = 0; c < 3; c++) v[c] /= 10.0;
+          if (flags & ASC_TRA_RADIANS)
+            for (int c = 3; c < 6; c++) v[c] *= M_PI / 180.0;
         }
-        v[15] = 1.0;
-        if (flags & ASC_TRA_CM)
-          for (int c = 12; c < 15; c++) v[c] /= 10.0;
-        if (flags & ASC_TRA_ROWS) {
-          double w[16];
-          for (int r = 0; r < 4; r++)
-            for (int c = 0; c < 4; c++) w[4 * r + c] = v[4 * c + r];
-          std::memcpy(v, w, sizeof v);
+        for (int k = 0; k < n; k++) {
+          if (k) std::fputs(s, f);
+          if (invalid)
+            std::fputs("NaN", f);
+          else
+            std::fprintf(f, "%.16f", v[k]);  // std::ios::fixed, precision 16
         }
-      } else if (!invalid) {
-        std::memcpy(v, p, 6 * sizeof(double));
-        if (flags & ASC_TRA_CM)
-          for (int c = 0; c < 3; c++) v[c] /= 10.0;
-        if (flags & ASC_TRA_RADIANS)
-          for (int c = 3; c < 6; c++) v[c] *= M_PI / 180.0;
+        if (j != n_volumes - 1) std::fputs(s, f);
       }
-      for (int k = 0; k < n; k++) {
-        if (k) std::fputs(s, f);
-        if (invalid)
-          std::fputs("NaN", f);
-        else
-          std::fprintf(f, "%.16f", v[k]);  // std::ios::fixed, precision 16
-      }
-      if (j != n_volumes - 1) std::fputs(s, f);
+      std::fputs("\n", f);
     }
-    std::fputs("\n", f);
-  }
-  const bool ok = std::fclose(f) == 0;
-  return ok ? ASC_OK : io_fail(std::string("Failed to save to file: ") + path);
+    const bool ok = std::fclose(f) == 0;
+    return ok ? ASC_OK : io_fail(std::string("Failed to save to file: ") + path);
+  });
 }
 
 ASC_API int asc_read_tra(const char* path, int n_frames, int n_volumes, int flags, double* poses, int* frames_read)
 {
-  if (!path || !poses || n_frames < 0 || n_volumes < 1) return fail(ASC_ERR_INVALID, "bad argument");
-  std::ifstream f(path);
-  if (!f.is_open()) return io_fail(std::string("Tracking Data File Not Found: ") + path);
-  const char s = (flags & ASC_TRA_COMMAS) ? ',' : ' ';
-  const bool matrix = flags & ASC_TRA_MATRIX;
-  const double nan = std::numeric_limits<double>::quiet_NaN();
-  std::fill(poses, poses + (size_t)n_frames * n_volumes * 6, nan);
-  std::string line, value;
-  int i = 0;
-  for (; i < n_frames && get_line(f, line); ++i) {
-    std::istringstream ls(line);
-    for (int k = 0; k < n_volumes; k++) {
-      double m[16] = {0};
-      for (int j = 0; j < (matrix ? 16 : 6) && std::getline(ls, value, s); ++j) {
-        std::istringstream vs(value);
-        vs >> m[j];
-      }
-      if (value.compare(0, 3, "NaN") == 0) continue;
-      if (matrix && (flags & ASC_TRA_ROWS)) {
-        double n[16];
-        std::memcpy(n, m, sizeof n);
-        for (int r = 0; r < 4; r++)
-          for (int c = 0; c < 4; c++) m[4 * c + r] = n[4 * r + c];
-      }
-      if (flags & ASC_TRA_CM) {
-        const int o = matrix ? 12 : 0;
-        for (int c = 0; c < 3; c++) m[o + c] *= 10.0;
-      }
-      if ((flags & ASC_TRA_RADIANS) && !matrix)
-        for (int c = 3; c < 6; c++) m[c] *= 180.0 / M_PI;
-      double* p = poses + ((size_t)i * n_volumes + k) * 6;
-      if (matrix) {
-        // CoordFrame::from_matrix (CoordFrame.cpp:191-201), then to_xyzypr
-        Frame fr;
-        for (int c = 0; c < 3; c++) {
-          for (int r = 0; r < 3; r++) fr.r[3 * c + r] = m[4 * c + r];
-          fr.t[c] = m[12 + c];
+  return guarded([&]() -> int {
+    if (!path || !poses || n_frames < 0 || n_volumes < 1) return fail(ASC_ERR_INVALID, "bad argument");
+    std::ifstream f(path);
+    if (!f.is_open()) return io_fail(std::string("Tracking Data File Not Found: ") + path);
+    const char s = (flags & ASC_TRA_COMMAS) ? ',' : ' ';
+    const bool matrix = flags & ASC_TRA_MATRIX;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    std::fill(poses, poses + (size_t)n_frames * n_volumes * 6, nan);
+    std::string line, value;
+    int i = 0;
+    for (; i < n_frames && get_line(f, line); ++i) {
+      std::istringstream ls(line);
+      for (int k = 0; k < n_volumes; k++) {
+        double m[16] = {0};
+        for (int j = 0; j < (matrix ? 16 : 6) && std::getline(ls, value, s); ++j) {
+          std::istringstream vs(value);
+          vs >> m[j];
         }
-        frame_to_xyzypr(fr, p);
-      } else {
-        std::memcpy(p, m, 6 * sizeof(double));
+        if (value.compare(0, 3, "NaN") == 0) continue;
+        if (matrix && (flags & ASC_TRA_ROWS)) {
+          double n[16];
+          std::memcpy(n, m, sizeof n);
+          for (int r = 0; r < 4; r++)
+            for (int c = 0; c < 4; c++) m[4 * c + r] = n[4 * r + c];
+        }
+        if (flags & ASC_TRA_CM) {
+          const int o = matrix ? 12 : 0;
+          for (int c = 0; c < 3; c++) m[o + c] *= 10.0;
+        }
+        if ((flags & ASC_TRA_RADIANS) && !matrix)
+          for (int c = 3; c < 6; c++) m[c] *= 180.0 / M_PI;
+        double* p = poses + ((size_t)i * n_volumes + k) * 6;
+        if (matrix) {
+          // CoordFrame::from_matrix (CoordFrame.cpp:191-201), then to_xyzypr
+          Frame fr;
+          for (int c = 0; c < 3; c++) {
+            for (int r = 0; r < 3; r++) fr.r[3 * c + r] = m[4 * c + r];
+            fr.t[c] = m[12 + c];
+          }
+          frame_to_xyzypr(fr, p);
+        } else {
+          std::memcpy(p, m, 6 * sizeof(double));
+        }
       }
     }
-  }
-  if (frames_read) *frames_read = i;
-  return ASC_OK;
+    if (frames_read) *frames_read = i;
+    return ASC_OK;
+  });
 }
 
 // ---- display output ----------------------------------------------------------
@@ -1333,24 +1412,28 @@ int write_png(const char* path, const uint8_t* pix, int w, int h, int channels)
 
 ASC_API int asc_write_png(const char* path, const uint8_t* pixels, int width, int height, int channels)
 {
-  return write_png(path, pixels, width, height, channels);
+  return guarded([&]() -> int {
+    return write_png(path, pixels, width, height, channels);
+  });
 }
 
 // The composite as glDrawPixels(GL_RGB, GL_FLOAT) shows it: row 0 at the
 // bottom, each component clamped to [0, 1] and rounded to 8 bits.
 ASC_API int asc_write_view_png(const char* path, const float* rgb, int width, int height)
 {
-  if (!rgb || width <= 0 || height <= 0) return fail(ASC_ERR_INVALID, "bad argument");
-  const size_t row = 3 * (size_t)width;
-  std::vector<uint8_t> pix(row * height);
-  for (int y = 0; y < height; y++) {
-    const float* src = rgb + row * (height - 1 - y);
-    for (size_t i = 0; i < row; i++) {
-      const float v = src[i];
-      pix[row * y + i] = v > 0.0f ? (uint8_t)std::floor(std::min(v, 1.0f) * 255.0f + 0.5f) : 0;
+  return guarded([&]() -> int {
+    if (!rgb || width <= 0 || height <= 0) return fail(ASC_ERR_INVALID, "bad argument");
+    const size_t row = 3 * (size_t)width;
+    std::vector<uint8_t> pix(row * height);
+    for (int y = 0; y < height; y++) {
+      const float* src = rgb + row * (height - 1 - y);
+      for (size_t i = 0; i < row; i++) {
+        const float v = src[i];
+        pix[row * y + i] = v > 0.0f ? (uint8_t)std::floor(std::min(v, 1.0f) * 255.0f + 0.5f) : 0;
+      }
     }
-  }
-  return write_png(path, pix.data(), width, height, 3);
+    return write_png(path, pix.data(), width, height, 3);
+  });
 }
 
 // save_full_drr (Tracker.cpp:155-204): the image stretched to 0..255 over
@@ -1360,25 +1443,27 @@ ASC_API int asc_write_view_png(const char* path, const float* rgb, int width, in
 // order. An image with no range (maximum == minimum) writes 255 everywhere.
 ASC_API int asc_write_full_drr_pgm(const char* path, const float* image, int width, int height)
 {
-  if (!path || !image || width <= 0 || height <= 0) return fail(ASC_ERR_INVALID, "bad argument");
-  const size_t n = (size_t)width * height;
-  float lo = std::numeric_limits<float>::max();
-  float hi = std::numeric_limits<float>::min();
-  for (size_t i = 0; i < n; i++) {
-    if (image[i] > hi) hi = image[i];
-    if (image[i] < lo) lo = image[i];
-  }
-  const float range = hi - lo;
-  std::string text = "P2\n" + std::to_string(width) + " " + std::to_string(height) + "\n255\n";
-  text.reserve(text.size() + 4 * n);
-  for (size_t i = 0; i < n; i++) {
-    unsigned char u = 0;
-    if (range > 0.0f) u = (unsigned char)(int)(255.0f * (image[i] - lo) / range);
-    text += std::to_string(255 - (int)u);
-    text += ' ';
-  }
-  std::FILE* f = std::fopen(path, "w");
-  if (!f) return io_fail(std::string("cannot write ") + path);
-  const bool ok = std::fwrite(text.data(), 1, text.size(), f) == text.size();
-  return (std::fclose(f) == 0 && ok) ? ASC_OK : io_fail(std::string("write failed: ") + path);
+  return guarded([&]() -> int {
+    if (!path || !image || width <= 0 || height <= 0) return fail(ASC_ERR_INVALID, "bad argument");
+    const size_t n = (size_t)width * height;
+    float lo = std::numeric_limits<float>::max();
+    float hi = std::numeric_limits<float>::min();
+    for (size_t i = 0; i < n; i++) {
+      if (image[i] > hi) hi = image[i];
+      if (image[i] < lo) lo = image[i];
+    }
+    const float range = hi - lo;
+    std::string text = "P2\n" + std::to_string(width) + " " + std::to_string(height) + "\n255\n";
+    text.reserve(text.size() + 4 * n);
+    for (size_t i = 0; i < n; i++) {
+      unsigned char u = 0;
+      if (range > 0.0f) u = (unsigned char)(int)(255.0f * (image[i] - lo) / range);
+      text += std::to_string(255 - (int)u);
+      text += ' ';
+    }
+    std::FILE* f = std::fopen(path, "w");
+    if (!f) return io_fail(std::string("cannot write ") + path);
+    const bool ok = std::fwrite(text.data(), 1, text.size(), f) == text.size();
+    return (std::fclose(f) == 0 && ok) ? ASC_OK : io_fail(std::string("write failed: ") + path);
+  });
 }

@@ -586,7 +586,7 @@ bool file_exists(const std::string& p)
 }
 
 // Answers one message; returns false when the connection is to be closed.
-bool handle(Session& S, const uint8_t* msg, size_t len, std::vector<uint8_t>& out)
+bool handle_message(Session& S, const uint8_t* msg, size_t len, std::vector<uint8_t>& out)
 {
   out.clear();
   if (len == 0) return true;
@@ -817,6 +817,19 @@ bool handle(Session& S, const uint8_t* msg, size_t len, std::vector<uint8_t>& ou
   return true;
 }
 
+// One message, whatever happens inside: a failed allocation or a library
+// exception answers the failure byte and keeps the connection.
+bool handle(Session& S, const uint8_t* msg, size_t len, std::vector<uint8_t>& out)
+{
+  try {
+    return handle_message(S, msg, len, out);
+  } catch (const std::exception& e) {
+    fail(ASC_ERR_IO, std::string("message failed: ") + e.what());
+    out.assign(1, 0);
+    return true;
+  }
+}
+
 }  // namespace
 }  // namespace asc
 
@@ -930,7 +943,7 @@ ASC_API int asc_remote_handle(asc_remote r, const void* message, size_t length, 
     keep = handle(r->session, (const uint8_t*)message, length, out);
   }
   *reply_length = out.size();
-  if (reply && capacity) std::memcpy(reply, out.data(), std::min(capacity, out.size()));
+  if (reply && capacity && !out.empty()) std::memcpy(reply, out.data(), std::min(capacity, out.size()));
   return keep ? ASC_OK : 1;
 }
 
