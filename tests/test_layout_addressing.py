@@ -40,7 +40,8 @@ def packed_array(vol, layout, tex):
     if layout == "brick":
         BX, BY, BZ = (W + 3) // 2, (H + 3) // 2, (D + 3) // 2
         for bz, by, bx, c in itertools.product(range(BZ), range(BY), range(BX), range(8)):
-            ci, cj, ck = 2 * bx + (c & 1) - 2, 2 * by + ((c >> 1) & 1) - 2, 2 * bz + (c >> 2) - 2
+            # cell c = (j&1) + 2(k&1) + 4(i&1) of the brick: x is the high bit
+            ci, cj, ck = 2 * bx + (c >> 2) - 2, 2 * by + (c & 1) - 2, 2 * bz + ((c >> 1) & 1) - 2
             e = ((bz * BY + by) * BX + bx) * 8 + c
             mem[e * esz] = tuple(voxel(vol, ci + (q & 1), cj + ((q >> 1) & 1), ck + (q >> 2)) for q in range(8))
     else:
@@ -62,8 +63,8 @@ def offset(i, j, k, layout, esz, pW, pS):
     if layout == "brick":
         c0 = sy + sz + 8 * esz
         E = esz
-        base = i * E + j * 2 * E + k * 4 * E + c0
-        return mad_i24(i >> 1, 6 * E, mad_i24(j >> 1, sy - 4 * E, mad_i24(k >> 1, sz - 8 * E, base)))
+        base = i * 4 * E + j * E + k * 2 * E + c0
+        return mad_i24(j >> 1, sy - 2 * E, mad_i24(k >> 1, sz - 4 * E, base))
     c0 = sy + sz + esz
     return mad_i24(k, sz, mad_i24(j, sy, i * esz + c0))
 
