@@ -1,0 +1,109 @@
+"""The benchmark's own workload at full size: a 256x256x320 u16 long-bone
+phantom at 0.3 mm seen by two 1024x1024 cameras, batches of 100 poses in
+the PSO start range around the hidden pose (the shape bench.py times).
+
+  * against the oracle's trackFrame on a few poses (the oracle runs this
+    size in well under a second per pose): 1 - NCC within 1e-5 per camera,
+    in-view flags, render sizes, mask counts and ray-sample counts exact;
+  * size-independent properties over a whole batch: a pose's cost is the
+    same bits alone and in the batch, hipGraph replays equal the eager
+    batch, every volume layout gives the same full-frame DRR bits, and
+    doubling the intensity exactly halves every DRR pixel below the clamp.
+"""
+import numpy as np
+import pytest
+
+from autoscoper_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cfg2(asc, ctx, orc):
+    wl = synth.workload("cfg2", frames=1)
+    RW, RH = wl.render
+    prep = orc.volume_prep(wl.volume, (wl.voxel_mm,) * 3)
+    cams = [orc.camera_from_mayacam2(*c.mayacam2()) for c in wl.cameras]
+    gt = wl.gt_poses[0]
+    rads = []
+    for i, cam in enumerate(cams):
+        _, imv, _, _ = orc.view_geometry(cam, gt, prep["inv_scale"], prep["inv_trans"], RW, RH)
+        fvp = synth.full_frame_viewport(cam[24:28], RW, RH)
+        img, _ = orc.drr(prep["volume"], imv[:12], fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, RW, RH)
+        rads.append(synth.target_from_drr(img, 500 + i)[1])
+    vol = asc.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3)
+    sc = asc.Scene(ctx, RW, RH)
+    pivot = asc.geometry.frame_from_xyzypr(list(vol.desc.trans_center) + [0, 0, 0])
+    sc.add_volume(vol, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, pivot)
+    for cam, rad in zip(cams, rads):
+        c = sc.add_camera(asc.AscCamera.from_array(cam), asc.Image(ctx, rad))
+        sc.set_filters(c, 0, wl.drr_filters)
+        sc.set_filters(c, 1, wl.rad_filters)
+    rng = np.random.default_rng(synth.SEED_START)
+    batch = gt + np.concatenate([rng.uniform(-3, 3, (100, 3)), rng.uniform(-3, 3, (100, 3))], 1)
+    yield dict(wl=wl, prep=prep, cams=cams, rads=rads, vol=vol, scene=sc, batch=batch, render=(RW, RH))
+    sc.close()
+    vol.close()
+
+
+def test_batch_matches_oracle(asc, orc, cfg2):
+    wl, sc, batch = cfg2["wl"], cfg2["scene"], cfg2["batch"]
+    costs, per = sc.eval_poses(batch)
+    info, _ = sc.last_info(len(batch) * 2)
+    recs = [(asc.filters.to_records(wl.drr_filters), asc.filters.to_records(wl.rad_filters))] * 2
+    S = orc.Scene(cfg2["prep"], (synth.DRR_STEP, synth.DRR_INTENSITY, 0.0), cfg2["cams"], cfg2["rads"], recs,
+                  cfg2["render"])
+    samples = 0
+    for i in (0, 17, 58, 99):
+        r = S.track_frame(batch[i])
+        for c in range(2):
+            assert info[2 * i + c, 0] == r["info"][c, 0]
+            assert tuple(info[2 * i + c, 1:4]) == tuple(r["info"][c, 1:4])
+            assert abs(per[i, c] - r["corr"][c]) <= 1e-5, (i, c, per[i, c], r["corr"][c])
+        samples += r["samples"]
+    # ray samples of those four poses, from a batch of just them
+    sc.eval_poses(batch[[0, 17, 58, 99]])
+    assert sc.last_info(8)[1] == samples
+    print(f"cfg2: 4 poses x 2 cameras against the oracle, {samples} ray samples")
+
+
+def test_alone_equals_in_batch_and_graph_replays(cfg2):
+    sc, batch = cfg2["scene"], cfg2["batch"]
+    costs, per = sc.eval_poses(batch)
+    for i in (3, 50, 97):
+        c1, p1 = sc.eval_poses(batch[i:i + 1])
+        assert np.array_equal(p1[0], per[i]) and c1[0] == costs[i]
+    sc.use_graph(True)
+    try:
+        for _ in range(3):
+            cg, pg = sc.eval_poses(batch)
+            assert np.array_equal(pg, per) and np.array_equal(cg, costs)
+            sc.eval_poses(batch[:7])  # other eager-sized work between replays
+    finally:
+        sc.use_graph(False)
+
+
+def test_layouts_and_intensity_scaling(asc, ctx, orc, cfg2):
+    wl, prep, vol = cfg2["wl"], cfg2["prep"], cfg2["vol"]
+    RW, RH = cfg2["render"]
+    cam = cfg2["cams"][1]
+    pose = cfg2["batch"][11]
+    _, imv, _, _ = orc.view_geometry(cam, pose, prep["inv_scale"], prep["inv_trans"], RW, RH)
+    fvp = synth.full_frame_viewport(cam[24:28], RW, RH)
+    out = ctx.alloc(4 * RW * RH)
+
+    def render(v, intensity):
+        ctx.drr_render(v, imv[:12], fvp, synth.DRR_STEP, intensity, 0.0, out, RW, RH)
+        return out.download((RH, RW))
+
+    base = render(vol, synth.DRR_INTENSITY)
+    assert (base > 0).mean() > 0.1
+    other = asc.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3)
+    for layout in (other.LINEAR, other.QUAD, other.OCTET):
+        other.set_layout(layout)
+        assert np.array_equal(render(other, synth.DRR_INTENSITY), base), layout
+    other.close()
+    half = render(vol, 2 * synth.DRR_INTENSITY)
+    below = base < 1.0
+    assert np.array_equal(half[below], base[below] / np.float32(2))
+    out.free()
