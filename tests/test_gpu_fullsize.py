@@ -107,3 +107,93 @@ def test_layouts_and_intensity_scaling(asc, ctx, orc, cfg2):
     below = base < 1.0
     assert np.array_equal(half[below], base[below] / np.float32(2))
     out.free()
+
+
+def test_cfg3_bones_fp32_match_oracle(asc, ctx, orc):
+    """The multi-bone shape: three bones (96^3 @0.5 mm, 160x160x220 @0.4 mm,
+    256x256x320 @0.3 mm) stored as FP32 volumes, two 1024^2 cameras, targets
+    from all bones merged; one fused batch mixes poses of every bone, and each
+    is checked against the oracle's trackFrame for its bone."""
+    wl = synth.multi_workload("cfg3", frames=1)
+    RW, RH = wl.render
+    cams = [orc.camera_from_mayacam2(*c.mayacam2()) for c in wl.cameras]
+    preps = [orc.volume_prep(v, (vx,) * 3) for v, vx in zip(wl.volumes, wl.voxel_mm)]
+    gts = wl.gt_poses[0]
+    rads = []
+    for i, cam in enumerate(cams):
+        fvp = synth.full_frame_viewport(cam[24:28], RW, RH)
+        acc = np.zeros((RH, RW), np.float32)
+        for b, prep in enumerate(preps):
+            _, imv, _, _ = orc.view_geometry(cam, gts[b], prep["inv_scale"], prep["inv_trans"], RW, RH)
+            img, _ = orc.drr(prep["volume"], imv[:12], fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, RW, RH)
+            acc = orc.merge(acc, img)
+        rads.append(synth.target_from_drr(acc, 500 + i)[1])
+    vols = [asc.Volume(ctx, v, scale=(vx,) * 3, storage=asc.STORAGE_F32) for v, vx in zip(wl.volumes, wl.voxel_mm)]
+    sc = asc.Scene(ctx, RW, RH)
+    for vol in vols:
+        sc.add_volume(vol, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0,
+                      asc.geometry.frame_from_xyzypr(list(vol.desc.trans_center) + [0, 0, 0]))
+    for cam, rad in zip(cams, rads):
+        c = sc.add_camera(asc.AscCamera.from_array(cam), asc.Image(ctx, rad))
+        sc.set_filters(c, 0, wl.drr_filters)
+        sc.set_filters(c, 1, wl.rad_filters)
+    rng = np.random.default_rng(41)
+    bones = np.repeat(np.arange(3), 2)
+    poses = gts[bones] + np.concatenate([rng.uniform(-2, 2, (6, 3)), rng.uniform(-3, 3, (6, 3))], 1)
+    costs, per = sc.eval_poses(poses, volumes=bones)
+    info, samples = sc.last_info(len(poses) * 2)
+    recs = [(asc.filters.to_records(wl.drr_filters), asc.filters.to_records(wl.rad_filters))] * 2
+    total = 0
+    for b in range(3):
+        S = orc.Scene(preps[b], (synth.DRR_STEP, synth.DRR_INTENSITY, 0.0), cams, rads, recs, (RW, RH))
+        for i in np.flatnonzero(bones == b):
+            r = S.track_frame(poses[i])
+            for c in range(2):
+                assert info[2 * i + c, 0] == r["info"][c, 0]
+                assert tuple(info[2 * i + c, 1:4]) == tuple(r["info"][c, 1:4]), (b, i, c)
+                assert abs(per[i, c] - r["corr"][c]) <= 1e-5, (b, i, c, per[i, c], r["corr"][c])
+            total += r["samples"]
+    assert samples == total
+    sc.close()
+    for v in vols:
+        v.close()
+
+
+def test_drr_parity_and_f16_error_at_full_size(asc, ctx, orc, cfg2):
+    """Full-frame DRRs of both cfg2 cameras, pixels in (1e-3, 1):
+      * the exact u16 path against the float-faithful oracle: within 1e-4
+        relative, zero pattern exact (the parity bar);
+      * an F16 copy against the u16 path: within the half-precision storage
+        roundoff 2^-11 plus 1e-4 for its entry-point positions (the DRR is a
+        non-negative weighted sum of stored texels);
+      * both against the oracle's all-FP64 DRR: rms within 1e-4 (the max is
+        printed: silhouette pixels, where FP64 and FP32 ray set-up can differ
+        by a sample, dominate it)."""
+    wl, prep = cfg2["wl"], cfg2["prep"]
+    RW, RH = cfg2["render"]
+    f16 = asc.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3, storage=asc.STORAGE_F16)
+    out = ctx.alloc(4 * RW * RH)
+
+    def render(v):
+        ctx.drr_render(v, imv[:12], fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, out, RW, RH)
+        return out.download((RH, RW)).astype(np.float64)
+
+    for cam in cfg2["cams"]:
+        _, imv, _, _ = orc.view_geometry(cam, cfg2["batch"][5], prep["inv_scale"], prep["inv_trans"], RW, RH)
+        fvp = synth.full_frame_viewport(cam[24:28], RW, RH)
+        ref32, _ = orc.drr(prep["volume"], imv[:12], fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, RW, RH)
+        ref64, _ = orc.drr(prep["volume"], imv[:12], fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, RW, RH, fp64=True)
+        u16, h16 = render(cfg2["vol"]), render(f16)
+        assert ((u16 == 0) == (ref32 == 0)).all()
+        nz = ref32 != 0
+        assert (np.abs(u16[nz] - ref32[nz]) / ref32[nz]).max() <= 1e-4
+        sel = (u16 > 1e-3) & (u16 < 1.0)
+        assert sel.mean() > 0.05
+        assert (np.abs(h16[sel] - u16[sel]) / u16[sel]).max() <= 2.0 ** -11 + 1e-4
+        for name, g in (("u16", u16), ("F16", h16)):
+            rel = np.abs(g[sel] - ref64[sel]) / ref64[sel]
+            rms = np.sqrt((rel ** 2).mean())
+            print(f"full-size DRR {name} vs FP64: max {rel.max():.2e}, rms {rms:.2e}")
+            assert rms <= 1e-4
+    out.free()
+    f16.close()
