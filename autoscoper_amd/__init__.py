@@ -150,6 +150,7 @@ _SIGS = {
     "asc_context_set_blocking": [_vp, C.c_int],
     "asc_context_set_ncc_form": [_vp, C.c_int],
     "asc_context_stream": [_vp],
+    "asc_context_device": [_vp, _P(_i)],
     "asc_synchronize": [_vp],
     "asc_malloc": [_vp, _sz, _P(_vp)],
     "asc_free": [_vp, _vp],

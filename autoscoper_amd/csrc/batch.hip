@@ -780,6 +780,8 @@ struct Request {
 // per_camera: n x stride values (camera k of request i at i*stride + k).
 int eval_requests(asc_scene s, const std::vector<Request>& reqs, double* costs, double* per_camera, int stride)
 {
+  // the calling thread may not have selected the scene's device yet
+  ASC_HIP(hipSetDevice(s->ctx->device));
   int rc = scene_upload_config(s);
   if (rc) return rc;
   const int n = (int)reqs.size();

@@ -237,6 +237,13 @@ ASC_API int asc_context_set_stream(asc_context ctx, void* s)
 
 ASC_API void* asc_context_stream(asc_context ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
+ASC_API int asc_context_device(asc_context ctx, int* device)
+{
+  ASC_CHECK(valid_ctx(ctx) && device, ASC_ERR_INVALID, "bad argument");
+  *device = ctx->device;
+  return ASC_OK;
+}
+
 ASC_API int asc_context_set_ncc_form(asc_context ctx, int form)
 {
   ASC_CHECK(valid_ctx(ctx), ASC_ERR_INVALID, "null context");

@@ -7,12 +7,15 @@
 // with guess 1 = previous frame and guess 2 = linear extrapolation,
 // :380-427) in its own host thread; every cost evaluation the optimiser
 // asks for becomes a request to the batcher. The batcher thread is the only
-// one that touches the GPU: it admits queued jobs when their device memory
-// fits the budget, uploads their volumes and frames into one shared scene,
-// and evaluates the pending requests of all running bones in one fused
-// launch (asc_scene_eval_requests) per tick. A request's cost does not
-// depend on what else is in the launch, so every job's trajectory is the
-// same alone and in the mix.
+// one that evaluates: it registers staged jobs in one shared scene and
+// evaluates the pending requests of all running bones in one fused launch
+// (asc_scene_eval_requests) per tick. A stager thread, with a HIP stream of
+// its own, admits queued jobs when their device memory fits the budget,
+// makes their device volumes and radiographs, and frees the device copies
+// of finished jobs, so that neither the host-side volume packing nor the
+// copies and frees stall the evaluations. A request's cost does not depend
+// on what else is in the launch, so every job's trajectory is the same
+// alone and in the mix.
 //
 // Per-frame latency of a job: from the moment the frame could start (job
 // arrival for the first frame, completion of the previous frame by all
@@ -91,25 +94,31 @@ struct Job {
   int bones_left = 0;
   int64_t evaluations = 0;
   int status = ASC_OK;
-  bool admitted = false, finished = false, released = false;
+  // admitted: taken from the queue by the stager; registered: in the scene
+  // (or failed) with bones_left set; released: out of the scene, queued for
+  // the stager to free; finished: device copies freed, results final
+  bool admitted = false, registered = false, finished = false, released = false;
 };
 
 }  // namespace
 
 struct asc_server_s {
   asc_context ctx;
+  asc_context up_ctx = nullptr;  // the stager's context (its own stream)
   asc_scene scene = nullptr;
   size_t budget = 0, used = 0;
   int max_active = 1, active = 0;
   std::mutex mu;
-  std::condition_variable cv_batcher, cv_req, cv_done;
+  std::condition_variable cv_batcher, cv_req, cv_done, cv_stager;
   std::deque<Job*> waiting;                  // submitted, not admitted
+  std::deque<Job*> staged;                   // device copies made, not registered
+  std::deque<Job*> to_free;                  // out of the scene, device copies to free
   std::map<int, std::unique_ptr<Job>> jobs;  // all not yet collected by asc_server_wait
   std::vector<Request*> pending;
   int running_bones = 0;
   int next_id = 1;
-  bool stop = false;
-  std::thread batcher;
+  bool stop = false, stop_stager = false;
+  std::thread batcher, stager;
   Clock::time_point t0;
   // statistics
   int64_t batches = 0, requests = 0, evaluations = 0, frames_done = 0, jobs_done = 0;
@@ -197,8 +206,10 @@ void bone_main(asc_server s, Job* j, int b)
   s->cv_batcher.notify_one();
 }
 
-// Device upload of an admitted job into the shared scene (batcher thread).
-int upload_job(asc_server s, Job& j)
+// Device volumes and radiographs of an admitted job (stager thread, on the
+// server's upload context). Each create call has finished its copies and
+// layout kernels when it returns, so the batcher's stream may read them.
+int upload_job(asc_context up, Job& j)
 {
   int rc;
   for (int b = 0; b < j.n_bones; b++) {
@@ -206,7 +217,7 @@ int upload_job(asc_server s, Job& j)
     asc_volume_desc d;
     const int flip[3] = {0, 0, 0};
     const std::array<int, 3>& dm = j.vol_dims[b];
-    if ((rc = asc_volume_create(s->ctx, j.vol_data[b].data(), dm[0], dm[1], dm[2], 16, flip, j.vol_scale[b].data(),
+    if ((rc = asc_volume_create(up, j.vol_data[b].data(), dm[0], dm[1], dm[2], 16, flip, j.vol_scale[b].data(),
                                 j.storage, &v, &d)))
       return rc;
     j.dvols.push_back(v);
@@ -215,8 +226,27 @@ int upload_job(asc_server s, Job& j)
     std::array<double, 12> pv;
     asc_frame_from_xyzypr(ypr0, pv.data());
     j.pivots.push_back(pv);
+  }
+  const int C = (int)j.cams.size();
+  for (int f = 0; f < j.n_frames; f++)
+    for (int c = 0; c < C; c++) {
+      asc_image im = nullptr;
+      if ((rc = asc_image_create(up, j.frames[(size_t)f * C + c].data(), j.frame_dims[c][0], j.frame_dims[c][1],
+                                 j.bps, &im)))
+        return rc;
+      j.images.push_back(im);
+    }
+  return ASC_OK;
+}
+
+// Scene entries of a staged job (batcher thread).
+int register_job(asc_server s, Job& j)
+{
+  int rc;
+  for (int b = 0; b < j.n_bones; b++) {
     int idx;
-    if ((rc = asc_scene_add_volume(s->scene, v, j.step, j.intensity, j.cutoff, pv.data(), &idx))) return rc;
+    if ((rc = asc_scene_add_volume(s->scene, j.dvols[b], j.step, j.intensity, j.cutoff, j.pivots[b].data(), &idx)))
+      return rc;
     j.svols.push_back(idx);
   }
   const int C = (int)j.cams.size();
@@ -229,50 +259,90 @@ int upload_job(asc_server s, Job& j)
   }
   if ((rc = asc_scene_add_camera_set(s->scene, j.scams.data(), C, j.RW, j.RH, &j.set))) return rc;
   for (int f = 0; f < j.n_frames; f++) {
-    std::vector<asc_image> imgs(C);
-    for (int c = 0; c < C; c++) {
-      if ((rc = asc_image_create(s->ctx, j.frames[(size_t)f * C + c].data(), j.frame_dims[c][0], j.frame_dims[c][1],
-                                 j.bps, &imgs[c])))
-        return rc;
-      j.images.push_back(imgs[c]);
-    }
     int fid;
-    if ((rc = asc_scene_add_frame(s->scene, j.set, imgs.data(), &fid))) return rc;
+    if ((rc = asc_scene_add_frame(s->scene, j.set, &j.images[(size_t)f * C], &fid))) return rc;
     j.sframes.push_back(fid);
   }
   return ASC_OK;
 }
 
-void release_job(asc_server s, Job& j)
+// Scene entries out (batcher thread). No later launch names the job's
+// device copies, so the stager may free them while the batcher evaluates.
+void unregister_job(asc_server s, Job& j)
 {
   for (int f : j.sframes) asc_scene_remove_frame(s->scene, f);
   if (j.set >= 0) asc_scene_remove_camera_set(s->scene, j.set);
   for (int c : j.scams) asc_scene_remove_camera(s->scene, c);
   for (int v : j.svols) asc_scene_remove_volume(s->scene, v);
-  for (asc_image im : j.images) asc_image_destroy(im);
-  for (asc_volume v : j.dvols) asc_volume_destroy(v);
   j.sframes.clear();
   j.scams.clear();
   j.svols.clear();
-  j.images.clear();
-  j.dvols.clear();
   j.set = -1;
-  j.released = true;
 }
 
-// Called with the lock held: admit waiting jobs that fit.
-void admit(asc_server s, std::unique_lock<std::mutex>& lk)
+// Device copies out (stager thread).
+void free_job(Job& j)
 {
-  while (!s->waiting.empty() && s->active < s->max_active && s->used + s->waiting.front()->device_bytes <= s->budget) {
-    Job* j = s->waiting.front();
-    s->waiting.pop_front();
-    s->active++;
-    s->used += j->device_bytes;
-    j->admit_ms = ms_since(j->arrival, Clock::now());
-    lk.unlock();
-    const int rc = upload_job(s, *j);
-    lk.lock();
-    j->admitted = true;
+  for (asc_image im : j.images) asc_image_destroy(im);
+  for (asc_volume v : j.dvols) asc_volume_destroy(v);
+  j.images.clear();
+  j.dvols.clear();
+}
+
+// Stager thread: frees finished jobs first (their memory may admit the
+// next one), then admits the queue's head when it fits the budget.
+void stager_main(asc_server s)
+{
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto fits = [&] {
+    return !s->waiting.empty() && s->active < s->max_active &&
+           s->used + s->waiting.front()->device_bytes <= s->budget;
+  };
+  for (;;) {
+    s->cv_stager.wait(lk, [&] { return s->stop_stager || !s->to_free.empty() || fits(); });
+    if (!s->to_free.empty()) {
+      Job* j = s->to_free.front();
+      s->to_free.pop_front();
+      lk.unlock();
+      free_job(*j);
+      lk.lock();
+      s->active--;
+      s->used -= j->device_bytes;
+      j->finished = true;
+      s->cv_done.notify_all();
+    } else if (fits()) {
+      Job* j = s->waiting.front();
+      s->waiting.pop_front();
+      s->active++;
+      s->used += j->device_bytes;
+      j->admitted = true;
+      j->admit_ms = ms_since(j->arrival, Clock::now());
+      lk.unlock();
+      const int rc = upload_job(s->up_ctx, *j);
+      lk.lock();
+      if (rc != ASC_OK) j->status = rc;
+      s->staged.push_back(j);
+      s->cv_batcher.notify_one();
+    } else {
+      break;  // stopping, nothing left to free
+    }
+  }
+}
+
+// Called with the lock held: scene entries and bone threads for the staged
+// jobs.
+void register_staged(asc_server s, std::unique_lock<std::mutex>& lk)
+{
+  while (!s->staged.empty()) {
+    Job* j = s->staged.front();
+    s->staged.pop_front();
+    int rc = j->status;
+    if (rc == ASC_OK) {
+      lk.unlock();
+      rc = register_job(s, *j);
+      lk.lock();
+    }
+    j->registered = true;
     if (rc != ASC_OK) {
       j->status = rc;
       j->bones_left = 0;
@@ -284,18 +354,20 @@ void admit(asc_server s, std::unique_lock<std::mutex>& lk)
   }
 }
 
-// Called with the lock held: finish jobs whose bones are all done.
+// Called with the lock held: take jobs whose bones are all done out of the
+// scene and hand their device copies to the stager.
 void reap(asc_server s, std::unique_lock<std::mutex>& lk)
 {
   for (auto& kv : s->jobs) {
     Job& j = *kv.second;
-    if (!j.admitted || j.finished || j.bones_left > 0) continue;
+    if (!j.registered || j.released || j.bones_left > 0) continue;
     std::vector<std::thread> th;
     th.swap(j.threads);
     lk.unlock();
     for (std::thread& t : th) t.join();
-    release_job(s, j);
+    unregister_job(s, j);
     lk.lock();
+    j.released = true;
     // frame latency: completion by all bones, from when the frame could start
     double prev = 0.0;
     for (int f = 0; f < j.n_frames; f++) {
@@ -307,10 +379,8 @@ void reap(asc_server s, std::unique_lock<std::mutex>& lk)
     }
     s->frames_done += j.status == ASC_OK ? j.n_frames : 0;
     s->jobs_done++;
-    s->active--;
-    s->used -= j.device_bytes;
-    j.finished = true;
-    s->cv_done.notify_all();
+    s->to_free.push_back(&j);
+    s->cv_stager.notify_one();
   }
 }
 
@@ -319,15 +389,13 @@ void batcher_main(asc_server s)
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
     reap(s, lk);
-    admit(s, lk);
+    register_staged(s, lk);
     // a tick: wait until every running bone has a request pending (they do
     // between two optimiser steps only host work), or briefly for stragglers
     auto ready = [&] {
       return s->stop || (!s->pending.empty() && (int)s->pending.size() >= s->running_bones) ||
-             (!s->waiting.empty() && s->active < s->max_active &&
-              s->used + s->waiting.front()->device_bytes <= s->budget) ||
-             std::any_of(s->jobs.begin(), s->jobs.end(), [](const auto& kv) {
-               return kv.second->admitted && !kv.second->finished && kv.second->bones_left == 0;
+             !s->staged.empty() || std::any_of(s->jobs.begin(), s->jobs.end(), [](const auto& kv) {
+               return kv.second->registered && !kv.second->released && kv.second->bones_left == 0;
              });
     };
     if (s->pending.empty())
@@ -391,12 +459,17 @@ ASC_API int asc_server_create(asc_context ctx, size_t hbm_budget, int max_active
   s->budget = hbm_budget;
   s->max_active = max_active_jobs;
   s->t0 = Clock::now();
-  int rc = asc_scene_create(ctx, 1, 1, &s->scene);
+  int device = 0;
+  int rc = asc_context_device(ctx, &device);
+  if (rc == ASC_OK) rc = asc_context_create(device, &s->up_ctx);
+  if (rc == ASC_OK) rc = asc_scene_create(ctx, 1, 1, &s->scene);
   if (rc) {
+    if (s->up_ctx) asc_context_destroy(s->up_ctx);
     delete s;
     return rc;
   }
   s->batcher = std::thread(batcher_main, s);
+  s->stager = std::thread(stager_main, s);
   *out = s;
   return ASC_OK;
 }
@@ -467,7 +540,7 @@ ASC_API int asc_server_submit(asc_server s, const asc_job_desc* d, int* job_id)
   *job_id = j->id;
   s->waiting.push_back(j.get());
   s->jobs[j->id] = std::move(j);
-  s->cv_batcher.notify_one();
+  s->cv_stager.notify_one();
   return ASC_OK;
 }
 
@@ -538,10 +611,14 @@ ASC_API int asc_server_destroy(asc_server s)
       return true;
     });
     s->stop = true;
+    s->stop_stager = true;
     s->cv_batcher.notify_one();
+    s->cv_stager.notify_one();
   }
   s->batcher.join();
+  s->stager.join();
   asc_scene_destroy(s->scene);
+  asc_context_destroy(s->up_ctx);
   delete s;
   return ASC_OK;
 }

@@ -99,6 +99,8 @@ ASC_API int asc_context_destroy(asc_context ctx);
 /* Install a caller-owned hipStream_t (NULL restores the context's own). */
 ASC_API int asc_context_set_stream(asc_context ctx, void* hip_stream);
 ASC_API void* asc_context_stream(asc_context ctx);
+/* The HIP device index the context was created on. */
+ASC_API int asc_context_device(asc_context ctx, int* device);
 /* Blocking mode: each per-operator call (asc_drr_render ... asc_composite)
  * waits for its launch to finish before it returns, as the reference's
  * OpenCL backend does after every kernel (Kernel::launch, clFinish,
