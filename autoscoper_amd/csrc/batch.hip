@@ -763,9 +763,9 @@ int issue_batch(asc_scene s, size_t batch_bytes, int grid_drr, int grid_post)
   finalize_items<<<std::min(n_items, 4096), kBlock, 0, st>>>(a);
   if (timed) ASC_HIP(hipEventRecord(s->ev[3], st));
   ASC_HIP(hipGetLastError());
+  // (the per-item sample counts stay on the device until asc_scene_last_info
+  // asks for them: one copy less per batch)
   ASC_HIP(hipMemcpyAsync(s->h_results.p, s->d_results.p, sizeof(double) * 10 * n_items, hipMemcpyDeviceToHost, st));
-  ASC_HIP(hipMemcpyAsync(s->h_samples.p, s->d_samples.p, sizeof(unsigned long long) * n_items,
-                         hipMemcpyDeviceToHost, st));
   return ASC_OK;
 }
 
@@ -1364,6 +1364,16 @@ ASC_API int asc_scene_last_timing(asc_scene s, float ms[3])
 ASC_API int asc_scene_last_info(asc_scene s, int* info, int capacity, int64_t* samples)
 {
   ASC_CHECK(s, ASC_ERR_INVALID, "null scene");
+  // the last batch's counts, still in device memory (the batch has been
+  // waited for; the next one overwrites them). A batch that failed before
+  // its buffers were sized leaves no counts to read.
+  ASC_CHECK((size_t)s->n_items <= s->d_samples.cap && (size_t)s->n_items <= s->h_samples.cap, ASC_ERR_INVALID,
+            "no batch results to report");
+  if (s->n_items > 0) {
+    ASC_HIP(hipStreamSynchronize(s->ctx->stream));
+    ASC_HIP(hipMemcpy(s->h_samples.p, s->d_samples.p, sizeof(unsigned long long) * s->n_items,
+                      hipMemcpyDeviceToHost));
+  }
   int64_t total = 0;
   for (int ii = 0; ii < s->n_items; ii++) {
     total += (int64_t)s->h_samples.p[s->korder[ii]];
