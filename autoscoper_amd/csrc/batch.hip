@@ -62,7 +62,7 @@ struct CamDev {
 
 struct VolParams {
   VolDev v;
-  float step, intensity, cutoff_raw, step_vscale;
+  float step, intensity, cutoff, step_vscale;
 };
 
 struct ItemDesc {
@@ -180,7 +180,7 @@ __device__ __forceinline__ float march_item(const VolParams& vp, const ItemDesc&
 {
   dev::Ray r;
   if (!dev::ray_setup(it.imv, it.vp, vp.step, x, y, (unsigned)it.w, (unsigned)it.h, r)) return 0.0f;
-  return dev::drr_finish(dev::march<UNROLL>(vp.v, r, vp.cutoff_raw, nsamp), vp.step_vscale, vp.intensity);
+  return dev::drr_finish(dev::march<UNROLL>(vp.v, r, vp.cutoff, nsamp), vp.step_vscale, vp.intensity);
 }
 
 template <int UNROLL>
@@ -694,7 +694,7 @@ int scene_upload_config(asc_scene s)
     vols[i].v = sv.vol->dev;
     vols[i].step = sv.step;
     vols[i].intensity = sv.intensity;
-    vols[i].cutoff_raw = sv.cutoff / sv.vol->dev.vscale;
+    vols[i].cutoff = sv.cutoff;
     vols[i].step_vscale = sv.step * sv.vol->dev.vscale;
   }
   if (pool.empty()) pool.push_back(0.0f);

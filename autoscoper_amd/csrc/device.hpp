@@ -310,7 +310,7 @@ __device__ __forceinline__ int sample_count(const Ray& r)
 
 template <int ST, int LY, bool CUT>
 __device__ __forceinline__ void accumulate(const VolDev& v, const VolRsrc& rs, const Ray& r, float t, float W,
-                                           float H, float D, float cutoff_raw, float& acc)
+                                           float H, float D, float cutoff, float vscale, float& acc)
 {
   const float px = fmaf(t, r.dx, r.ox);
   const float py = fmaf(t, r.dy, r.oy);
@@ -320,8 +320,10 @@ __device__ __forceinline__ void accumulate(const VolDev& v, const VolRsrc& rs, c
   const float Y = fmaf(1.0f - py, H, -0.5f);
   const float Z = fmaf(-pz, D, -0.5f);
   const float val = trilinear<ST, LY>(v, rs, X, Y, Z);
+  // the reference's `val > cutoff` on the normalised value (raw * vscale),
+  // while the sum stays raw: step * vscale is applied once per ray
   if constexpr (CUT)
-    acc += val > cutoff_raw ? val : 0.0f;
+    acc += val * vscale > cutoff ? val : 0.0f;
   else
     acc += val;
 }
@@ -358,9 +360,10 @@ __device__ __forceinline__ void accumulate_line(const VolDev& v, const VolRsrc& 
 // so a wave keeps 4*UNROLL loads in flight; the sum keeps the reference's
 // sample order.
 template <int ST, int LY, bool CUT, int UNROLL>
-__device__ __forceinline__ float march_loop(const VolDev& v, const Ray& r, float cutoff_raw, int& nsamples)
+__device__ __forceinline__ float march_loop(const VolDev& v, const Ray& r, float cutoff, int& nsamples)
 {
   const VolRsrc rs = vol_rsrc(v);
+  const float vscale = v.vscale;
   const float W = (float)v.W, H = (float)v.H, D = (float)v.D;
   const int n = sample_count(r);
   float acc = 0.0f;
@@ -369,11 +372,11 @@ __device__ __forceinline__ float march_loop(const VolDev& v, const Ray& r, float
     float t = r.t_far;
     for (; k + UNROLL <= n; k += UNROLL) {
 #pragma unroll
-      for (int u = 0; u < UNROLL; u++) accumulate<ST, LY, CUT>(v, rs, r, t - (float)u, W, H, D, cutoff_raw, acc);
+      for (int u = 0; u < UNROLL; u++) accumulate<ST, LY, CUT>(v, rs, r, t - (float)u, W, H, D, cutoff, vscale, acc);
       t -= (float)UNROLL;
     }
     for (; k < n; k++) {
-      accumulate<ST, LY, CUT>(v, rs, r, t, W, H, D, cutoff_raw, acc);
+      accumulate<ST, LY, CUT>(v, rs, r, t, W, H, D, cutoff, vscale, acc);
       t -= 1.0f;
     }
   } else {
@@ -393,31 +396,32 @@ __device__ __forceinline__ float march_loop(const VolDev& v, const Ray& r, float
   return acc;
 }
 
+// cutoff is on normalised values, as the reference compares them
 template <int ST, int LY, int UNROLL>
-__device__ __forceinline__ float march_st(const VolDev& v, const Ray& r, float cutoff_raw, int& nsamples)
+__device__ __forceinline__ float march_st(const VolDev& v, const Ray& r, float cutoff, int& nsamples)
 {
-  if (cutoff_raw > 0.0f) return march_loop<ST, LY, true, UNROLL>(v, r, cutoff_raw, nsamples);
-  return march_loop<ST, LY, false, UNROLL>(v, r, cutoff_raw, nsamples);
+  if (cutoff > 0.0f) return march_loop<ST, LY, true, UNROLL>(v, r, cutoff, nsamples);
+  return march_loop<ST, LY, false, UNROLL>(v, r, cutoff, nsamples);
 }
 
 // Dispatch on the volume's storage and layout (uniform per wave).
 template <int UNROLL>
-__device__ __forceinline__ float march(const VolDev& v, const Ray& r, float cutoff_raw, int& nsamples)
+__device__ __forceinline__ float march(const VolDev& v, const Ray& r, float cutoff, int& nsamples)
 {
 #define ASC_MARCH_ST(ST)                                                                  \
   switch (v.layout) {                                                                     \
-    case ASC_LAYOUT_QUAD: return march_st<ST, ASC_LAYOUT_QUAD, UNROLL>(v, r, cutoff_raw, nsamples);   \
-    case ASC_LAYOUT_OCTET: return march_st<ST, ASC_LAYOUT_OCTET, UNROLL>(v, r, cutoff_raw, nsamples); \
-    case ASC_LAYOUT_BRICK: return march_st<ST, ASC_LAYOUT_BRICK, UNROLL>(v, r, cutoff_raw, nsamples); \
-    default: return march_st<ST, ASC_LAYOUT_LINEAR, UNROLL>(v, r, cutoff_raw, nsamples);              \
+    case ASC_LAYOUT_QUAD: return march_st<ST, ASC_LAYOUT_QUAD, UNROLL>(v, r, cutoff, nsamples);   \
+    case ASC_LAYOUT_OCTET: return march_st<ST, ASC_LAYOUT_OCTET, UNROLL>(v, r, cutoff, nsamples); \
+    case ASC_LAYOUT_BRICK: return march_st<ST, ASC_LAYOUT_BRICK, UNROLL>(v, r, cutoff, nsamples); \
+    default: return march_st<ST, ASC_LAYOUT_LINEAR, UNROLL>(v, r, cutoff, nsamples);              \
   }
   switch (v.storage) {
     case ASC_STORAGE_U16:
       if (v.pair_diff) {
         switch (v.layout) {
-          case ASC_LAYOUT_QUAD: return march_st<kStorageU16Diff, ASC_LAYOUT_QUAD, UNROLL>(v, r, cutoff_raw, nsamples);
-          case ASC_LAYOUT_OCTET: return march_st<kStorageU16Diff, ASC_LAYOUT_OCTET, UNROLL>(v, r, cutoff_raw, nsamples);
-          default: return march_st<kStorageU16Diff, ASC_LAYOUT_BRICK, UNROLL>(v, r, cutoff_raw, nsamples);
+          case ASC_LAYOUT_QUAD: return march_st<kStorageU16Diff, ASC_LAYOUT_QUAD, UNROLL>(v, r, cutoff, nsamples);
+          case ASC_LAYOUT_OCTET: return march_st<kStorageU16Diff, ASC_LAYOUT_OCTET, UNROLL>(v, r, cutoff, nsamples);
+          default: return march_st<kStorageU16Diff, ASC_LAYOUT_BRICK, UNROLL>(v, r, cutoff, nsamples);
         }
       }
       ASC_MARCH_ST(ASC_STORAGE_U16)
@@ -441,7 +445,7 @@ __device__ __forceinline__ float drr_finish(float acc_raw, float step_vscale, fl
 // Images hold texels already normalised at upload (raw / (2^bits - 1),
 // correctly rounded): the sampler's UNORM read is a plain load here.
 // Bilinear sample at sampler coordinates (s, t) in [0, 1] (every caller
-// returns before sampling outside), OpenCL 1.2 section 8.2: x = s*W - 0.5,
+// returns before sampling outside), OpenCL 1.2 section 8.2: x = fma(s, W, -0.5),
 // i0 = floor(x), a = x - i0, texels outside the image read the border 0
 // (CLK_ADDRESS_CLAMP). The texels come through buffer loads, whose
 // out-of-range offsets read 0: a row outside [0, H) lands out of range by
@@ -455,8 +459,9 @@ __device__ __forceinline__ float img_sample(const ImgDev& im, float s, float t)
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
     (void*)(uintptr_t)(((uint64_t)hi << 32) | lo), 0, W * H * 4, 0x00020000);
-  const float x = s * (float)W - 0.5f;
-  const float y = t * (float)H - 0.5f;
+  // the un-normalisation as in the 3-D sampler: one FMA per axis
+  const float x = fmaf(s, (float)W, -0.5f);
+  const float y = fmaf(t, (float)H, -0.5f);
   const float fx = floorf(x), fy = floorf(y);
   const float a = x - fx, b = y - fy;
   const int i0 = (int)fx, j0 = (int)fy;

@@ -23,7 +23,7 @@ __device__ __forceinline__ void tile_pixel(unsigned bx, unsigned by, unsigned& x
 struct DrrArgs {
   float imv[12];
   float vp[4];
-  float step, intensity, cutoff_raw, step_vscale;
+  float step, intensity, cutoff, step_vscale;
   unsigned w, h;
 };
 
@@ -36,7 +36,7 @@ __global__ __launch_bounds__(kBlock) void drr_kernel(VolDev v, DrrArgs a, float*
   float val = 0.0f;
   if (dev::ray_setup(a.imv, a.vp, a.step, x, y, a.w, a.h, r)) {
     int n = 0;
-    val = dev::drr_finish(dev::march<2>(v, r, a.cutoff_raw, n), a.step_vscale, a.intensity);
+    val = dev::drr_finish(dev::march<2>(v, r, a.cutoff, n), a.step_vscale, a.intensity);
   }
   out[(size_t)y * a.w + x] = val;
 }
@@ -279,8 +279,7 @@ int launch_drr(asc_context ctx, const VolDev& v, const float* imv, const float* 
   for (int i = 0; i < 4; i++) a.vp[i] = vp[i];
   a.step = step;
   a.intensity = intensity;
-  // val > cutoff on normalised values  <=>  raw > cutoff / vscale
-  a.cutoff_raw = cutoff / v.vscale;
+  a.cutoff = cutoff;
   a.step_vscale = step * v.vscale;
   a.w = w;
   a.h = h;

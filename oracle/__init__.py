@@ -36,6 +36,7 @@ def lib():
                                  _P(_i)], _i),
             "orc_drr_f32": ([_vp, _P(_i), _P(_f), _P(_f), _f, _f, _f, _u, _u, _P(_f)], C.c_int64),
             "orc_drr_f64": ([_vp, _P(_i), _P(_f), _P(_f), _f, _f, _f, _u, _u, _P(_d)], C.c_int64),
+            "orc_drr_margin_f32": ([_vp, _P(_i), _P(_f), _P(_f), _f, _f, _f, _f, _u, _u, _P(_f), _P(_f)], C.c_int64),
             "orc_rad_f32": ([_vp, _i, _i, _i, _P(_f), _P(_f), _u, _u, _P(_f)], None),
             "orc_background_f32": ([_P(_f), _i, _i, _P(_f), _P(_f), _f, _u, _u, _P(_f)], None),
             "orc_sobel_f32": ([_P(_f), _P(_f), _i, _i, _f, _f], None),
@@ -138,6 +139,20 @@ def drr(vol: np.ndarray, imv, vp, step, intensity, cutoff, w, h, fp64=False):
         n = lib().orc_drr_f32(vol.ctypes.data_as(_vp), _p(vd, _i), _p(imv, _f), _p(vp, _f), step, intensity, cutoff,
                               w, h, _p(out, _f))
     return out, int(n)
+
+
+def drr_cut_margin(vol: np.ndarray, imv, vp, step, intensity, cutoff, w, h, delta=4e-6):
+    """-> (float DRR, per-pixel margin): the margin is the DRR contribution
+    of the samples whose value lies within delta * max(val, cutoff) of the
+    cutoff, i.e. what a sampler rounding differently could flip."""
+    vol = np.ascontiguousarray(vol, np.uint16)
+    vd = np.array([vol.shape[2], vol.shape[1], vol.shape[0]], np.int32)
+    imv, vp = f32(imv), f32(vp)
+    out = np.zeros((h, w), np.float32)
+    margin = np.zeros((h, w), np.float32)
+    lib().orc_drr_margin_f32(vol.ctypes.data_as(_vp), _p(vd, _i), _p(imv, _f), _p(vp, _f), step, intensity, cutoff,
+                             delta, w, h, _p(out, _f), _p(margin, _f))
+    return out, margin
 
 
 def sample3(vol: np.ndarray, s, t, r, fp64=False):

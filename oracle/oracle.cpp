@@ -140,7 +140,8 @@ inline R vtex(const uint16_t* v, int W, int H, int D, int i, int j, int k)
 
 // The un-normalisation s*W - 0.5 as an OpenCL compiler emits it: FP_CONTRACT
 // is on by default in OpenCL C, so clang (PoCL's compiler) fuses it into one
-// FMA, one rounding.
+// FMA, one rounding. The 2-D sampler (sample2) follows the same convention:
+// both are the one read_imagef rule of OpenCL 1.2 section 8.2.
 template <typename R>
 R sample3(const uint16_t* v, int W, int H, int D, R s, R t, R r)
 {
@@ -172,11 +173,12 @@ inline R itex(const void* img, int bps, int W, int H, int i, int j)
   return R(((const float*)img)[idx]);
 }
 
+// The same un-normalisation as sample3: one FMA per axis.
 template <typename R>
 R sample2(const void* img, int bps, int W, int H, R s, R t)
 {
-  R x = s * R(W) - R(0.5);
-  R y = t * R(H) - R(0.5);
+  R x = std::fma(s, R(W), R(-0.5));
+  R y = std::fma(t, R(H), R(-0.5));
   R a, b;
   int i0 = floor_index(x, a), j0 = floor_index(y, b);
   R T00 = itex<R>(img, bps, W, H, i0, j0), T10 = itex<R>(img, bps, W, H, i0 + 1, j0);
@@ -189,9 +191,14 @@ R sample2(const void* img, int bps, int W, int H, R s, R t)
 // DRR ray march (a5). Reference: RayCaster.cl:2-70; launch RayCaster.cpp:158-193.
 // min/max on the slab distances use the NaN-ignoring fmin/fmax.
 // ---------------------------------------------------------------------------
+// margin (optional): per pixel, the DRR contribution step*val/intensity of
+// the samples whose value lies within delta*max(val, cutoff) of the cutoff.
+// The sampler's arithmetic is implementation-defined in OpenCL, so two
+// admissible evaluations of the same sample may fall on either side of the
+// cutoff there; the margin bounds what such flips can change.
 template <typename R>
 int64_t drr(const uint16_t* vol, int W, int H, int D, const float* imv, const float* vp, R step,
-            R intensity, R cutoff, unsigned w, unsigned h, R* out)
+            R intensity, R cutoff, unsigned w, unsigned h, R* out, R* margin = nullptr, R delta = R(0))
 {
   int64_t total = 0;
 #pragma omp parallel for schedule(dynamic, 4) reduction(+ : total)
@@ -212,23 +219,26 @@ int64_t drr(const uint16_t* vol, int W, int H, int D, const float* imv, const fl
       R nearT = std::fmax(std::fmax(nx, ny), nz);
       const R farT = std::fmin(std::fmin(fx, fy), fz);
       R* o = out + (size_t)y * w + x;
+      if (margin) margin[(size_t)y * w + x] = R(0);
       if (!(farT > nearT)) {
         *o = R(0);
         continue;
       }
       if (nearT < R(0)) nearT = R(0);
       R t = farT;
-      R density = R(0);
+      R density = R(0), near_cut = R(0);
       while (t > nearT) {
         // ray.origin + t*ray.direction, contracted as above: fma(t, d, o)
         const R px = std::fma(t, dx, ox), py = std::fma(t, dy, oy), pz = std::fma(t, dz, oz);
         const R val = sample3<R>(vol, W, H, D, px, R(1) - py, -pz);
         density += val > cutoff ? step * val : R(0);
+        if (margin && std::fabs(val - cutoff) <= delta * std::max(val, cutoff)) near_cut += step * val;
         t -= R(1);
         total++;
       }
       R r = density / intensity;
       *o = r < R(0) ? R(0) : (r > R(1) ? R(1) : r);
+      if (margin) margin[(size_t)y * w + x] = near_cut / intensity;
     }
   }
   return total;
@@ -1260,6 +1270,14 @@ ORC_API int64_t orc_drr_f32(const uint16_t* vol, const int* vdim, const float* i
                             float intensity, float cutoff, unsigned w, unsigned h, float* out)
 {
   return drr<float>(vol, vdim[0], vdim[1], vdim[2], imv, vp, step, intensity, cutoff, w, h, out);
+}
+
+// orc_drr_f32 plus the cutoff margin of every pixel (see drr)
+ORC_API int64_t orc_drr_margin_f32(const uint16_t* vol, const int* vdim, const float* imv, const float* vp,
+                                   float step, float intensity, float cutoff, float delta, unsigned w, unsigned h,
+                                   float* out, float* margin)
+{
+  return drr<float>(vol, vdim[0], vdim[1], vdim[2], imv, vp, step, intensity, cutoff, w, h, out, margin, delta);
 }
 
 ORC_API int64_t orc_drr_f64(const uint16_t* vol, const int* vdim, const float* imv, const float* vp, float step,

@@ -2,7 +2,8 @@
 oracle on the same inputs.
 
 Tolerances (stated per the parity bar):
-  DRR pixels            |gpu - oracle| <= 1e-4 * |oracle|, zero pattern exact
+  DRR pixels            |gpu - oracle| <= 1e-4 * |oracle|, zero pattern exact;
+                        with a cutoff, plus the oracle's cutoff margin
   radiograph/background bit-exact (same float operation order)
   Sobel/Gaussian/Sharpen bit-exact; Contrast <= 4 ulp-scale (powf)
   NCC                   |gpu - oracle FP64| <= 1e-5, mask count exact
@@ -114,9 +115,15 @@ def test_drr_cutoff_and_full_frame(asc, ctx, orc, tiny, vol_pair):
         if c == 0.0:
             assert_drr_close(g, ref)
         else:
-            # near the cutoff a sample can flip; require agreement on >= 99.5 %
-            ok = np.isclose(g, ref, rtol=1e-4, atol=1e-7)
-            assert ok.mean() >= 0.995
+            # the sampler's rounding is implementation-defined, so a sample
+            # within a few ulps of the cutoff may be kept by one evaluation
+            # and dropped by another: beyond the 1e-4 bar a pixel may move by
+            # at most the contribution of such samples (the oracle's margin)
+            ref_m, margin = orc.drr_cut_margin(prep["volume"], imv[:12], fvp, 0.37, 3.0, c, RW, RH)
+            assert np.array_equal(ref_m, ref)
+            err = np.abs(g.astype(np.float64) - ref)
+            assert (err <= 1e-4 * np.abs(ref) + margin).all(), float((err - 1e-4 * np.abs(ref) - margin).max())
+            assert (margin > 0).mean() < 0.05 and (g > 0).mean() > 0.1
 
 
 @pytest.mark.parametrize("kind", ["u8", "u16", "f32"])

@@ -391,3 +391,180 @@ def test_pso_finds_bowl_centre(orc):
     assert bn < 1e-3
     assert np.abs(best - centre).max() < 0.05
     assert epochs <= 201
+
+
+# --------------------------------------------------------------------------
+# a6/a12: radiograph resample and background mask on a ramp, closed form
+# --------------------------------------------------------------------------
+def _ramp_image(IW, IH, kind):
+    j, i = np.meshgrid(np.arange(IH), np.arange(IW), indexing="ij")
+    if kind == "u16":
+        return (1000 + 370 * i + 510 * j).astype(np.uint16), lambda X, Y: (1000 + 370 * X + 510 * Y) / 65535.0
+    return (0.1 + 0.3 * i / IW + 0.5 * j / IH).astype(np.float32), lambda X, Y: 0.1 + 0.3 * X / IW + 0.5 * Y / IH
+
+
+@pytest.mark.parametrize("kind", ["f32", "u16"])
+@pytest.mark.parametrize("shift", [0.0, 0.3])
+def test_rad_resample_of_ramp_closed_form(orc, kind, shift):
+    """RadRenderer.cl:19-29 with the viewport equal to the image plane
+    (shifted by `shift` pixels): pixel (x, y) samples texel coordinates
+    X = x + shift - 1/2 and Y = IH - y - 1/2 (the half-texel offset of the
+    sampler and the t = 1/2 - (v - v0)/v1 flip). A linear ramp is reproduced
+    exactly at interior X, Y; within one texel of the image edge it falls
+    off linearly to the border colour 0 at the first texel centre outside;
+    beyond the image plane the output is 0. The resample returns 1 - value."""
+    IW, IH = 64, 48
+    img, ramp = _ramp_image(IW, IH, kind)
+    u0, v0, u1, v1 = 0.13, -0.07, 0.8, 0.6
+    plane = np.array([u0, v0, u1, v1], np.float32)
+    vp = np.array([u0 - u1 / 2 + shift * u1 / IW, v0 - v1 / 2, u1, v1], np.float32)
+    out = orc.rad(img, plane, vp, IW, IH)
+    y, x = np.meshgrid(np.arange(IH), np.arange(IW), indexing="ij")
+    X, Y = x + shift - 0.5, IH - y - 0.5
+    s = (x + shift) / IW
+    val = ramp(np.clip(X, 0, IW - 1), np.clip(Y, 0, IH - 1))
+    # border falloff: weight (X + 1) below 0, (IW - X) above IW - 1 (same in Y)
+    val = val * np.where(X < 0, X + 1, 1.0) * np.where(X > IW - 1, IW - X, 1.0)
+    val = val * np.where(Y < 0, Y + 1, 1.0) * np.where(Y > IH - 1, IH - Y, 1.0)
+    expect = np.where(s > 1.0, 0.0, 1.0 - val)
+    assert np.abs(out - expect).max() <= 2e-6
+    # the first row samples the top texel row of the image (the t flip):
+    # a missing flip would read row 0, which is 0.5 below it on this ramp
+    assert abs(out[1, IW // 2] - (1 - ramp(IW // 2 + shift - 0.5, IH - 1.5))) <= 2e-6
+
+
+def test_rad_resample_outside_plane_is_zero(orc):
+    IW, IH = 32, 24
+    img, _ = _ramp_image(IW, IH, "f32")
+    plane = np.array([0.0, 0.0, 1.0, 1.0], np.float32)
+    # viewport twice the plane: a quarter-width margin on each side is outside
+    vp = np.array([-1.0, -1.0, 2.0, 2.0], np.float32)
+    out = orc.rad(img, plane, vp, 2 * IW, 2 * IH)
+    u = -1.0 + 2.0 * np.arange(2 * IW) / (2 * IW)
+    v = -1.0 + 2.0 * np.arange(2 * IH) / (2 * IH)
+    outside = (np.abs(v)[:, None] > 0.5) | (np.abs(u)[None, :] > 0.5)
+    assert (out[outside] == 0).all()
+    assert (out[~outside] > 0).all()
+
+
+def test_background_mask_of_ramp_closed_form(orc):
+    """BackgroundRenderer.cl: 1 where threshold <= sampled value, else 0,
+    with the same sampler rule as the resample."""
+    IW, IH = 64, 48
+    img, ramp = _ramp_image(IW, IH, "f32")
+    u0, v0, u1, v1 = 0.0, 0.0, 1.0, 1.0
+    plane = np.array([u0, v0, u1, v1], np.float32)
+    vp = np.array([-0.5, -0.5, 1.0, 1.0], np.float32)
+    thr = 0.45
+    out = orc.background(img, plane, vp, thr, IW, IH)
+    y, x = np.meshgrid(np.arange(1, IH), np.arange(1, IW), indexing="ij")
+    val = ramp(x - 0.5, IH - y - 0.5)
+    m = out[1:, 1:]
+    clear = np.abs(val - thr) > 1e-6
+    assert np.array_equal(m[clear], (val[clear] >= thr).astype(np.float32))
+    assert 0.2 < m.mean() < 0.8
+
+
+# --------------------------------------------------------------------------
+# a19/a20: Compositor and Merger on hand-computed channels
+# --------------------------------------------------------------------------
+def test_composite_and_merge_hand_values(orc):
+    """Compositor.cl:2-24: R = src1, G = src1/2 + src2/2 where src3 >= 0.5
+    else 0, B = src2. Merger.cl: min(a + b, 1)."""
+    s1 = np.array([0.25, 0.75, 1.0, 0.0, 0.5], np.float32)
+    s2 = np.array([0.5, 0.25, 0.0, 1.0, 0.5], np.float32)
+    s3 = np.array([0.5, 0.4999, 1.0, 0.0, 0.75], np.float32)
+    rgb = orc.composite(s1, s2, s3)
+    assert rgb.tolist() == [[0.25, 0.375, 0.5], [0.75, 0.0, 0.25], [1.0, 0.5, 0.0], [0.0, 0.0, 1.0],
+                            [0.5, 0.5, 0.5]]
+    assert orc.merge(s1, s2).tolist() == [0.75, 1.0, 1.0, 1.0, 1.0]
+
+
+# --------------------------------------------------------------------------
+# a5 + a6 + a15 + a18 closed loop: tracking recovers the hidden pose
+# --------------------------------------------------------------------------
+def _bead_scene(orc, shift_px=0.0, filters=None):
+    """Seven smooth beads (u16, zero background) in a 48^3 volume at 0.5 mm,
+    two 160^2 cameras 90 degrees apart. The target of each camera is
+    1 - (the all-FP64 full-frame DRR at the hidden pose), rows flipped to
+    the radiograph's order, so that with no filters the cost is 1 - NCC of
+    a DRR and its own resampled rendering. `shift_px` moves the rendering
+    grid of the target by that fraction of a pixel (a planted half-texel
+    mistake)."""
+    N, vox = 48, 0.5
+    k, j, i = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    c = (N - 1) / 2
+    rng = np.random.default_rng(5)
+    vol = np.zeros((N, N, N))
+    for _ in range(7):
+        p = rng.uniform(-0.3, 0.3, 3) * N + c
+        s = rng.uniform(1.5, 2.5) / vox
+        vol += rng.uniform(0.5, 1.0) * np.exp(-((i - p[0]) ** 2 + (j - p[1]) ** 2 + (k - p[2]) ** 2) / (2 * s * s))
+    vol = np.rint(vol / vol.max() * 30000).astype(np.uint16)
+    prep = orc.volume_prep(vol, (vox,) * 3)
+    cams = [orc.camera_from_mayacam2(*s.mayacam2())
+            for s in synth.biplane(sod_mm=400.0, size=(160, 160), pixel_mm=0.25, sid_mm=550.0)]
+    RW = RH = 160
+    step, inten = synth.DRR_STEP, 6.0
+    gt = synth.volume_centre_pose(vol.shape, vox) + np.array([0.3, -0.2, 0.1, 3.0, -2.0, 4.0])
+    rads = []
+    for cam in cams:
+        _, imv, _, _ = orc.view_geometry(cam, gt, prep["inv_scale"], prep["inv_trans"], RW, RH)
+        fvp = synth.full_frame_viewport(cam[24:28], RW, RH).astype(np.float64)
+        fvp[0] += shift_px * fvp[2] / RW
+        img, _ = orc.drr(prep["volume"], imv[:12], fvp, step, inten, 0.0, RW, RH, fp64=True)
+        assert 0.3 < img.max() < 1.0  # not clamped
+        rads.append((1.0 - np.flipud(img)).astype(np.float32))
+    recs = filters if filters is not None else np.zeros((0, 6), np.float32)
+    S = orc.Scene(prep, (step, inten, 0.0), cams, rads, [(recs, recs)] * 2, (RW, RH))
+    pivot = orc.frame_from_xyzypr(list(prep["trans_center"]) + [0, 0, 0])
+    return S, gt, pivot
+
+
+def _pose_gap(orc, a, b):
+    fa, fb = orc.frame_from_xyzypr(a), orc.frame_from_xyzypr(b)
+    Ra, Rb = fa[:9].reshape(3, 3), fb[:9].reshape(3, 3)
+    cosang = (np.trace(Ra.T @ Rb) - 1.0) / 2.0
+    return float(np.linalg.norm(fa[9:] - fb[9:])), float(np.degrees(np.arccos(np.clip(cosang, -1.0, 1.0))))
+
+
+START = np.array([1.5, -1.2, 0.8, 2.0, -1.5, 1.0])  # ~2 mm / 2 deg off the hidden pose
+
+
+def test_tracking_recovers_hidden_pose(orc):
+    """Tracker::optimize's downhill simplex (DownhillSimplex.cpp, Tracker.cpp:
+    476-570) on the oracle's trackFrame, started 2 mm / 2 deg off, lands on
+    the pose the target was rendered at within 0.05 mm / 0.05 deg. This
+    holds only if the DRR's pixel grid (RayCaster.cl), the resample's
+    sampler rule and t flip (RadRenderer.cl), the mask and NCC composition
+    (Tracker.cpp:583-773) and the pose plumbing (minimizationFunc) agree;
+    the same target rendered half a pixel off moves the optimum by more
+    than the bar."""
+    S, gt, pivot = _bead_scene(orc)
+    s, st, n = S.optimize_frame(orc.pose_store_set(gt + START), pivot, method=1, max_iter=100,
+                                offsets=(1.0,) * 6, seed=7)
+    dt, dr = _pose_gap(orc, orc.pose_store_get(s), gt)
+    assert dt <= 0.05 and dr <= 0.05, (dt, dr)
+    assert st[3] <= S.track_frame(gt)["corr"].sum() + 1e-6
+    # planted mistake: the target's grid half a pixel off along u
+    S2, _, _ = _bead_scene(orc, shift_px=0.5)
+    s2, _, _ = S2.optimize_frame(orc.pose_store_set(gt + START), pivot, method=1, max_iter=100,
+                                 offsets=(1.0,) * 6, seed=7)
+    dt2, dr2 = _pose_gap(orc, orc.pose_store_get(s2), gt)
+    assert dt2 > 0.05 or dr2 > 0.05, (dt2, dr2)
+
+
+def test_filtered_cost_is_minimal_at_hidden_pose(orc):
+    """With the same Sobel chain on both images the cost at the hidden pose
+    is below the cost at every pose 0.2 mm or 0.2 deg off along one axis:
+    the mask is taken after the DRR filters and both images are masked
+    alike (Tracker.cpp:746-752)."""
+    from autoscoper_amd import filters as F
+
+    S, gt, _ = _bead_scene(orc, filters=F.to_records([F.sobel(1.7, 0.1)]))
+    c0 = S.track_frame(gt)["corr"].sum()
+    for ax in range(6):
+        for sgn in (-1, 1):
+            d = np.zeros(6)
+            d[ax] = 0.2 * sgn
+            assert S.track_frame(gt + d)["corr"].sum() > c0
