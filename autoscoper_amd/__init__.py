@@ -190,6 +190,7 @@ _SIGS = {
     "asc_multiply": [_vp, _vp, _vp, _vp, _u, _u],
     "asc_mask_multiply": [_vp, _vp, _vp, _vp, _vp, _u, _u],
     "asc_ncc": [_vp, _vp, _vp, _vp, _u, _P(_d), _P(_d)],
+    "asc_ncc_noalloc": [_vp, _vp, _vp, _vp, _u, _vp],
     "asc_ncc_init": [_vp, _u],
     "asc_ncc_deinit": [_vp],
     "asc_merge": [_vp, _vp, _vp, _vp, _u, _u],
@@ -935,6 +936,11 @@ class Context:
         mom = np.zeros(9)
         _check(lib().asc_ncc(self.handle, f.ptr, g.ptr, m.ptr, n, C.byref(v), _ptr(mom, _d)))
         return v.value, mom
+
+    def ncc_noalloc(self, f: DeviceArray, g: DeviceArray, m: DeviceArray, n: int, out10: DeviceArray):
+        """asc_ncc_noalloc: moments then NCC into the device buffer out10
+        (10 doubles), queued on the context's stream; nothing allocated."""
+        _check(lib().asc_ncc_noalloc(self.handle, f.ptr, g.ptr, m.ptr, n, out10.ptr))
 
 
 class Volume:

@@ -135,6 +135,10 @@ int ncc_tree(asc_context ctx, const float* f, const float* g, const float* m, un
              float* sums6, float* ncc);
 int ncc_moments(asc_context ctx, const float* f, const float* g, const float* m, unsigned n, double* out9,
                 double* ncc);
+// asc_ncc_noalloc: the static scratch's address for `device` (current), once;
+// then the two moment kernels into the caller's out10, nothing allocated.
+int noalloc_scratch_init(int device);
+int ncc_noalloc(asc_context ctx, const float* f, const float* g, const float* m, unsigned n, double* out10);
 
 // NCC from the nine moments, combined in double (Ncc.cpp:180-220 semantics).
 __host__ __device__ inline double ncc_from_moments(const double* m)

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <mutex>
 
 #include "device.hpp"
 
@@ -448,6 +449,54 @@ int ncc_moments(asc_context ctx, const float* f, const float* g, const float* m,
   if (out9)
     for (int k = 0; k < 9; k++) out9[k] = ctx->ncc_host[k];
   if (ncc) *ncc = ctx->ncc_host[9];
+  return ASC_OK;
+}
+
+// ---- the moments NCC without allocation (asc_ncc_noalloc) -------------------
+// Its block partials live in `scratch`, a static device array: every GPU the
+// library runs on holds its own copy (ASC_NCC_NOALLOC_MAX_N elements need
+// kNoAllocBlocks partials of 9 doubles, 316 KB) until the process ends. Host
+// code never passes the array's name to a kernel (on the host it names a
+// stand-in, not device memory): noalloc_scratch_init takes the address of
+// the current device's copy once, and the launches use that address.
+constexpr unsigned kNoAllocBlocks = (ASC_NCC_NOALLOC_MAX_N + kBlock * kNccPerThread - 1) / (kBlock * kNccPerThread);
+static __device__ __align__(256) double scratch[9 * kNoAllocBlocks];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int noalloc_scratch_init(int device)
+{
+  ASC_CHECK(device >= 0 && device < kMaxDevices, ASC_ERR_INVALID, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return ASC_OK;
+  void* p = nullptr;
+  ASC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = static_cast<double*>(p);
+  return ASC_OK;
+}
+
+int ncc_noalloc(asc_context ctx, const float* f, const float* g, const float* m, unsigned n, double* out10)
+{
+  ASC_CHECK(n <= ASC_NCC_NOALLOC_MAX_N, ASC_ERR_TOO_LARGE,
+            "asc_ncc_noalloc: n exceeds ASC_NCC_NOALLOC_MAX_N (9000000 elements)");
+  double* part;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    part = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratch_ptr[ctx->device] : nullptr;
+  }
+  ASC_CHECK(part, ASC_ERR_INVALID, "asc_ncc_noalloc: scratch address not set up for this device");
+  const unsigned per_block = kBlock * kNccPerThread;
+  const unsigned nblocks = n == 0 ? 1 : (n + per_block - 1) / per_block;
+  // every partial the final kernel reads is written by the first: nothing
+  // left by an earlier call is read, so nothing needs clearing
+  ncc_partial_kernel<<<nblocks, kBlock, 0, ctx->stream>>>(f, g, m, n, part);
+  ASC_HIP(hipGetLastError());
+  ncc_final_kernel<<<1, kBlock, 0, ctx->stream>>>(part, nblocks, out10);
+  ASC_HIP(hipGetLastError());
   return ASC_OK;
 }
 

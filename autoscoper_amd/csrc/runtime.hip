@@ -201,6 +201,8 @@ ASC_API int asc_context_create(int device, asc_context* out)
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(ASC_ERR_NO_DEVICE, "no HIP device");
   ASC_CHECK(device >= 0 && device < n, ASC_ERR_INVALID, "device index out of range");
   ASC_HIP(hipSetDevice(device));
+  // the no-allocation entry point's scratch: its address on this device
+  if (int rc = noalloc_scratch_init(device)) return rc;
   asc_context c = new asc_context_s();
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
@@ -836,6 +838,13 @@ ASC_API int asc_ncc(asc_context ctx, const float* f, const float* g, const float
     return ASC_OK;
   }
   return ncc_moments(ctx, f, g, mask, n, moments, ncc);  // waits for its result already
+}
+
+ASC_API int asc_ncc_noalloc(asc_context ctx, const float* f, const float* g, const float* mask, unsigned n,
+                            double* out10)
+{
+  ASC_CHECK(valid_ctx(ctx) && f && g && mask && out10, ASC_ERR_INVALID, "bad argument");
+  return finish(ctx, ncc_noalloc(ctx, f, g, mask, n, out10));
 }
 
 ASC_API int asc_ncc_init(asc_context ctx, unsigned max_n)

@@ -240,6 +240,22 @@ enum { ASC_NCC_MOMENTS = 0, ASC_NCC_TREE_CUDA = 1, ASC_NCC_TREE_OPENCL = 2 };
 ASC_API int asc_context_set_ncc_form(asc_context ctx, int form);
 ASC_API int asc_ncc(asc_context ctx, const float* f, const float* g, const float* mask, unsigned n, double* ncc,
                     double moments[9]);
+/* No counterpart in the reference. asc_ncc (ASC_NCC_MOMENTS form, whatever
+ * asc_context_set_ncc_form says) for callers that cannot allocate: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * It calls no allocation routine and does not wait: the two moment kernels
+ * run on the context's stream (blocking contexts wait, as every per-operator
+ * call does), and out10 (caller-owned device memory, 10 doubles) receives the
+ * nine moments in asc_ncc's order, then the NCC, the same bits as asc_ncc.
+ * The block partials live in a static device array of the library, one per
+ * GPU, held until the process ends (316 KB): n is limited to
+ * ASC_NCC_NOALLOC_MAX_N (3000 x 3000, the reference View's buffer size);
+ * larger n returns ASC_ERR_TOO_LARGE. All calls on one GPU share that array:
+ * calls that overlap on one GPU (two streams, two threads) race, so keep
+ * them on one stream. */
+#define ASC_NCC_NOALLOC_MAX_N 9000000u
+ASC_API int asc_ncc_noalloc(asc_context ctx, const float* f, const float* g, const float* mask, unsigned n,
+                            double* out10);
 /* ncc_init / ncc_deinit (Ncc.cpp:151-178): reserve scratch for n elements. */
 ASC_API int asc_ncc_init(asc_context ctx, unsigned max_n);
 ASC_API int asc_ncc_deinit(asc_context ctx);
