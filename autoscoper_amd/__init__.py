@@ -1126,7 +1126,7 @@ class Scene:
                                            _ptr(costs, _d), _ptr(per, _d)))
         return costs, per
 
-    TUNE_UNROLL, TUNE_GRID, TUNE_WAVE_PATCH = 0, 1, 2
+    TUNE_UNROLL, TUNE_GRID, TUNE_WAVE_PATCH, TUNE_GATHER_PROBE = 0, 1, 2, 3
 
     def autotune(self, poses, volumes=None, reps=3) -> dict:
         """Time the launch-shape variants on this batch of poses and keep the
@@ -1194,7 +1194,7 @@ class Scene:
         return cost.value, per
 
     def optimize_frame(self, store, method=0, max_iter=100, max_stall=25, min_limit=-3.0, max_limit=3.0,
-                       offsets=(0.1,) * 6, seed=1, particles=100, synchronous=False, volume=0):
+                       offsets=(0.1,) * 6, seed=1, particles=100, synchronous=False, volume=0, speculative=False):
         s = _f32(store).copy()
         o = AscOptimizeOptions()
         o.method, o.max_iter, o.max_stall = method, max_iter, max_stall
@@ -1202,6 +1202,7 @@ class Scene:
         for k in range(6):
             o.offsets[k] = offsets[k]
         o.seed, o.particles, o.synchronous, o.volume = seed, particles, int(synchronous), volume
+        o.speculative = int(speculative)
         st = AscOptimizeStats()
         _check(lib().asc_optimize_frame(self.handle, _ptr(s, _f), C.byref(o), C.byref(st)))
         return s, st

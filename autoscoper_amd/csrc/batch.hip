@@ -183,7 +183,17 @@ __device__ __forceinline__ float march_item(const VolParams& vp, const ItemDesc&
   return dev::drr_finish(dev::march<UNROLL>(vp.v, r, vp.cutoff, nsamp), vp.step_vscale, vp.intensity);
 }
 
-template <int UNROLL>
+__device__ __forceinline__ float probe_item(const VolParams& vp, const ItemDesc& it, unsigned x, unsigned y,
+                                            int& nsamp)
+{
+  dev::Ray r;
+  if (!dev::ray_setup(it.imv, it.vp, vp.step, x, y, (unsigned)it.w, (unsigned)it.h, r)) return 0.0f;
+  return dev::gather_probe(vp.v, r, nsamp);
+}
+
+// PROBE: the gather probe in place of the march (same tiles, rays and
+// sample counts; the images it leaves are not DRRs).
+template <int UNROLL, bool PROBE = false>
 __global__ __launch_bounds__(kBlock) void drr_tiles(BatchArgs a)
 {
   __shared__ int s_tile;
@@ -209,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void drr_tiles(BatchArgs a)
     int nsamp = 0;
     if (x < (unsigned)it.w && y < (unsigned)it.h) {
       const VolParams& vp = a.vols[it.vol];
-      const float val = march_item<UNROLL>(vp, it, x, y, nsamp);
+      const float val = PROBE ? probe_item(vp, it, x, y, nsamp) : march_item<UNROLL>(vp, it, x, y, nsamp);
       a.drr[it.drr_off + (long long)y * it.w + x] = val;
     }
     // per-tile sample count (for the algorithmic-bytes figure)
@@ -619,6 +629,7 @@ struct asc_scene_s {
   bool timing = false;
   int unroll = 4;  // ray samples per loop iteration (1, 2 or 4)
   int wave_shift = 3;  // a wave marches an 8 x 8 ray patch
+  int gather_probe = 0;  // 1: the gather probe replaces the march (measurement)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   float last_ms[3] = {0, 0, 0};
 };
@@ -749,11 +760,14 @@ int issue_batch(asc_scene s, size_t batch_bytes, int grid_drr, int grid_post)
   zero_u64<<<(n_items + 255) / 256, 256, 0, st>>>(s->d_samples.p, n_items);
   const bool timed = s->timing && s->ev[0];
   if (timed) ASC_HIP(hipEventRecord(s->ev[0], st));
-  switch (s->unroll) {
-    case 1: drr_tiles<1><<<grid_drr, kBlock, 0, st>>>(a); break;
-    case 4: drr_tiles<4><<<grid_drr, kBlock, 0, st>>>(a); break;
-    default: drr_tiles<2><<<grid_drr, kBlock, 0, st>>>(a); break;
-  }
+  if (s->gather_probe)
+    drr_tiles<4, true><<<grid_drr, kBlock, 0, st>>>(a);
+  else
+    switch (s->unroll) {
+      case 1: drr_tiles<1><<<grid_drr, kBlock, 0, st>>>(a); break;
+      case 4: drr_tiles<4><<<grid_drr, kBlock, 0, st>>>(a); break;
+      default: drr_tiles<2><<<grid_drr, kBlock, 0, st>>>(a); break;
+    }
   if (timed) ASC_HIP(hipEventRecord(s->ev[1], st));
   if (s->post_small)
     post_tiles<kSmallHalo><<<grid_post, kBlock, 0, st>>>(a);
@@ -1337,6 +1351,10 @@ ASC_API int asc_scene_set_tuning(asc_scene s, int key, int value)
     case ASC_TUNE_WAVE_PATCH:
       ASC_CHECK(value == 4 || value == 8 || value == 16, ASC_ERR_INVALID, "wave patch width must be 4, 8 or 16");
       s->wave_shift = value == 4 ? 2 : value == 8 ? 3 : 4;
+      break;
+    case ASC_TUNE_GATHER_PROBE:
+      ASC_CHECK(value == 0 || value == 1, ASC_ERR_INVALID, "gather probe is 0 or 1");
+      s->gather_probe = value;
       break;
     default: return fail(ASC_ERR_INVALID, "unknown tuning key");
   }

@@ -202,6 +202,30 @@ __device__ __forceinline__ void gather8(const VolRsrc& r, int voff, float* t)
   }
 }
 
+// Byte offset of the element holding cell (i, j, k) (corner (i, j, k) of
+// the trilinear footprint) in layout LY. In-box samples have
+// -1 <= i <= W-1 (likewise j, k), so the padded element (i+1, j+1, k+1) is
+// in the array and its offset i*esz + j*sy + k*sz + c0 is >= 0; the +1s are
+// folded into c0.
+template <int ST, int LY>
+__device__ __forceinline__ int cell_offset(const VolRsrc& r, int i, int j, int k)
+{
+  if constexpr (LY == ASC_LAYOUT_BRICK) {
+    // cell (i, j, k) lives in brick (i/2+1, j/2+1, k/2+1) (floor division),
+    // at cell (j&1) + 2(k&1) + 4(i&1) of the brick; with E bytes per cell, a
+    // brick of 8E bytes, and row/layer strides SY = r.sy, SZ = r.sz:
+    //   4E*(i&1) + 8E*((i>>1)+1) = 4E*i + 8E,
+    //   E*(j&1) + SY*((j>>1)+1) = E*j + (SY-2E)*(j>>1) + SY,
+    //   2E*(k&1) + SZ*((k>>1)+1) = 2E*k + (SZ-4E)*(k>>1) + SZ;
+    // the constants sum to r.c0, and x needs no split at all
+    constexpr int S = elem_shift<ST, LY>(), E = 1 << S;
+    const int base = (i << (S + 2)) + (j << S) + (k << (S + 1)) + r.c0;
+    return mad_i24(j >> 1, r.sy - 2 * E, mad_i24(k >> 1, r.sz - 4 * E, base));
+  } else {
+    return mad_i24(k, r.sz, mad_i24(j, r.sy, (i << elem_shift<ST, LY>()) + r.c0));
+  }
+}
+
 // Trilinear sample at voxel-space position (X, Y, Z) = (s*W-0.5, t*H-0.5,
 // r*D-0.5) with the sampler's weights (OpenCL 1.2 section 8.2): i0 =
 // floor(X), a = X - floor(X). Each lerp is one FMA: fma(a, T1 - T0, T0) is
@@ -216,22 +240,7 @@ __device__ __forceinline__ float trilinear(const VolDev& v, const VolRsrc& r, fl
   // in-box samples have -1 <= i0 <= W-1, so the padded element
   // (i0+1, j0+1, k0+1) is in the array and its byte offset
   // i0*esz + j0*sy + k0*sz + c0 is >= 0; the +1s are folded into c0
-  const int i = cvt_flr(X), j = cvt_flr(Y), k = cvt_flr(Z);
-  int voff;
-  if constexpr (LY == ASC_LAYOUT_BRICK) {
-    // cell (i, j, k) lives in brick (i/2+1, j/2+1, k/2+1) (floor division),
-    // at cell (j&1) + 2(k&1) + 4(i&1) of the brick; with E bytes per cell, a
-    // brick of 8E bytes, and row/layer strides SY = r.sy, SZ = r.sz:
-    //   4E*(i&1) + 8E*((i>>1)+1) = 4E*i + 8E,
-    //   E*(j&1) + SY*((j>>1)+1) = E*j + (SY-2E)*(j>>1) + SY,
-    //   2E*(k&1) + SZ*((k>>1)+1) = 2E*k + (SZ-4E)*(k>>1) + SZ;
-    // the constants sum to r.c0, and x needs no split at all
-    constexpr int S = elem_shift<ST, LY>(), E = 1 << S;
-    const int base = (i << (S + 2)) + (j << S) + (k << (S + 1)) + r.c0;
-    voff = mad_i24(j >> 1, r.sy - 2 * E, mad_i24(k >> 1, r.sz - 4 * E, base));
-  } else {
-    voff = mad_i24(k, r.sz, mad_i24(j, r.sy, (i << elem_shift<ST, LY>()) + r.c0));
-  }
+  const int voff = cell_offset<ST, LY>(r, cvt_flr(X), cvt_flr(Y), cvt_flr(Z));
   if constexpr (ST == kStorageU16Diff) {
     // pairs stored as (x0, x1 - x0): each x-lerp is fma(a, x1 - x0, x0) on
     // two converted halves, the same values as the u16 path below
@@ -397,6 +406,54 @@ __device__ __forceinline__ float march_loop(const VolDev& v, const Ray& r, float
 }
 
 // cutoff is on normalised values, as the reference compares them
+// Gather probe: the march's memory access stream with as little arithmetic
+// as it needs. The same rays and sample counts visit the same cells (their
+// positions from the ray's entry point, one FMA per axis), one 16-byte cell
+// load per sample, 4 in flight per lane, folded by XOR so that no load is
+// dead. What it returns is not a DRR; its duration measures what the memory
+// hierarchy delivers for this access pattern (16-bit storages in the OCTET
+// and BRICK layouts, whose cells are 16 bytes; other volumes return 0 and
+// count no samples).
+template <int LY>
+__device__ __forceinline__ uint32_t gather_probe_st(const VolDev& v, const Ray& r, int& nsamples)
+{
+  const VolRsrc rs = vol_rsrc(v);
+  const VoxelLine l = voxel_line(r, (float)v.W, (float)v.H, (float)v.D);
+  const int n = sample_count(r);
+  uint32_t acc = 0;
+  float kf = 0.0f;
+  int k = 0;
+  auto one = [&](float kk) {
+    const float X = fmaf(kk, l.nbx, l.ax), Y = fmaf(kk, l.nby, l.ay), Z = fmaf(kk, l.nbz, l.az);
+    const int voff = cell_offset<ASC_STORAGE_U16, LY>(rs, cvt_flr(X), cvt_flr(Y), cvt_flr(Z));
+    const auto q = __builtin_amdgcn_raw_buffer_load_b128(rs.rsrc, voff, 0, 0);
+    acc ^= q[0] ^ q[1] ^ q[2] ^ q[3];
+  };
+  for (; k + 4 <= n; k += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) one(kf + (float)u);
+    kf += 4.0f;
+  }
+  for (; k < n; k++) {
+    one(kf);
+    kf += 1.0f;
+  }
+  nsamples += n;
+  return acc;
+}
+
+__device__ __forceinline__ float gather_probe(const VolDev& v, const Ray& r, int& nsamples)
+{
+  uint32_t acc = 0;
+  if (v.storage != ASC_STORAGE_F32) {
+    if (v.layout == ASC_LAYOUT_BRICK)
+      acc = gather_probe_st<ASC_LAYOUT_BRICK>(v, r, nsamples);
+    else if (v.layout == ASC_LAYOUT_OCTET)
+      acc = gather_probe_st<ASC_LAYOUT_OCTET>(v, r, nsamples);
+  }
+  return __uint_as_float(acc & 0x007FFFFFu);  // a finite value depending on every load
+}
+
 template <int ST, int LY, int UNROLL>
 __device__ __forceinline__ float march_st(const VolDev& v, const Ray& r, float cutoff, int& nsamples)
 {

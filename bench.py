@@ -32,6 +32,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _jsonable(o):
+    """numpy scalars and arrays in the result line"""
+    if hasattr(o, "tolist"):
+        return o.tolist()
+    raise TypeError(f"not JSON serializable: {type(o).__name__}")
+
+
 LAYOUTS = {"linear": 0, "quad": 1, "octet": 2, "brick": 3}
 
 
@@ -131,28 +138,33 @@ def cpu_baseline(wl, cams, rads, gt, budget_s: float):
     return n / el, n, el, threads
 
 
-SERVE_RATES = (1.0, 4.0, 16.0)  # job arrivals per second
-# trial templates of the serving stream: cameras, detector side, bones (phantom pool indices), frames
+SERVE_LOAD = (0.5, 0.8, 1.0)  # arrival rates as fractions of the measured capacity
+# trial templates of the serving stream: cameras, detector side, bones (indices into SERVE_POOL)
 SERVE_TEMPLATES = [
-    dict(ncam=1, det=512, bones=(0,), frames=8),
-    dict(ncam=2, det=1024, bones=(2,), frames=5),
-    dict(ncam=4, det=768, bones=(0, 1), frames=4),
-    dict(ncam=3, det=512, bones=(1,), frames=6),
-    dict(ncam=2, det=768, bones=(0, 1, 2), frames=4),
-    dict(ncam=2, det=512, bones=(1, 0), frames=6),
+    dict(ncam=1, det=512, bones=(0,)),
+    dict(ncam=2, det=1024, bones=(2,)),
+    dict(ncam=4, det=768, bones=(0, 1)),
+    dict(ncam=3, det=512, bones=(1,)),
+    dict(ncam=2, det=768, bones=(0, 1, 2)),
+    dict(ncam=4, det=1024, bones=(2,)),
 ]
+# bone volumes: 64^3 @0.5 mm up to 256x256x400 @0.3 mm (u16)
+SERVE_POOL = [((64, 64, 64), 0.5, "ellipsoid", -60.0), ((160, 128, 128), 0.4, "longbone", 0.0),
+              ((400, 256, 256), 0.3, "longbone", 70.0)]
+SERVE_WALK = 24  # distinct target frames per template; a job walks them back and forth
 SERVE_OPTIONS = dict(method=0, max_iter=6, max_stall=3, synchronous=True, speculative=True)
 
 
 def serve_trials(A, synth, ctx):
     """Seeded trials of mixed shapes (1-4 cameras, 512^2-1024^2 detectors,
-    1-3 bones from 64^3 to 256x256x320 u16) with rendered target frames."""
+    1-3 bones from 64^3 to 256x256x400 u16), each with SERVE_WALK target
+    frames rendered at the poses of a random walk (0.5 mm / 1 deg a frame).
+    A job of F frames visits them back and forth (0, 1, .., 23, 22, .., 0,
+    1, ..), so consecutive frames are always neighbouring walk poses."""
     t0 = time.time()
-    pool = [((64, 64, 64), 0.5, "ellipsoid", -60.0), ((160, 128, 128), 0.4, "longbone", 0.0),
-            ((320, 256, 256), 0.3, "longbone", 70.0)]
     vols = [synth.phantom(shape, vox, kind, seed=synth.SEED_PHANTOM + 20 * i, noise_seed=synth.SEED_NOISE + 20 * i)
-            for i, (shape, vox, kind, _) in enumerate(pool)]
-    dvols = [A.Volume(ctx, v, scale=(p[1],) * 3) for v, p in zip(vols, pool)]
+            for i, (shape, vox, kind, _) in enumerate(SERVE_POOL)]
+    dvols = [A.Volume(ctx, v, scale=(p[1],) * 3) for v, p in zip(vols, SERVE_POOL)]
     trials = []
     for ti, tp in enumerate(SERVE_TEMPLATES):
         det = tp["det"]
@@ -160,13 +172,13 @@ def serve_trials(A, synth, ctx):
         cams = [A.geometry.camera_from_mayacam2(*c.mayacam2()) for c in specs]
         gts = []
         for k, b in enumerate(tp["bones"]):
-            shape, vox, _, z = pool[b]
+            shape, vox, _, z = SERVE_POOL[b]
             base = synth.volume_centre_pose(shape, vox) + np.array([0.0, 0.0, z, 0.0, 0.0, 0.0])
-            gts.append(synth.random_walk(base, tp["frames"], 0.5, 1.0, 5000 + 10 * ti + k))
+            gts.append(synth.random_walk(base, SERVE_WALK, 0.5, 1.0, 5000 + 10 * ti + k))
         gts = np.stack(gts, 1)  # frames x bones x 6
         full, acc = ctx.alloc(4 * det * det), ctx.alloc(4 * det * det)
         frames = []
-        for f in range(tp["frames"]):
+        for f in range(SERVE_WALK):
             per = []
             for c, cam in enumerate(cams):
                 fvp = synth.full_frame_viewport(list(cam.viewport), det, det)
@@ -183,63 +195,81 @@ def serve_trials(A, synth, ctx):
         rng = np.random.default_rng(8000 + ti)
         start = gts[0] + np.concatenate([rng.uniform(-2, 2, (len(tp["bones"]), 3)),
                                          rng.uniform(-2, 2, (len(tp["bones"]), 3))], 1)
-        trials.append(dict(vols=[vols[b] for b in tp["bones"]], vox=[pool[b][1] for b in tp["bones"]], start=start,
-                           cams=cams, frames=frames, render=(det, det)))
+        trials.append(dict(vols=[vols[b] for b in tp["bones"]], vox=[SERVE_POOL[b][1] for b in tp["bones"]],
+                           start=start, cams=cams, frames=frames, render=(det, det)))
     for v in dvols:
         v.close()
     log(f"[serve] {len(trials)} trial templates ready in {time.time() - t0:.1f}s")
     return trials
 
 
-def serve_bench(A, synth, ctx, trials, rates, arrival_s: float, seed: int = 9000):
-    """Poisson job arrivals at each rate for `arrival_s` seconds, served by
-    one server (continuous batching); frames/s and per-frame latency."""
+def walk_frames(frames, n):
+    """n frames visiting `frames` back and forth."""
+    T = len(frames)
+    period = 2 * (T - 1)
+    return [frames[f % period if f % period < T else period - f % period] for f in range(n)]
+
+
+def serve_run(A, synth, ctx, trials, jobs, arrival=True):
+    """Serve `jobs` = [(arrival s, template, frames, particles, seed)] with one
+    server (continuous batching) -> result dict."""
     import threading
 
     dflt, rflt = synth.default_filters()
-    out = []
-    for ri, rate in enumerate(rates):
-        rng = np.random.default_rng(seed + ri)
-        t, arrivals = 0.0, []
-        while True:
+    srv = A.Server(ctx, 64 << 30, 16)
+    ids = [None] * len(jobs)
+    t0 = time.perf_counter()
+
+    def submitter():
+        for k, (a, ti, nframes, particles, s) in enumerate(jobs):
+            delay = t0 + a - time.perf_counter()
+            if arrival and delay > 0:
+                time.sleep(delay)
+            tr = trials[ti]
+            ids[k] = srv.submit(tr["vols"], tr["vox"], tr["start"], tr["cams"], walk_frames(tr["frames"], nframes),
+                                tr["render"], drr_filters=dflt, rad_filters=rflt,
+                                drr=(synth.DRR_STEP, synth.DRR_INTENSITY, synth.DRR_CUTOFF), guess=1,
+                                particles=particles, seed=s, **SERVE_OPTIONS)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    frames = 0
+    for jid in ids:
+        _, lat, st = srv.wait(jid)
+        frames += st.frames
+    el = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return {"jobs": len(jobs), "frames": frames, "seconds": el, "frames_per_s": frames / el,
+            "latency_p50_ms": st["latency_p50_ms"], "latency_p99_ms": st["latency_p99_ms"], "launches": st["batches"],
+            "requests_per_launch": st["requests"] / max(st["batches"], 1), "gpu_busy_frac": st["busy_ms"] / (1e3 * el)}
+
+
+def serve_bench(A, synth, ctx, trials, loads, jobs_per_rate: int, max_frames: int, seed: int = 9000):
+    """cfg5: first the capacity (frames/s with 8 jobs submitted at once),
+    then Poisson job arrivals at each fraction of it, `jobs_per_rate` jobs of
+    20..max_frames frames each; frames/s and per-frame latency per rate."""
+    def draw(rng, n, fmax):
+        return [(int(rng.integers(len(trials))), int(rng.integers(20, fmax + 1)), int(rng.choice([50, 100, 200])),
+                 int(rng.integers(1 << 20))) for _ in range(n)]
+
+    rng = np.random.default_rng(seed)
+    sat = serve_run(A, synth, ctx, trials, [(0.0, *j) for j in draw(rng, 8, 40)], arrival=False)
+    cap = sat["frames_per_s"]
+    mean_frames = (20 + max_frames) / 2
+    log(f"[serve] capacity {cap:.1f} frames/s (8 jobs at once, p99 {sat['latency_p99_ms']:.0f} ms)")
+    out = {"capacity": sat, "rates": []}
+    for ri, load in enumerate(loads):
+        rate = load * cap / mean_frames  # jobs per second
+        rng = np.random.default_rng(seed + 1 + ri)
+        t, jobs = 0.0, []
+        for j in draw(rng, jobs_per_rate, max_frames):
             t += rng.exponential(1.0 / rate)
-            if t > arrival_s:
-                break
-            arrivals.append(t)
-        jobs = [(a, int(rng.integers(len(trials))), int(rng.choice([50, 100, 200])), int(rng.integers(1 << 20)))
-                for a in arrivals]
-        srv = A.Server(ctx, 64 << 30, 16)
-        ids = [None] * len(jobs)
-        t0 = time.perf_counter()
-
-        def submitter():
-            for k, (a, ti, particles, s) in enumerate(jobs):
-                delay = t0 + a - time.perf_counter()
-                if delay > 0:
-                    time.sleep(delay)
-                tr = trials[ti]
-                ids[k] = srv.submit(tr["vols"], tr["vox"], tr["start"], tr["cams"], tr["frames"], tr["render"],
-                                    drr_filters=dflt, rad_filters=rflt,
-                                    drr=(synth.DRR_STEP, synth.DRR_INTENSITY, synth.DRR_CUTOFF), guess=1,
-                                    particles=particles, seed=s, **SERVE_OPTIONS)
-
-        th = threading.Thread(target=submitter)
-        th.start()
-        th.join()
-        frames = 0
-        for jid in ids:
-            _, lat, st = srv.wait(jid)
-            frames += st.frames
-        el = time.perf_counter() - t0
-        st = srv.stats()
-        srv.close()
-        res = {"rate_jobs_per_s": rate, "jobs": len(jobs), "frames": frames, "seconds": el,
-               "frames_per_s": frames / el, "latency_p50_ms": st["latency_p50_ms"],
-               "latency_p99_ms": st["latency_p99_ms"], "launches": st["batches"],
-               "requests_per_launch": st["requests"] / max(st["batches"], 1),
-               "gpu_busy_frac": st["busy_ms"] / (1e3 * el)}
+            jobs.append((t, *j))
+        res = {"load": load, "rate_jobs_per_s": rate, **serve_run(A, synth, ctx, trials, jobs)}
         log(f"[serve] {res}")
-        out.append(res)
+        out["rates"].append(res)
     return out
 
 
@@ -314,14 +344,16 @@ def plain_facade(synth, wl, cams, rads, poses, evals: int):
     (u16, here in the densest layout) and as a plain FP32 copy in the linear
     layout; "reference_form" adds gpu::ncc as the reference computes it
     (six float tree sums of two or three launches each, each ended by a
-    blocking read). Returns None when the program is not built."""
+    blocking read). Raises when the program is not built: the batching gain
+    is quoted against this form."""
     import subprocess
     import tempfile
 
     root = os.path.dirname(os.path.abspath(__file__))
     exe = os.path.join(root, "tests", "cpp", "facade_check.bin")
     if not os.path.exists(exe):
-        return None
+        raise FileNotFoundError(f"{exe} is not built (python -m autoscoper_amd.build builds it); the plain "
+                                "reference-form baseline cannot be timed without it")
     D, H, W = wl.volume.shape
     RW, RH = wl.render
     rh, rw = rads[0][1].shape
@@ -349,6 +381,244 @@ def plain_facade(synth, wl, cams, rads, poses, evals: int):
                 raise RuntimeError(f"facade_check failed: {out.stderr.strip()}")
             res[key] = json.loads(out.stdout.strip().splitlines()[-1])["evals_per_s"]
     return res
+
+
+def pose_gap(A, a, b):
+    """(translation distance mm, rotation angle deg) between two xyzypr poses."""
+    fa, fb = A.geometry.frame_from_xyzypr(a), A.geometry.frame_from_xyzypr(b)
+    Ra, Rb = fa[:9].reshape(3, 3), fb[:9].reshape(3, 3)
+    c = (np.trace(Ra.T @ Rb) - 1.0) / 2.0
+    return float(np.linalg.norm(fa[9:] - fb[9:])), float(np.degrees(np.arccos(np.clip(c, -1.0, 1.0))))
+
+
+def frame_targets(A, synth, ctx, wl, vol, cams, frames, seed0):
+    """Targets of every frame: the full-frame DRR at the frame's hidden pose
+    (product path), blurred, noised and given a background; -> per frame,
+    per camera (float32 radiograph, u16 copy)."""
+    RW, RH = wl.render
+    full = ctx.alloc(4 * RW * RH)
+    out = []
+    for f in range(frames):
+        per = []
+        for i, cam in enumerate(cams):
+            inview, imv, _, _ = A.geometry.view_geometry(cam, wl.gt_poses[f], vol.desc, RW, RH)
+            assert inview
+            fvp = synth.full_frame_viewport(list(cam.viewport), RW, RH)
+            ctx.drr_render(vol, imv, fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, full, RW, RH)
+            per.append(synth.target_from_drr(full.download((RH, RW)), seed0 + 10 * f + i))
+        out.append(per)
+    full.free()
+    return out
+
+
+def frame_loop(A, synth, ctx, name, storage, frames, opts, seed, restart=None, fp64_frames=0, fp64_budget=60.0,
+               wl=None, targets=None):
+    """Tracker's frame loop on one GPU: for each frame of the hidden random
+    walk, the scene's radiographs are the frame's targets, the start is the
+    previous frame's result (guess 1; frame 0 starts 2 mm / 2 deg off its
+    hidden pose), and asc_optimize_frame runs `opts` (PSO then simplex, or
+    the simplex alone). With `restart`, every frame starts from its hidden
+    pose plus a fresh perturbation of that size instead. Timed from the first
+    frame's start to the last frame's end, radiograph switches included
+    (targets are uploaded before). Returns frames/s, evaluations and the
+    final pose's distance to the hidden pose per frame; for the first
+    `fp64_frames` frames also the distance to the oracle's all-FP64 simplex
+    polish started at the GPU's pose (the FP64 cost's own minimum nearby)."""
+    wl = wl or synth.workload(name, frames=frames)
+    vol = A.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3, storage=STORAGES[storage])
+    RW, RH = wl.render
+    cams = [A.geometry.camera_from_mayacam2(*c.mayacam2()) for c in wl.cameras]
+    if targets is None:
+        targets = frame_targets(A, synth, ctx, wl, vol, cams, frames, 900)
+    imgs = [[A.Image(ctx, t[1]) for t in per] for per in targets]
+    scene = A.Scene(ctx, RW, RH)
+    pivot = A.geometry.frame_from_xyzypr(list(vol.desc.trans_center) + [0, 0, 0])
+    scene.add_volume(vol, synth.DRR_STEP, synth.DRR_INTENSITY, synth.DRR_CUTOFF, pivot)
+    for i, cam in enumerate(cams):
+        c = scene.add_camera(cam, imgs[0][i])
+        scene.set_filters(c, 0, wl.drr_filters)
+        scene.set_filters(c, 1, wl.rad_filters)
+    rng = np.random.default_rng(synth.SEED_START)
+    pert = lambda mm, deg: np.concatenate([rng.uniform(-mm, mm, 3), rng.uniform(-deg, deg, 3)])  # noqa: E731
+    guess = wl.gt_poses[0] + pert(2.0, 2.0)
+    starts = [wl.gt_poses[f] + pert(*restart) for f in range(frames)] if restart else None
+    gaps, evals, finals, costs = [], 0, [], []
+    t0 = time.perf_counter()
+    for f in range(frames):
+        for i in range(len(cams)):
+            scene.set_rad(i, imgs[f][i])
+        start = starts[f] if restart else guess
+        s, st = scene.optimize_frame(A.geometry.pose_store_set(start), seed=seed, **opts)
+        p = A.geometry.pose_store_get(s)
+        evals += st.evaluations
+        costs.append([st.initial_cost, st.pso_best_cost, st.final_cost, st.pso_epochs, st.simplex_iterations])
+        gaps.append(pose_gap(A, p, wl.gt_poses[f]))
+        finals.append((p, s))
+        guess = p
+    el = time.perf_counter() - t0
+    res = {"frames": frames, "seconds": el, "frames_per_s": frames / el, "pose_evals": int(evals),
+           "pose_evals_per_s": evals / el,
+           "gap_to_hidden_mm": [g[0] for g in gaps], "gap_to_hidden_deg": [g[1] for g in gaps],
+           "max_gap_to_hidden": [max(g[0] for g in gaps), max(g[1] for g in gaps)],
+           "per_frame": {"fields": ["initial_cost", "pso_best_cost", "final_cost", "pso_epochs", "simplex_iterations"],
+                         "values": costs}}
+    if fp64_frames:
+        import oracle as O
+        from autoscoper_amd import filters as F
+
+        prep = O.volume_prep(wl.volume, (wl.voxel_mm,) * 3)
+        recs = [(F.to_records(wl.drr_filters), F.to_records(wl.rad_filters)) for _ in cams]
+        fp = []
+        t1 = time.perf_counter()
+        for f in range(min(fp64_frames, frames)):
+            if time.perf_counter() - t1 > fp64_budget:
+                break
+            S = O.Scene(prep, (synth.DRR_STEP, synth.DRR_INTENSITY, synth.DRR_CUTOFF), [c.to_array() for c in cams],
+                        [t[1] for t in targets[f]], recs, wl.render)
+            p_gpu, s_gpu = finals[f]
+            s64, st64, n64 = S.optimize_frame(s_gpu, pivot, method=1, fp64=True)
+            g = pose_gap(A, p_gpu, O.pose_store_get(s64))
+            fp.append({"frame": f, "gap_mm": g[0], "gap_deg": g[1], "oracle_evals": n64,
+                       "fp64_cost": float(st64[3])})
+        res["fp64_polish"] = {"what": "oracle all-FP64 simplex (offsets 0.1) started at the GPU's final pose; the "
+                                      "gap is how far the FP64 cost's own minimum lies from the GPU's pose",
+                              "frames": fp, "seconds": time.perf_counter() - t1}
+    scene.close()
+    for per in imgs:
+        for im in per:
+            im.close()
+    vol.close()
+    return res
+
+
+CFG2_TRACK = dict(method=0, max_iter=100, max_stall=101, synchronous=True, speculative=True, particles=100)
+CFG1_TRACK = dict(method=1, speculative=True)
+
+
+def cfg1_leg(A, synth, ctx, args):
+    """configs[0] on the GPU and the CPU: the 128^3 u16 ellipsoid, two 512^2
+    cameras; the fused batch rate (100 poses per launch), the simplex frame
+    loop of 10 frames (each started 2 mm / 2 deg off its hidden pose), and
+    the CPU oracle's evaluations and simplex frames on the host cores."""
+    import oracle as O
+    from autoscoper_amd import filters as F
+
+    wl = synth.workload("small", frames=10)
+    vol = A.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3)
+    cams = [A.geometry.camera_from_mayacam2(*c.mayacam2()) for c in wl.cameras]
+    targets = frame_targets(A, synth, ctx, wl, vol, cams, 10, 300)
+    RW, RH = wl.render
+    scene = A.Scene(ctx, RW, RH)
+    pivot = A.geometry.frame_from_xyzypr(list(vol.desc.trans_center) + [0, 0, 0])
+    scene.add_volume(vol, synth.DRR_STEP, synth.DRR_INTENSITY, synth.DRR_CUTOFF, pivot)
+    imgs = [A.Image(ctx, t[1]) for t in targets[0]]
+    for i, cam in enumerate(cams):
+        c = scene.add_camera(cam, imgs[i])
+        scene.set_filters(c, 0, wl.drr_filters)
+        scene.set_filters(c, 1, wl.rad_filters)
+    rng = np.random.default_rng(synth.SEED_START + 7)
+    batches = [wl.gt_poses[0] + np.concatenate([rng.uniform(-3, 3, (100, 3)), rng.uniform(-3, 3, (100, 3))], 1)
+               for _ in range(args.warmup + args.steps)]
+    ev_s, ms, kern, _ = time_fused(ctx, scene, batches, np.zeros(100, np.int32), len(cams), args)
+    scene.close()
+    for im in imgs:
+        im.close()
+    vol.close()
+    loop = frame_loop(A, synth, ctx, "small", "u16", 10, CFG1_TRACK, 1, restart=(2.0, 2.0), wl=wl, targets=targets)
+    out = {"workload": "cfg1: 128x128x128 u16 ellipsoid @0.5mm, 2 cams 512x512",
+           "fused": {"value": ev_s, "unit": "pose-evals/s", "ms_per_step": ms, "poses_per_step": 100,
+                     "kernel_ms": {"drr_tiles": kern[0] / args.steps, "post_tiles": kern[1] / args.steps}},
+           "simplex_frames": loop}
+    if not args.no_cpu:
+        prep = O.volume_prep(wl.volume, (wl.voxel_mm,) * 3)
+        recs = [(F.to_records(wl.drr_filters), F.to_records(wl.rad_filters)) for _ in cams]
+        S = O.Scene(prep, (synth.DRR_STEP, synth.DRR_INTENSITY, synth.DRR_CUTOFF), [c.to_array() for c in cams],
+                    [t[1] for t in targets[0]], recs, wl.render)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < args.cpu_seconds / 2:
+            S.track_frame(batches[0][n % 100])
+            n += 1
+        ev_cpu = n / (time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        rng = np.random.default_rng(synth.SEED_START)
+        start = wl.gt_poses[0] + np.concatenate([rng.uniform(-2, 2, 3), rng.uniform(-2, 2, 3)])
+        s_c, st_c, n_c = S.optimize_frame(A.geometry.pose_store_set(start), pivot, method=1)
+        el = time.perf_counter() - t0
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {"value": ev_cpu, "unit": "pose-evals/s", "cores": threads, "kind": "port",
+                               "sample": f"{n} evaluations (float-faithful oracle trackFrame)",
+                               "simplex_frame_s": el, "simplex_frame_evals": n_c,
+                               "simplex_frame_gap_to_hidden": pose_gap(A, O.pose_store_get(s_c), wl.gt_poses[0])}
+    return out
+
+
+def gather_probe(ctx, scene, batches, bone_of, ncams, reps=5):
+    """The ray march's launch with its arithmetic replaced by the bare memory
+    stream (ASC_TUNE_GATHER_PROBE): the same tiles, rays, sample counts and
+    16-byte cell loads. -> (ms per launch, ray samples per launch)."""
+    P = len(bone_of)
+    scene.use_graph(False)
+    scene.set_timing(True)
+    scene.set_tuning(scene.TUNE_GATHER_PROBE, 1)
+    ms, samples = 0.0, 0
+    try:
+        for k in range(reps + 1):
+            scene.eval_poses(batches[k % len(batches)], volumes=bone_of)
+            if k:
+                ms += float(scene.last_timing()[0])
+                samples += scene.last_info(P * ncams)[1]
+    finally:
+        scene.set_tuning(scene.TUNE_GATHER_PROBE, 0)
+    return ms / reps, samples / reps
+
+
+VALU_ISSUE_PER_S = 256 * 2.4e9  # VALU issue cycles per second over the chip: 256 CUs at the 2.4 GHz peak clock
+
+
+def roofline(drr_ms, samples, bpv, probe, prof, vol_bytes, pixels):
+    """Which roof binds drr_tiles, per launch:
+      gather  16 B x samples (8 texels x 2 B) at BW_gather, the bandwidth the
+              gather probe achieved on the same access stream;
+      valu    the VALU issue cycles the kernel needs (rocprof
+              SQ_ACTIVE_INST_VALU - SQ_ACTIVE_INST_VALU2 per wave-sample, from
+              profiles/, times the launch's wave-samples) at one issue cycle
+              per CU per clock;
+      hbm     compulsory bytes (the volume once, 4 B per bbox pixel) at 8 TB/s.
+    frac = the largest roof time / the measured time."""
+    S = samples
+    Bg = 8 * bpv * S
+    t_meas = drr_ms * 1e-3
+    probe_ms, probe_samples = probe
+    bw_gather = 8 * bpv * probe_samples / (probe_ms * 1e-3)
+    t = {"gather": Bg / bw_gather, "hbm": (vol_bytes + 4 * pixels) / 8.0e12}
+    valu = None
+    if prof and "valu_issue_cycles_per_wave_sample" in prof:
+        cyc = prof["valu_issue_cycles_per_wave_sample"] * S / 64
+        t["valu"] = cyc / VALU_ISSUE_PER_S
+        valu = {"issue_cycles_per_wave_sample": prof["valu_issue_cycles_per_wave_sample"],
+                "instr_per_wave_sample": prof.get("valu_instr_per_wave_sample"),
+                "achieved_G_issue_cycles_per_s": cyc / t_meas / 1e9, "peak_G_issue_cycles_per_s": VALU_ISSUE_PER_S / 1e9,
+                "source": prof.get("source")}
+    bound = max(t, key=t.get)
+    out = {"bound": bound, "frac": t[bound] / t_meas, "t_roof_ms": {k: 1e3 * v for k, v in t.items()},
+           "measured_ms": drr_ms, "kernel": "drr_tiles", "ray_samples_per_launch": S,
+           "algorithmic_bytes_per_launch": Bg,
+           "gather": {"bw_gather_GBps": bw_gather / 1e9, "probe_ms": probe_ms, "achieved_GBps": Bg / t_meas / 1e9,
+                      "frac": Bg / t_meas / bw_gather},
+           "valu": valu}
+    if bound == "gather":
+        out.update(achieved=Bg / t_meas / 1e9, peak=bw_gather / 1e9, unit="GB/s")
+    elif bound == "valu":
+        out.update(achieved=valu["achieved_G_issue_cycles_per_s"], peak=valu["peak_G_issue_cycles_per_s"],
+                   unit="G VALU issue-cycles/s")
+    else:
+        out.update(achieved=(vol_bytes + 4 * pixels) / t_meas / 1e9, peak=8000.0, unit="GB/s")
+    traffic = prof.get("traffic_bytes_per_launch") if prof else None
+    out["traffic"] = traffic
+    out["hbm"] = {"compulsory_bytes": vol_bytes + 4 * pixels, "measured_traffic_bytes": traffic,
+                  "achieved_GBps": traffic / t_meas / 1e9 if traffic else None,
+                  "frac_of_8TBps": traffic / t_meas / 8.0e12 if traffic else None}
+    return out
 
 
 def workload_name(args, wl, cams, P, multi) -> str:
@@ -379,8 +649,9 @@ def main(argv=None) -> int:
                     help="pose evaluations timed through the C++ facade's trackFrame (0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serve-seconds", type=float, default=6.0,
-                    help="serving stream: seconds of Poisson job arrivals per rate (0 = skip)")
+    ap.add_argument("--serve-jobs", type=int, default=3,
+                    help="serving stream: jobs per arrival rate (0 = skip the stream)")
+    ap.add_argument("--serve-frames", type=int, default=200, help="serving stream: frames per job, 20 to this")
     ap.add_argument("--serve-only", action="store_true", help="only the serving stream")
     ap.add_argument("--autotune", type=int, default=1,
                     help="choose the ray march's launch shape with asc_scene_autotune before timing (0 = defaults)")
@@ -388,6 +659,14 @@ def main(argv=None) -> int:
                     help="also time the F16-volume variant of a u16 workload (0 = skip)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run the device copy kernel over MIB MiB 5 times (counter calibration)")
+    ap.add_argument("--frames", type=int, default=3,
+                    help="cfg2 frame loop (PSO 100 x 100 + simplex per frame): frames per storage (0 = skip)")
+    ap.add_argument("--fp64-frames", type=int, default=1,
+                    help="frames of the u16 frame loop checked against the oracle's all-FP64 simplex polish")
+    ap.add_argument("--cfg1", type=int, default=1, help="also run configs[0] (128^3, 2 x 512^2) on GPU and CPU")
+    ap.add_argument("--track-only", default="", metavar="STORAGES",
+                    help="only the frame loop, for these comma-separated storages (with --seeds)")
+    ap.add_argument("--seeds", default="1", help="PSO seeds of the --track-only frame loops, comma-separated")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -408,10 +687,28 @@ def main(argv=None) -> int:
         ctx.synchronize()
         log(f"[bench] copied {nbytes} bytes x 5")
         return 0
+    if args.track_only:
+        wl = synth.workload(args.workload, frames=args.frames)
+        vol = A.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3)
+        cams = [A.geometry.camera_from_mayacam2(*c.mayacam2()) for c in wl.cameras]
+        targets = frame_targets(A, synth, ctx, wl, vol, cams, args.frames, 900)
+        vol.close()
+        res = {}
+        for st in args.track_only.split(","):
+            for seed in [int(x) for x in args.seeds.split(",")]:
+                log(f"[track] {st} seed {seed}: {args.frames} frames")
+                r = frame_loop(A, synth, ctx, args.workload, st, args.frames, CFG2_TRACK, seed,
+                               fp64_frames=args.fp64_frames, wl=wl, targets=targets)
+                log(f"[track] {st} seed {seed}: {r['frames_per_s']:.2f} frames/s, max gap {r['max_gap_to_hidden']}")
+                res[f"{st}/seed{seed}"] = r
+        print(json.dumps({"frame_loops": res, "workload": args.workload, "options": CFG2_TRACK}, default=_jsonable),
+              flush=True)
+        ctx.close()
+        return 0
     if args.serve_only:
         trials = serve_trials(A, synth, ctx)
-        print(json.dumps({"serving": serve_bench(A, synth, ctx, trials, SERVE_RATES, args.serve_seconds)}),
-              flush=True)
+        print(json.dumps({"serving": serve_bench(A, synth, ctx, trials, SERVE_LOAD, args.serve_jobs,
+                                                 args.serve_frames)}, default=_jsonable), flush=True)
         ctx.close()
         return 0
     multi = args.workload in MULTI
@@ -453,10 +750,10 @@ def main(argv=None) -> int:
     info, last_samples = scene.last_info(P * len(cams))
     drr_ms = kern_ms[0] / args.steps
     bpv = BYTES_PER_VOXEL[args.storage]
-    bytes_per_launch = 8 * bpv * samples / args.steps  # 8 texels per trilinear sample
-    achieved = bytes_per_launch / (drr_ms * 1e-3)
-    peak = 8.0e12
     log(f"[bench] {evals_s:.1f} evals/s, {ms_step:.2f} ms/step, kernels ms {kern_ms / args.steps}")
+    probe = gather_probe(ctx, scene, batches, bone_of, len(cams)) if bpv == 2 else None
+    if probe:
+        log(f"[bench] gather probe: {probe[0]:.3f} ms per launch, {16 * probe[1] / probe[0] / 1e9:.1f} TB/s")
 
     # plain one-eval-at-a-time form
     t0 = time.perf_counter()
@@ -477,24 +774,23 @@ def main(argv=None) -> int:
                "sample": f"{n} pose evaluations of the same workload (float-faithful oracle trackFrame, "
                          f"OpenMP over rows, {el_c:.1f}s)"}
 
-    # HBM traffic per launch of the ray-march kernel, from the committed
-    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (calibrated with a copy kernel)
-    traffic, valu = None, None
-    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "drr_tiles_cfg2_traffic.json")
-    if args.workload == "cfg2" and args.storage == "u16" and P == 100 and os.path.exists(prof):
+    # per-launch counters of the ray-march kernel from the committed rocprofv3
+    # runs: calibrated HBM traffic (FETCH_SIZE / WRITE_SIZE) and VALU issue
+    # cycles per wave-sample (SQ_ACTIVE_INST_VALU - SQ_ACTIVE_INST_VALU2)
+    pj = None
+    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"drr_tiles_{args.workload}_{args.storage}_counters.json")
+    if P == 100 and os.path.exists(prof):
         with open(prof) as fh:
             pj = json.load(fh)
-        traffic = pj["traffic_bytes_per_launch"]
-        # the issue-bound view: VALU lane-instructions of the launch at one
-        # instruction per lane per cycle on every SIMD lane (256 CUs x 64
-        # lanes x 2.4 GHz); gfx950 also dual-issues part of the time, which
-        # the measured busy fraction accounts for
-        ipws = pj["valu_instr_per_wave_sample"]
-        t_valu = ipws * (samples / args.steps) / (256 * 64 * 2.4e9)
-        valu = {"instr_per_wave_sample": ipws, "single_issue_ms": 1e3 * t_valu,
-                "frac_single_issue": 1e3 * t_valu / drr_ms, "busy_frac": pj.get("valu_busy_frac"),
-                "ta_busy_frac": pj.get("ta_busy_frac"), "source": "profiles/drr_tiles_cfg2_traffic.json"}
+        pj["source"] = os.path.relpath(prof, os.path.dirname(os.path.abspath(__file__)))
     tiles = info[:, 4].sum()
+    pixels = int((info[:, 1] * info[:, 2]).sum())
+    vol_bytes = sum(int(np.prod(v.desc.dims)) * bpv for v in vols)
+    roof = None
+    if probe:
+        roof = roofline(drr_ms, samples / args.steps, bpv, probe, pj, vol_bytes, pixels)
+        log(f"[bench] roofline: bound {roof['bound']}, frac {roof['frac']:.3f}, roofs ms {roof['t_roof_ms']}")
     out = {
         "metric": "pose evals/sec (DRR+filters+NCC, 2 cams 1024\u00b2); served frames/s + p99 latency",
         "value": evals_s,
@@ -518,20 +814,9 @@ def main(argv=None) -> int:
             "graph": bool(args.graph),
             "launch_shape": tuning,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved / 1e9,
-            "peak": peak / 1e9,
-            "unit": "GB/s",
-            "frac": achieved / peak,
-            "traffic": traffic,
-            "kernel": "drr_tiles",
-            "algorithmic_bytes_per_launch": bytes_per_launch,
-            "ray_samples_per_launch": samples / args.steps,
-            "kernel_ms": {"drr_tiles": drr_ms, "post_tiles": kern_ms[1] / args.steps,
-                          "finalize_items": kern_ms[2] / args.steps},
-            "valu": valu,
-        },
+        "roofline": roof,
+        "kernel_ms": {"drr_tiles": drr_ms, "post_tiles": kern_ms[1] / args.steps,
+                      "finalize_items": kern_ms[2] / args.steps},
         "cpu_baseline": cpu,
         "graph_replay": graph_replay,
         "plain_per_eval": {"value": plain_s, "unit": "pose-evals/s",
@@ -555,20 +840,38 @@ def main(argv=None) -> int:
             out["plain_facade"]["speedup_of_fused_f16"] = f16["value"] / facade["reference_form"]
             if f16["graph_replay_value"]:
                 out["plain_facade"]["speedup_of_fused_f16_graph"] = f16["graph_replay_value"] / facade["reference_form"]
-    if args.serve_seconds > 0:
+    if args.frames > 0 and not multi:
+        # Tracker's frame loop at this workload: PSO 100 particles x 100
+        # epochs (every epoch runs: stall limit above the epoch count), then
+        # the simplex, each frame started from the previous frame's result
+        wl_f = synth.workload(args.workload, frames=args.frames)
+        targets = frame_targets(A, synth, ctx, wl_f, vol, cams, args.frames, 900)
+        out["frame_loop"] = {"options": CFG2_TRACK, "guess": "previous frame's result (frame 0: 2 mm / 2 deg off)"}
+        for st in ("u16", "f16"):
+            log(f"[bench] frame loop, {st} volume, {args.frames} frames")
+            out["frame_loop"][st] = frame_loop(A, synth, ctx, args.workload, st, args.frames, CFG2_TRACK, 1,
+                                               fp64_frames=args.fp64_frames if st == "u16" else 0,
+                                               wl=wl_f, targets=targets)
+            log(f"[bench] {st}: {out['frame_loop'][st]['frames_per_s']:.2f} frames/s, "
+                f"max gap to hidden {out['frame_loop'][st]['max_gap_to_hidden']}")
+    if args.cfg1 and not multi:
+        log("[bench] cfg1 leg")
+        out["cfg1"] = cfg1_leg(A, synth, ctx, args)
+    if args.serve_jobs > 0:
         scene.close()
         for v in vols:
             v.close()
         vols = []
         trials = serve_trials(A, synth, ctx)
         out["serving"] = {
-            "workload": "cfg5-style stream: Poisson job arrivals; jobs of 1-4 cameras (512^2-1024^2), 1-3 bones "
-                        "(64^3 .. 256x256x320 u16), 4-8 frames, PSO 50-200 particles (synchronous, <= 7 epochs) "
-                        "+ simplex per frame; targets rendered from hidden random-walk poses",
+            "workload": f"cfg5 stream: Poisson job arrivals at 0.5/0.8/1.0 x measured capacity, {args.serve_jobs} "
+                        f"jobs per rate; jobs of 1-4 cameras (512^2-1024^2), 1-3 bones (64^3 .. 256x256x400 u16), "
+                        f"20-{args.serve_frames} frames, PSO 50-200 particles (synchronous, <= 7 epochs) + simplex "
+                        "per frame; targets rendered from hidden random-walk poses",
             "per_frame_latency": "frame start (job arrival or previous frame done) to frame done",
-            "rates": serve_bench(A, synth, ctx, trials, SERVE_RATES, args.serve_seconds),
+            **serve_bench(A, synth, ctx, trials, SERVE_LOAD, args.serve_jobs, args.serve_frames),
         }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(out, default=_jsonable), flush=True)
     scene.close()
     for v in vols:
         v.close()

@@ -331,8 +331,14 @@ ASC_API int asc_scene_set_debug(asc_scene scene, int enable);
  * occupancy);
  * ASC_TUNE_WAVE_PATCH: width of the ray patch one wave marches inside a
  * 16 x 16 tile: 8 (8 x 8, default), 16 (16 x 4) or 4 (4 x 16).
- * None of them changes a result. */
-enum { ASC_TUNE_UNROLL = 0, ASC_TUNE_GRID = 1, ASC_TUNE_WAVE_PATCH = 2 };
+ * None of them changes a result.
+ * ASC_TUNE_GATHER_PROBE (no counterpart in the reference; a measurement
+ * tool): 1 replaces the ray march by its memory access stream alone (the
+ * same rays, sample counts and 16-byte cell loads, with as little
+ * arithmetic as the addresses need), so that the kernel's duration is the
+ * gather bandwidth the memory hierarchy sustains for this access pattern;
+ * results are meaningless while it is set (16-bit OCTET/BRICK volumes). */
+enum { ASC_TUNE_UNROLL = 0, ASC_TUNE_GRID = 1, ASC_TUNE_WAVE_PATCH = 2, ASC_TUNE_GATHER_PROBE = 3 };
 ASC_API int asc_scene_set_tuning(asc_scene scene, int key, int value);
 
 /* Autotuner: times every combination of unroll {2, 4}, wave patch {8, 16,

@@ -1,0 +1,143 @@
+"""The benchmark configurations that the other GPU tests do not reach, at
+their stated shapes, through the C ABI:
+
+* cfg1 (128^3 u16 ellipsoid at 0.5 mm, two 512^2 cameras): a fused batch
+  against the oracle's trackFrame (1 - NCC within 1e-5; in-view flags,
+  render sizes, mask counts and ray-sample counts exact), and the
+  downhill-simplex frame of the configuration (started 2 mm / 2 deg off the
+  hidden pose) against the oracle's simplex from the same start: final
+  poses within 0.05 mm / 0.05 deg.
+* cfg5 shapes in the registration server: a 256x256x400 u16 volume tracked
+  in four 1024^2 cameras, alone and in a mix with a 512^2 one-camera job
+  and a 2-camera 768^2 job; its tracked poses are the same bits in both.
+"""
+import numpy as np
+import pytest
+
+from autoscoper_amd import synth
+
+from test_gpu_optimize import pose_gap
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cfg1(asc, ctx, orc):
+    wl = synth.workload("small", frames=1)
+    prep = orc.volume_prep(wl.volume, (wl.voxel_mm,) * 3)
+    cams = [orc.camera_from_mayacam2(*c.mayacam2()) for c in wl.cameras]
+    RW, RH = wl.render
+    rads = []
+    for i, cam in enumerate(cams):
+        _, imv, _, _ = orc.view_geometry(cam, wl.gt_poses[0], prep["inv_scale"], prep["inv_trans"], RW, RH)
+        img, _ = orc.drr(prep["volume"], imv[:12], synth.full_frame_viewport(cam[24:28], RW, RH), synth.DRR_STEP,
+                         synth.DRR_INTENSITY, 0.0, RW, RH)
+        rads.append(synth.target_from_drr(img, 300 + i)[1])
+    vol = asc.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3)
+    sc = asc.Scene(ctx, RW, RH)
+    pivot = asc.geometry.frame_from_xyzypr(list(vol.desc.trans_center) + [0, 0, 0])
+    sc.add_volume(vol, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, pivot)
+    for i, cam in enumerate(cams):
+        c = sc.add_camera(asc.AscCamera.from_array(cam), asc.Image(ctx, rads[i]))
+        sc.set_filters(c, 0, wl.drr_filters)
+        sc.set_filters(c, 1, wl.rad_filters)
+    recs = [(asc.filters.to_records(wl.drr_filters), asc.filters.to_records(wl.rad_filters)) for _ in cams]
+    S = orc.Scene(prep, (synth.DRR_STEP, synth.DRR_INTENSITY, 0.0), cams, rads, recs, (RW, RH))
+    yield dict(wl=wl, sc=sc, S=S, pivot=pivot, assert_pivot=orc.frame_from_xyzypr(
+        list(prep["trans_center"]) + [0, 0, 0]))
+    sc.close()
+    vol.close()
+
+
+def test_cfg1_batch_matches_oracle(cfg1):
+    sc, S, wl = cfg1["sc"], cfg1["S"], cfg1["wl"]
+    rng = np.random.default_rng(41)
+    poses = wl.gt_poses[0] + np.concatenate([rng.uniform(-3, 3, (6, 3)), rng.uniform(-3, 3, (6, 3))], 1)
+    costs, per = sc.eval_poses(poses)
+    info, samples = sc.last_info(len(poses) * 2)
+    tot = 0
+    for i, p in enumerate(poses):
+        r = S.track_frame(p)
+        for c in range(2):
+            assert info[2 * i + c, 0] == r["info"][c, 0] == 1
+            assert tuple(info[2 * i + c, 1:4]) == tuple(r["info"][c, 1:4])  # w, h, mask count
+            assert abs(per[i, c] - r["corr"][c]) <= 1e-5
+        assert abs(costs[i] - r["corr"].sum()) <= 2e-5
+        tot += r["samples"]
+    assert samples == tot
+    assert info[:, 1].min() > 100  # boxes of cfg1 size
+
+
+def test_cfg1_simplex_frame_matches_oracle(asc, orc, cfg1):
+    sc, S, wl = cfg1["sc"], cfg1["S"], cfg1["wl"]
+    assert np.array_equal(cfg1["pivot"], cfg1["assert_pivot"])
+    rng = np.random.default_rng(synth.SEED_START)
+    start = wl.gt_poses[0] + np.concatenate([rng.uniform(-2, 2, 3), rng.uniform(-2, 2, 3)])
+    store0 = asc.geometry.pose_store_set(start)
+    s_gpu, st = sc.optimize_frame(store0, method=1)
+    s_orc, ost, n = S.optimize_frame(store0, cfg1["pivot"], method=1)
+    dt, dr = pose_gap(orc, asc.geometry.pose_store_get(s_gpu), orc.pose_store_get(s_orc))
+    print(f"cfg1 simplex: gpu {st.evaluations} evals, final {st.final_cost:.6f}; oracle {n} evals, final "
+          f"{ost[3]:.6f}; gap {dt:.2e} mm {dr:.2e} deg")
+    assert dt <= 0.05 and dr <= 0.05
+    assert abs(st.final_cost - ost[3]) <= 1e-4
+
+
+def _targets(asc, ctx, vol, cams, poses, size, seed):
+    """Full-frame DRRs at `poses` (frames) rendered on the device, made into
+    u16 targets."""
+    W, H = size
+    out = ctx.alloc(4 * W * H)
+    frames = []
+    for f, p in enumerate(poses):
+        per = []
+        for i, cam in enumerate(cams):
+            inview, imv, _, _ = asc.geometry.view_geometry(cam, p, vol.desc, W, H)
+            assert inview
+            fvp = synth.full_frame_viewport(list(cam.viewport), W, H)
+            ctx.drr_render(vol, imv, fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, out, W, H)
+            per.append(synth.target_from_drr(out.download((H, W)), seed + 10 * f + i)[1])
+        frames.append(per)
+    out.free()
+    return frames
+
+
+def _job(asc, ctx, shape, vox, ncam, det, seed):
+    vol_data = synth.phantom(shape, vox, "longbone" if shape[0] > 128 else "ellipsoid", seed=seed,
+                             noise_seed=seed + 1)
+    specs = synth.biplane(sod_mm=800.0, size=(det, det), pixel_mm=307.2 / det, sid_mm=1100.0, n=ncam)
+    cams = [asc.geometry.camera_from_mayacam2(*c.mayacam2()) for c in specs]
+    gts = synth.random_walk(synth.volume_centre_pose(shape, vox), 2, 0.5, 1.0, seed + 2)
+    v = asc.Volume(ctx, vol_data, scale=(vox,) * 3)
+    frames = _targets(asc, ctx, v, cams, gts, (det, det), seed + 3)
+    v.close()
+    start = gts[0] + np.array([1.0, -0.8, 0.6, 1.2, -1.0, 0.8])
+    return dict(volumes=[vol_data], voxel_mm=[vox], initial_poses=start[None], cameras=cams, frames=frames,
+                render=(det, det))
+
+
+def _submit(asc, srv, job, seed):
+    dflt, rflt = synth.default_filters()
+    return srv.submit(job["volumes"], job["voxel_mm"], job["initial_poses"], job["cameras"], job["frames"],
+                      job["render"], drr_filters=dflt, rad_filters=rflt,
+                      drr=(synth.DRR_STEP, synth.DRR_INTENSITY, 0.0), guess=1, method=0, max_iter=2, max_stall=3,
+                      seed=seed, particles=50, synchronous=True, speculative=True)
+
+
+def test_cfg5_large_job_alone_equals_in_the_mix(asc, ctx):
+    big = _job(asc, ctx, (400, 256, 256), 0.3, 4, 1024, 2101)
+    small = _job(asc, ctx, (64, 64, 64), 0.5, 1, 512, 2201)
+    mid = _job(asc, ctx, (128, 128, 160), 0.4, 2, 768, 2301)
+    srv = asc.Server(ctx, 64 << 30, 8)
+    alone, lat, st = srv.wait(_submit(asc, srv, big, 5))
+    srv.close()
+    assert st.frames == 2 and (lat > 0).all()
+    srv = asc.Server(ctx, 64 << 30, 8)
+    ids = [_submit(asc, srv, small, 6), _submit(asc, srv, big, 5), _submit(asc, srv, mid, 7)]
+    res = [srv.wait(i) for i in ids]
+    stats = srv.stats()
+    srv.close()
+    print("cfg5 mix:", {k: stats[k] for k in ("batches", "requests", "evaluations", "latency_p50_ms",
+                                              "latency_p99_ms")})
+    assert np.array_equal(res[1][0], alone)
+    assert stats["requests"] > stats["batches"]  # requests of different jobs shared launches
