@@ -252,7 +252,7 @@ void Tracker::load(const std::vector<VolumeSpec>& volumes, std::vector<Camera>& 
   // Tracker.cpp:281-290: describe each volume and centre its pivot
   for (const VolumeSpec& vs : volumes) {
     volumeDescription_.emplace_back(new gpu::VolumeDescription(*vs.volume, vs.storage));
-    if (vs.layout >= 0) check(asc_volume_set_layout(volumeDescription_.back()->image(), vs.layout));
+    if (vs.layout >= 0) check(asc_volume_set_layout(volumeDescription_.back()->image()->volume(), vs.layout));
     CoordFrame pivot = vs.pivot;
     pivot.translate(volumeDescription_.back()->transCenter());
     pivots_.push_back(pivot);
@@ -287,7 +287,7 @@ void Tracker::load(const std::vector<VolumeSpec>& volumes, std::vector<Camera>& 
     double pv[12];
     std::memcpy(pv, pivots_[v].rotation(), 9 * sizeof(double));
     std::memcpy(pv + 9, pivots_[v].translation(), 3 * sizeof(double));
-    check(asc_scene_add_volume(scene_, volumeDescription_[v]->image(), 0.5f, 10.0f, 0.0f, pv, nullptr));
+    check(asc_scene_add_volume(scene_, volumeDescription_[v]->image()->volume(), 0.5f, 10.0f, 0.0f, pv, nullptr));
   }
   for (size_t i = 0; i < cameras_.size(); i++) {
     asc_image im = nullptr;
@@ -311,7 +311,8 @@ bool Tracker::calculate_viewport(const CoordFrame& modelview, const Camera& came
   double mv[12];
   std::memcpy(mv, modelview.rotation(), 9 * sizeof(double));
   std::memcpy(mv + 9, modelview.translation(), 3 * sizeof(double));
-  const int rc = asc_calculate_viewport(mv, &camera.record(), &volumeDescription_[current_volume_]->desc(), viewport);
+  const int rc =
+    asc_calculate_viewport(mv, &camera.record(), &volumeDescription_[current_volume_]->image()->desc(), viewport);
   check(rc);
   return rc == 1;
 }
@@ -330,8 +331,8 @@ std::vector<double> Tracker::trackFrame(unsigned volumeID, const double* xyzypr)
     double mv[12];
     std::memcpy(mv, modelview.rotation(), 9 * sizeof(double));
     std::memcpy(mv + 9, modelview.translation(), 3 * sizeof(double));
-    const int inview = asc_calculate_viewport(mv, &view->camera()->record(), &volumeDescription_[volumeID]->desc(),
-                                              viewport);
+    const int inview = asc_calculate_viewport(mv, &view->camera()->record(),
+                                              &volumeDescription_[volumeID]->image()->desc(), viewport);
     check(inview);
     if (!inview) {
       correlations.push_back(9999.0);

@@ -124,19 +124,50 @@ void Buffer::fill(const char c) const { check(asc_fill_u8(context(), ptr_, (unsi
 
 void Buffer::fill(const float v) const { check(asc_fill_f32(context(), (float*)ptr_, v, size_ / sizeof(float))); }
 
-// ---- VolumeDescription -----------------------------------------------------
-VolumeDescription::VolumeDescription(const Volume& v, int storage) : vol_(nullptr)
-{
-  const int flip[3] = {v.flipX(), v.flipY(), v.flipZ()};
-  const double scale[3] = {v.scaleX(), v.scaleY(), v.scaleZ()};
-  check(asc_volume_create(context(), v.data(), (int)v.width(), (int)v.height(), (int)v.depth(), v.bps(), flip, scale,
-                          storage, &vol_, &desc_));
-}
-
-VolumeDescription::~VolumeDescription()
+// ---- Image, VolumeDescription (VolumeDescription.cpp:59-305) ----------------
+Image::~Image()
 {
   if (vol_) asc_volume_destroy(vol_);
 }
+
+namespace {
+// flip, crop, normalise and upload (asc_volume_create), then the
+// description's fields as the reference's constructor sets them
+Image* make_image(const Volume& v, int storage, float& minV, float& maxV, float* invScale, float* invTrans,
+                  double* transCenter)
+{
+  const int flip[3] = {v.flipX(), v.flipY(), v.flipZ()};
+  const double scale[3] = {v.scaleX(), v.scaleY(), v.scaleZ()};
+  asc_volume vol = nullptr;
+  asc_volume_desc d;
+  check(asc_volume_create(context(), v.data(), (int)v.width(), (int)v.height(), (int)v.depth(), v.bps(), flip, scale,
+                          storage, &vol, &d));
+  minV = d.min_value;
+  maxV = d.max_value;
+  for (int i = 0; i < 3; i++) {
+    invScale[i] = d.inv_scale[i];
+    invTrans[i] = d.inv_trans[i];
+    transCenter[i] = d.trans_center[i];
+  }
+  return new Image(vol, d);
+}
+}  // namespace
+
+#ifdef AUTOSCOPER_XROMM_HOST_TYPES
+// the library's class (VolumeDescription.hpp:62-87): u16 storage, the
+// reference's UNORM16
+VolumeDescription::VolumeDescription(const Volume& v)
+  : image_(make_image(v, ASC_STORAGE_U16, minValue_, maxValue_, invScale_, invTrans_, transCenter_))
+{
+}
+#else
+VolumeDescription::VolumeDescription(const Volume& v, int storage)
+  : image_(make_image(v, storage, minValue_, maxValue_, invScale_, invTrans_, transCenter_))
+{
+}
+#endif
+
+VolumeDescription::~VolumeDescription() { delete image_; }
 
 // ---- RayCaster (RayCaster.cpp:60-193) ---------------------------------------
 RayCaster::RayCaster()
@@ -176,8 +207,8 @@ void RayCaster::render(const Buffer* buffer, unsigned width, unsigned height)
     buffer->fill((char)0x00);
     return;
   }
-  check(asc_drr_render(context(), volumeDescription_->image(), imv_, viewport_, sampleDistance_, rayIntensity_, cutoff_,
-                       buffer->get(), width, height));
+  check(asc_drr_render(context(), volumeDescription_->image()->volume(), imv_, viewport_, sampleDistance_,
+                       rayIntensity_, cutoff_, buffer->get(), width, height));
 }
 
 float RayCaster::getMinCutoff() const { return volumeDescription_ ? volumeDescription_->minValue() : 0.0f; }

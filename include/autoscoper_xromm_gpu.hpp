@@ -23,68 +23,16 @@
 // std::runtime_error) here.
 #pragma once
 
-#include <stddef.h>
-
-#include <stdexcept>
-#include <string>
-#include <utility>
-#include <vector>
-
-#include "autoscoper_hip.h"
-
-#define XROMM_API __attribute__((visibility("default")))
-
-namespace xromm {
-namespace gpu {
-
-struct XROMM_API Error : std::runtime_error {
-  int code;
-  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-// The process-wide context of the backend (the reference keeps one global
-// OpenCL context and queue, OpenCL.cpp:83-102). Created on first use on the
-// selected device (0 unless set_device() / setUsedPlatform() chose another).
-XROMM_API asc_context context();
-XROMM_API void set_device(int device);
-
-// Device selection as the reference GUI drives it (OpenCL.hpp:63-66): one
-// platform ("HIP"), whose devices are the visible HIP devices.
-XROMM_API std::vector<std::vector<std::string>> get_platforms();
-XROMM_API void setUsedPlatform(int platform_idx);
-XROMM_API void setUsedPlatform(int platform, int device);
-XROMM_API std::pair<int, int> getUsedPlatform();
-
-// Device buffer (OpenCL.hpp:124-141). `access` is accepted for signature
-// compatibility (cl_mem_flags there) and ignored: device memory is
-// read-write.
-class XROMM_API Buffer {
- public:
-  explicit Buffer(size_t size, int access = 0);
-  ~Buffer();
-  Buffer(const Buffer&) = delete;
-  Buffer& operator=(const Buffer&) = delete;
-  void read(const void* buf, size_t size = 0) const;  // host -> device
-  void write(void* buf, size_t size = 0) const;       // device -> host
-  void copy(const Buffer* dst, size_t size = 0) const;
-  void fill(const char c) const;
-  void fill(const float val) const;
-  float* get() const { return (float*)ptr_; }
-  size_t size() const { return size_; }
-
- private:
-  void* ptr_;
-  size_t size_;
-};
-
-}  // namespace gpu
-}  // namespace xromm
+#include "autoscoper_xromm_gpu_types.hpp"
 
 #ifdef AUTOSCOPER_XROMM_HOST_TYPES
-// libautoscoper's own Volume (TIFF stack) and Filter base; Filter.hpp needs
-// xromm::gpu::Buffer, declared above.
+// libautoscoper's own Volume (TIFF stack), Filter base and VolumeDescription
+// (whose constructor and destructor this backend implements, as the
+// reference's gpu/opencl/VolumeDescription.cpp does); they include
+// autoscoper_xromm_gpu_types.hpp at their backend include sites.
 #include "Filter.hpp"
 #include "Volume.hpp"
+#include "VolumeDescription.hpp"
 #else
 namespace xromm {
 
@@ -140,6 +88,31 @@ class XROMM_API Filter {
   bool enabled_;
 };
 
+// VolumeDescription (VolumeDescription.hpp:62-87), with the library's
+// members; `storage` selects the device copy (ASC_STORAGE_U16 is the
+// reference's UNORM16, exact).
+class XROMM_API VolumeDescription {
+ public:
+  explicit VolumeDescription(const Volume& volume, int storage = ASC_STORAGE_U16);
+  ~VolumeDescription();
+  VolumeDescription(const VolumeDescription&) = delete;
+  VolumeDescription& operator=(const VolumeDescription&) = delete;
+  const float* invScale() const { return invScale_; }
+  const float* invTrans() const { return invTrans_; }
+  const double* transCenter() const { return transCenter_; }
+  float minValue() const { return minValue_; }
+  float maxValue() const { return maxValue_; }
+  const Image* image() const { return image_; }
+
+ private:
+  float minValue_;
+  float maxValue_;
+  float invScale_[3];
+  float invTrans_[3];
+  double transCenter_[3];
+  Image* image_;
+};
+
 }  // namespace gpu
 }  // namespace xromm
 #endif
@@ -147,26 +120,6 @@ class XROMM_API Filter {
 namespace xromm {
 namespace gpu {
 
-// VolumeDescription (VolumeDescription.hpp:62-87). `storage` selects the
-// device copy (ASC_STORAGE_U16 is the reference's UNORM16, exact).
-class XROMM_API VolumeDescription {
- public:
-  explicit VolumeDescription(const Volume& volume, int storage = ASC_STORAGE_U16);
-  ~VolumeDescription();
-  VolumeDescription(const VolumeDescription&) = delete;
-  VolumeDescription& operator=(const VolumeDescription&) = delete;
-  const float* invScale() const { return desc_.inv_scale; }
-  const float* invTrans() const { return desc_.inv_trans; }
-  const double* transCenter() const { return desc_.trans_center; }
-  float minValue() const { return desc_.min_value; }
-  float maxValue() const { return desc_.max_value; }
-  asc_volume image() const { return vol_; }
-  const asc_volume_desc& desc() const { return desc_; }
-
- private:
-  asc_volume vol_;
-  asc_volume_desc desc_;
-};
 
 // RayCaster (RayCaster.hpp:53-96).
 class XROMM_API RayCaster {
