@@ -235,6 +235,11 @@ _SIGS = {
     "asc_server_submit": [_vp, _P(AscJobDesc), _P(_i)],
     "asc_server_wait": [_vp, _i, _P(_d), _P(_d), _P(AscJobStats)],
     "asc_server_stats": [_vp, _P(AscServeStats)],
+    "asc_server_lock_scene": [_vp, _P(_vp)],
+    "asc_server_unlock_scene": [_vp],
+    "asc_server_eval_requests": [_vp, _i, _P(_i), _P(_i), _P(_i), _P(_d), _P(_d), _P(_d)],
+    "asc_scene_scratch_bytes": [_vp, _P(C.c_size_t)],
+    "asc_remote_create_shared": [_vp, _vp, _P(_vp)],
     "asc_server_destroy": [_vp],
     "asc_scene_autotune": [_vp, _i, _P(_i), _P(_d), _i, _P(AscTuneResult)],
     "asc_remote_create": [_vp, _P(_vp)],
@@ -742,11 +747,17 @@ class Remote:
     """A session answering the reference's socket protocol (version 2):
     handle() one message directly, or listen() on 127.0.0.1."""
 
-    def __init__(self, ctx: "Context | None" = None):
+    def __init__(self, ctx: "Context | None" = None, server: "Server | None" = None):
+        """server: answer costs and optimisations through this registration
+        server's batcher (launches shared with its other clients and jobs)."""
         h = _vp()
-        _check(lib().asc_remote_create(ctx.handle if ctx else None, C.byref(h)))
+        if server is not None:
+            _check(lib().asc_remote_create_shared(ctx.handle, server.handle, C.byref(h)))
+        else:
+            _check(lib().asc_remote_create(ctx.handle if ctx else None, C.byref(h)))
         self.h = h
         self._ctx = ctx
+        self._server = server
 
     def handle(self, message: bytes) -> bytes:
         n = C.c_size_t()

@@ -1302,6 +1302,15 @@ ASC_API int asc_scene_eval_requests(asc_scene s, int n, const int* volumes, cons
   return eval_requests(s, reqs, costs, per_camera, kMaxSetCams);
 }
 
+ASC_API int asc_scene_scratch_bytes(asc_scene s, size_t* bytes)
+{
+  ASC_CHECK(s && bytes, ASC_ERR_INVALID, "bad argument");
+  *bytes = s->d_batch.cap + s->d_drr.cap * sizeof(float) + s->d_partials.cap * sizeof(double) +
+           s->d_results.cap * sizeof(double) + s->d_samples.cap * sizeof(unsigned long long) +
+           s->d_dbg.cap * sizeof(float) + s->p_bufs.cap * sizeof(float);
+  return ASC_OK;
+}
+
 ASC_API int asc_scene_eval_manips(asc_scene s, const int* volumes, const float* stores, const double* manips, int n,
                                   double* costs, double* per_camera)
 {

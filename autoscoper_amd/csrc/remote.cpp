@@ -242,6 +242,12 @@ struct Session {
   double bg_threshold = -1.0;
   asc_scene scene = nullptr;
   bool scene_dirty = true;
+  // shared mode: the scene is the registration server's, and evaluations are
+  // requests to its batcher (several sessions share launches). The session
+  // owns only its entries in that scene.
+  asc_server srv = nullptr;
+  std::vector<int> vol_slot, cam_slot;  // session volume / camera -> scene slot
+  int set = -1;                         // the session's camera set (shared mode)
   int current_volume = 0;       // Trial::current_volume, set by optimizeFrame
   std::string full_drr_dir = ".";  // where message 12 writes its images
   int full_drr_count = 0;        // save_full_drr's running file number
@@ -250,8 +256,23 @@ struct Session {
 
   void close_trial()
   {
-    if (scene) asc_scene_destroy(scene);
+    if (srv && set >= 0) {
+      // out of the shared scene before the device copies go (the lock
+      // waits for a launch that may read them)
+      asc_scene sc = nullptr;
+      asc_server_lock_scene(srv, &sc);
+      asc_scene_remove_camera_set(sc, set);
+      for (int c : cam_slot) asc_scene_remove_camera(sc, c);
+      for (int v : vol_slot)
+        if (v >= 0) asc_scene_remove_volume(sc, v);
+      asc_server_unlock_scene(srv);
+    } else if (scene && !srv) {
+      asc_scene_destroy(scene);
+    }
     scene = nullptr;
+    set = -1;
+    vol_slot.clear();
+    cam_slot.clear();
     for (auto& c : cam_state) {
       if (c.rad) asc_image_destroy(c.rad);
       if (c.bg) asc_image_destroy(c.bg);
@@ -304,27 +325,35 @@ int open_trial(Session& S, const std::string& path)
   return ASC_OK;
 }
 
-// Device side of the session, created on the first request that needs it.
-int ensure_scene(Session& S)
+// The session's scene for the length of one operation: its own, or the
+// registration server's under the server's scene lock (shared mode).
+struct SceneUse {
+  Session& S;
+  explicit SceneUse(Session& s) : S(s)
+  {
+    if (S.srv) asc_server_lock_scene(S.srv, &S.scene);
+  }
+  ~SceneUse()
+  {
+    if (S.srv) asc_server_unlock_scene(S.srv);
+  }
+};
+
+// Scene entries and radiographs, with the scene in use.
+int update_scene(Session& S)
 {
-  if (!S.trial) return fail(ASC_ERR_INVALID, "no trial loaded");
-  if (!S.ctx) return fail(ASC_ERR_NO_DEVICE, "the remote session has no device context");
   int rc;
-  if (!S.scene) {
-    for (Bone& b : S.bones) {
-      if (b.bps != 8 && b.bps != 16) return fail(ASC_ERR_UNSUPPORTED, "volumes must be 8 or 16 bit");
-      if ((rc = asc_volume_create(S.ctx, b.data.data(), b.dims[0], b.dims[1], b.dims[2], b.bps, b.flip, b.voxel,
-                                  ASC_STORAGE_U16, &b.dev, &b.desc)))
-        return rc;
-      // Tracker.cpp:281-290: the pivot is the volume centre
-      const double c[6] = {b.desc.trans_center[0], b.desc.trans_center[1], b.desc.trans_center[2], 0, 0, 0};
-      asc_frame_from_xyzypr(c, b.pivot);
-    }
-    if ((rc = asc_scene_create(S.ctx, S.info.render_width, S.info.render_height, &S.scene))) return rc;
+  if (S.cam_slot.empty()) {
     for (size_t c = 0; c < S.cams.size(); c++) {
       int idx;
       if ((rc = asc_scene_add_camera(S.scene, &S.cams[c], nullptr, &idx))) return rc;
+      S.cam_slot.push_back(idx);
     }
+    if (S.srv &&
+        (rc = asc_scene_add_camera_set(S.scene, S.cam_slot.data(), (int)S.cam_slot.size(), S.info.render_width,
+                                       S.info.render_height, &S.set)))
+      return rc;
+    S.vol_slot.assign(S.bones.size(), -1);
     S.scene_dirty = true;
   }
   if (S.scene_dirty) {
@@ -335,15 +364,14 @@ int ensure_scene(Session& S)
       Bone& b = S.bones[v];
       float cutoff;
       asc_drr_cutoff(v0.cutoff_fraction, &b.desc, &cutoff);
-      int idx;
-      asc_scene_remove_volume(S.scene, (int)v);  // a missing slot is fine here
-      if ((rc = asc_scene_add_volume(S.scene, b.dev, v0.drr_step, v0.drr_intensity, cutoff, b.pivot, &idx))) return rc;
-      if (idx != (int)v) return fail(ASC_ERR_INVALID, "volume slot order");
+      if (S.vol_slot[v] >= 0) asc_scene_remove_volume(S.scene, S.vol_slot[v]);
+      if ((rc = asc_scene_add_volume(S.scene, b.dev, v0.drr_step, v0.drr_intensity, cutoff, b.pivot, &S.vol_slot[v])))
+        return rc;
     }
     for (size_t c = 0; c < S.cams.size(); c++) {
       const asc_filter_settings& s = S.cam_state[c].vie;
-      if ((rc = asc_scene_set_filters(S.scene, (int)c, 0, s.drr_filters, s.n_drr_filters))) return rc;
-      if ((rc = asc_scene_set_filters(S.scene, (int)c, 1, s.rad_filters, s.n_rad_filters))) return rc;
+      if ((rc = asc_scene_set_filters(S.scene, S.cam_slot[c], 0, s.drr_filters, s.n_drr_filters))) return rc;
+      if ((rc = asc_scene_set_filters(S.scene, S.cam_slot[c], 1, s.rad_filters, s.n_rad_filters))) return rc;
     }
     S.scene_dirty = false;
   }
@@ -357,7 +385,7 @@ int ensure_scene(Session& S)
     if ((rc = asc_trial_frame_read(S.trial, (int)c, S.frame, px.data(), px.size()))) return rc;
     asc_image img = nullptr;
     if ((rc = asc_image_create(S.ctx, px.data(), dims[0], dims[1], bps, &img))) return rc;
-    if ((rc = asc_scene_set_rad(S.scene, (int)c, img))) {
+    if ((rc = asc_scene_set_rad(S.scene, S.cam_slot[c], img))) {
       asc_image_destroy(img);
       return rc;
     }
@@ -366,6 +394,44 @@ int ensure_scene(Session& S)
     cs.rad_frame = S.frame;
   }
   return ASC_OK;
+}
+
+// Device side of the session, created on the first request that needs it.
+int ensure_scene(Session& S)
+{
+  if (!S.trial) return fail(ASC_ERR_INVALID, "no trial loaded");
+  if (!S.ctx) return fail(ASC_ERR_NO_DEVICE, "the remote session has no device context");
+  int rc;
+  if (S.cam_slot.empty()) {
+    for (Bone& b : S.bones) {
+      if (b.dev) continue;
+      if (b.bps != 8 && b.bps != 16) return fail(ASC_ERR_UNSUPPORTED, "volumes must be 8 or 16 bit");
+      if ((rc = asc_volume_create(S.ctx, b.data.data(), b.dims[0], b.dims[1], b.dims[2], b.bps, b.flip, b.voxel,
+                                  ASC_STORAGE_U16, &b.dev, &b.desc)))
+        return rc;
+      // Tracker.cpp:281-290: the pivot is the volume centre
+      const double c[6] = {b.desc.trans_center[0], b.desc.trans_center[1], b.desc.trans_center[2], 0, 0, 0};
+      asc_frame_from_xyzypr(c, b.pivot);
+    }
+    if (!S.srv && !S.scene &&
+        (rc = asc_scene_create(S.ctx, S.info.render_width, S.info.render_height, &S.scene)))
+      return rc;
+  }
+  SceneUse use(S);
+  return update_scene(S);
+}
+
+// n poses of one volume at the current frame -> costs and per-camera
+// values (n x 8): through the server's batcher in shared mode, else the
+// session's own scene.
+int eval_poses(Session& S, int volume, const double* poses, int n, double* costs, double* per8)
+{
+  std::vector<int> vols(n, S.vol_slot[volume]);
+  if (S.srv) {
+    std::vector<int> sets(n, S.set);
+    return asc_server_eval_requests(S.srv, n, vols.data(), sets.data(), nullptr, poses, costs, per8);
+  }
+  return asc_scene_eval_poses(S.scene, vols.data(), poses, n, costs, per8);
 }
 
 // AutoscoperMainWindow::setBackground (:876-894) with Video::create_background_image.
@@ -382,7 +448,8 @@ int set_background(Session& S, double threshold)
       if ((rc = asc_trial_background(S.trial, (int)c, bg.data(), bg.size()))) return rc;
       if ((rc = asc_image_create(S.ctx, bg.data(), dims[0], dims[1], 32, &cs.bg))) return rc;
     }
-    if ((rc = asc_scene_set_background(S.scene, (int)c, cs.bg, (float)threshold))) return rc;
+    SceneUse use(S);
+    if ((rc = asc_scene_set_background(S.scene, S.cam_slot[c], cs.bg, (float)threshold))) return rc;
   }
   S.bg_threshold = threshold;
   return ASC_OK;
@@ -393,6 +460,26 @@ unsigned seed_from_env()
   // initializeRandom (PSO.cpp:131-149): Autoscoper_RANDOM_SEED, else the clock
   if (const char* s = std::getenv("Autoscoper_RANDOM_SEED")) return (unsigned)std::atoi(s);
   return (unsigned)std::time(nullptr);
+}
+
+// The optimiser's cost in shared mode: manipulator offsets composed with
+// the current store and the volume's pivot (Tracker::minimizationFunc,
+// Tracker.cpp:775-827, as the scene's own path composes them), evaluated
+// by the server's batcher.
+struct SharedCost {
+  Session* S;
+  int volume;
+  const float* store;
+};
+
+int shared_cost(const double* manips, int n, double* costs, void* user)
+{
+  const SharedCost* c = (const SharedCost*)user;
+  Session& S = *c->S;
+  std::vector<double> poses((size_t)n * 6), per((size_t)n * 8);
+  for (int i = 0; i < n; i++)
+    asc_compose_pose(c->store, S.bones[c->volume].pivot, manips + 6 * i, &poses[(size_t)i * 6]);
+  return eval_poses(S, c->volume, poses.data(), n, costs, per.data());
 }
 
 // Tracker::optimize (Tracker.cpp:337-580) for one volume and frame.
@@ -433,12 +520,18 @@ int optimize(Session& S, int volume, int frame, int dframe, int repeats, int met
   for (int k = 0; k < 6; k++) o.offsets[k] = S.info.offsets[k];
   o.seed = seed_from_env();
   o.particles = 100;
-  o.volume = volume;
+  o.volume = S.vol_slot[volume];
   for (int r = 0; r < repeats; r++) {
     float store[7];
     cv.store_at(frame, store);
     asc_optimize_stats st;
-    if ((rc = asc_optimize_frame(S.scene, store, &o, &st))) return rc;
+    if (S.srv) {
+      SharedCost c{&S, volume, store};
+      rc = asc_optimize_frame_fn(shared_cost, &c, store, S.bones[volume].pivot, &o, &st);
+    } else {
+      rc = asc_optimize_frame(S.scene, store, &o, &st);
+    }
+    if (rc) return rc;
     cv.insert_store(frame, store);
   }
   return ASC_OK;
@@ -730,16 +823,16 @@ bool handle_message(Session& S, const uint8_t* msg, size_t len, std::vector<uint
       const int volume = r.get<int32_t>();
       double p[6];
       for (double& v : p) v = r.get<double>();
-      std::vector<double> per(S.cams.size());
+      double per[8];
       double cost;
       if (!r.ok || volume < 0 || volume >= (int)S.bones.size() || ensure_scene(S) ||
-          asc_scene_eval_poses(S.scene, &volume, p, 1, &cost, per.data())) {
+          eval_poses(S, volume, p, 1, &cost, per)) {
         reply(false);
         break;
       }
       reply(true);
-      out.push_back((uint8_t)per.size());
-      for (double v : per) put(out, v);
+      out.push_back((uint8_t)S.cams.size());
+      for (size_t c = 0; c < S.cams.size(); c++) put(out, per[c]);
       break;
     }
     case 9: {  // background threshold
@@ -894,6 +987,16 @@ ASC_API int asc_remote_create(asc_context ctx, asc_remote* out)
   if (!out) return fail(ASC_ERR_INVALID, "null pointer");
   asc_remote r = new asc_remote_s();
   r->session.ctx = ctx;
+  *out = r;
+  return ASC_OK;
+}
+
+ASC_API int asc_remote_create_shared(asc_context ctx, asc_server server, asc_remote* out)
+{
+  if (!out || !server) return fail(ASC_ERR_INVALID, "null pointer");
+  asc_remote r = new asc_remote_s();
+  r->session.ctx = ctx;
+  r->session.srv = server;
   *out = r;
   return ASC_OK;
 }

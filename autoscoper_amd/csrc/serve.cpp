@@ -17,6 +17,12 @@
 // on what else is in the launch, so every job's trajectory is the same
 // alone and in the mix.
 //
+// Front ends that keep their own trial state (a socket session of the
+// reference's protocol) share the same scene and launches: they register
+// their volumes, cameras and camera sets in the server's scene under its
+// lock, and their evaluations are requests like the bones' (external
+// requests), so several clients' optimisations ride in one fused launch.
+//
 // Per-frame latency of a job: from the moment the frame could start (job
 // arrival for the first frame, completion of the previous frame by all
 // bones for the next ones) to its completion by all bones.
@@ -42,8 +48,9 @@ double ms_since(Clock::time_point t0, Clock::time_point t) { return std::chrono:
 
 struct Job;
 
-// One cost evaluation batch of one bone: n manipulator offsets relative to
-// the bone's current float pose store.
+// One cost evaluation batch: of a bone (job set), n manipulator offsets
+// relative to the bone's current float pose store; of a front end (job
+// null), n (volume, camera set, frame, pose) requests of the shared scene.
 struct Request {
   Job* job;
   int bone;
@@ -52,6 +59,12 @@ struct Request {
   int n;
   const float* store;
   double* costs;
+  // external requests
+  const int* ext_vols = nullptr;
+  const int* ext_sets = nullptr;
+  const int* ext_frames = nullptr;
+  const double* ext_poses = nullptr;
+  double* per_camera = nullptr;  // n x 8, optional
   bool done = false;
   int status = ASC_OK;
 };
@@ -94,6 +107,8 @@ struct Job {
   int bones_left = 0;
   int64_t evaluations = 0;
   int status = ASC_OK;
+  size_t data_bytes = 0;     // volumes and frames
+  size_t scratch_share = 0;  // one PSO epoch of batch scratch
   // admitted: taken from the queue by the stager; registered: in the scene
   // (or failed) with bones_left set; released: out of the scene, queued for
   // the stager to free; finished: device copies freed, results final
@@ -106,7 +121,14 @@ struct asc_server_s {
   asc_context ctx;
   asc_context up_ctx = nullptr;  // the stager's context (its own stream)
   asc_scene scene = nullptr;
-  size_t budget = 0, used = 0;
+  // held while the scene is launched or edited (the batcher, or a front
+  // end between asc_server_lock_scene and asc_server_unlock_scene)
+  std::mutex scene_mu;
+  std::thread::id scene_owner;
+  // admission: device bytes of the active jobs' volumes and frames, and
+  // their shares of the batch scratch; the scene's scratch only grows, so
+  // what it holds counts when it exceeds the shares
+  size_t budget = 0, used_data = 0, used_share = 0;
   int max_active = 1, active = 0;
   std::mutex mu;
   std::condition_variable cv_batcher, cv_req, cv_done, cv_stager;
@@ -121,6 +143,7 @@ struct asc_server_s {
   std::thread batcher, stager;
   Clock::time_point t0;
   // statistics
+  size_t scene_scratch = 0;  // the scene's device scratch after the last launch
   int64_t batches = 0, requests = 0, evaluations = 0, frames_done = 0, jobs_done = 0;
   double busy_ms = 0.0;
   std::vector<double> latencies_ms;
@@ -295,8 +318,9 @@ void stager_main(asc_server s)
 {
   std::unique_lock<std::mutex> lk(s->mu);
   auto fits = [&] {
-    return !s->waiting.empty() && s->active < s->max_active &&
-           s->used + s->waiting.front()->device_bytes <= s->budget;
+    if (s->waiting.empty() || s->active >= s->max_active) return false;
+    const Job* j = s->waiting.front();
+    return s->used_data + j->data_bytes + std::max(s->scene_scratch, s->used_share + j->scratch_share) <= s->budget;
   };
   for (;;) {
     s->cv_stager.wait(lk, [&] { return s->stop_stager || !s->to_free.empty() || fits(); });
@@ -307,14 +331,16 @@ void stager_main(asc_server s)
       free_job(*j);
       lk.lock();
       s->active--;
-      s->used -= j->device_bytes;
+      s->used_data -= j->data_bytes;
+      s->used_share -= j->scratch_share;
       j->finished = true;
       s->cv_done.notify_all();
     } else if (fits()) {
       Job* j = s->waiting.front();
       s->waiting.pop_front();
       s->active++;
-      s->used += j->device_bytes;
+      s->used_data += j->data_bytes;
+      s->used_share += j->scratch_share;
       j->admitted = true;
       j->admit_ms = ms_since(j->arrival, Clock::now());
       lk.unlock();
@@ -339,7 +365,10 @@ void register_staged(asc_server s, std::unique_lock<std::mutex>& lk)
     int rc = j->status;
     if (rc == ASC_OK) {
       lk.unlock();
-      rc = register_job(s, *j);
+      {
+        std::lock_guard<std::mutex> sl(s->scene_mu);
+        rc = register_job(s, *j);
+      }
       lk.lock();
     }
     j->registered = true;
@@ -365,7 +394,10 @@ void reap(asc_server s, std::unique_lock<std::mutex>& lk)
     th.swap(j.threads);
     lk.unlock();
     for (std::thread& t : th) t.join();
-    unregister_job(s, j);
+    {
+      std::lock_guard<std::mutex> sl(s->scene_mu);
+      unregister_job(s, j);
+    }
     lk.lock();
     j.released = true;
     // frame latency: completion by all bones, from when the frame could start
@@ -391,9 +423,12 @@ void batcher_main(asc_server s)
     reap(s, lk);
     register_staged(s, lk);
     // a tick: wait until every running bone has a request pending (they do
-    // between two optimiser steps only host work), or briefly for stragglers
+    // between two optimiser steps only host work), or briefly for stragglers;
+    // external requests wait for the bones the same way
     auto ready = [&] {
-      return s->stop || (!s->pending.empty() && (int)s->pending.size() >= s->running_bones) ||
+      const int bone_reqs = (int)std::count_if(s->pending.begin(), s->pending.end(),
+                                               [](const Request* r) { return r->job != nullptr; });
+      return s->stop || (!s->pending.empty() && bone_reqs >= s->running_bones) ||
              !s->staged.empty() || std::any_of(s->jobs.begin(), s->jobs.end(), [](const auto& kv) {
                return kv.second->registered && !kv.second->released && kv.second->bones_left == 0;
              });
@@ -411,9 +446,18 @@ void batcher_main(asc_server s)
     size_t n = 0;
     for (Request* r : reqs) n += r->n;
     std::vector<int> vols(n), sets(n), frames(n);
-    std::vector<double> poses(n * 6), costs(n);
+    std::vector<double> poses(n * 6), costs(n), per(n * 8);
     size_t k = 0;
     for (Request* r : reqs) {
+      if (!r->job) {
+        for (int i = 0; i < r->n; i++, k++) {
+          vols[k] = r->ext_vols[i];
+          sets[k] = r->ext_sets[i];
+          frames[k] = r->ext_frames ? r->ext_frames[i] : -1;
+          std::memcpy(&poses[k * 6], r->ext_poses + 6 * i, 6 * sizeof(double));
+        }
+        continue;
+      }
       Job& j = *r->job;
       for (int i = 0; i < r->n; i++, k++) {
         vols[k] = j.svols[r->bone];
@@ -423,21 +467,32 @@ void batcher_main(asc_server s)
       }
     }
     const Clock::time_point t = Clock::now();
-    const int rc = asc_scene_eval_requests(s->scene, (int)n, vols.data(), sets.data(), frames.data(), poses.data(),
-                                           costs.data(), nullptr);
+    int rc;
+    size_t scratch = 0;
+    {
+      std::lock_guard<std::mutex> sl(s->scene_mu);
+      rc = asc_scene_eval_requests(s->scene, (int)n, vols.data(), sets.data(), frames.data(), poses.data(),
+                                   costs.data(), per.data());
+      asc_scene_scratch_bytes(s->scene, &scratch);
+    }
     const double busy = ms_since(t, Clock::now());
     lk.lock();
     s->batches++;
     s->requests += (int64_t)reqs.size();
     s->evaluations += (int64_t)n;
     s->busy_ms += busy;
+    s->scene_scratch = scratch;
     k = 0;
     for (Request* r : reqs) {
-      for (int i = 0; i < r->n; i++, k++) r->costs[i] = costs[k];
+      for (int i = 0; i < r->n; i++, k++) {
+        r->costs[i] = costs[k];
+        if (r->per_camera) std::memcpy(r->per_camera + 8 * i, &per[k * 8], 8 * sizeof(double));
+      }
       r->status = rc;
       r->done = true;
     }
     s->cv_req.notify_all();
+    s->cv_stager.notify_one();  // the scratch may have grown: re-check admission
   }
 }
 
@@ -525,9 +580,15 @@ ASC_API int asc_server_submit(asc_server s, const asc_job_desc* d, int* job_id)
                                                : bytes_of((const float*)src, n));
       dev += n * 4;
     }
-  // the job's share of the batch scratch: one PSO epoch of DRR images
-  dev += (size_t)d->options.particles * d->n_volumes * d->n_cameras * d->render_width * d->render_height * 4;
-  j->device_bytes = dev;
+  // the job's share of the batch scratch, one PSO epoch of items: DRR
+  // images, per-tile moment partials (9 doubles per 32 x 32 tile), results
+  // and descriptors
+  const size_t items = (size_t)d->options.particles * d->n_volumes * d->n_cameras;
+  const size_t tiles = (size_t)((d->render_width + 31) / 32) * ((d->render_height + 31) / 32);
+  j->data_bytes = dev;
+  j->scratch_share = items * ((size_t)d->render_width * d->render_height * 4 + tiles * 72 + 80 + 256);
+  j->device_bytes = j->data_bytes + j->scratch_share;
+  dev = j->device_bytes;
   j->opt = d->options;
   j->guess = d->guess;
   j->poses.assign((size_t)d->n_frames * d->n_volumes * 6, 0.0);
@@ -569,6 +630,52 @@ ASC_API int asc_server_wait(asc_server s, int job_id, double* poses, double* fra
   return ASC_OK;
 }
 
+ASC_API int asc_server_lock_scene(asc_server s, asc_scene* scene)
+{
+  if (!s || !scene) return asc::fail(ASC_ERR_INVALID, "bad argument");
+  s->scene_mu.lock();
+  s->scene_owner = std::this_thread::get_id();
+  *scene = s->scene;
+  return ASC_OK;
+}
+
+ASC_API int asc_server_unlock_scene(asc_server s)
+{
+  if (!s) return asc::fail(ASC_ERR_INVALID, "null server");
+  if (s->scene_owner != std::this_thread::get_id())
+    return asc::fail(ASC_ERR_INVALID, "the scene is not locked by this thread");
+  s->scene_owner = std::thread::id();
+  s->scene_mu.unlock();
+  return ASC_OK;
+}
+
+ASC_API int asc_server_eval_requests(asc_server s, int n, const int* volumes, const int* sets, const int* frames,
+                                     const double* xyzypr, double* costs, double* per_camera)
+{
+  if (!s || n < 1 || !volumes || !sets || !xyzypr || !costs) return asc::fail(ASC_ERR_INVALID, "bad argument");
+  if (s->scene_owner == std::this_thread::get_id())
+    return asc::fail(ASC_ERR_INVALID, "asc_server_eval_requests while this thread holds the scene lock");
+  Request r;
+  r.job = nullptr;
+  r.bone = -1;
+  r.frame = -1;
+  r.manips = nullptr;
+  r.n = n;
+  r.store = nullptr;
+  r.costs = costs;
+  r.ext_vols = volumes;
+  r.ext_sets = sets;
+  r.ext_frames = frames;
+  r.ext_poses = xyzypr;
+  r.per_camera = per_camera;
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (s->stop) return asc::fail(ASC_ERR_INVALID, "server is stopping");
+  s->pending.push_back(&r);
+  s->cv_batcher.notify_one();
+  s->cv_req.wait(lk, [&] { return r.done; });
+  return r.status;
+}
+
 ASC_API int asc_server_stats(asc_server s, asc_serve_stats* out)
 {
   if (!s || !out) return asc::fail(ASC_ERR_INVALID, "bad argument");
@@ -589,7 +696,7 @@ ASC_API int asc_server_stats(asc_server s, asc_serve_stats* out)
   out->latency_max_ms = l.empty() ? 0.0 : l.back();
   out->active_jobs = s->active;
   out->waiting_jobs = (int)s->waiting.size();
-  out->device_bytes_in_use = s->used;
+  out->device_bytes_in_use = s->used_data + std::max(s->scene_scratch, s->used_share);
   return ASC_OK;
 }
 

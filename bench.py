@@ -412,7 +412,7 @@ def frame_targets(A, synth, ctx, wl, vol, cams, frames, seed0):
 
 
 def frame_loop(A, synth, ctx, name, storage, frames, opts, seed, restart=None, fp64_frames=0, fp64_budget=60.0,
-               wl=None, targets=None):
+               wl=None, targets=None, filters=None):
     """Tracker's frame loop on one GPU: for each frame of the hidden random
     walk, the scene's radiographs are the frame's targets, the start is the
     previous frame's result (guess 1; frame 0 starts 2 mm / 2 deg off its
@@ -425,6 +425,7 @@ def frame_loop(A, synth, ctx, name, storage, frames, opts, seed, restart=None, f
     `fp64_frames` frames also the distance to the oracle's all-FP64 simplex
     polish started at the GPU's pose (the FP64 cost's own minimum nearby)."""
     wl = wl or synth.workload(name, frames=frames)
+    drr_filters, rad_filters = filters or (wl.drr_filters, wl.rad_filters)
     vol = A.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3, storage=STORAGES[storage])
     RW, RH = wl.render
     cams = [A.geometry.camera_from_mayacam2(*c.mayacam2()) for c in wl.cameras]
@@ -436,8 +437,8 @@ def frame_loop(A, synth, ctx, name, storage, frames, opts, seed, restart=None, f
     scene.add_volume(vol, synth.DRR_STEP, synth.DRR_INTENSITY, synth.DRR_CUTOFF, pivot)
     for i, cam in enumerate(cams):
         c = scene.add_camera(cam, imgs[0][i])
-        scene.set_filters(c, 0, wl.drr_filters)
-        scene.set_filters(c, 1, wl.rad_filters)
+        scene.set_filters(c, 0, drr_filters)
+        scene.set_filters(c, 1, rad_filters)
     rng = np.random.default_rng(synth.SEED_START)
     pert = lambda mm, deg: np.concatenate([rng.uniform(-mm, mm, 3), rng.uniform(-deg, deg, 3)])  # noqa: E731
     guess = wl.gt_poses[0] + pert(2.0, 2.0)
@@ -461,13 +462,14 @@ def frame_loop(A, synth, ctx, name, storage, frames, opts, seed, restart=None, f
            "gap_to_hidden_mm": [g[0] for g in gaps], "gap_to_hidden_deg": [g[1] for g in gaps],
            "max_gap_to_hidden": [max(g[0] for g in gaps), max(g[1] for g in gaps)],
            "per_frame": {"fields": ["initial_cost", "pso_best_cost", "final_cost", "pso_epochs", "simplex_iterations"],
-                         "values": costs}}
+                         "values": costs},
+           "final_poses": [f[0].tolist() for f in finals]}
     if fp64_frames:
         import oracle as O
         from autoscoper_amd import filters as F
 
         prep = O.volume_prep(wl.volume, (wl.voxel_mm,) * 3)
-        recs = [(F.to_records(wl.drr_filters), F.to_records(wl.rad_filters)) for _ in cams]
+        recs = [(F.to_records(drr_filters), F.to_records(rad_filters)) for _ in cams]
         fp = []
         t1 = time.perf_counter()
         for f in range(min(fp64_frames, frames)):
@@ -492,6 +494,22 @@ def frame_loop(A, synth, ctx, name, storage, frames, opts, seed, restart=None, f
 
 
 CFG2_TRACK = dict(method=0, max_iter=100, max_stall=101, synchronous=True, speculative=True, particles=100)
+
+
+def track_filters(A, synth, name):
+    """Filter chains of a frame loop: "vie" = the sample .vie settings
+    (radiograph Sobel(3, 0.43) -> Contrast(2.5, 0.9), DRR Sobel(1.7, 0.1));
+    "matched" = the DRR's Sobel(1.7, 0.1) -> Contrast(2.5, 0.9) on both
+    images; "sobel" = Sobel(1.7, 0.1) on both."""
+    F = A.filters
+    if name == "vie":
+        return synth.default_filters()
+    if name == "matched":
+        c = [F.sobel(1.7, 0.1), F.contrast(2.5, 0.9, 3)]
+        return c, list(c)
+    if name == "sobel":
+        return [F.sobel(1.7, 0.1)], [F.sobel(1.7, 0.1)]
+    raise ValueError(name)
 CFG1_TRACK = dict(method=1, speculative=True)
 
 
@@ -667,6 +685,8 @@ def main(argv=None) -> int:
     ap.add_argument("--track-only", default="", metavar="STORAGES",
                     help="only the frame loop, for these comma-separated storages (with --seeds)")
     ap.add_argument("--seeds", default="1", help="PSO seeds of the --track-only frame loops, comma-separated")
+    ap.add_argument("--track-filters", default="vie", help="filter chains of the frame loops: vie, matched or sobel "
+                                                          "(comma-separated for --track-only)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -694,13 +714,16 @@ def main(argv=None) -> int:
         targets = frame_targets(A, synth, ctx, wl, vol, cams, args.frames, 900)
         vol.close()
         res = {}
-        for st in args.track_only.split(","):
-            for seed in [int(x) for x in args.seeds.split(",")]:
-                log(f"[track] {st} seed {seed}: {args.frames} frames")
-                r = frame_loop(A, synth, ctx, args.workload, st, args.frames, CFG2_TRACK, seed,
-                               fp64_frames=args.fp64_frames, wl=wl, targets=targets)
-                log(f"[track] {st} seed {seed}: {r['frames_per_s']:.2f} frames/s, max gap {r['max_gap_to_hidden']}")
-                res[f"{st}/seed{seed}"] = r
+        for fl in args.track_filters.split(","):
+            for st in args.track_only.split(","):
+                for seed in [int(x) for x in args.seeds.split(",")]:
+                    log(f"[track] {fl} {st} seed {seed}: {args.frames} frames")
+                    r = frame_loop(A, synth, ctx, args.workload, st, args.frames, CFG2_TRACK, seed,
+                                   fp64_frames=args.fp64_frames, wl=wl, targets=targets,
+                                   filters=track_filters(A, synth, fl))
+                    log(f"[track] {fl} {st} seed {seed}: {r['frames_per_s']:.2f} frames/s, "
+                        f"max gap {r['max_gap_to_hidden']}")
+                    res[f"{fl}/{st}/seed{seed}"] = r
         print(json.dumps({"frame_loops": res, "workload": args.workload, "options": CFG2_TRACK}, default=_jsonable),
               flush=True)
         ctx.close()

@@ -316,6 +316,11 @@ ASC_API int asc_scene_remove_frame(asc_scene scene, int frame);
  * Tracker.cpp:614-618, 814-826). */
 ASC_API int asc_scene_eval_requests(asc_scene scene, int n, const int* volumes, const int* sets, const int* frames,
                                     const double* xyzypr, double* costs, double* per_camera);
+/* Device bytes the scene holds for batch scratch (DRR images, moment
+ * partials, results, descriptors); they grow with the largest batch and are
+ * kept for reuse. No counterpart in the reference (memory accounting of the
+ * registration server). */
+ASC_API int asc_scene_scratch_bytes(asc_scene scene, size_t* bytes);
 /* Record the fused launches of an n-pose batch in a hipGraph and replay it
  * for later batches of the same size (1 = on, 0 = off). */
 ASC_API int asc_scene_use_graph(asc_scene scene, int enable);
@@ -508,6 +513,20 @@ ASC_API int asc_server_submit(asc_server server, const asc_job_desc* job, int* j
 ASC_API int asc_server_wait(asc_server server, int job_id, double* poses, double* frame_latency_ms,
                             asc_job_stats* stats);
 ASC_API int asc_server_stats(asc_server server, asc_serve_stats* out);
+/* Front ends that keep their own trial state (the socket session,
+ * asc_remote_create_shared; reference: the socket's getNCC and optimizeFrame
+ * messages, Socket.cpp:282-302,345-372, which run one client's requests
+ * alone there) share the server's scene and its fused launches. Under
+ * asc_server_lock_scene the batcher does not launch: the front end adds or
+ * removes its volumes, cameras, camera sets and frames with asc_scene_*
+ * calls, then unlocks (same thread). asc_server_eval_requests evaluates n
+ * requests as asc_scene_eval_requests does, in the batcher's next launch
+ * together with every other pending request, and blocks until done; a
+ * thread holding the scene lock may not call it. */
+ASC_API int asc_server_lock_scene(asc_server server, asc_scene* scene);
+ASC_API int asc_server_unlock_scene(asc_server server);
+ASC_API int asc_server_eval_requests(asc_server server, int n, const int* volumes, const int* sets,
+                                     const int* frames, const double* xyzypr, double* costs, double* per_camera);
 /* Drops queued jobs, lets running ones finish, then frees everything. */
 ASC_API int asc_server_destroy(asc_server server);
 
@@ -618,6 +637,15 @@ ASC_API int asc_write_full_drr_pgm(const char* path, const float* image, int wid
  * built (message 11 with cf_model 1, SAD, fails). */
 typedef struct asc_remote_s* asc_remote;
 ASC_API int asc_remote_create(asc_context ctx, asc_remote* out);
+/* A session whose costs and optimisations (messages 8 and 11,
+ * Socket.cpp:282-302,345-372) run as requests of the registration server
+ * `server` (asc_server_eval_requests): its volumes, cameras and camera set
+ * live in the server's scene, so the requests of several sessions and of
+ * the server's own jobs share fused launches; a request's cost does not
+ * depend on what shares its launch, so the answers are the same bits as a
+ * session of its own. `ctx` (same device) holds the session's uploads and
+ * its full-view renders (messages 10 and 12). */
+ASC_API int asc_remote_create_shared(asc_context ctx, asc_server server, asc_remote* out);
 /* Serves on 127.0.0.1:port from a thread of its own (port 0: any free
  * port; the reference's client uses 30007). */
 ASC_API int asc_remote_listen(asc_remote remote, int port);

@@ -139,3 +139,65 @@ def test_background_threshold(asc, ctx, wrist, remote, tmp_path):
     masked = r2.handle(msg(8, "i6d", 0, *pose))
     assert masked[0] == 8 and masked != unmasked
     r2.close()
+
+
+def _open(asc, ctx, server=None):
+    r = asc.Remote(ctx, server=server)
+    assert r.handle(msg(1, tail=str(GOLD / "wrist.cfg").encode())) == b"\x01"
+    assert r.handle(msg(4, "i", -1, tail=str(GOLD / "xParameters" / "control_settings.vie").encode())) == b"\x04"
+    assert r.handle(msg(5, "i", 0)) == b"\x05"
+    return r
+
+
+def _track(r, volume, start):
+    """getNCC at the start, optimizeFrame (PSO 6 epochs + simplex, seed from
+    the environment), then the tracked pose and its getNCC."""
+    assert r.handle(msg(7, "ii6d", volume, 0, *start)) == b"\x07"
+    before = r.handle(msg(8, "i6d", volume, *start))
+    rep = r.handle(msg(11, "iiiiddiiiii", volume, 0, 1, 6, -3.0, 3.0, 3, 1, 0, 0, 0))
+    assert rep == b"\x0b"
+    pose = struct.unpack("<6d", r.handle(msg(6, "ii", volume, 0))[1:])
+    return before, pose, r.handle(msg(8, "i6d", volume, *pose))
+
+
+def test_sessions_share_the_servers_launches(asc, ctx, wrist):
+    """Two socket sessions on one registration server (asc_remote_create_shared)
+    optimise different bones at the same time: their messages ride in shared
+    fused launches, and every answer is the same bits as a session of its own."""
+    import threading
+
+    starts = {0: base_pose(wrist, 0) + np.array([0.8, -0.5, 0.4, 1.5, -1.0, 0.8]),
+              3: base_pose(wrist, 3) + np.array([-0.6, 0.7, -0.3, -1.0, 1.2, 0.5])}
+    os.environ["Autoscoper_RANDOM_SEED"] = "5"
+    try:
+        alone = {}
+        for v, start in starts.items():
+            r = _open(asc, ctx)
+            alone[v] = _track(r, v, start)
+            r.close()
+        srv = asc.Server(ctx, 16 << 30, 4)
+        sessions = {v: _open(asc, ctx, server=srv) for v in starts}
+        shared, errors = {}, []
+
+        def run(v):
+            try:
+                shared[v] = _track(sessions[v], v, starts[v])
+            except Exception as e:  # reported below
+                errors.append(e)
+
+        threads = [threading.Thread(target=run, args=(v,)) for v in starts]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        st = srv.stats()
+        for r in sessions.values():
+            r.close()
+        srv.close()
+    finally:
+        del os.environ["Autoscoper_RANDOM_SEED"]
+    assert not errors, errors
+    print("shared sessions:", {k: st[k] for k in ("batches", "requests", "evaluations")})
+    for v in starts:
+        assert shared[v] == alone[v], v
+    assert st["requests"] > st["batches"]  # the two sessions' requests shared launches

@@ -133,7 +133,12 @@ def test_admission_one_job_at_a_time(asc, ctx, tiny, trial, rule):
     b = submit_tiny(asc, srv, tiny, trial)
     pa, _, sa = srv.wait(a)
     pb, _, sb = srv.wait(b)
+    st = srv.stats()
     srv.close()
+    # the scene's batch scratch (kept after the jobs) is what the server counts
+    assert 0 < st["device_bytes_in_use"]
+    if rule == "memory_budget":
+        assert st["device_bytes_in_use"] <= int(one * 1.5)
     print(f"{rule}: a queued {sa.queue_ms:.1f} ms, ran to {sa.total_ms:.1f} ms; b queued {sb.queue_ms:.1f} ms")
     # b was admitted only after a finished (b arrived a few ms after a)
     assert sb.queue_ms >= sa.total_ms - 20.0
