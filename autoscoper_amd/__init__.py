@@ -238,6 +238,8 @@ _SIGS = {
     "asc_server_lock_scene": [_vp, _P(_vp)],
     "asc_server_unlock_scene": [_vp],
     "asc_server_eval_requests": [_vp, _i, _P(_i), _P(_i), _P(_i), _P(_d), _P(_d), _P(_d)],
+    "asc_server_stream_begin": [_vp],
+    "asc_server_stream_end": [_vp],
     "asc_scene_scratch_bytes": [_vp, _P(C.c_size_t)],
     "asc_remote_create_shared": [_vp, _vp, _P(_vp)],
     "asc_server_destroy": [_vp],

@@ -505,9 +505,9 @@ __device__ __forceinline__ float drr_finish(float acc_raw, float step_vscale, fl
 // returns before sampling outside), OpenCL 1.2 section 8.2: x = fma(s, W, -0.5),
 // i0 = floor(x), a = x - i0, texels outside the image read the border 0
 // (CLK_ADDRESS_CLAMP). The texels come through buffer loads, whose
-// out-of-range offsets read 0: a row outside [0, H) lands out of range by
-// itself (the image holds fewer than 2^29 texels, checked on the host), a
-// column outside [0, W) is sent there explicitly.
+// out-of-range offsets read 0: a texel whose row or column lies outside the
+// image is sent out of range explicitly (the image holds fewer than 2^29
+// texels, checked on the host).
 __device__ __forceinline__ float img_sample(const ImgDev& im, float s, float t)
 {
   const int W = __builtin_amdgcn_readfirstlane(im.W), H = __builtin_amdgcn_readfirstlane(im.H);
@@ -524,10 +524,16 @@ __device__ __forceinline__ float img_sample(const ImgDev& im, float s, float t)
   const int i0 = (int)fx, j0 = (int)fy;
   const int row0 = (j0 * W + i0) * 4, row1 = row0 + W * 4;
   const bool c0 = (unsigned)i0 < (unsigned)W, c1 = (unsigned)(i0 + 1) < (unsigned)W;
-  const float T00 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, c0 ? row0 : -1, 0, 0));
-  const float T10 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, c1 ? row0 + 4 : -1, 0, 0));
-  const float T01 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, c0 ? row1 : -1, 0, 0));
-  const float T11 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, c1 ? row1 + 4 : -1, 0, 0));
+  const bool r0 = (unsigned)j0 < (unsigned)H, r1 = (unsigned)(j0 + 1) < (unsigned)H;
+  // a texel outside the image is loaded at offset 2^31: past the image
+  // (fewer than 2^31 bytes), and far enough from 2^32 that the range check
+  // (offset plus the access size, in 32 bits) cannot wrap into range, as
+  // an offset such as -1 would
+  constexpr int kOut = (int)0x80000000u;
+  const float T00 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, c0 && r0 ? row0 : kOut, 0, 0));
+  const float T10 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, c1 && r0 ? row0 + 4 : kOut, 0, 0));
+  const float T01 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, c0 && r1 ? row1 : kOut, 0, 0));
+  const float T11 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, c1 && r1 ? row1 + 4 : kOut, 0, 0));
   return (1.0f - a) * (1.0f - b) * T00 + a * (1.0f - b) * T10 + (1.0f - a) * b * T01 + a * b * T11;
 }
 

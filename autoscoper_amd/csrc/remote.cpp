@@ -526,8 +526,12 @@ int optimize(Session& S, int volume, int frame, int dframe, int repeats, int met
     cv.store_at(frame, store);
     asc_optimize_stats st;
     if (S.srv) {
+      // announced, so that the optimisations of other sessions share its launches
       SharedCost c{&S, volume, store};
+      if ((rc = asc_server_stream_begin(S.srv))) return rc;
       rc = asc_optimize_frame_fn(shared_cost, &c, store, S.bones[volume].pivot, &o, &st);
+      const int rc_end = asc_server_stream_end(S.srv);
+      if (!rc) rc = rc_end;
     } else {
       rc = asc_optimize_frame(S.scene, store, &o, &st);
     }

@@ -138,6 +138,7 @@ struct asc_server_s {
   std::map<int, std::unique_ptr<Job>> jobs;  // all not yet collected by asc_server_wait
   std::vector<Request*> pending;
   int running_bones = 0;
+  int streams = 0;  // front ends inside asc_server_stream_begin / _end
   int next_id = 1;
   bool stop = false, stop_stager = false;
   std::thread batcher, stager;
@@ -422,13 +423,16 @@ void batcher_main(asc_server s)
   for (;;) {
     reap(s, lk);
     register_staged(s, lk);
-    // a tick: wait until every running bone has a request pending (they do
-    // between two optimiser steps only host work), or briefly for stragglers;
-    // external requests wait for the bones the same way
+    // a tick: wait until every running bone and every announced stream of
+    // external requests has a request pending (between two optimiser steps
+    // they do only host work), or briefly for stragglers; any other
+    // external request waits the same way
     auto ready = [&] {
       const int bone_reqs = (int)std::count_if(s->pending.begin(), s->pending.end(),
                                                [](const Request* r) { return r->job != nullptr; });
-      return s->stop || (!s->pending.empty() && bone_reqs >= s->running_bones) ||
+      const int ext_reqs = (int)s->pending.size() - bone_reqs;
+      return s->stop ||
+             (!s->pending.empty() && bone_reqs >= s->running_bones && ext_reqs >= s->streams) ||
              !s->staged.empty() || std::any_of(s->jobs.begin(), s->jobs.end(), [](const auto& kv) {
                return kv.second->registered && !kv.second->released && kv.second->bones_left == 0;
              });
@@ -646,6 +650,25 @@ ASC_API int asc_server_unlock_scene(asc_server s)
     return asc::fail(ASC_ERR_INVALID, "the scene is not locked by this thread");
   s->scene_owner = std::thread::id();
   s->scene_mu.unlock();
+  return ASC_OK;
+}
+
+ASC_API int asc_server_stream_begin(asc_server s)
+{
+  if (!s) return asc::fail(ASC_ERR_INVALID, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->streams++;
+  return ASC_OK;
+}
+
+ASC_API int asc_server_stream_end(asc_server s)
+{
+  if (!s) return asc::fail(ASC_ERR_INVALID, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->streams <= 0) return asc::fail(ASC_ERR_INVALID, "asc_server_stream_end without asc_server_stream_begin");
+  s->streams--;
+  // the batcher may be waiting for this stream's next request
+  s->cv_batcher.notify_one();
   return ASC_OK;
 }
 

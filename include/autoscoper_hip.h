@@ -527,6 +527,15 @@ ASC_API int asc_server_lock_scene(asc_server server, asc_scene* scene);
 ASC_API int asc_server_unlock_scene(asc_server server);
 ASC_API int asc_server_eval_requests(asc_server server, int n, const int* volumes, const int* sets,
                                      const int* frames, const double* xyzypr, double* costs, double* per_camera);
+/* A front end about to post a stream of requests, one after another (an
+ * optimisation: Tracker::optimize's cost calls, Tracker.cpp:476-570),
+ * announces it; until the matching asc_server_stream_end the batcher, before
+ * it launches, waits for a request from every announced stream as it does
+ * for the running jobs' bones (at most 0.5 ms for stragglers), so that
+ * concurrent streams share launches. Requests of streams and any other
+ * external requests count alike. */
+ASC_API int asc_server_stream_begin(asc_server server);
+ASC_API int asc_server_stream_end(asc_server server);
 /* Drops queued jobs, lets running ones finish, then frees everything. */
 ASC_API int asc_server_destroy(asc_server server);
 

@@ -13,14 +13,17 @@ Bars:
     1.6e-4 / 3.4e-5, 2.3e-3 / 5.0e-4);
   * 1 - NCC per camera: U16/F32 within 1e-5 (the parity bar); F16 within
     5e-4 and BF16 within 5e-3 (measured errors are several times smaller);
-  * final pose of a converged optimisation (simplex alone from the start
-    pose; synchronous batched PSO run until it stalls, then the simplex)
-    with the same seed: F16 and U16 within 0.05 mm / 0.05 deg of the FP64
-    oracle's. A PSO cut short after a few epochs hands the simplex whichever
-    particle leads at that moment, and two near-equal leaders can swap
-    under any cost change (u16 costs, 7e-7 from FP64, already move such a
-    run by 0.04 mm), so only converged runs measure the storage's effect
-    on the pose.
+  * final pose of a converged optimisation (synchronous batched PSO run
+    until it stalls, then the simplex) with the same seed: F16 and U16
+    within 0.05 mm / 0.05 deg of the FP64 oracle's. A PSO cut short after
+    a few epochs hands the simplex whichever particle leads at that moment,
+    and two near-equal leaders can swap under any cost change (u16 costs,
+    7e-7 from FP64, already move such a run by 0.04 mm), so only converged
+    runs measure the storage's effect on the pose. The simplex alone is not
+    one: from the 2 mm / 2 deg start it stops where its simplex collapses,
+    0.3 to 1.9 mm from the hidden pose on this 96 x 80 scene, on a floor
+    rough enough that FP32 and FP64 costs take different branches there
+    (0.32 mm apart with the oracle alone, FP32 against FP64).
 """
 import numpy as np
 import pytest
@@ -144,8 +147,7 @@ def test_final_pose_vs_fp64_oracle(asc, ctx, orc, tiny, storage):
     S = oracle_scene(orc, asc, tiny)
     store0 = asc.geometry.pose_store_set(tiny["wl"].gt_poses[0] + START)
     gaps = []
-    runs = [dict(method=1, max_iter=100, max_stall=25, seed=3)]
-    runs += [dict(method=0, max_iter=30, max_stall=10, seed=seed) for seed in (3, 11)]
+    runs = [dict(method=0, max_iter=30, max_stall=10, seed=seed) for seed in (3, 11)]
     for opts in runs:
         s_gpu, _ = sc.optimize_frame(store0, synchronous=True, **opts)
         s64, _, _ = S.optimize_frame(store0, pivot, nparticles=100, synchronous=True, fp64=True, **opts)
