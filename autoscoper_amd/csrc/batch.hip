@@ -209,11 +209,19 @@ __global__ __launch_bounds__(kBlock) void drr_tiles(BatchArgs a)
     if (!decode_tile<false>(hd, a.items, t, ii, tx, ty)) continue;
     const ItemDesc& it = a.items[ii];
     // each of the tile's 4 waves marches a pw x (64 / pw) patch of rays,
-    // pw = 2^wave_shift; the patches cover the 16 x 16 tile row by row
+    // pw = 2^wave_shift; the patches cover the 16 x 16 tile row by row.
+    // Inside a patch, lanes 4q..4q+3 take the q-th 2 x 2 block of rays (row
+    // by row): the vector memory unit serves a quad of lanes per cycle for
+    // each cache line its four addresses touch, and the samples of a 2 x 2
+    // block of neighbouring rays share a brick's line more often than those
+    // of a row of four.
     const int wsh = a.wave_shift, pw = 1 << wsh, ph = 64 >> wsh;
     const int wpr = kTile >> wsh;  // waves per tile row
-    const unsigned x = tx * kTile + (wave % wpr) * pw + (lane & (pw - 1));
-    const unsigned y = ty * kTile + (wave / wpr) * ph + (lane >> wsh);
+    const unsigned q = lane >> 2, qsh = (unsigned)wsh - 1u;
+    const unsigned lx = (lane & 1u) | ((q & ((1u << qsh) - 1u)) << 1);
+    const unsigned ly = ((lane >> 1) & 1u) | ((q >> qsh) << 1);
+    const unsigned x = tx * kTile + (wave % wpr) * pw + lx;
+    const unsigned y = ty * kTile + (wave / wpr) * ph + ly;
     if (threadIdx.x == 0) s_samples = 0;
     __syncthreads();
     int nsamp = 0;
