@@ -9,7 +9,7 @@
 //   B  post_tiles  : persistent; per 32x32 tile: DRR filter chain and
 //                    radiograph resample + filter chain in LDS with halos,
 //                    DRR mask, multiplies, nine NCC moments (four pixels
-//                    per thread, compensated) -> per-tile partial
+//                    per thread, summed in double) -> per-tile partial
 //                    (9 doubles).
 //   C  finalize    : one block per item: fixed-order sum of its partials,
 //                    NCC in double; resets the work counters for the next
@@ -275,6 +275,32 @@ __device__ __forceinline__ void for_region(int side, int rx0, int ry0, int w, in
   }
 }
 
+// for_region for a square wholly inside the image: every position is
+// visited, fn gets the LDS index c = base + yy * R + xx (stepped along with
+// xx and yy, no multiply) and (xx, yy).
+template <int R, class F>
+__device__ __forceinline__ void for_square(int side, int base, F&& fn)
+{
+  const int n = side * side;
+  const float inv = 1.0f / (float)side;
+  int yy = (int)(((float)threadIdx.x + 0.5f) * inv);
+  int xx = (int)threadIdx.x - yy * side;
+  const int q = (int)(((float)kBlock + 0.5f) * inv);
+  const int r = kBlock - q * side;
+  int c = base + yy * R + xx;
+  for (int k = threadIdx.x; k < n; k += kBlock) {
+    fn(c, xx, yy);
+    xx += r;
+    yy += q;
+    c += q * R + r;
+    if (xx >= side) {
+      xx -= side;
+      ++yy;
+      c += R - side;
+    }
+  }
+}
+
 // Run a filter chain over the region in LDS (row stride R). On entry `cur`
 // holds stage-0 values at every in-image position of the post tile grown by
 // the chain's halo; on exit the returned buffer holds the chain output for
@@ -299,7 +325,23 @@ __device__ __forceinline__ float* run_chain(const ChainDev& ch, const float* __r
     float* const out = nxt;
     const float* const in = cur;
     auto at = [&](int px, int py) { return (py - oy) * R + (px - ox); };
-    if (f.enabled && f.type == ASC_FILTER_SOBEL) {
+    // the stage's outputs and their 3 x 3 neighbourhoods all inside the
+    // image (a block-uniform test): no per-pixel bounds or border tests
+    const bool inner = rx0 >= 1 && ry0 >= 1 && rx0 + side <= w - 1 && ry0 + side <= h - 1;
+    const int base = at(rx0, ry0);
+    if (f.enabled && f.type == ASC_FILTER_SOBEL && inner) {
+      const float scale = f.p[0], blend = f.p[1];
+      for_square<R>(side, base, [&](int c, int, int) {
+        out[c] = dev::sobel_core(in[c - R - 1], in[c - R], in[c - R + 1], in[c - 1], in[c], in[c + 1], in[c + R - 1],
+                                 in[c + R], in[c + R + 1], scale, blend);
+      });
+    } else if (f.enabled && f.type == ASC_FILTER_CONTRAST && inner && (int)f.p[2] == 3) {
+      const float alpha = f.p[0], beta = f.p[1];
+      for_square<R>(side, base, [&](int c, int, int) {
+        out[c] = dev::contrast3_core(in[c - R - 1], in[c - R], in[c - R + 1], in[c - 1], in[c], in[c + 1],
+                                     in[c + R - 1], in[c + R], in[c + R + 1], alpha, beta);
+      });
+    } else if (f.enabled && f.type == ASC_FILTER_SOBEL) {
       const float scale = f.p[0], blend = f.p[1];
       for_region(side, rx0, ry0, w, h, [&](int, int, int px, int py) {
         const int c = at(px, py);
@@ -365,6 +407,39 @@ __device__ __forceinline__ float rad_t(const float* __restrict__ plane, const fl
   return (t < 0.0f || t > 1.0f) ? -1.0f : t;
 }
 
+// The radiograph sampler's arithmetic (device.hpp img_sample) for one
+// column or one row of a tile region, done once per column and row: the
+// texel index i0 = floor(s n - 1/2) becomes the byte offsets of texel
+// columns i0 and i0 + 1 (of texel rows j0 and j0 + 1 for a row), kSampOut
+// where they lie outside the image, and the weight a with 1 - a. A texel's
+// offset is the sum of its column's and its row's: in the image when both
+// are, otherwise at least 2^30 (past the image, which holds at most 2^30
+// bytes) and at most 2^31 (where the range check, offset plus access size
+// in 32 bits, cannot wrap), so that it reads the border 0. A coordinate
+// outside [0, 1] (-1 from rad_s / rad_t) gets w = -1: its pixels are 0.
+constexpr unsigned kSampOut = 0x40000000u;
+struct SampAxis {
+  unsigned o0, o1;
+  float w, nw;  // a and 1 - a
+};
+
+__device__ __forceinline__ SampAxis samp_axis(float s, int n, unsigned stride)
+{
+  SampAxis r;
+  const float x = fmaf(s, (float)n, -0.5f);
+  const float fx = floorf(x);
+  const int i0 = (int)fx;
+  r.w = x - fx;
+  r.nw = 1.0f - r.w;
+  r.o0 = (unsigned)i0 < (unsigned)n ? (unsigned)i0 * stride : kSampOut;
+  r.o1 = (unsigned)(i0 + 1) < (unsigned)n ? (unsigned)(i0 + 1) * stride : kSampOut;
+  if (s < 0.0f) {
+    r.o0 = r.o1 = kSampOut;
+    r.w = -1.0f;
+  }
+  return r;
+}
+
 template <int MAXH>
 __global__ __launch_bounds__(kBlock) void post_tiles(BatchArgs a)
 {
@@ -373,6 +448,7 @@ __global__ __launch_bounds__(kBlock) void post_tiles(BatchArgs a)
   __shared__ float buf1[R * R];
   __shared__ float sdrr[kPTile * kPTile];
   __shared__ float s_col[R], s_row[R];
+  __shared__ SampAxis s_scol[R], s_srow[R];
   __shared__ double lds_red[4 * 9];
   __shared__ int s_tile;
   const BatchHeader& hd = *a.hdr;
@@ -397,8 +473,14 @@ __global__ __launch_bounds__(kBlock) void post_tiles(BatchArgs a)
       const int H0 = ch.halo, side = kPTile + 2 * H0;
       const int ox = x0 - H0, oy = y0 - H0;
       const float* src = a.drr + it.drr_off;
-      for_region(side, ox, oy, w, h,
-                 [&](int xx, int yy, int px, int py) { buf0[yy * R + xx] = src[(long long)py * w + px]; });
+      if (ox >= 0 && oy >= 0 && ox + side <= w && oy + side <= h) {
+        // the region wholly inside the image: rows of the DRR straight in
+        const float* s0 = src + (long long)oy * w + ox;
+        for_square<R>(side, 0, [&](int c, int xx, int yy) { buf0[c] = s0[yy * w + xx]; });
+      } else {
+        for_region(side, ox, oy, w, h,
+                   [&](int xx, int yy, int px, int py) { buf0[yy * R + xx] = src[(long long)py * w + px]; });
+      }
       __syncthreads();
       const float* outb = run_chain<R>(ch, a.weights, buf0, buf1, x0, y0, w, h);
 #pragma unroll
@@ -418,17 +500,37 @@ __global__ __launch_bounds__(kBlock) void post_tiles(BatchArgs a)
       for (int k = threadIdx.x; k < 2 * side; k += kBlock) {
         if (k < side) {
           const int px = ox + k;
-          if (px >= 0 && px < w) s_col[k] = rad_s(cam.plane, it.vp, (unsigned)px, (unsigned)w);
+          if (px >= 0 && px < w) {
+            const float ss = rad_s(cam.plane, it.vp, (unsigned)px, (unsigned)w);
+            s_col[k] = ss;
+            s_scol[k] = samp_axis(ss, it.rad.W, 4u);
+          }
         } else {
           const int py = oy + (k - side);
-          if (py >= 0 && py < h) s_row[k - side] = rad_t(cam.plane, it.vp, (unsigned)py, (unsigned)h);
+          if (py >= 0 && py < h) {
+            const float tt = rad_t(cam.plane, it.vp, (unsigned)py, (unsigned)h);
+            s_row[k - side] = tt;
+            s_srow[k - side] = samp_axis(tt, it.rad.H, 4u * (unsigned)it.rad.W);
+          }
         }
       }
       __syncthreads();
-      for_region(side, ox, oy, w, h, [&](int xx, int yy, int, int) {
-        const float ss = s_col[xx], tt = s_row[yy];
-        buf0[yy * R + xx] = (ss < 0.0f || tt < 0.0f) ? 0.0f : 1.0f - dev::img_sample(it.rad, ss, tt);
-      });
+      // img_sample's texels and weighted sum, in its order, from the
+      // column's and the row's sampler values
+      const __amdgpu_buffer_rsrc_t rrs = dev::img_rsrc(it.rad);
+      auto resample = [&](int xx, int yy) {
+        const SampAxis c = s_scol[xx], r = s_srow[yy];
+        const float T00 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rrs, (int)(c.o0 + r.o0), 0, 0));
+        const float T10 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rrs, (int)(c.o1 + r.o0), 0, 0));
+        const float T01 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rrs, (int)(c.o0 + r.o1), 0, 0));
+        const float T11 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rrs, (int)(c.o1 + r.o1), 0, 0));
+        const float v = (c.nw * r.nw) * T00 + (c.w * r.nw) * T10 + (c.nw * r.w) * T01 + (c.w * r.w) * T11;
+        return (c.w < 0.0f || r.w < 0.0f) ? 0.0f : 1.0f - v;
+      };
+      if (ox >= 0 && oy >= 0 && ox + side <= w && oy + side <= h)
+        for_square<R>(side, 0, [&](int c, int xx, int yy) { buf0[c] = resample(xx, yy); });
+      else
+        for_region(side, ox, oy, w, h, [&](int xx, int yy, int, int) { buf0[yy * R + xx] = resample(xx, yy); });
       __syncthreads();
       radb = run_chain<R>(ch, a.weights, buf0, buf1, x0, y0, w, h);
     }

@@ -506,16 +506,22 @@ __device__ __forceinline__ float drr_finish(float acc_raw, float step_vscale, fl
 // i0 = floor(x), a = x - i0, texels outside the image read the border 0
 // (CLK_ADDRESS_CLAMP). The texels come through buffer loads, whose
 // out-of-range offsets read 0: a texel whose row or column lies outside the
-// image is sent out of range explicitly (the image holds fewer than 2^29
+// image is sent out of range explicitly (the image holds at most 2^28
 // texels, checked on the host).
-__device__ __forceinline__ float img_sample(const ImgDev& im, float s, float t)
+// A buffer resource over an image's float texels (range-checked loads).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t img_rsrc(const ImgDev& im)
 {
   const int W = __builtin_amdgcn_readfirstlane(im.W), H = __builtin_amdgcn_readfirstlane(im.H);
   const uint64_t base = (uint64_t)(uintptr_t)im.data;
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(base >> 32));
-  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-    (void*)(uintptr_t)(((uint64_t)hi << 32) | lo), 0, W * H * 4, 0x00020000);
+  return __builtin_amdgcn_make_buffer_rsrc((void*)(uintptr_t)(((uint64_t)hi << 32) | lo), 0, W * H * 4, 0x00020000);
+}
+
+__device__ __forceinline__ float img_sample(const ImgDev& im, float s, float t)
+{
+  const int W = __builtin_amdgcn_readfirstlane(im.W), H = __builtin_amdgcn_readfirstlane(im.H);
+  const __amdgpu_buffer_rsrc_t rs = img_rsrc(im);
   // the un-normalisation as in the 3-D sampler: one FMA per axis
   const float x = fmaf(s, (float)W, -0.5f);
   const float y = fmaf(t, (float)H, -0.5f);
@@ -526,7 +532,7 @@ __device__ __forceinline__ float img_sample(const ImgDev& im, float s, float t)
   const bool c0 = (unsigned)i0 < (unsigned)W, c1 = (unsigned)(i0 + 1) < (unsigned)W;
   const bool r0 = (unsigned)j0 < (unsigned)H, r1 = (unsigned)(j0 + 1) < (unsigned)H;
   // a texel outside the image is loaded at offset 2^31: past the image
-  // (fewer than 2^31 bytes), and far enough from 2^32 that the range check
+  // (at most 2^30 bytes), and far enough from 2^32 that the range check
   // (offset plus the access size, in 32 bits) cannot wrap into range, as
   // an offset such as -1 would
   constexpr int kOut = (int)0x80000000u;
@@ -701,80 +707,68 @@ __host__ __device__ inline int filter_radius(const asc_filter& f, int gauss_n)
 }
 
 // ---------------------------------------------------------------------------
-// NCC moments: per-thread fp32 partials with compensation (the running sum and
-// its exact rounding errors), combined
+// NCC moments: per-thread partial sums in double of the float inputs and of
+// their products, which are exact in double (24-bit significands), combined
 // in double across lanes, waves and blocks.
 // ---------------------------------------------------------------------------
-struct Comp {
-  float s = 0.0f, c = 0.0f;
-  __device__ __forceinline__ void add(float x)
-  {
-    // Knuth's TwoSum: the exact rounding error of s + x, without a branch
-    // (Neumaier's two cases give the same value)
-    const float t = s + x;
-    const float bp = t - s;
-    c += (s - (t - bp)) + (x - bp);
-    s = t;
-  }
-  __device__ __forceinline__ double value() const { return (double)s + (double)c; }
-};
-
 struct Moments {
   // {sum mask, sum f, sum g, count, S_f, S_g, S_ff, S_gg, S_fg}
-  Comp m, f, g, sf, sg, sff, sgg, sfg;
+  double m = 0.0, f = 0.0, g = 0.0, sf = 0.0, sg = 0.0, sff = 0.0, sgg = 0.0, sfg = 0.0;
   int count = 0;
   __device__ __forceinline__ void add(float fv, float gv, float mv)
   {
-    m.add(mv);
-    f.add(fv);
-    g.add(gv);
+    const double x = fv, y = gv;
+    m += (double)mv;
+    f += x;
+    g += y;
     if (mv > 0.5f) {
       ++count;
-      sf.add(fv);
-      sg.add(gv);
-      sff.add(fv * fv);
-      sgg.add(gv * gv);
-      sfg.add(fv * gv);
+      sf += x;
+      sg += y;
+      sff += x * x;
+      sgg += y * y;
+      sfg += x * y;
     }
   }
   __device__ __forceinline__ void to_double(double* o) const
   {
-    o[0] = m.value();
-    o[1] = f.value();
-    o[2] = g.value();
+    o[0] = m;
+    o[1] = f;
+    o[2] = g;
     o[3] = (double)count;
-    o[4] = sf.value();
-    o[5] = sg.value();
-    o[6] = sff.value();
-    o[7] = sgg.value();
-    o[8] = sfg.value();
+    o[4] = sf;
+    o[5] = sg;
+    o[6] = sff;
+    o[7] = sgg;
+    o[8] = sfg;
   }
 };
 
 // Moments of a post-processing tile, where the mask is 0 or 1 and f and g
 // are already multiplied by it: a masked-out pixel adds zeros, which leave
-// a compensated sum as it was, so sum f = S_f, sum g = S_g and sum mask =
-// count, and six sums give the nine of Moments with the same bits.
+// a sum as it was, so sum f = S_f, sum g = S_g and sum mask = count, and six
+// sums give the nine of Moments with the same bits.
 struct MaskedMoments {
   int m = 0;  // the mask is 0 or 1: its sum is a count
-  Comp f, g, ff, gg, fg;
+  double f = 0.0, g = 0.0, ff = 0.0, gg = 0.0, fg = 0.0;
   __device__ __forceinline__ void add(float fv, float gv, float mv)
   {
+    const double x = fv, y = gv;
     m += mv > 0.5f;
-    f.add(fv);
-    g.add(gv);
-    ff.add(fv * fv);
-    gg.add(gv * gv);
-    fg.add(fv * gv);
+    f += x;
+    g += y;
+    ff += x * x;
+    gg += y * y;
+    fg += x * y;
   }
   __device__ __forceinline__ void to_double6(double* o) const
   {
     o[0] = (double)m;
-    o[1] = f.value();
-    o[2] = g.value();
-    o[3] = ff.value();
-    o[4] = gg.value();
-    o[5] = fg.value();
+    o[1] = f;
+    o[2] = g;
+    o[3] = ff;
+    o[4] = gg;
+    o[5] = fg;
   }
 };
 

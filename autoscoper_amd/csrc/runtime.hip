@@ -540,8 +540,9 @@ ASC_API int asc_image_create(asc_context ctx, const void* data, int width, int h
 {
   ASC_CHECK(valid_ctx(ctx) && out, ASC_ERR_INVALID, "bad argument");
   ASC_CHECK(width > 0 && height > 0, ASC_ERR_INVALID, "bad image shape");
-  // the sampler addresses texels with 32-bit byte offsets
-  ASC_CHECK((int64_t)width * height * 4 < ((int64_t)1 << 31), ASC_ERR_TOO_LARGE, "image larger than 2^29 texels");
+  // the samplers address texels with 32-bit byte offsets, and mark texels
+  // outside the image with offsets of 2^30 and more
+  ASC_CHECK((int64_t)width * height * 4 <= ((int64_t)1 << 30), ASC_ERR_TOO_LARGE, "image larger than 2^28 texels");
   ASC_CHECK(bps == 8 || bps == 16 || bps == 32, ASC_ERR_UNSUPPORTED, "unsupported image bit depth (8, 16, 32)");
   asc_image im = new asc_image_s();
   im->ctx = ctx;
