@@ -9,7 +9,7 @@
 //   B  post_tiles  : persistent; per 32x32 tile: DRR filter chain and
 //                    radiograph resample + filter chain in LDS with halos,
 //                    DRR mask, multiplies, nine NCC moments (four pixels
-//                    per thread, summed in double) -> per-tile partial
+//                    per thread, compensated) -> per-tile partial
 //                    (9 doubles).
 //   C  finalize    : one block per item: fixed-order sum of its partials,
 //                    NCC in double; resets the work counters for the next

@@ -707,68 +707,80 @@ __host__ __device__ inline int filter_radius(const asc_filter& f, int gauss_n)
 }
 
 // ---------------------------------------------------------------------------
-// NCC moments: per-thread partial sums in double of the float inputs and of
-// their products, which are exact in double (24-bit significands), combined
+// NCC moments: per-thread fp32 partials with compensation (the running sum and
+// its exact rounding errors), combined
 // in double across lanes, waves and blocks.
 // ---------------------------------------------------------------------------
+struct Comp {
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float x)
+  {
+    // Knuth's TwoSum: the exact rounding error of s + x, without a branch
+    // (Neumaier's two cases give the same value)
+    const float t = s + x;
+    const float bp = t - s;
+    c += (s - (t - bp)) + (x - bp);
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return (double)s + (double)c; }
+};
+
 struct Moments {
   // {sum mask, sum f, sum g, count, S_f, S_g, S_ff, S_gg, S_fg}
-  double m = 0.0, f = 0.0, g = 0.0, sf = 0.0, sg = 0.0, sff = 0.0, sgg = 0.0, sfg = 0.0;
+  Comp m, f, g, sf, sg, sff, sgg, sfg;
   int count = 0;
   __device__ __forceinline__ void add(float fv, float gv, float mv)
   {
-    const double x = fv, y = gv;
-    m += (double)mv;
-    f += x;
-    g += y;
+    m.add(mv);
+    f.add(fv);
+    g.add(gv);
     if (mv > 0.5f) {
       ++count;
-      sf += x;
-      sg += y;
-      sff += x * x;
-      sgg += y * y;
-      sfg += x * y;
+      sf.add(fv);
+      sg.add(gv);
+      sff.add(fv * fv);
+      sgg.add(gv * gv);
+      sfg.add(fv * gv);
     }
   }
   __device__ __forceinline__ void to_double(double* o) const
   {
-    o[0] = m;
-    o[1] = f;
-    o[2] = g;
+    o[0] = m.value();
+    o[1] = f.value();
+    o[2] = g.value();
     o[3] = (double)count;
-    o[4] = sf;
-    o[5] = sg;
-    o[6] = sff;
-    o[7] = sgg;
-    o[8] = sfg;
+    o[4] = sf.value();
+    o[5] = sg.value();
+    o[6] = sff.value();
+    o[7] = sgg.value();
+    o[8] = sfg.value();
   }
 };
 
 // Moments of a post-processing tile, where the mask is 0 or 1 and f and g
 // are already multiplied by it: a masked-out pixel adds zeros, which leave
-// a sum as it was, so sum f = S_f, sum g = S_g and sum mask = count, and six
-// sums give the nine of Moments with the same bits.
+// a compensated sum as it was, so sum f = S_f, sum g = S_g and sum mask =
+// count, and six sums give the nine of Moments with the same bits.
 struct MaskedMoments {
   int m = 0;  // the mask is 0 or 1: its sum is a count
-  double f = 0.0, g = 0.0, ff = 0.0, gg = 0.0, fg = 0.0;
+  Comp f, g, ff, gg, fg;
   __device__ __forceinline__ void add(float fv, float gv, float mv)
   {
-    const double x = fv, y = gv;
     m += mv > 0.5f;
-    f += x;
-    g += y;
-    ff += x * x;
-    gg += y * y;
-    fg += x * y;
+    f.add(fv);
+    g.add(gv);
+    ff.add(fv * fv);
+    gg.add(gv * gv);
+    fg.add(fv * gv);
   }
   __device__ __forceinline__ void to_double6(double* o) const
   {
     o[0] = (double)m;
-    o[1] = f;
-    o[2] = g;
-    o[3] = ff;
-    o[4] = gg;
-    o[5] = fg;
+    o[1] = f.value();
+    o[2] = g.value();
+    o[3] = ff.value();
+    o[4] = gg.value();
+    o[5] = fg.value();
   }
 };
 

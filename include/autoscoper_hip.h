@@ -226,8 +226,7 @@ ASC_API int asc_mask_multiply(asc_context ctx, float* drr, float* rad, const flo
  * sum g, count(mask>0.5), S_f, S_g, S_ff, S_gg, S_fg} with the S_ over
  * mask > 0.5. Blocking (returns a host value, like the reference).
  * asc_context_set_ncc_form selects how asc_ncc computes:
- *   ASC_NCC_MOMENTS (default): the moments summed in FP64 (products of
- *     the float inputs exact in FP64);
+ *   ASC_NCC_MOMENTS (default): compensated moments, FP64 combine;
  *   ASC_NCC_TREE_CUDA, ASC_NCC_TREE_OPENCL: the reference's own float
  *     computation, launch by launch: six tree sums (NccSum.cl / the CUDA
  *     sum kernel; 256-thread blocks with contiguous indexing, or 128-thread
