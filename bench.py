@@ -514,7 +514,7 @@ CFG1_TRACK = dict(method=1, speculative=True)
 
 
 def cfg1_leg(A, synth, ctx, args):
-    """configs[0] on the GPU and the CPU: the 128^3 u16 ellipsoid, two 512^2
+    """The small workload (cfg1) on the GPU and the CPU: the 128^3 u16 ellipsoid, two 512^2
     cameras; the fused batch rate (100 poses per launch), the simplex frame
     loop of 10 frames (each started 2 mm / 2 deg off its hidden pose), and
     the CPU oracle's evaluations and simplex frames on the host cores."""
@@ -681,7 +681,7 @@ def main(argv=None) -> int:
                     help="cfg2 frame loop (PSO 100 x 100 + simplex per frame): frames per storage (0 = skip)")
     ap.add_argument("--fp64-frames", type=int, default=1,
                     help="frames of the u16 frame loop checked against the oracle's all-FP64 simplex polish")
-    ap.add_argument("--cfg1", type=int, default=1, help="also run configs[0] (128^3, 2 x 512^2) on GPU and CPU")
+    ap.add_argument("--cfg1", type=int, default=1, help="also run the small workload (cfg1: 128^3, 2 x 512^2) on GPU and CPU")
     ap.add_argument("--track-only", default="", metavar="STORAGES",
                     help="only the frame loop, for these comma-separated storages (with --seeds)")
     ap.add_argument("--seeds", default="1", help="PSO seeds of the --track-only frame loops, comma-separated")
