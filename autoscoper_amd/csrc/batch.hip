@@ -39,6 +39,9 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kTile = 16;   // ray-march tile side
 constexpr int kPTile = 32;  // post-processing tile side (4 pixels per thread)
+constexpr int kChainCopies = 0, kChainSobel = 1, kChainGeneral = 2;
+// pixels of a thread whose Sobel neighbourhoods are loaded together (9 loads each)
+constexpr int kDirectGroup = 4;
 constexpr int kMaxChain = 8;
 constexpr int kMaxHalo = 12;
 constexpr int kSmallHalo = 4;  // chains up to this halo use the smaller-LDS post kernel
@@ -46,6 +49,9 @@ constexpr int kSmallHalo = 4;  // chains up to this halo use the smaller-LDS pos
 struct ChainDev {
   int n;
   int halo;
+  // the chain's stages other than copies: kChainCopies (none: the output is
+  // the input), kChainSobel (one Sobel) or kChainGeneral
+  int shape;
   asc_filter f[kMaxChain];
   int gn[kMaxChain];    // Gaussian/sharpen size N (1 = copy)
   int woff[kMaxChain];  // offset of the weights in the weight pool
@@ -468,7 +474,51 @@ __global__ __launch_bounds__(kBlock) void post_tiles(BatchArgs a)
     const int w = it.w, h = it.h;
 
     // --- DRR chain: stage 0 = DRR scratch ---
-    {
+    if (cam.chain[0].shape != kChainGeneral) {
+      // no stage, or one Sobel: each thread's 4 outputs straight from the
+      // DRR scratch (Sobel's neighbourhood clamped to the image as the
+      // reference's kernel clamps it, the same values the LDS form reads),
+      // every load issued before any result is stored; positions outside
+      // the image read an in-image pixel and are not stored
+      const ChainDev& ch = cam.chain[0];
+      const float* src = a.drr + it.drr_off;
+      int sob = 0;
+      for (int k = 0; k < ch.n; k++)
+        if (ch.f[k].enabled && ch.f[k].type == ASC_FILTER_SOBEL) sob = k;
+      const float scale = ch.f[sob].p[0], blend = ch.f[sob].p[1];
+      const int cx = min(x0 + tx, w - 1), xm = max(cx - 1, 0), xp = min(cx + 1, w - 1);
+      float v[4];
+      if (ch.shape == kChainCopies) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = src[(long long)min(y0 + ty + 8 * k, h - 1) * w + cx];
+      } else {
+#pragma unroll
+        for (int k0 = 0; k0 < 4; k0 += kDirectGroup) {
+          float p[kDirectGroup][9];
+#pragma unroll
+          for (int k = k0; k < k0 + kDirectGroup; k++) {
+            const int cy = min(y0 + ty + 8 * k, h - 1), ym = max(cy - 1, 0), yp = min(cy + 1, h - 1);
+            const float* r0 = src + (long long)ym * w;
+            const float* r1 = src + (long long)cy * w;
+            const float* r2 = src + (long long)yp * w;
+            float* q = p[k - k0];
+            q[0] = r0[xm], q[1] = r0[cx], q[2] = r0[xp];
+            q[3] = r1[xm], q[4] = r1[cx], q[5] = r1[xp];
+            q[6] = r2[xm], q[7] = r2[cx], q[8] = r2[xp];
+          }
+#pragma unroll
+          for (int k = k0; k < k0 + kDirectGroup; k++) {
+            const float* q = p[k - k0];
+            v[k] = dev::sobel_core(q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], scale, blend);
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int ly = ty + 8 * k;
+        if (x0 + tx < w && y0 + ly < h) sdrr[ly * kPTile + tx] = v[k];
+      }
+    } else {
       const ChainDev& ch = cam.chain[0];
       const int H0 = ch.halo, side = kPTile + 2 * H0;
       const int ox = x0 - H0, oy = y0 - H0;
@@ -796,6 +846,17 @@ int scene_upload_config(asc_scene s)
         ch.radius[k] = dev::filter_radius(fs[k], ch.gn[k]);
         ch.halo += ch.radius[k];
       }
+      // run_chain's copies: a disabled stage, or a Gaussian / Sharpen of size <= 1
+      int n_work = 0, n_sobel = 0;
+      for (int k = 0; k < ch.n; k++) {
+        const bool copy = !fs[k].enabled ||
+                          ((fs[k].type == ASC_FILTER_GAUSSIAN || fs[k].type == ASC_FILTER_SHARPEN) && ch.gn[k] <= 1);
+        if (!copy) {
+          ++n_work;
+          n_sobel += fs[k].type == ASC_FILTER_SOBEL;
+        }
+      }
+      ch.shape = n_work == 0 ? kChainCopies : (n_work == 1 && n_sobel == 1) ? kChainSobel : kChainGeneral;
       if (ch.halo > kMaxHalo) return fail(ASC_ERR_UNSUPPORTED, "filter chain halo exceeds 12 pixels");
       max_halo = std::max(max_halo, ch.halo);
     }
