@@ -152,6 +152,52 @@ __device__ __forceinline__ int next_tile(unsigned int* counters, int total, int*
   return -1;
 }
 
+// next_tile with the following claim issued ahead: thread 0 holds the
+// counter value it claimed for the next tile while the block works on the
+// current one, so the atomic's round trip overlaps that work. The claim's
+// partition and the count of partitions this block has exhausted live in
+// LDS (`st`: {partition, exhausted}), written by thread 0 only. A claim past
+// its partition's range moves the block on to the next partition, as in
+// next_tile; every index inside a range is handed out once, and the block
+// stops when all partitions are drained. next_tile_ahead_init makes the
+// first claim.
+__device__ __forceinline__ unsigned next_tile_ahead_init(unsigned int* counters, int* st)
+{
+  unsigned claim = 0;
+  if (threadIdx.x == 0) {
+    st[0] = (int)(blockIdx.x % kParts);
+    st[1] = 0;
+    claim = atomicAdd(&counters[st[0]], 1u);
+  }
+  return claim;
+}
+
+__device__ __forceinline__ int next_tile_ahead(unsigned int* counters, int total, int* slot, int* st, unsigned& claim)
+{
+  const int chunk = (total + kParts - 1) / kParts;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = -1;
+    for (;;) {
+      const int p = st[0];
+      const int cand = p * chunk + (int)claim;
+      if (cand < min((p + 1) * chunk, total)) {
+        t = cand;
+        break;
+      }
+      const int part = st[1] + 1;
+      st[1] = part;
+      if (part >= kParts) break;
+      st[0] = (int)((blockIdx.x + part) % kParts);
+      claim = atomicAdd(&counters[st[0]], 1u);
+    }
+    *slot = t;
+    if (t >= 0) claim = atomicAdd(&counters[st[0]], 1u);
+  }
+  __syncthreads();
+  return __builtin_amdgcn_readfirstlane(*slot);
+}
+
 // Enumeration index -> (item, tile column, tile row); false when the tile
 // lies outside that item's bounding box.
 template <bool POST>
@@ -447,7 +493,7 @@ __device__ __forceinline__ SampAxis samp_axis(float s, int n, unsigned stride)
 }
 
 template <int MAXH>
-__global__ __launch_bounds__(kBlock) void post_tiles(BatchArgs a)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void post_tiles(BatchArgs a)
 {
   constexpr int R = kPTile + 2 * MAXH;  // region row stride
   __shared__ float buf0[R * R];
@@ -461,9 +507,10 @@ __global__ __launch_bounds__(kBlock) void post_tiles(BatchArgs a)
   const int total = hd.post.total;
   const int debug = hd.debug;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  int part = 0;
+  __shared__ int s_claim[2];
+  unsigned claim = next_tile_ahead_init(a.counters->post_next, s_claim);
   for (;;) {
-    const int t = next_tile(a.counters->post_next, total, &s_tile, part);
+    const int t = next_tile_ahead(a.counters->post_next, total, &s_tile, s_claim, claim);
     if (t < 0) break;
     int ii, ptx, pty;
     if (!decode_tile<true>(hd, a.items, t, ii, ptx, pty)) continue;
