@@ -152,9 +152,11 @@ __device__ __forceinline__ int next_tile(unsigned int* counters, int total, int*
   return -1;
 }
 
-// next_tile with the following claim issued ahead: thread 0 holds the
-// counter value it claimed for the next tile while the block works on the
-// current one, so the atomic's round trip overlaps that work. The claim's
+// next_tile with the claim for the following tile issued ahead: thread 0
+// holds the counter value it claimed while the block works on the current
+// tile, so the atomic's round trip overlaps that work (post_tiles, whose
+// tiles are latency-bound; drr_tiles keeps next_tile: holding a claimed
+// tile there slowed the march by 5%). The claim's
 // partition and the count of partitions this block has exhausted live in
 // LDS (`st`: {partition, exhausted}), written by thread 0 only. A claim past
 // its partition's range moves the block on to the next partition, as in
