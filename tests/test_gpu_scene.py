@@ -207,6 +207,9 @@ def test_launch_shapes_and_autotune_do_not_change_results(asc, ctx, tiny):
     ("sharpen/gauss3-sobel", lambda F: ([F.sharpen(2.0, 0.5, 0.05)], [F.gaussian(3.0), F.sobel(2.0, 0.7)])),
     # no filters at all; an even-sized Contrast window
     ("none/contrast4", lambda F: ([], [F.contrast(2.5, 0.9, 4)])),
+    # one Sobel behind a disabled stage (the DRR chain straight from the
+    # scratch, blend >= 0.5); no radiograph filter
+    ("off-sobel/none", lambda F: ([F.gaussian(1.0, enabled=False), F.sobel(1.2, 0.6)], [])),
 ], ids=lambda c: c[0])
 def test_fused_filter_kinds_match_oracle_pipeline(asc, ctx, orc, tiny, chains):
     """Every filter kind through the fused per-tile chain (LDS stages, the
