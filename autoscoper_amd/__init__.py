@@ -11,6 +11,7 @@ native layer without it raises, there is no CPU fallback.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 from pathlib import Path
@@ -517,7 +518,47 @@ class Server:
         h = C.c_void_p()
         _check(lib().asc_server_create(ctx.handle, budget_bytes, max_active_jobs, C.byref(h)))
         self.handle = h.value
+        self.ctx = ctx
         self._jobs = {}
+        self._scene_keep = []  # volumes and images added to the server's scene by front ends
+
+    @contextlib.contextmanager
+    def scene(self):
+        """The server's scene under asc_server_lock_scene, for a front end's
+        volumes, cameras, camera sets and frames: a Scene that does not own
+        its handle; the batcher does not launch until the block ends (same
+        thread)."""
+        h = C.c_void_p()
+        _check(lib().asc_server_lock_scene(self.handle, C.byref(h)))
+        try:
+            yield Scene._shared(h, self.ctx, self._scene_keep)
+        finally:
+            _check(lib().asc_server_unlock_scene(self.handle))
+
+    def eval_requests(self, volumes, sets, frames, poses):
+        """n requests (scene volume, camera set, frame or -1, xyzypr), in the
+        batcher's next fused launch with every other pending request ->
+        (costs[n], per_camera[n, 8])."""
+        p = _f64(poses).reshape(-1, 6)
+        n = p.shape[0]
+        v = np.ascontiguousarray(volumes, np.int32)
+        st = np.ascontiguousarray(sets, np.int32)
+        fr = np.ascontiguousarray(frames if frames is not None else np.full(n, -1), np.int32)
+        costs = np.zeros(n)
+        per = np.zeros((n, 8))
+        _check(lib().asc_server_eval_requests(self.handle, n, _ptr(v, _i), _ptr(st, _i), _ptr(fr, _i), _ptr(p, _d),
+                                              _ptr(costs, _d), _ptr(per, _d)))
+        return costs, per
+
+    @contextlib.contextmanager
+    def stream(self):
+        """asc_server_stream_begin / _end around a stream of requests (an
+        optimisation), so that the batcher waits for its next request."""
+        _check(lib().asc_server_stream_begin(self.handle))
+        try:
+            yield
+        finally:
+            _check(lib().asc_server_stream_end(self.handle))
 
     def submit(self, volumes, voxel_mm, initial_poses, cameras, frames, render, drr_filters=(), rad_filters=(),
                drr=(0.5, 1.0, 0.0), storage=STORAGE_U16, guess=1, **options) -> int:
@@ -1026,6 +1067,21 @@ class Scene:
         self.nvol = 0
         self.render = (render_width, render_height)
         self._keep = []
+        self._owned = True
+
+    @classmethod
+    def _shared(cls, handle, ctx: Context, keep: list) -> "Scene":
+        """A scene owned elsewhere (a registration server's): close() leaves it,
+        and the objects added through it live in `keep`."""
+        sc = cls.__new__(cls)
+        sc.handle = handle
+        sc.ctx = ctx
+        sc.ncam = 0
+        sc.nvol = 0
+        sc.render = (0, 0)
+        sc._keep = keep
+        sc._owned = False
+        return sc
 
     def add_volume(self, vol: Volume, step: float, intensity: float, cutoff: float, pivot=None) -> int:
         idx = C.c_int()
@@ -1221,6 +1277,6 @@ class Scene:
         return s, st
 
     def close(self):
-        if self.handle:
+        if self.handle and self._owned:
             lib().asc_scene_destroy(self.handle)
-            self.handle = None
+        self.handle = None

@@ -143,3 +143,62 @@ def test_admission_one_job_at_a_time(asc, ctx, tiny, trial, rule):
     # b was admitted only after a finished (b arrived a few ms after a)
     assert sb.queue_ms >= sa.total_ms - 20.0
     assert np.array_equal(pa, ref) and np.array_equal(pb, ref)
+
+
+def test_front_ends_share_launches(asc, ctx, tiny, trial):
+    """Two front ends on one server (the server's scene under
+    asc_server_lock_scene; asc_server_eval_requests inside
+    asc_server_stream_begin / _end), each posting a chain of single-pose
+    requests from its own thread: their requests share launches, and every
+    cost is the same bits as the chain run with nothing beside it."""
+    import threading
+
+    wl = tiny["wl"]
+    RW, RH = tiny["render"]
+    srv = asc.Server(ctx, 16 << 30, 4)
+    vol = asc.Volume(ctx, wl.volume, scale=(wl.voxel_mm,) * 3)
+    with srv.scene() as sc:
+        v = sc.add_volume(vol, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0)
+        cams = []
+        for cam in tiny["cams"]:
+            c = sc.add_camera(asc.AscCamera.from_array(cam), None)
+            sc.set_filters(c, 0, wl.drr_filters)
+            sc.set_filters(c, 1, wl.rad_filters)
+            cams.append(c)
+        cs = sc.add_camera_set(cams, RW, RH)
+        frames = [sc.add_frame(cs, [asc.Image(ctx, r) for r in trial["frames"][f]]) for f in range(2)]
+    rng = np.random.default_rng(17)
+    chains = [wl.gt_poses[0] + np.concatenate([rng.uniform(-2, 2, (40, 3)), rng.uniform(-3, 3, (40, 3))], 1)
+              for _ in range(2)]
+
+    def run(k):
+        return [srv.eval_requests([v], [cs], [frames[k]], p[None])[0][0] for p in chains[k]]
+
+    alone = [run(k) for k in range(2)]
+    before = srv.stats()
+    shared = [None, None]
+    errors = []
+
+    def worker(k):
+        try:
+            with srv.stream():
+                shared[k] = run(k)
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    after = srv.stats()
+    srv.close()
+    vol.close()
+    assert not errors, errors
+    batches = after["batches"] - before["batches"]
+    requests = after["requests"] - before["requests"]
+    print(f"front ends: {requests} requests in {batches} launches")
+    assert requests == 80
+    assert requests > batches
+    for k in range(2):
+        assert np.array_equal(np.array(shared[k]), np.array(alone[k])), k
