@@ -7,6 +7,8 @@ their stated shapes, through the C ABI:
   downhill-simplex frame of the configuration (started 2 mm / 2 deg off the
   hidden pose) against the oracle's simplex from the same start: final
   poses within 0.05 mm / 0.05 deg.
+* the fused kernel's images at cfg1 size, where most post tiles take the
+  inner-tile loops, against the oracle's pipeline for two filter settings;
 * cfg5 shapes in the registration server: a 256x256x400 u16 volume tracked
   in four 1024^2 cameras, alone and in a mix with a 512^2 one-camera job
   and a 2-camera 768^2 job; its tracked poses are the same bits in both.
@@ -141,3 +143,51 @@ def test_cfg5_large_job_alone_equals_in_the_mix(asc, ctx):
                                               "latency_p99_ms")})
     assert np.array_equal(res[1][0], alone)
     assert stats["requests"] > stats["batches"]  # requests of different jobs shared launches
+
+
+@pytest.mark.parametrize("chains", [
+    # the reference's sample settings: the DRR Sobel straight from the scratch
+    ("sobel/sobel-contrast3", lambda F: ([F.sobel(1.7, 0.1)], [F.sobel(3.0, 0.43), F.contrast(2.5, 0.9, 3)])),
+    # LDS chains on both sides: Gaussian then Sobel on the DRR, Contrast 3x3 then Sharpen on the radiograph
+    ("gauss-sobel/contrast3-sharpen", lambda F: ([F.gaussian(1.0), F.sobel(1.2, 0.6)],
+                                                 [F.contrast(2.0, 1.1, 3), F.sharpen(1.5, 1.5, 0.01)])),
+], ids=lambda c: c[0])
+def test_cfg1_fused_images_match_oracle_pipeline(asc, orc, cfg1, chains):
+    """At cfg1 size the bounding boxes span many 32 x 32 post tiles, so most
+    tiles take the inner-tile loops (no per-pixel bounds or border tests)
+    and the rest the border forms. The fused kernel's DRR, radiograph and
+    mask images for one pose, against the oracle's pipeline: mask exact,
+    DRR channel within the bound the 1e-4 DRR bar propagates to through the
+    chain, radiograph within 1e-5."""
+    from chain_bound import assert_within, chain_bound, drr_bar
+
+    name, make = chains
+    drr_fs, rad_fs = make(asc.filters)
+    sc, S, wl = cfg1["sc"], cfg1["S"], cfg1["wl"]
+    prep = orc.volume_prep(wl.volume, (wl.voxel_mm,) * 3)
+    cams = [orc.camera_from_mayacam2(*c.mayacam2()) for c in wl.cameras]
+    RW, RH = wl.render
+    for c in range(len(cams)):
+        sc.set_filters(c, 0, drr_fs)
+        sc.set_filters(c, 1, rad_fs)
+    sc.set_debug(True)
+    pose = wl.gt_poses[0] + np.array([0.6, -0.4, 0.3, 1.5, -1.0, 2.0])
+    sc.eval_poses(pose[None])
+    try:
+        for c, cam in enumerate(cams):
+            _, imv, vp, (w, h) = orc.view_geometry(cam, pose, prep["inv_scale"], prep["inv_trans"], RW, RH)
+            assert w > 4 * 32 and h > 4 * 32  # inner tiles exist
+            d, g, m = sc.last_images(0, c, w, h)
+            drr, _ = orc.drr(prep["volume"], imv[:12], vp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, w, h)
+            drr_f = orc.filter_chain(asc.filters.to_records(drr_fs), drr)
+            rad_f = orc.filter_chain(asc.filters.to_records(rad_fs), orc.rad(S.rads[c], cam[24:28], vp, w, h))
+            mask = (drr_f != 0).astype(np.float32)
+            assert np.array_equal(m, mask), name
+            bound = chain_bound(asc.filters.to_records(drr_fs), drr, drr_bar(drr))
+            assert_within(d, drr_f * mask, bound * mask, name)
+            assert np.allclose(g, rad_f * mask, rtol=1e-5, atol=1e-7), name
+    finally:
+        sc.set_debug(False)
+        for c in range(len(cams)):
+            sc.set_filters(c, 0, wl.drr_filters)
+            sc.set_filters(c, 1, wl.rad_filters)
