@@ -45,6 +45,11 @@ constexpr int kDirectGroup = 4;
 constexpr int kMaxChain = 8;
 constexpr int kMaxHalo = 12;
 constexpr int kSmallHalo = 4;  // chains up to this halo use the smaller-LDS post kernel
+// region elements of a thread whose loads are issued together in a post
+// tile's stage-0 fill: 6 in the small-halo kernel (a 36 x 36 region is 5
+// or 6 per thread), 4 in the other (whose registers hold no more)
+template <int MAXH>
+constexpr int resample_group() { return MAXH <= kSmallHalo ? 6 : 4; }
 
 struct ChainDev {
   int n;
@@ -111,9 +116,14 @@ struct BatchHeader {
   TileEnum post;  // kPTile tiles
 };
 
+// Work counters, one per XCD partition and kernel, each on a 256-byte
+// line of its own: device-scope atomics to one address are serialised
+// where they are performed, so counters sharing a line would queue every
+// claim of the launch behind one another.
+constexpr int kCtrStride = 64;  // unsigned ints between counters (256 B)
 struct Counters {
-  unsigned int drr_next[kParts];
-  unsigned int post_next[kParts];
+  unsigned int drr_next[kParts * kCtrStride];
+  unsigned int post_next[kParts * kCtrStride];
 };
 
 struct BatchArgs {
@@ -134,14 +144,17 @@ struct BatchArgs {
 };
 
 // Next enumeration index for the whole block, or -1 once every partition
-// is drained. `part` counts the partitions this block has exhausted.
+// is drained. `part` counts the partitions this block has exhausted; it
+// starts with its own XCD's. One atomic per tile: claiming several tiles at
+// once, or the next tile ahead, widens the range of tiles in flight and so
+// the volume region the L2s must hold (both measured slower).
 __device__ __forceinline__ int next_tile(unsigned int* counters, int total, int* slot, int& part)
 {
   const int chunk = (total + kParts - 1) / kParts;
   while (part < kParts) {
     const int p = (int)((blockIdx.x + part) % kParts);
     __syncthreads();
-    if (threadIdx.x == 0) *slot = p * chunk + (int)atomicAdd(&counters[p], 1u);
+    if (threadIdx.x == 0) *slot = p * chunk + (int)atomicAdd(&counters[p * kCtrStride], 1u);
     __syncthreads();
     // the slot is block-uniform: say so, so that everything derived from it
     // (item, volume, descriptors) lives in scalar registers
@@ -150,54 +163,6 @@ __device__ __forceinline__ int next_tile(unsigned int* counters, int total, int*
     ++part;
   }
   return -1;
-}
-
-// next_tile with the claim for the following tile issued ahead: thread 0
-// holds the counter value it claimed while the block works on the current
-// tile, so the atomic's round trip overlaps that work (post_tiles, whose
-// tiles are latency-bound; drr_tiles keeps next_tile: holding a claimed
-// tile there slowed the march by 5%). The claim's
-// partition and the count of partitions this block has exhausted live in
-// LDS (`st`: {partition, exhausted}), written by thread 0 only. A claim past
-// its partition's range moves the block on to the next partition, as in
-// next_tile; every index inside a range is handed out once, and the block
-// stops when all partitions are drained. next_tile_ahead_init makes the
-// first claim.
-__device__ __forceinline__ unsigned next_tile_ahead_init(unsigned int* counters, int* st)
-{
-  unsigned claim = 0;
-  if (threadIdx.x == 0) {
-    st[0] = (int)(blockIdx.x % kParts);
-    st[1] = 0;
-    claim = atomicAdd(&counters[st[0]], 1u);
-  }
-  return claim;
-}
-
-__device__ __forceinline__ int next_tile_ahead(unsigned int* counters, int total, int* slot, int* st, unsigned& claim)
-{
-  const int chunk = (total + kParts - 1) / kParts;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int t = -1;
-    for (;;) {
-      const int p = st[0];
-      const int cand = p * chunk + (int)claim;
-      if (cand < min((p + 1) * chunk, total)) {
-        t = cand;
-        break;
-      }
-      const int part = st[1] + 1;
-      st[1] = part;
-      if (part >= kParts) break;
-      st[0] = (int)((blockIdx.x + part) % kParts);
-      claim = atomicAdd(&counters[st[0]], 1u);
-    }
-    *slot = t;
-    if (t >= 0) claim = atomicAdd(&counters[st[0]], 1u);
-  }
-  __syncthreads();
-  return __builtin_amdgcn_readfirstlane(*slot);
 }
 
 // Enumeration index -> (item, tile column, tile row); false when the tile
@@ -355,6 +320,45 @@ __device__ __forceinline__ void for_square(int side, int base, F&& fn)
   }
 }
 
+// for_square in groups of G consecutive steps of a thread, in two phases:
+// get(xx, yy) for all G positions of the group first (loads and
+// arithmetic, no stores), then put(c, value) for those inside the square,
+// so that a thread's loads for G elements are in flight together. A step
+// past the square's end calls get(0, 0), a position of the square, and its
+// value is dropped.
+template <int R, int G, class Get, class Put>
+__device__ __forceinline__ void for_square_grouped(int side, int base, Get&& get, Put&& put)
+{
+  const int n = side * side;
+  const float inv = 1.0f / (float)side;
+  int yy = (int)(((float)threadIdx.x + 0.5f) * inv);
+  int xx = (int)threadIdx.x - yy * side;
+  const int q = (int)(((float)kBlock + 0.5f) * inv);
+  const int r = kBlock - q * side;
+  int c = base + yy * R + xx;
+  for (int k0 = threadIdx.x; k0 < n; k0 += G * kBlock) {
+    float v[G];
+    int cs[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      const bool in = k0 + g * kBlock < n;
+      v[g] = get(in ? xx : 0, in ? yy : 0);
+      cs[g] = c;
+      xx += r;
+      yy += q;
+      c += q * R + r;
+      if (xx >= side) {
+        xx -= side;
+        ++yy;
+        c += R - side;
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++)
+      if (k0 + g * kBlock < n) put(cs[g], v[g]);
+  }
+}
+
 // Run a filter chain over the region in LDS (row stride R). On entry `cur`
 // holds stage-0 values at every in-image position of the post tile grown by
 // the chain's halo; on exit the returned buffer holds the chain output for
@@ -509,10 +513,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
   const int total = hd.post.total;
   const int debug = hd.debug;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  __shared__ int s_claim[2];
-  unsigned claim = next_tile_ahead_init(a.counters->post_next, s_claim);
+  int part = 0;
   for (;;) {
-    const int t = next_tile_ahead(a.counters->post_next, total, &s_tile, s_claim, claim);
+    const int t = next_tile(a.counters->post_next, total, &s_tile, part);
     if (t < 0) break;
     int ii, ptx, pty;
     if (!decode_tile<true>(hd, a.items, t, ii, ptx, pty)) continue;
@@ -575,7 +578,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
       if (ox >= 0 && oy >= 0 && ox + side <= w && oy + side <= h) {
         // the region wholly inside the image: rows of the DRR straight in
         const float* s0 = src + (long long)oy * w + ox;
-        for_square<R>(side, 0, [&](int c, int xx, int yy) { buf0[c] = s0[yy * w + xx]; });
+        for_square_grouped<R, resample_group<MAXH>()>(
+          side, 0, [&](int xx, int yy) { return s0[yy * w + xx]; }, [&](int c, float v) { buf0[c] = v; });
       } else {
         for_region(side, ox, oy, w, h,
                    [&](int xx, int yy, int px, int py) { buf0[yy * R + xx] = src[(long long)py * w + px]; });
@@ -627,7 +631,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         return (c.w < 0.0f || r.w < 0.0f) ? 0.0f : 1.0f - v;
       };
       if (ox >= 0 && oy >= 0 && ox + side <= w && oy + side <= h)
-        for_square<R>(side, 0, [&](int c, int xx, int yy) { buf0[c] = resample(xx, yy); });
+        for_square_grouped<R, resample_group<MAXH>()>(side, 0, resample, [&](int c, float v) { buf0[c] = v; });
       else
         for_region(side, ox, oy, w, h, [&](int xx, int yy, int, int) { buf0[yy * R + xx] = resample(xx, yy); });
       __syncthreads();
@@ -696,8 +700,8 @@ __global__ __launch_bounds__(kBlock) void finalize_items(BatchArgs a)
     __syncthreads();
   }
   if (blockIdx.x == 0 && threadIdx.x < kParts) {
-    a.counters->drr_next[threadIdx.x] = 0;
-    a.counters->post_next[threadIdx.x] = 0;
+    a.counters->drr_next[threadIdx.x * kCtrStride] = 0;
+    a.counters->post_next[threadIdx.x * kCtrStride] = 0;
   }
 }
 

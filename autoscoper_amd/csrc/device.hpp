@@ -105,9 +105,10 @@ __device__ __forceinline__ VolRsrc vol_rsrc(const VolDev& v)
   return r;
 }
 
-// Internal storage code of a u16 volume whose pairs hold (x0, x1 - x0)
-// (VolDev::pair_diff); never seen by the C ABI.
+// Internal storage codes of u16 volumes in the encodings 1 and 2 of
+// VolDev::pair_diff (texel differences); never seen by the C ABI.
 constexpr int kStorageU16Diff = 16;
+constexpr int kStorageU16Grad = 17;
 
 // Two 16-bit texels packed in a dword -> raw floats.
 template <int ST>
@@ -241,18 +242,59 @@ __device__ __forceinline__ float trilinear(const VolDev& v, const VolRsrc& r, fl
   // (i0+1, j0+1, k0+1) is in the array and its byte offset
   // i0*esz + j0*sy + k0*sz + c0 is >= 0; the +1s are folded into c0
   const int voff = cell_offset<ST, LY>(r, cvt_flr(X), cvt_flr(Y), cvt_flr(Z));
-  if constexpr (ST == kStorageU16Diff) {
-    // pairs stored as (x0, x1 - x0): each x-lerp is fma(a, x1 - x0, x0) on
-    // two converted halves, the same values as the u16 path below
-    uint32_t q[4];
-    gather_pairs16<LY>(r, voff, q);
-    float x[4];
+  if constexpr (ST == kStorageU16Grad || ST == kStorageU16Diff || ST == ASC_STORAGE_U16) {
+    // u16 volumes (integer texels). Per slice, with the slice's 2 x 2
+    // block {t00, t10, t01, t11}: the x-lerp of row 0 is
+    // r0 = fma(a, t10 - t00, t00), and the y-lerp's difference of the two
+    // x-lerps, (t01 + a (t11 - t01)) - (t00 + a (t10 - t00)), is the exact
+    // dy + a dxy with the integers dy = t01 - t00 and
+    // dxy = (t11 - t01) - (t10 - t00): one FMA rounds it once. So the
+    // slice's value is fma(b, fma(a, dxy, dy), r0). Since that difference
+    // is >= -r0 before rounding, it is >= -r0 after (rounding is monotonic
+    // and odd), and the slice value is >= 0; it is 0 exactly when every
+    // texel of the slice with a positive weight is 0. The z-lerp is
+    // fma(c, v1 - v0, v0) on the two slice values, which keeps both
+    // properties. Every u16 encoding and layout computes these same
+    // integers (differences of integers below 2^24 are exact in float), so
+    // all give the same bits.
+    float x0[2], dx[2], dy[2], dxy[2];  // per slice
+    if constexpr (ST == kStorageU16Grad) {
+      // {t00 | t10 - t00, t01 - t00 | dxy} per slice
+      uint32_t q[4];
+      gather_pairs16<LY>(r, voff, q);
 #pragma unroll
-    for (int e = 0; e < 4; e++)
-      x[e] = fmaf(a, (float)(int)(int16_t)(uint16_t)(q[e] >> 16), (float)(q[e] & 0xFFFFu));
-    const float c0 = fmaf(b, x[1] - x[0], x[0]);
-    const float c1 = fmaf(b, x[3] - x[2], x[2]);
-    return fmaf(c, c1 - c0, c0);
+      for (int sl = 0; sl < 2; sl++) {
+        x0[sl] = (float)(q[2 * sl] & 0xFFFFu);
+        dx[sl] = (float)(int)(int16_t)(uint16_t)(q[2 * sl] >> 16);
+        dy[sl] = (float)(int)(int16_t)(uint16_t)(q[2 * sl + 1] & 0xFFFFu);
+        dxy[sl] = (float)(int)(int16_t)(uint16_t)(q[2 * sl + 1] >> 16);
+      }
+    } else if constexpr (ST == kStorageU16Diff) {
+      // {t00 | t10 - t00, t01 - t00 | t11 - t01} per slice
+      uint32_t q[4];
+      gather_pairs16<LY>(r, voff, q);
+#pragma unroll
+      for (int sl = 0; sl < 2; sl++) {
+        x0[sl] = (float)(q[2 * sl] & 0xFFFFu);
+        dx[sl] = (float)(int)(int16_t)(uint16_t)(q[2 * sl] >> 16);
+        dy[sl] = (float)(int)(int16_t)(uint16_t)(q[2 * sl + 1] & 0xFFFFu);
+        dxy[sl] = (float)(int)(int16_t)(uint16_t)(q[2 * sl + 1] >> 16) - dx[sl];
+      }
+    } else {
+      float t[8];
+      gather8<ST, LY>(r, voff, t);
+#pragma unroll
+      for (int sl = 0; sl < 2; sl++) {
+        const float* u = t + 4 * sl;
+        x0[sl] = u[0];
+        dx[sl] = u[1] - u[0];
+        dy[sl] = u[2] - u[0];
+        dxy[sl] = (u[3] - u[2]) - dx[sl];
+      }
+    }
+    const float v0 = fmaf(b, fmaf(a, dxy[0], dy[0]), fmaf(a, dx[0], x0[0]));
+    const float v1 = fmaf(b, fmaf(a, dxy[1], dy[1]), fmaf(a, dx[1], x0[1]));
+    return fmaf(c, v1 - v0, v0);
   }
   if constexpr (ST == ASC_STORAGE_F16) {
     // F16 volumes: the x-lerps read the half-precision texels directly in
@@ -474,6 +516,13 @@ __device__ __forceinline__ float march(const VolDev& v, const Ray& r, float cuto
   }
   switch (v.storage) {
     case ASC_STORAGE_U16:
+      if (v.pair_diff == 2) {
+        switch (v.layout) {
+          case ASC_LAYOUT_QUAD: return march_st<kStorageU16Grad, ASC_LAYOUT_QUAD, UNROLL>(v, r, cutoff, nsamples);
+          case ASC_LAYOUT_OCTET: return march_st<kStorageU16Grad, ASC_LAYOUT_OCTET, UNROLL>(v, r, cutoff, nsamples);
+          default: return march_st<kStorageU16Grad, ASC_LAYOUT_BRICK, UNROLL>(v, r, cutoff, nsamples);
+        }
+      }
       if (v.pair_diff) {
         switch (v.layout) {
           case ASC_LAYOUT_QUAD: return march_st<kStorageU16Diff, ASC_LAYOUT_QUAD, UNROLL>(v, r, cutoff, nsamples);

@@ -44,10 +44,14 @@ int hip_fail(hipError_t e, const char* what);
 // block (8 texels), so a trilinear sample reads 4, 2 or 1 aligned vectors.
 // BRICK stores OCTET cells grouped in 2x2x2 bricks (pW and pS then count
 // the cells of one brick row and one brick layer; see layout_shape()).
-// pair_diff (u16 volumes in QUAD, OCTET or BRICK whose stored values are all
-// <= 32767): every (x0, x1) texel pair of an element is stored as x0 and the
-// signed 16-bit difference x1 - x0, which is exactly the integer the x-lerp
-// needs.
+// pair_diff, the encoding of u16 volumes in QUAD, OCTET or BRICK (the
+// integers each 2x2 (x, y) block {t00, t10, t01, t11} of a slice holds):
+//   0: the texels as they are;
+//   1 (every texel <= 32767): the first texel and three signed 16-bit
+//      differences: {t00, t10 - t00, t01 - t00, t11 - t01};
+//   2 (every texel <= 16383): the row differences too:
+//      {t00, t10 - t00, t01 - t00, (t11 - t01) - (t10 - t00)}.
+// Each is exactly the integers the march's lerps need (device.hpp).
 struct VolDev {
   const void* data;
   int W, H, D;
@@ -57,7 +61,7 @@ struct VolDev {
   int layout;    // ASC_LAYOUT_*
   int esz;       // bytes per element of the layout
   int nbytes;    // bytes of the layout's array (the buffer's extent)
-  int pair_diff; // 1: x1 texels of the pairs hold x1 - x0 (u16 paired layouts)
+  int pair_diff; // u16 paired layouts: the encoding 0, 1 or 2 above
 };
 
 struct ImgDev {

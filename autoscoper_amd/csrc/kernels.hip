@@ -209,13 +209,24 @@ __global__ __launch_bounds__(kBlock) void ncc_final_kernel(const double* __restr
   }
 }
 
-// The value stored for corner q of an element: with pair_diff (u16 only),
-// odd corners (x1) hold x1 - x0 as a signed 16-bit integer.
+// Stores the corner texels v[0..per) of one element (x fastest, then y,
+// then z: 2x2 (x, y) blocks of one slice each) in the encoding `enc` of
+// VolDev::pair_diff. Differences are stored as signed 16-bit integers (the
+// host picks an encoding whose differences fit); enc is 0 for F32 storage.
 template <typename T>
-__device__ __forceinline__ T pair_value(T v, T prev, int q, int diff)
+__device__ __forceinline__ void store_element(T* __restrict__ dst, const T* v, int per, int enc)
 {
-  if (diff && (q & 1)) return (T)((int)v - (int)prev);
-  return v;
+  for (int s = 0; s < per; s += 4) {
+    if (per == 1 || enc == 0) {
+      for (int q = s; q < s + 4 && q < per; q++) dst[q] = v[q];
+      continue;
+    }
+    const int t00 = (int)v[s], t10 = (int)v[s + 1], t01 = (int)v[s + 2], t11 = (int)v[s + 3];
+    dst[s] = (T)t00;
+    dst[s + 1] = (T)(t10 - t00);
+    dst[s + 2] = (T)(t01 - t00);
+    dst[s + 3] = (T)(enc == 2 ? (t11 - t01) - (t10 - t00) : t11 - t01);
+  }
 }
 
 // QUAD / OCTET packing: element (i, j, k) of the padded grid receives the
@@ -228,15 +239,14 @@ __global__ __launch_bounds__(kBlock) void pack_kernel(const T* __restrict__ src,
   const size_t n = (size_t)pW * pH * pD;
   for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (size_t)gridDim.x * kBlock) {
     const int i = (int)(e % pW), j = (int)((e / pW) % pH), k = (int)(e / ((size_t)pW * pH));
-    T prev = T(0);
+    T v[8];
     for (int q = 0; q < per; q++) {
       const int di = q & 1, dj = (q >> 1) & 1, dk = (q >> 2) & 1;
       const int ii = i + di, jj = j + dj, kk = k + dk;
-      T v = T(0);
-      if (ii < pW && jj < pH && kk < pD) v = src[((size_t)kk * pH + jj) * pW + ii];
-      dst[e * per + q] = pair_value(v, prev, q, diff);
-      prev = v;
+      v[q] = T(0);
+      if (ii < pW && jj < pH && kk < pD) v[q] = src[((size_t)kk * pH + jj) * pW + ii];
     }
+    store_element(dst + e * per, v, per, diff);
   }
 }
 
@@ -255,15 +265,14 @@ __global__ __launch_bounds__(kBlock) void pack_brick_kernel(const T* __restrict_
     const int c = (int)(e & 7);
     const int bx = (int)(b % BX), by = (int)((b / BX) % BY), bz = (int)(b / ((size_t)BX * BY));
     const int ci = 2 * bx + (c >> 2) - 2, cj = 2 * by + (c & 1) - 2, ck = 2 * bz + ((c >> 1) & 1) - 2;
-    T prev = T(0);
+    T v[8];
     for (int q = 0; q < 8; q++) {
       // padded coordinates of corner q
       const int ii = ci + (q & 1) + 1, jj = cj + ((q >> 1) & 1) + 1, kk = ck + (q >> 2) + 1;
-      T v = T(0);
-      if (ii >= 0 && jj >= 0 && kk >= 0 && ii < pW && jj < pH && kk < pD) v = src[((size_t)kk * pH + jj) * pW + ii];
-      dst[e * 8 + q] = pair_value(v, prev, q, diff);
-      prev = v;
+      v[q] = T(0);
+      if (ii >= 0 && jj >= 0 && kk >= 0 && ii < pW && jj < pH && kk < pD) v[q] = src[((size_t)kk * pH + jj) * pW + ii];
     }
+    store_element(dst + e * 8, v, 8, diff);
   }
 }
 

@@ -489,9 +489,11 @@ ASC_API int asc_volume_set_layout(asc_volume vol, int layout)
     vol->packed = nullptr;
     return fail(ASC_ERR_NOMEM, "packed volume allocation failed");
   }
-  // x1 - x0 of two u16 texels fits a signed 16-bit integer when every
-  // texel (and the zero border) lies in [0, 32767]
-  const int pair_diff = lin.storage == ASC_STORAGE_U16 && vol->u16_max <= 32767;
+  // the differences of u16 texels fit signed 16-bit integers: x1 - x0
+  // when every texel (and the zero border) lies in [0, 32767], and the row
+  // difference (t11 - t01) - (t10 - t00) too when they lie in [0, 16383]
+  const int pair_diff =
+    lin.storage != ASC_STORAGE_U16 ? 0 : vol->u16_max <= 16383 ? 2 : vol->u16_max <= 32767 ? 1 : 0;
   int rc = launch_pack_volume(ctx, lin, vol->packed, layout, pair_diff);
   if (rc) return rc;
   ASC_HIP(hipStreamSynchronize(ctx->stream));

@@ -570,24 +570,31 @@ def cfg1_leg(A, synth, ctx, args):
     return out
 
 
-def gather_probe(ctx, scene, batches, bone_of, ncams, reps=5):
+def gather_probe(ctx, scene, batches, bone_of, ncams, grids=(0,), reps=5):
     """The ray march's launch with its arithmetic replaced by the bare memory
     stream (ASC_TUNE_GATHER_PROBE): the same tiles, rays, sample counts and
-    16-byte cell loads. -> (ms per launch, ray samples per launch)."""
+    16-byte cell loads, at each persistent grid of `grids` (0 = one block
+    per CU slot); the fastest is the roof. The scene is left at the last
+    grid of `grids`. -> (ms per launch, ray samples per launch)."""
     P = len(bone_of)
     scene.use_graph(False)
     scene.set_timing(True)
     scene.set_tuning(scene.TUNE_GATHER_PROBE, 1)
-    ms, samples = 0.0, 0
+    best = None
     try:
-        for k in range(reps + 1):
-            scene.eval_poses(batches[k % len(batches)], volumes=bone_of)
-            if k:
-                ms += float(scene.last_timing()[0])
-                samples += scene.last_info(P * ncams)[1]
+        for grid in grids:
+            scene.set_tuning(scene.TUNE_GRID, grid)
+            ms, samples = 0.0, 0
+            for k in range(reps + 1):
+                scene.eval_poses(batches[k % len(batches)], volumes=bone_of)
+                if k:
+                    ms += float(scene.last_timing()[0])
+                    samples += scene.last_info(P * ncams)[1]
+            if best is None or ms / reps < best[0]:
+                best = (ms / reps, samples / reps)
     finally:
         scene.set_tuning(scene.TUNE_GATHER_PROBE, 0)
-    return ms / reps, samples / reps
+    return best
 
 
 VALU_ISSUE_PER_S = 256 * 2.4e9  # VALU issue cycles per second over the chip: 256 CUs at the 2.4 GHz peak clock
@@ -774,7 +781,10 @@ def main(argv=None) -> int:
     drr_ms = kern_ms[0] / args.steps
     bpv = BYTES_PER_VOXEL[args.storage]
     log(f"[bench] {evals_s:.1f} evals/s, {ms_step:.2f} ms/step, kernels ms {kern_ms / args.steps}")
-    probe = gather_probe(ctx, scene, batches, bone_of, len(cams)) if bpv == 2 else None
+    # the probe at the default grid and at the march's tuned one (the scene
+    # keeps the tuned grid afterwards)
+    tuned_grid = args.grid or (tuning or {}).get("grid", 0)
+    probe = gather_probe(ctx, scene, batches, bone_of, len(cams), (0, tuned_grid)) if bpv == 2 else None
     if probe:
         log(f"[bench] gather probe: {probe[0]:.3f} ms per launch, {16 * probe[1] / probe[0] / 1e9:.1f} TB/s")
 

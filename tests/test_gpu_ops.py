@@ -74,15 +74,16 @@ def test_drr_matches_oracle(asc, ctx, orc, tiny, storage):
     v.close()
 
 
-@pytest.mark.parametrize("storage,gain", [(0, 1), (0, 16), (1, 1), (2, 1), (3, 1)])
+@pytest.mark.parametrize("storage,gain", [(0, 1), (0, 8), (0, 16), (1, 1), (2, 1), (3, 1)])
 def test_layouts_are_bit_identical(asc, ctx, orc, tiny, storage, gain):
     """QUAD/OCTET/BRICK corner-replicated layouts change only the access
-    pattern. u16 volumes with every value <= 32767 store their x-pairs as
-    (x0, x1 - x0); gain 16 lifts the phantom's values past 32767, so the
-    pairs are stored plain: both encodings equal the LINEAR layout."""
+    pattern. The paired layouts of a u16 volume store differences of its
+    texels: the row differences too when every value is <= 16383 (gain 1),
+    the x-differences alone up to 32767 (gain 8), and the texels as they are
+    above that (gain 16). Every encoding gives the LINEAR layout's bits."""
     wl, prep = tiny["wl"], tiny["prep"]
     data = np.minimum(wl.volume.astype(np.uint32) * gain, 65535).astype(np.uint16)
-    assert (data.max() > 32767) == (gain > 1)
+    assert (data.max() <= 16383) == (gain == 1) and (data.max() > 32767) == (gain == 16)
     v = asc.Volume(ctx, data, scale=(wl.voxel_mm,) * 3, storage=storage)
     RW, RH = tiny["render"]
     cam = tiny["cams"][0]
