@@ -195,19 +195,19 @@ __device__ __forceinline__ bool decode_tile(const BatchHeader& hd, const ItemDes
 
 template <int UNROLL>
 __device__ __forceinline__ float march_item(const VolParams& vp, const ItemDesc& it, unsigned x, unsigned y,
-                                            int& nsamp)
+                                            int& nsamp, const int* tab)
 {
   dev::Ray r;
   if (!dev::ray_setup(it.imv, it.vp, vp.step, x, y, (unsigned)it.w, (unsigned)it.h, r)) return 0.0f;
-  return dev::drr_finish(dev::march<UNROLL>(vp.v, r, vp.cutoff, nsamp), vp.step_vscale, vp.intensity);
+  return dev::drr_finish(dev::march<UNROLL>(vp.v, r, vp.cutoff, nsamp, tab), vp.step_vscale, vp.intensity);
 }
 
 __device__ __forceinline__ float probe_item(const VolParams& vp, const ItemDesc& it, unsigned x, unsigned y,
-                                            int& nsamp)
+                                            int& nsamp, const int* tab)
 {
   dev::Ray r;
   if (!dev::ray_setup(it.imv, it.vp, vp.step, x, y, (unsigned)it.w, (unsigned)it.h, r)) return 0.0f;
-  return dev::gather_probe(vp.v, r, nsamp);
+  return dev::gather_probe(vp.v, r, nsamp, tab);
 }
 
 // PROBE: the gather probe in place of the march (same tiles, rays and
@@ -217,16 +217,24 @@ __global__ __launch_bounds__(kBlock) void drr_tiles(BatchArgs a)
 {
   __shared__ int s_tile;
   __shared__ unsigned long long s_samples;
+  __shared__ int s_tab[kBrickTab];  // BRICK offset tables of volume tab_vol
   const BatchHeader& hd = *a.hdr;
   const int total = hd.drr.total;
   const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int part = 0;
+  int part = 0, tab_vol = -1;
   for (;;) {
     const int t = next_tile(a.counters->drr_next, total, &s_tile, part);
     if (t < 0) break;
     int ii, tx, ty;
     if (!decode_tile<false>(hd, a.items, t, ii, tx, ty)) continue;
     const ItemDesc& it = a.items[ii];
+    if (it.vol != tab_vol) {
+      // every wave has passed next_tile's barrier, so none still reads the
+      // tables of the previous volume
+      dev::brick_tables(a.vols[it.vol].v, s_tab);
+      __syncthreads();
+      tab_vol = it.vol;
+    }
     // each of the tile's 4 waves marches a pw x (64 / pw) patch of rays,
     // pw = 2^wave_shift; the patches cover the 16 x 16 tile row by row.
     // Inside a patch, lanes 4q..4q+3 take the q-th 2 x 2 block of rays (row
@@ -246,7 +254,8 @@ __global__ __launch_bounds__(kBlock) void drr_tiles(BatchArgs a)
     int nsamp = 0;
     if (x < (unsigned)it.w && y < (unsigned)it.h) {
       const VolParams& vp = a.vols[it.vol];
-      const float val = PROBE ? probe_item(vp, it, x, y, nsamp) : march_item<UNROLL>(vp, it, x, y, nsamp);
+      const float val =
+        PROBE ? probe_item(vp, it, x, y, nsamp, s_tab) : march_item<UNROLL>(vp, it, x, y, nsamp, s_tab);
       a.drr[it.drr_off + (long long)y * it.w + x] = val;
     }
     // per-tile sample count (for the algorithmic-bytes figure)

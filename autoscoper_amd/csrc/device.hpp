@@ -59,6 +59,8 @@ struct VolRsrc {
   __amdgpu_buffer_rsrc_t rsrc;
   int sy, sz;  // byte strides of +1 row and +1 slice (< 2^23, checked on the host)
   int c0;      // byte offset of padded element (1, 1, 1): voxel (0, 0, 0)
+  const int* tab;  // BRICK: the block's LDS offset tables (brick_tables)
+  int tz;          // index of the z table in `tab` (H + 1)
 };
 
 // log2 of the bytes per element of layout LY over storage ST: one texel
@@ -87,7 +89,7 @@ __device__ __forceinline__ int mad_i24(int a, int b, int c)
 
 // The volume of a ray is uniform across the wave (one volume per tile);
 // readfirstlane states that, so the descriptor and strides are scalar.
-__device__ __forceinline__ VolRsrc vol_rsrc(const VolDev& v)
+__device__ __forceinline__ VolRsrc vol_rsrc(const VolDev& v, const int* tab)
 {
   const uint64_t base = (uint64_t)(uintptr_t)v.data;
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)base);
@@ -102,7 +104,32 @@ __device__ __forceinline__ VolRsrc vol_rsrc(const VolDev& v)
   // byte offset of the cell of voxel (0, 0, 0): padded element (1, 1, 1),
   // or brick (1, 1, 1) cell 0 in the BRICK layout (8 cells per brick)
   r.c0 = r.sy + r.sz + __builtin_amdgcn_readfirstlane(v.layout == ASC_LAYOUT_BRICK ? 8 * v.esz : v.esz);
+  r.tab = tab;
+  r.tz = __builtin_amdgcn_readfirstlane(v.H + 1);
   return r;
+}
+
+// The BRICK layout's y and z offset terms for cells j = -1 .. H-1 and
+// k = -1 .. D-1 (every cell an in-box sample can touch), filled by the
+// whole block into `tab` (kBrickTab ints of LDS): with E bytes per cell and
+// row/layer strides SY, SZ, cell (i, j, k) lives at
+//   4E i + [E j + (SY - 2E)(j >> 1)] + [2E k + (SZ - 4E)(k >> 1)] + c0
+// (cell_offset), and the bracketed terms are tab[j + 1] and
+// tab[H + 2 + k]. Other layouts need no table. The caller synchronises the
+// block before the tables are read.
+__device__ __forceinline__ void brick_tables(const VolDev& v, int* tab)
+{
+  if (v.layout != ASC_LAYOUT_BRICK) return;
+  const int E = v.esz, SY = v.pW * v.esz, SZ = v.pS * v.esz, H = v.H, n = v.H + v.D + 2;
+  for (int e = (int)threadIdx.x; e < n; e += (int)blockDim.x) {
+    if (e <= H) {
+      const int j = e - 1;
+      tab[e] = E * j + (SY - 2 * E) * (j >> 1);
+    } else {
+      const int k = e - H - 2;
+      tab[e] = 2 * E * k + (SZ - 4 * E) * (k >> 1);
+    }
+  }
 }
 
 // Internal storage codes of u16 volumes in the encodings 1 and 2 of
@@ -218,10 +245,10 @@ __device__ __forceinline__ int cell_offset(const VolRsrc& r, int i, int j, int k
     //   4E*(i&1) + 8E*((i>>1)+1) = 4E*i + 8E,
     //   E*(j&1) + SY*((j>>1)+1) = E*j + (SY-2E)*(j>>1) + SY,
     //   2E*(k&1) + SZ*((k>>1)+1) = 2E*k + (SZ-4E)*(k>>1) + SZ;
-    // the constants sum to r.c0, and x needs no split at all
-    constexpr int S = elem_shift<ST, LY>(), E = 1 << S;
-    const int base = (i << (S + 2)) + (j << S) + (k << (S + 1)) + r.c0;
-    return mad_i24(j >> 1, r.sy - 2 * E, mad_i24(k >> 1, r.sz - 4 * E, base));
+    // the constants sum to r.c0, x needs no split at all, and the y and z
+    // terms come from the block's LDS tables (brick_tables)
+    constexpr int S = elem_shift<ST, LY>();
+    return (i << (S + 2)) + r.c0 + r.tab[j + 1] + r.tab[r.tz + 1 + k];
   } else {
     return mad_i24(k, r.sz, mad_i24(j, r.sy, (i << elem_shift<ST, LY>()) + r.c0));
   }
@@ -411,9 +438,10 @@ __device__ __forceinline__ void accumulate_line(const VolDev& v, const VolRsrc& 
 // so a wave keeps 4*UNROLL loads in flight; the sum keeps the reference's
 // sample order.
 template <int ST, int LY, bool CUT, int UNROLL>
-__device__ __forceinline__ float march_loop(const VolDev& v, const Ray& r, float cutoff, int& nsamples)
+__device__ __forceinline__ float march_loop(const VolDev& v, const Ray& r, float cutoff, int& nsamples,
+                                            const int* tab)
 {
-  const VolRsrc rs = vol_rsrc(v);
+  const VolRsrc rs = vol_rsrc(v, tab);
   const float vscale = v.vscale;
   const float W = (float)v.W, H = (float)v.H, D = (float)v.D;
   const int n = sample_count(r);
@@ -457,9 +485,9 @@ __device__ __forceinline__ float march_loop(const VolDev& v, const Ray& r, float
 // and BRICK layouts, whose cells are 16 bytes; other volumes return 0 and
 // count no samples).
 template <int LY>
-__device__ __forceinline__ uint32_t gather_probe_st(const VolDev& v, const Ray& r, int& nsamples)
+__device__ __forceinline__ uint32_t gather_probe_st(const VolDev& v, const Ray& r, int& nsamples, const int* tab)
 {
-  const VolRsrc rs = vol_rsrc(v);
+  const VolRsrc rs = vol_rsrc(v, tab);
   const VoxelLine l = voxel_line(r, (float)v.W, (float)v.H, (float)v.D);
   const int n = sample_count(r);
   uint32_t acc = 0;
@@ -484,50 +512,52 @@ __device__ __forceinline__ uint32_t gather_probe_st(const VolDev& v, const Ray& 
   return acc;
 }
 
-__device__ __forceinline__ float gather_probe(const VolDev& v, const Ray& r, int& nsamples)
+__device__ __forceinline__ float gather_probe(const VolDev& v, const Ray& r, int& nsamples, const int* tab)
 {
   uint32_t acc = 0;
   if (v.storage != ASC_STORAGE_F32) {
     if (v.layout == ASC_LAYOUT_BRICK)
-      acc = gather_probe_st<ASC_LAYOUT_BRICK>(v, r, nsamples);
+      acc = gather_probe_st<ASC_LAYOUT_BRICK>(v, r, nsamples, tab);
     else if (v.layout == ASC_LAYOUT_OCTET)
-      acc = gather_probe_st<ASC_LAYOUT_OCTET>(v, r, nsamples);
+      acc = gather_probe_st<ASC_LAYOUT_OCTET>(v, r, nsamples, tab);
   }
   return __uint_as_float(acc & 0x007FFFFFu);  // a finite value depending on every load
 }
 
 template <int ST, int LY, int UNROLL>
-__device__ __forceinline__ float march_st(const VolDev& v, const Ray& r, float cutoff, int& nsamples)
+__device__ __forceinline__ float march_st(const VolDev& v, const Ray& r, float cutoff, int& nsamples,
+                                          const int* tab)
 {
-  if (cutoff > 0.0f) return march_loop<ST, LY, true, UNROLL>(v, r, cutoff, nsamples);
-  return march_loop<ST, LY, false, UNROLL>(v, r, cutoff, nsamples);
+  if (cutoff > 0.0f) return march_loop<ST, LY, true, UNROLL>(v, r, cutoff, nsamples, tab);
+  return march_loop<ST, LY, false, UNROLL>(v, r, cutoff, nsamples, tab);
 }
 
 // Dispatch on the volume's storage and layout (uniform per wave).
+// tab: the block's BRICK tables for v (brick_tables, synchronised)
 template <int UNROLL>
-__device__ __forceinline__ float march(const VolDev& v, const Ray& r, float cutoff, int& nsamples)
+__device__ __forceinline__ float march(const VolDev& v, const Ray& r, float cutoff, int& nsamples, const int* tab)
 {
 #define ASC_MARCH_ST(ST)                                                                  \
   switch (v.layout) {                                                                     \
-    case ASC_LAYOUT_QUAD: return march_st<ST, ASC_LAYOUT_QUAD, UNROLL>(v, r, cutoff, nsamples);   \
-    case ASC_LAYOUT_OCTET: return march_st<ST, ASC_LAYOUT_OCTET, UNROLL>(v, r, cutoff, nsamples); \
-    case ASC_LAYOUT_BRICK: return march_st<ST, ASC_LAYOUT_BRICK, UNROLL>(v, r, cutoff, nsamples); \
-    default: return march_st<ST, ASC_LAYOUT_LINEAR, UNROLL>(v, r, cutoff, nsamples);              \
+    case ASC_LAYOUT_QUAD: return march_st<ST, ASC_LAYOUT_QUAD, UNROLL>(v, r, cutoff, nsamples, tab);   \
+    case ASC_LAYOUT_OCTET: return march_st<ST, ASC_LAYOUT_OCTET, UNROLL>(v, r, cutoff, nsamples, tab); \
+    case ASC_LAYOUT_BRICK: return march_st<ST, ASC_LAYOUT_BRICK, UNROLL>(v, r, cutoff, nsamples, tab); \
+    default: return march_st<ST, ASC_LAYOUT_LINEAR, UNROLL>(v, r, cutoff, nsamples, tab);              \
   }
   switch (v.storage) {
     case ASC_STORAGE_U16:
       if (v.pair_diff == 2) {
         switch (v.layout) {
-          case ASC_LAYOUT_QUAD: return march_st<kStorageU16Grad, ASC_LAYOUT_QUAD, UNROLL>(v, r, cutoff, nsamples);
-          case ASC_LAYOUT_OCTET: return march_st<kStorageU16Grad, ASC_LAYOUT_OCTET, UNROLL>(v, r, cutoff, nsamples);
-          default: return march_st<kStorageU16Grad, ASC_LAYOUT_BRICK, UNROLL>(v, r, cutoff, nsamples);
+          case ASC_LAYOUT_QUAD: return march_st<kStorageU16Grad, ASC_LAYOUT_QUAD, UNROLL>(v, r, cutoff, nsamples, tab);
+          case ASC_LAYOUT_OCTET: return march_st<kStorageU16Grad, ASC_LAYOUT_OCTET, UNROLL>(v, r, cutoff, nsamples, tab);
+          default: return march_st<kStorageU16Grad, ASC_LAYOUT_BRICK, UNROLL>(v, r, cutoff, nsamples, tab);
         }
       }
       if (v.pair_diff) {
         switch (v.layout) {
-          case ASC_LAYOUT_QUAD: return march_st<kStorageU16Diff, ASC_LAYOUT_QUAD, UNROLL>(v, r, cutoff, nsamples);
-          case ASC_LAYOUT_OCTET: return march_st<kStorageU16Diff, ASC_LAYOUT_OCTET, UNROLL>(v, r, cutoff, nsamples);
-          default: return march_st<kStorageU16Diff, ASC_LAYOUT_BRICK, UNROLL>(v, r, cutoff, nsamples);
+          case ASC_LAYOUT_QUAD: return march_st<kStorageU16Diff, ASC_LAYOUT_QUAD, UNROLL>(v, r, cutoff, nsamples, tab);
+          case ASC_LAYOUT_OCTET: return march_st<kStorageU16Diff, ASC_LAYOUT_OCTET, UNROLL>(v, r, cutoff, nsamples, tab);
+          default: return march_st<kStorageU16Diff, ASC_LAYOUT_BRICK, UNROLL>(v, r, cutoff, nsamples, tab);
         }
       }
       ASC_MARCH_ST(ASC_STORAGE_U16)

@@ -52,6 +52,11 @@ int hip_fail(hipError_t e, const char* what);
 //   2 (every texel <= 16383): the row differences too:
 //      {t00, t10 - t00, t01 - t00, (t11 - t01) - (t10 - t00)}.
 // Each is exactly the integers the march's lerps need (device.hpp).
+// The ray march addresses BRICK cells through per-block LDS tables of the
+// y and z offset terms (H + 1 and D + 1 ints); a volume whose H + D + 2
+// exceeds this many ints is stored in another layout.
+constexpr int kBrickTab = 4096;
+
 struct VolDev {
   const void* data;
   int W, H, D;

@@ -30,6 +30,9 @@ struct DrrArgs {
 
 __global__ __launch_bounds__(kBlock) void drr_kernel(VolDev v, DrrArgs a, float* __restrict__ out)
 {
+  __shared__ int s_tab[kBrickTab];
+  dev::brick_tables(v, s_tab);
+  __syncthreads();
   unsigned x, y;
   tile_pixel(blockIdx.x, blockIdx.y, x, y);
   if (x >= a.w || y >= a.h) return;
@@ -37,7 +40,7 @@ __global__ __launch_bounds__(kBlock) void drr_kernel(VolDev v, DrrArgs a, float*
   float val = 0.0f;
   if (dev::ray_setup(a.imv, a.vp, a.step, x, y, a.w, a.h, r)) {
     int n = 0;
-    val = dev::drr_finish(dev::march<2>(v, r, a.cutoff, n), a.step_vscale, a.intensity);
+    val = dev::drr_finish(dev::march<2>(v, r, a.cutoff, n, s_tab), a.step_vscale, a.intensity);
   }
   out[(size_t)y * a.w + x] = val;
 }

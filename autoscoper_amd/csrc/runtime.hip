@@ -361,6 +361,7 @@ static LayoutShape layout_shape(const VolDev& lin, int layout)
 static bool layout_fits(const VolDev& lin, int layout)
 {
   const LayoutShape s = layout_shape(lin, layout);
+  if (layout == ASC_LAYOUT_BRICK && lin.H + lin.D + 2 > kBrickTab) return false;  // the march's LDS tables
   return s.pS * s.esz < (int64_t(1) << 23) && s.bytes < (int64_t(1) << 31);
 }
 
