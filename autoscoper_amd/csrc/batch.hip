@@ -45,11 +45,6 @@ constexpr int kDirectGroup = 4;
 constexpr int kMaxChain = 8;
 constexpr int kMaxHalo = 12;
 constexpr int kSmallHalo = 4;  // chains up to this halo use the smaller-LDS post kernel
-// region elements of a thread whose loads are issued together in a post
-// tile's stage-0 fill: 6 in the small-halo kernel (a 36 x 36 region is 5
-// or 6 per thread), 4 in the other (whose registers hold no more)
-template <int MAXH>
-constexpr int resample_group() { return MAXH <= kSmallHalo ? 6 : 4; }
 
 struct ChainDev {
   int n;
@@ -329,45 +324,6 @@ __device__ __forceinline__ void for_square(int side, int base, F&& fn)
   }
 }
 
-// for_square in groups of G consecutive steps of a thread, in two phases:
-// get(xx, yy) for all G positions of the group first (loads and
-// arithmetic, no stores), then put(c, value) for those inside the square,
-// so that a thread's loads for G elements are in flight together. A step
-// past the square's end calls get(0, 0), a position of the square, and its
-// value is dropped.
-template <int R, int G, class Get, class Put>
-__device__ __forceinline__ void for_square_grouped(int side, int base, Get&& get, Put&& put)
-{
-  const int n = side * side;
-  const float inv = 1.0f / (float)side;
-  int yy = (int)(((float)threadIdx.x + 0.5f) * inv);
-  int xx = (int)threadIdx.x - yy * side;
-  const int q = (int)(((float)kBlock + 0.5f) * inv);
-  const int r = kBlock - q * side;
-  int c = base + yy * R + xx;
-  for (int k0 = threadIdx.x; k0 < n; k0 += G * kBlock) {
-    float v[G];
-    int cs[G];
-#pragma unroll
-    for (int g = 0; g < G; g++) {
-      const bool in = k0 + g * kBlock < n;
-      v[g] = get(in ? xx : 0, in ? yy : 0);
-      cs[g] = c;
-      xx += r;
-      yy += q;
-      c += q * R + r;
-      if (xx >= side) {
-        xx -= side;
-        ++yy;
-        c += R - side;
-      }
-    }
-#pragma unroll
-    for (int g = 0; g < G; g++)
-      if (k0 + g * kBlock < n) put(cs[g], v[g]);
-  }
-}
-
 // Run a filter chain over the region in LDS (row stride R). On entry `cur`
 // holds stage-0 values at every in-image position of the post tile grown by
 // the chain's halo; on exit the returned buffer holds the chain output for
@@ -587,8 +543,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
       if (ox >= 0 && oy >= 0 && ox + side <= w && oy + side <= h) {
         // the region wholly inside the image: rows of the DRR straight in
         const float* s0 = src + (long long)oy * w + ox;
-        for_square_grouped<R, resample_group<MAXH>()>(
-          side, 0, [&](int xx, int yy) { return s0[yy * w + xx]; }, [&](int c, float v) { buf0[c] = v; });
+        for_square<R>(side, 0, [&](int c, int xx, int yy) { buf0[c] = s0[yy * w + xx]; });
       } else {
         for_region(side, ox, oy, w, h,
                    [&](int xx, int yy, int px, int py) { buf0[yy * R + xx] = src[(long long)py * w + px]; });
@@ -640,7 +595,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
         return (c.w < 0.0f || r.w < 0.0f) ? 0.0f : 1.0f - v;
       };
       if (ox >= 0 && oy >= 0 && ox + side <= w && oy + side <= h)
-        for_square_grouped<R, resample_group<MAXH>()>(side, 0, resample, [&](int c, float v) { buf0[c] = v; });
+        for_square<R>(side, 0, [&](int c, int xx, int yy) { buf0[c] = resample(xx, yy); });
       else
         for_region(side, ox, oy, w, h, [&](int xx, int yy, int, int) { buf0[yy * R + xx] = resample(xx, yy); });
       __syncthreads();
