@@ -67,7 +67,7 @@ struct CamDev {
 };
 
 struct VolParams {
-  VolDev v;
+  VolDev v[3];  // the volume's view for rays along x, y, z (asc_volume_s::odev)
   float step, intensity, cutoff, step_vscale;
 };
 
@@ -77,6 +77,7 @@ struct ItemDesc {
   ImgDev rad;  // the radiograph of this item's frame and camera
   int w, h;
   int vol, cam;
+  int axis;                // view of the volume: ray_axis of imv
   int tiles_x, tiles_y;    // ray-march tiles (kTile)
   int ptiles_x, ptiles_y;  // post tiles (kPTile)
   int tile_base;  // first tile of this item in the batch's tile list
@@ -194,7 +195,8 @@ __device__ __forceinline__ float march_item(const VolParams& vp, const ItemDesc&
 {
   dev::Ray r;
   if (!dev::ray_setup(it.imv, it.vp, vp.step, x, y, (unsigned)it.w, (unsigned)it.h, r)) return 0.0f;
-  return dev::drr_finish(dev::march<UNROLL>(vp.v, r, vp.cutoff, nsamp, tab), vp.step_vscale, vp.intensity);
+  return dev::drr_finish(dev::march<UNROLL>(vp.v[it.axis], r, vp.cutoff, nsamp, tab), vp.step_vscale,
+                         vp.intensity);
 }
 
 __device__ __forceinline__ float probe_item(const VolParams& vp, const ItemDesc& it, unsigned x, unsigned y,
@@ -202,7 +204,7 @@ __device__ __forceinline__ float probe_item(const VolParams& vp, const ItemDesc&
 {
   dev::Ray r;
   if (!dev::ray_setup(it.imv, it.vp, vp.step, x, y, (unsigned)it.w, (unsigned)it.h, r)) return 0.0f;
-  return dev::gather_probe(vp.v, r, nsamp, tab);
+  return dev::gather_probe(vp.v[it.axis], r, nsamp, tab);
 }
 
 // PROBE: the gather probe in place of the march (same tiles, rays and
@@ -212,23 +214,23 @@ __global__ __launch_bounds__(kBlock) void drr_tiles(BatchArgs a)
 {
   __shared__ int s_tile;
   __shared__ unsigned long long s_samples;
-  __shared__ int s_tab[kBrickTab];  // BRICK offset tables of volume tab_vol
+  __shared__ int s_tab[kBrickTab];  // BRICK offset tables of view tab_view
   const BatchHeader& hd = *a.hdr;
   const int total = hd.drr.total;
   const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int part = 0, tab_vol = -1;
+  int part = 0, tab_view = -1;
   for (;;) {
     const int t = next_tile(a.counters->drr_next, total, &s_tile, part);
     if (t < 0) break;
     int ii, tx, ty;
     if (!decode_tile<false>(hd, a.items, t, ii, tx, ty)) continue;
     const ItemDesc& it = a.items[ii];
-    if (it.vol != tab_vol) {
+    if (it.vol * 3 + it.axis != tab_view) {
       // every wave has passed next_tile's barrier, so none still reads the
-      // tables of the previous volume
-      dev::brick_tables(a.vols[it.vol].v, s_tab);
+      // tables of the previous view
+      dev::brick_tables(a.vols[it.vol].v[it.axis], s_tab);
       __syncthreads();
-      tab_vol = it.vol;
+      tab_view = it.vol * 3 + it.axis;
     }
     // each of the tile's 4 waves marches a pw x (64 / pw) patch of rays,
     // pw = 2^wave_shift; the patches cover the 16 x 16 tile row by row.
@@ -890,7 +892,7 @@ int scene_upload_config(asc_scene s)
     std::memset(&vols[i], 0, sizeof(VolParams));
     if (!sv.in_use) continue;
     sv.generation_seen = sv.vol->generation;
-    vols[i].v = sv.vol->dev;
+    for (int a = 0; a < 3; a++) vols[i].v[a] = sv.vol->odev[a];
     vols[i].step = sv.step;
     vols[i].intensity = sv.intensity;
     vols[i].cutoff = sv.cutoff;
@@ -1034,6 +1036,7 @@ int eval_requests(asc_scene s, const std::vector<Request>& reqs, double* costs, 
         view_geometry(s->cams[c].cam, q.pose, d.inv_scale, d.inv_trans, RW, RH, it.imv, vp, sz);
       it.vol = q.vol;
       it.cam = c;
+      it.axis = s->vols[q.vol].vol->dev.layout == ASC_LAYOUT_BRICK ? ray_axis(s->vols[q.vol].vol->dev, it.imv) : 0;
       if (inview) {
         if (sz[0] > (unsigned)RW || sz[1] > (unsigned)RH)
           return fail(ASC_ERR_TOO_LARGE, "Rendered image is larger than the viewport buffer");
@@ -1673,7 +1676,8 @@ ASC_API int asc_scene_track_frame_plain(asc_scene s, int volume, const double xy
       const int w = (int)sz[0], h = (int)sz[1];
       // renderDrrSingle: DRR then the DRR chain, into `mask` as the filtered image holder
       float* drr_f = mask;
-      if ((rc = launch_drr(ctx, sv.vol->dev, imv, vpf, sv.step, sv.intensity, sv.cutoff, drr_raw, w, h))) return rc;
+      if ((rc = launch_drr(ctx, sv.vol->ray_view(imv), imv, vpf, sv.step, sv.intensity, sv.cutoff, drr_raw, w, h)))
+        return rc;
       if ((rc = asc_filter_chain(ctx, sc.chain[0].data(), (int)sc.chain[0].size(), drr_raw, drr_f, scratch, w, h)))
         return rc;
       // renderRad: resample then the rad chain
@@ -1755,7 +1759,8 @@ ASC_API int asc_scene_render_view(asc_scene s, int camera, const double* poses, 
     frame_row_order(frame_inverse(mv), m16);
     float imv[16];
     inv_model_view(m16, sv.vol->desc.inv_scale, sv.vol->desc.inv_trans, imv);
-    if ((rc = launch_drr(ctx, sv.vol->dev, imv, vp, sv.step, sv.intensity, sv.cutoff, drr, w, h))) return rc;
+    if ((rc = launch_drr(ctx, sv.vol->ray_view(imv), imv, vp, sv.step, sv.intensity, sv.cutoff, drr, w, h)))
+      return rc;
     if ((rc = launch_binary(ctx, BIN_MERGE, merged, drr, merged, n))) return rc;
   }
   if ((rc = asc_filter_chain(ctx, sc.chain[0].data(), (int)sc.chain[0].size(), merged, drr_f, scratch, (int)w,

@@ -60,7 +60,7 @@ struct VolRsrc {
   int sy, sz;  // byte strides of +1 row and +1 slice (< 2^23, checked on the host)
   int c0;      // byte offset of padded element (1, 1, 1): voxel (0, 0, 0)
   const int* tab;  // BRICK: the block's LDS offset tables (brick_tables)
-  int tz;          // index of the z table in `tab` (H + 1)
+  int ty, tz;      // indices of the y and z tables in `tab` (W + 1, W + H + 2)
 };
 
 // log2 of the bytes per element of layout LY over storage ST: one texel
@@ -101,34 +101,35 @@ __device__ __forceinline__ VolRsrc vol_rsrc(const VolDev& v, const int* tab)
   r.rsrc = __builtin_amdgcn_make_buffer_rsrc(p, 0, bytes, 0x00020000);
   r.sy = __builtin_amdgcn_readfirstlane(v.pW * v.esz);
   r.sz = __builtin_amdgcn_readfirstlane(v.pS * v.esz);
-  // byte offset of the cell of voxel (0, 0, 0): padded element (1, 1, 1),
-  // or brick (1, 1, 1) cell 0 in the BRICK layout (8 cells per brick)
-  r.c0 = r.sy + r.sz + __builtin_amdgcn_readfirstlane(v.layout == ASC_LAYOUT_BRICK ? 8 * v.esz : v.esz);
+  // byte offset of the element of voxel (0, 0, 0): padded element (1, 1, 1)
+  // (BRICK copies use their offset tables instead)
+  r.c0 = r.sy + r.sz + __builtin_amdgcn_readfirstlane(v.esz);
   r.tab = tab;
-  r.tz = __builtin_amdgcn_readfirstlane(v.H + 1);
+  r.ty = __builtin_amdgcn_readfirstlane(v.W + 1);
+  r.tz = __builtin_amdgcn_readfirstlane(v.W + v.H + 2);
   return r;
 }
 
-// The BRICK layout's y and z offset terms for cells j = -1 .. H-1 and
+// The BRICK copy's offset terms for cells i = -1 .. W-1, j = -1 .. H-1 and
 // k = -1 .. D-1 (every cell an in-box sample can touch), filled by the
-// whole block into `tab` (kBrickTab ints of LDS): with E bytes per cell and
-// row/layer strides SY, SZ, cell (i, j, k) lives at
-//   4E i + [E j + (SY - 2E)(j >> 1)] + [2E k + (SZ - 4E)(k >> 1)] + c0
-// (cell_offset), and the bracketed terms are tab[j + 1] and
-// tab[H + 2 + k]. Other layouts need no table. The caller synchronises the
-// block before the tables are read.
+// whole block into `tab` (kBrickTab ints of LDS): cell c of axis a lies at
+// bin[a] (c mod 2^blog[a]) + bstr[a] (floor(c / 2^blog[a]) + 1) bytes of
+// that axis, and a cell's offset is the sum of its three terms, tab[i + 1]
+// + tab[W + 2 + j] + tab[W + H + 3 + k] (cell_offset; the pack kernel
+// places cells by the same terms). Other layouts need no table. The caller
+// synchronises the block before the tables are read.
 __device__ __forceinline__ void brick_tables(const VolDev& v, int* tab)
 {
   if (v.layout != ASC_LAYOUT_BRICK) return;
-  const int E = v.esz, SY = v.pW * v.esz, SZ = v.pS * v.esz, H = v.H, n = v.H + v.D + 2;
-  for (int e = (int)threadIdx.x; e < n; e += (int)blockDim.x) {
-    if (e <= H) {
-      const int j = e - 1;
-      tab[e] = E * j + (SY - 2 * E) * (j >> 1);
-    } else {
-      const int k = e - H - 2;
-      tab[e] = 2 * E * k + (SZ - 4 * E) * (k >> 1);
+  const int n[3] = {v.W, v.H, v.D};
+  int base = 0;
+  for (int a = 0; a < 3; a++) {
+    const int l = v.blog[a], m = (1 << l) - 1, bi = v.bin[a], bs = v.bstr[a];
+    for (int e = (int)threadIdx.x; e <= n[a]; e += (int)blockDim.x) {
+      const int c = e - 1;
+      tab[base + e] = bi * (c & m) + bs * ((c >> l) + 1);
     }
+    base += n[a] + 1;
   }
 }
 
@@ -239,16 +240,9 @@ template <int ST, int LY>
 __device__ __forceinline__ int cell_offset(const VolRsrc& r, int i, int j, int k)
 {
   if constexpr (LY == ASC_LAYOUT_BRICK) {
-    // cell (i, j, k) lives in brick (i/2+1, j/2+1, k/2+1) (floor division),
-    // at cell (j&1) + 2(k&1) + 4(i&1) of the brick; with E bytes per cell, a
-    // brick of 8E bytes, and row/layer strides SY = r.sy, SZ = r.sz:
-    //   4E*(i&1) + 8E*((i>>1)+1) = 4E*i + 8E,
-    //   E*(j&1) + SY*((j>>1)+1) = E*j + (SY-2E)*(j>>1) + SY,
-    //   2E*(k&1) + SZ*((k>>1)+1) = 2E*k + (SZ-4E)*(k>>1) + SZ;
-    // the constants sum to r.c0, x needs no split at all, and the y and z
-    // terms come from the block's LDS tables (brick_tables)
-    constexpr int S = elem_shift<ST, LY>();
-    return (i << (S + 2)) + r.c0 + r.tab[j + 1] + r.tab[r.tz + 1 + k];
+    // the copy's per-axis offset terms from the block's LDS tables
+    // (brick_tables); in-box samples have -1 <= i <= W-1 (likewise j, k)
+    return r.tab[i + 1] + r.tab[r.ty + 1 + j] + r.tab[r.tz + 1 + k];
   } else {
     return mad_i24(k, r.sz, mad_i24(j, r.sy, (i << elem_shift<ST, LY>()) + r.c0));
   }

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -42,8 +43,13 @@ int hip_fail(hipError_t e, const char* what);
 // Layouts (ASC_LAYOUT_*): LINEAR stores one texel per element; QUAD stores
 // the 2x2 (x, y) corner block of each element (4 texels), OCTET the 2x2x2
 // block (8 texels), so a trilinear sample reads 4, 2 or 1 aligned vectors.
-// BRICK stores OCTET cells grouped in 2x2x2 bricks (pW and pS then count
-// the cells of one brick row and one brick layer; see layout_shape()).
+// BRICK stores OCTET cells grouped in bricks of 8 cells (128 bytes for
+// 16-bit texels), oriented across the rays: a volume in BRICK keeps one copy
+// per ray axis a (x, y, z), whose bricks are 1 cell along a, 2 along the
+// next axis and 4 along the third (x: 1x2x4, y: 2x1x4, z: 2x4x1), laid out
+// with a fastest; the march reads the copy across each item's central ray
+// (ray_axis). A cell's byte offset is the sum over the axes of
+// bin[a] * (c_a mod 2^blog[a]) + bstr[a] * (floor(c_a / 2^blog[a]) + 1).
 // pair_diff, the encoding of u16 volumes in QUAD, OCTET or BRICK (the
 // integers each 2x2 (x, y) block {t00, t10, t01, t11} of a slice holds):
 //   0: the texels as they are;
@@ -53,8 +59,8 @@ int hip_fail(hipError_t e, const char* what);
 //      {t00, t10 - t00, t01 - t00, (t11 - t01) - (t10 - t00)}.
 // Each is exactly the integers the march's lerps need (device.hpp).
 // The ray march addresses BRICK cells through per-block LDS tables of the
-// y and z offset terms (H + 1 and D + 1 ints); a volume whose H + D + 2
-// exceeds this many ints is stored in another layout.
+// x, y and z offset terms (W + 1, H + 1 and D + 1 ints); a volume whose
+// W + H + D + 3 exceeds this many ints is stored in another layout.
 constexpr int kBrickTab = 4096;
 
 struct VolDev {
@@ -67,7 +73,19 @@ struct VolDev {
   int esz;       // bytes per element of the layout
   int nbytes;    // bytes of the layout's array (the buffer's extent)
   int pair_diff; // u16 paired layouts: the encoding 0, 1 or 2 above
+  // BRICK: per axis the log2 of the brick's extent in cells, the in-brick
+  // stride and the brick stride in bytes
+  int blog[3], bin[3], bstr[3];
 };
+
+// The axis (0 x, 1 y, 2 z) along which the central ray of a view with
+// inverse model-view `imv` advances most voxels per unit step: its
+// direction is imv's third column, and the unit box spans W, H, D voxels.
+__host__ __device__ inline int ray_axis(const VolDev& v, const float* imv)
+{
+  const float ax = fabsf(imv[2]) * (float)v.W, ay = fabsf(imv[6]) * (float)v.H, az = fabsf(imv[10]) * (float)v.D;
+  return ax >= ay && ax >= az ? 0 : (ay >= az ? 1 : 2);
+}
 
 struct ImgDev {
   const void* data;  // float texels, already normalised (UNORM -> [0, 1])
@@ -107,10 +125,19 @@ struct asc_volume_s {
   asc::VolDev dev;
   void* dptr;       // LINEAR copy (always kept)
   size_t bytes;
-  void* packed = nullptr;  // QUAD / OCTET copy when selected
+  // the packed copies of the selected layout: one (QUAD, OCTET) or one per
+  // ray axis (BRICK), and their device views; in LINEAR all three views are
+  // `dev`
+  void* packed[3] = {nullptr, nullptr, nullptr};
+  asc::VolDev odev[3];
   size_t packed_bytes = 0;
   int generation = 0;      // bumped whenever `dev` changes
   int u16_max = 0;         // largest stored texel of a u16 volume
+  // the view the march reads for rays of a view with inverse model-view imv
+  const asc::VolDev& ray_view(const float* imv) const
+  {
+    return odev[dev.layout == ASC_LAYOUT_BRICK ? asc::ray_axis(dev, imv) : 0];
+  }
 };
 
 struct asc_image_s {
@@ -126,6 +153,7 @@ namespace asc {
 // ---------------------------------------------------------------------------
 // Kernel launchers (kernels.hip).
 // ---------------------------------------------------------------------------
+// v: the view of the volume for imv's rays (asc_volume_s::ray_view)
 int launch_drr(asc_context ctx, const VolDev& v, const float* imv, const float* vp, float step, float intensity,
                float cutoff, float* out, unsigned w, unsigned h);
 int launch_rad(asc_context ctx, const ImgDev& img, const float* plane, const float* vp, float* out, unsigned w,
@@ -137,6 +165,8 @@ int launch_composite(asc_context ctx, const float* s1, const float* s2, const fl
 int launch_fill_f32(asc_context ctx, float* dst, float v, size_t n);
 int launch_copy(asc_context ctx, void* dst, const void* src, size_t bytes);
 int launch_pack_volume(asc_context ctx, const VolDev& linear, void* dst, int layout, int pair_diff);
+// BRICK copy `dst` described by `brick` (its blog, bin, bstr and nbytes)
+int launch_pack_brick(asc_context ctx, const VolDev& linear, const VolDev& brick, void* dst, int pair_diff);
 // gpu::ncc as the reference computes it (float tree sums, Ncc.cpp:180-220):
 // sums6 = {mask, f, g, products, f dev^2, g dev^2} tree sums (products NaN
 // when the denominator is below 1e-5 and the reference skips that sum).

@@ -253,29 +253,45 @@ __global__ __launch_bounds__(kBlock) void pack_kernel(const T* __restrict__ src,
   }
 }
 
-// BRICK layout from the padded linear volume (pW = W+2, pH = H+2, pD = D+2,
-// voxel x at padded x+1): brick (bx, by, bz), cell c = py + 2pz + 4px holds
-// the 2x2x2 corner texels of voxel cell (2bx+px-2, 2by+py-2, 2bz+pz-2);
-// texels outside the padded grid are 0. The x bit is the highest of the
-// cell index, so along x a cell's offset is simply 4 cells per step.
+// A BRICK copy from the padded linear volume (pW = W+2, pH = H+2, pD = D+2,
+// voxel x at padded x+1). Every element of the copy is visited once: the
+// bricks of an axis of extent 2^l cover cells -2^l .. nb 2^l - 2^l - 1, and
+// cell c of the axis lies at bin (c mod 2^l) + bstr (floor(c / 2^l) + 1)
+// (the march's offset tables hold the same terms). The element receives
+// the 2x2x2 corner texels of its cell (0 outside the padded grid) in the
+// order trilinear() reads them.
+struct BrickArgs {
+  int pW, pH, pD;  // padded linear grid
+  int cells[3];    // covered cells per axis (bricks x extent)
+  int blog[3], bin[3], bstr[3];
+  int esz;         // bytes per cell
+};
+
 template <typename T>
-__global__ __launch_bounds__(kBlock) void pack_brick_kernel(const T* __restrict__ src, T* __restrict__ dst, int pW,
-                                                            int pH, int pD, int BX, int BY, int BZ, int diff)
+__global__ __launch_bounds__(kBlock) void pack_brick_kernel(const T* __restrict__ src, T* __restrict__ dst, BrickArgs a,
+                                                            int diff)
 {
-  const size_t n = (size_t)BX * BY * BZ * 8;
+  const size_t n = (size_t)a.cells[0] * a.cells[1] * a.cells[2];
   for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (size_t)gridDim.x * kBlock) {
-    const size_t b = e >> 3;
-    const int c = (int)(e & 7);
-    const int bx = (int)(b % BX), by = (int)((b / BX) % BY), bz = (int)(b / ((size_t)BX * BY));
-    const int ci = 2 * bx + (c >> 2) - 2, cj = 2 * by + (c & 1) - 2, ck = 2 * bz + ((c >> 1) & 1) - 2;
+    int c[3];
+    c[0] = (int)(e % a.cells[0]);
+    c[1] = (int)((e / a.cells[0]) % a.cells[1]);
+    c[2] = (int)(e / ((size_t)a.cells[0] * a.cells[1]));
+    size_t off = 0;
+    for (int k = 0; k < 3; k++) {
+      c[k] -= 1 << a.blog[k];
+      off += (size_t)a.bin[k] * (size_t)(c[k] & ((1 << a.blog[k]) - 1)) +
+             (size_t)a.bstr[k] * (size_t)((c[k] >> a.blog[k]) + 1);
+    }
     T v[8];
     for (int q = 0; q < 8; q++) {
       // padded coordinates of corner q
-      const int ii = ci + (q & 1) + 1, jj = cj + ((q >> 1) & 1) + 1, kk = ck + (q >> 2) + 1;
+      const int ii = c[0] + (q & 1) + 1, jj = c[1] + ((q >> 1) & 1) + 1, kk = c[2] + (q >> 2) + 1;
       v[q] = T(0);
-      if (ii >= 0 && jj >= 0 && kk >= 0 && ii < pW && jj < pH && kk < pD) v[q] = src[((size_t)kk * pH + jj) * pW + ii];
+      if (ii >= 0 && jj >= 0 && kk >= 0 && ii < a.pW && jj < a.pH && kk < a.pD)
+        v[q] = src[((size_t)kk * a.pH + jj) * a.pW + ii];
     }
-    store_element(dst + e * 8, v, 8, diff);
+    store_element(dst + off / a.esz * 8, v, 8, diff);
   }
 }
 
@@ -396,23 +412,39 @@ int launch_pack_volume(asc_context ctx, const VolDev& v, void* dst, int layout, 
   const int per = layout == ASC_LAYOUT_QUAD ? 4 : 8;
   const int pH = v.pS / v.pW, pD = v.D + 2;
   const unsigned grid = 256 * 8;
-  if (layout == ASC_LAYOUT_BRICK) {
-    const int BX = (v.W + 3) / 2, BY = (v.H + 3) / 2, BZ = (v.D + 3) / 2;
-    if (v.storage == ASC_STORAGE_F32)
-      pack_brick_kernel<uint32_t><<<grid, kBlock, 0, ctx->stream>>>((const uint32_t*)v.data, (uint32_t*)dst, v.pW,
-                                                                    pH, pD, BX, BY, BZ, 0);
-    else
-      pack_brick_kernel<uint16_t><<<grid, kBlock, 0, ctx->stream>>>((const uint16_t*)v.data, (uint16_t*)dst, v.pW,
-                                                                    pH, pD, BX, BY, BZ, pair_diff);
-    ASC_HIP(hipGetLastError());
-    return ASC_OK;
-  }
+  if (layout == ASC_LAYOUT_BRICK) return fail(ASC_ERR_INVALID, "BRICK copies are packed by launch_pack_brick");
   if (v.storage == ASC_STORAGE_F32)
     pack_kernel<uint32_t><<<grid, kBlock, 0, ctx->stream>>>((const uint32_t*)v.data, (uint32_t*)dst, v.pW, pH, pD,
                                                             per, 0);
   else
     pack_kernel<uint16_t><<<grid, kBlock, 0, ctx->stream>>>((const uint16_t*)v.data, (uint16_t*)dst, v.pW, pH, pD,
                                                             per, pair_diff);
+  ASC_HIP(hipGetLastError());
+  return ASC_OK;
+}
+
+int launch_pack_brick(asc_context ctx, const VolDev& v, const VolDev& b, void* dst, int pair_diff)
+{
+  if (pair_diff && v.storage != ASC_STORAGE_U16)
+    return fail(ASC_ERR_INVALID, "texel differences need a u16 volume");
+  BrickArgs a;
+  a.pW = v.pW;
+  a.pH = v.pS / v.pW;
+  a.pD = v.D + 2;
+  const int n[3] = {v.W, v.H, v.D};
+  for (int k = 0; k < 3; k++) {
+    a.blog[k] = b.blog[k];
+    a.bin[k] = b.bin[k];
+    a.bstr[k] = b.bstr[k];
+    a.cells[k] = (((n[k] - 1) >> b.blog[k]) + 2) << b.blog[k];
+  }
+  a.esz = b.esz;
+  const unsigned grid = 256 * 8;
+  if (v.storage == ASC_STORAGE_F32)
+    pack_brick_kernel<uint32_t><<<grid, kBlock, 0, ctx->stream>>>((const uint32_t*)v.data, (uint32_t*)dst, a, 0);
+  else
+    pack_brick_kernel<uint16_t><<<grid, kBlock, 0, ctx->stream>>>((const uint16_t*)v.data, (uint16_t*)dst, a,
+                                                                  pair_diff);
   ASC_HIP(hipGetLastError());
   return ASC_OK;
 }

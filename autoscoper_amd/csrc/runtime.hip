@@ -167,6 +167,8 @@ int upload_volume(asc_context ctx, const Cropped<T>& c, float full, int storage,
   v->dev.esz = (int)esize;
   v->dev.nbytes = (int)std::min<size_t>(host.size(), 0x7fffffff);
   v->dev.pair_diff = 0;
+  for (int a = 0; a < 3; a++) v->dev.blog[a] = v->dev.bin[a] = v->dev.bstr[a] = 0;
+  for (VolDev& o : v->odev) o = v->dev;
   v->u16_max = u16_max;
   return ASC_OK;
 }
@@ -330,27 +332,19 @@ ASC_API int asc_volume_set_layout(asc_volume vol, int layout);
 // The ray march addresses an element with signed 24-bit multiplies of the
 // row and slice byte strides and a 32-bit buffer offset: a layout is usable
 // when both strides are below 2^23 bytes and the whole array below 2^31.
+// BRICK copies are addressed through LDS tables instead (no stride limit).
 struct LayoutShape {
   int64_t esz;    // bytes per element
   int64_t pW;     // elements per row step (bytes = pW * esz)
-  int64_t pS;     // elements per slice (or brick-layer) step
+  int64_t pS;     // elements per slice step
   int64_t bytes;  // whole array
 };
 
 static LayoutShape layout_shape(const VolDev& lin, int layout)
 {
   const int64_t tex = lin.storage == ASC_STORAGE_F32 ? 4 : 2;
-  LayoutShape s;
-  if (layout == ASC_LAYOUT_BRICK) {
-    // bricks of 2x2x2 cells over cells -2 .. 2B-3 per axis (cells -1 .. n-1 are sampled)
-    const int64_t BX = (lin.W + 3) / 2, BY = (lin.H + 3) / 2, BZ = (lin.D + 3) / 2;
-    s.esz = 8 * tex;
-    s.pW = 8 * BX;
-    s.pS = 8 * BX * BY;
-    s.bytes = s.pS * BZ * s.esz;
-    return s;
-  }
   const int64_t per = layout == ASC_LAYOUT_LINEAR ? 1 : layout == ASC_LAYOUT_QUAD ? 4 : 8;
+  LayoutShape s;
   s.esz = per * tex;
   s.pW = lin.W + 2;
   s.pS = (int64_t)(lin.W + 2) * (lin.H + 2);
@@ -358,10 +352,53 @@ static LayoutShape layout_shape(const VolDev& lin, int layout)
   return s;
 }
 
+// The BRICK copy for rays along `axis`: bricks of 1 cell along the axis, 2
+// along the next (x -> y -> z -> x) and 4 along the third, ordered with
+// the ray axis fastest. Cells -1 .. n-1 of each axis are sampled; brick b
+// of an axis of extent 2^l holds cells (b - 1) 2^l .. b 2^l - 1, so
+// floor((n - 1) / 2^l) + 2 bricks cover them. In-brick cells are ordered
+// with the extent-2 axis fastest. Fills blog, bin, bstr, esz and nbytes
+// (bytes in `bytes`).
+static VolDev brick_view(const VolDev& lin, int axis, int64_t* bytes)
+{
+  VolDev b = lin;
+  const int64_t E = 8 * (lin.storage == ASC_STORAGE_F32 ? 4 : 2);  // bytes per cell
+  const int n[3] = {lin.W, lin.H, lin.D};
+  const int u = (axis + 1) % 3, w = (axis + 2) % 3;
+  b.blog[axis] = 0;
+  b.blog[u] = 1;
+  b.blog[w] = 2;
+  b.bin[axis] = 0;
+  b.bin[u] = (int)E;
+  b.bin[w] = (int)(2 * E);
+  int64_t nb[3];
+  for (int a = 0; a < 3; a++) nb[a] = ((int64_t)(n[a] - 1) >> b.blog[a]) + 2;
+  const int64_t sa = 8 * E, su = nb[axis] * sa, sw = nb[u] * su;
+  *bytes = nb[w] * sw;
+  b.bstr[axis] = (int)sa;
+  b.bstr[u] = (int)std::min<int64_t>(su, INT32_MAX);
+  b.bstr[w] = (int)std::min<int64_t>(sw, INT32_MAX);
+  b.layout = ASC_LAYOUT_BRICK;
+  b.esz = (int)E;
+  b.pW = 0;
+  b.pS = 0;
+  b.nbytes = (int)std::min<int64_t>(*bytes, INT32_MAX);
+  return b;
+}
+
 static bool layout_fits(const VolDev& lin, int layout)
 {
+  if (layout == ASC_LAYOUT_BRICK) {
+    // the march's LDS offset tables, and 32-bit buffer offsets
+    if (lin.W + lin.H + lin.D + 3 > kBrickTab) return false;
+    for (int a = 0; a < 3; a++) {
+      int64_t bytes = 0;
+      brick_view(lin, a, &bytes);
+      if (bytes >= (int64_t(1) << 31)) return false;
+    }
+    return true;
+  }
   const LayoutShape s = layout_shape(lin, layout);
-  if (layout == ASC_LAYOUT_BRICK && lin.H + lin.D + 2 > kBrickTab) return false;  // the march's LDS tables
   return s.pS * s.esz < (int64_t(1) << 23) && s.bytes < (int64_t(1) << 31);
 }
 
@@ -462,13 +499,15 @@ ASC_API int asc_volume_set_layout(asc_volume vol, int layout)
   ASC_CHECK(vol, ASC_ERR_INVALID, "null volume");
   ASC_CHECK(layout >= ASC_LAYOUT_LINEAR && layout <= ASC_LAYOUT_BRICK, ASC_ERR_INVALID, "bad layout");
   ASC_CHECK(layout_fits(vol->dev, layout), ASC_ERR_TOO_LARGE,
-            "volume too large for this layout (slice >= 8 MiB or array >= 2 GiB)");
-  if (layout == vol->dev.layout) return ASC_OK;
+            "volume too large for this layout (slice >= 8 MiB, array >= 2 GiB, or W + H + D > 4093 for BRICK)");
+  if (layout == vol->dev.layout && vol->dev.data) return ASC_OK;
   asc_context ctx = vol->ctx;
   vol->generation++;
   ASC_HIP(hipStreamSynchronize(ctx->stream));
-  if (vol->packed) ASC_HIP(hipFree(vol->packed));
-  vol->packed = nullptr;
+  for (void*& p : vol->packed) {
+    if (p) ASC_HIP(hipFree(p));
+    p = nullptr;
+  }
   vol->packed_bytes = 0;
   VolDev lin = vol->dev;
   const LayoutShape lin_shape = layout_shape(lin, ASC_LAYOUT_LINEAR);
@@ -479,34 +518,48 @@ ASC_API int asc_volume_set_layout(asc_volume vol, int layout)
   lin.pS = (int)lin_shape.pS;
   lin.nbytes = (int)lin_shape.bytes;
   lin.pair_diff = 0;
-  if (layout == ASC_LAYOUT_LINEAR) {
-    vol->dev = lin;
-    return ASC_OK;
-  }
-  const LayoutShape shape = layout_shape(lin, layout);
-  const size_t bytes = (size_t)shape.bytes;
-  hipError_t e = hipMalloc(&vol->packed, bytes);
-  if (e != hipSuccess) {
-    vol->packed = nullptr;
-    return fail(ASC_ERR_NOMEM, "packed volume allocation failed");
-  }
+  vol->dev = lin;
+  for (VolDev& o : vol->odev) o = lin;
+  if (layout == ASC_LAYOUT_LINEAR) return ASC_OK;
   // the differences of u16 texels fit signed 16-bit integers: x1 - x0
   // when every texel (and the zero border) lies in [0, 32767], and the row
   // difference (t11 - t01) - (t10 - t00) too when they lie in [0, 16383]
   const int pair_diff =
     lin.storage != ASC_STORAGE_U16 ? 0 : vol->u16_max <= 16383 ? 2 : vol->u16_max <= 32767 ? 1 : 0;
-  int rc = launch_pack_volume(ctx, lin, vol->packed, layout, pair_diff);
-  if (rc) return rc;
+  const int copies = layout == ASC_LAYOUT_BRICK ? 3 : 1;
+  for (int a = 0; a < copies; a++) {
+    VolDev v;
+    int64_t bytes = 0;
+    if (layout == ASC_LAYOUT_BRICK) {
+      v = brick_view(lin, a, &bytes);
+    } else {
+      const LayoutShape shape = layout_shape(lin, layout);
+      v = lin;
+      v.layout = layout;
+      v.esz = (int)shape.esz;
+      v.pW = (int)shape.pW;
+      v.pS = (int)shape.pS;
+      v.nbytes = (int)shape.bytes;
+      bytes = shape.bytes;
+    }
+    hipError_t e = hipMalloc(&vol->packed[a], (size_t)bytes);
+    if (e != hipSuccess) {
+      vol->packed[a] = nullptr;
+      return fail(ASC_ERR_NOMEM, "packed volume allocation failed");
+    }
+    v.data = vol->packed[a];
+    v.pair_diff = pair_diff;
+    const int rc = layout == ASC_LAYOUT_BRICK ? launch_pack_brick(ctx, lin, v, vol->packed[a], pair_diff)
+                                              : launch_pack_volume(ctx, lin, vol->packed[a], layout, pair_diff);
+    if (rc) return rc;
+    vol->packed_bytes += (size_t)bytes;
+    vol->odev[a] = v;
+  }
   ASC_HIP(hipStreamSynchronize(ctx->stream));
-  vol->packed_bytes = bytes;
-  vol->dev = lin;
-  vol->dev.data = vol->packed;
-  vol->dev.layout = layout;
-  vol->dev.esz = (int)shape.esz;
-  vol->dev.pW = (int)shape.pW;
-  vol->dev.pS = (int)shape.pS;
-  vol->dev.nbytes = (int)shape.bytes;
-  vol->dev.pair_diff = pair_diff;
+  if (copies == 1) vol->odev[1] = vol->odev[2] = vol->odev[0];
+  // `dev` describes the volume (size, storage, layout); the march reads the
+  // view ray_view() picks
+  vol->dev = vol->odev[0];
   return ASC_OK;
 }
 
@@ -516,7 +569,8 @@ ASC_API int asc_volume_destroy(asc_volume vol)
   hipSetDevice(vol->ctx->device);
   hipStreamSynchronize(vol->ctx->stream);
   if (vol->dptr) hipFree(vol->dptr);
-  if (vol->packed) hipFree(vol->packed);
+  for (void* p : vol->packed)
+    if (p) hipFree(p);
   delete vol;
   return ASC_OK;
 }
@@ -718,7 +772,7 @@ ASC_API int asc_drr_render(asc_context ctx, asc_volume vol, const float imv[12],
                            float step, float intensity, float cutoff, float* out, unsigned width, unsigned height)
 {
   ASC_CHECK(valid_ctx(ctx) && vol && imv && viewport && out, ASC_ERR_INVALID, "bad argument");
-  return finish(ctx, launch_drr(ctx, vol->dev, imv, viewport, step, intensity, cutoff, out, width, height));
+  return finish(ctx, launch_drr(ctx, vol->ray_view(imv), imv, viewport, step, intensity, cutoff, out, width, height));
 }
 
 ASC_API int asc_rad_render(asc_context ctx, asc_image img, const float image_plane[4], const float viewport[4],

@@ -558,9 +558,11 @@ ASC_API int asc_server_submit(asc_server s, const asc_job_desc* d, int* job_id)
     std::array<double, 6> p;
     std::memcpy(p.data(), d->initial_poses + 6 * b, sizeof(double) * 6);
     j->init_pose.push_back(p);
-    // linear copy plus the 8-corner copy of the march layout
+    // the padded linear copy plus three 8-corner BRICK copies, one per ray
+    // axis, each covering at most (n + 8) cells per axis
     const size_t tex = d->storage == ASC_STORAGE_F32 ? 4 : 2;
-    dev += (size_t)(dm[0] + 3) * (dm[1] + 3) * (dm[2] + 3) * tex * 9;
+    dev += (size_t)(dm[0] + 2) * (dm[1] + 2) * (dm[2] + 2) * tex;
+    dev += (size_t)(dm[0] + 8) * (dm[1] + 8) * (dm[2] + 8) * tex * 8 * 3;
   }
   j->storage = d->storage;
   j->step = d->drr_step;

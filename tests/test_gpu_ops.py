@@ -86,18 +86,25 @@ def test_layouts_are_bit_identical(asc, ctx, orc, tiny, storage, gain):
     assert (data.max() <= 16383) == (gain == 1) and (data.max() > 32767) == (gain == 16)
     v = asc.Volume(ctx, data, scale=(wl.voxel_mm,) * 3, storage=storage)
     RW, RH = tiny["render"]
-    cam = tiny["cams"][0]
-    _, imv, _, _ = orc.view_geometry(cam, wl.gt_poses[0], prep["inv_scale"], prep["inv_trans"], RW, RH)
-    fvp = synth.full_frame_viewport(cam[24:28], RW, RH)
     out = ctx.alloc(4 * RW * RH)
-    imgs = []
-    for layout in (v.LINEAR, v.QUAD, v.OCTET, v.BRICK, v.LINEAR):
-        v.set_layout(layout)
-        ctx.drr_render(v, imv[:12], fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, out, RW, RH)
-        imgs.append(out.download((RH, RW)))
-    assert imgs[0].max() > 0
-    for im in imgs[1:]:
-        assert np.array_equal(im, imgs[0])
+    # camera 0 at the hidden pose looks along x; pitched 90 degrees, along z;
+    # camera 1 looks along y: each of BRICK's three copies is read
+    W, H, D = prep["dims"]
+    axes = set()
+    for cam, turn in ((tiny["cams"][0], (0, 0, 0)), (tiny["cams"][0], (0, 90, 0)), (tiny["cams"][1], (0, 0, 0))):
+        pose = wl.gt_poses[0] + np.array([0, 0, 0, *turn])
+        _, imv, _, _ = orc.view_geometry(cam, pose, prep["inv_scale"], prep["inv_trans"], RW, RH)
+        axes.add(int(np.argmax([abs(imv[2]) * W, abs(imv[6]) * H, abs(imv[10]) * D])))
+        fvp = synth.full_frame_viewport(cam[24:28], RW, RH)
+        imgs = []
+        for layout in (v.LINEAR, v.QUAD, v.OCTET, v.BRICK, v.LINEAR):
+            v.set_layout(layout)
+            ctx.drr_render(v, imv[:12], fvp, synth.DRR_STEP, synth.DRR_INTENSITY, 0.0, out, RW, RH)
+            imgs.append(out.download((RH, RW)))
+        assert imgs[0].max() > 0
+        for im in imgs[1:]:
+            assert np.array_equal(im, imgs[0])
+    assert axes == {0, 1, 2}
     v.close()
 
 
@@ -299,15 +306,29 @@ def test_ncc_reference_tree_forms_bit_exact(asc, ctx, orc, n):
     assert abs(v - ref) <= 1e-5 or (ref == 1e5 and v == 1e5)  # the default form is back
 
 
-def test_wide_slices_fall_back_to_a_layout_that_fits(asc, ctx):
-    """A 1100 x 1100 slice is too wide for BRICK/OCTET/QUAD addressing
-    (>= 8 MiB per slice step): creation falls back to LINEAR and asking for
-    a packed layout is an error, not a wrong result."""
+def test_wide_slices_take_a_layout_that_fits(asc, ctx):
+    """A 1100 x 1100 slice is too wide for OCTET/QUAD addressing (>= 8 MiB
+    per slice step): asking for those is an error, not a wrong result.
+    BRICK, addressed through its offset tables, takes it (creation picks
+    it) and renders the LINEAR layout's bits."""
     vol = np.full((2, 1100, 1100), 3000, np.uint16)
+    vol[:, 400:700, 300:350] = 1000
     v = asc.Volume(ctx, vol, scale=(0.5, 0.5, 0.5))
-    for layout in (v.QUAD, v.OCTET, v.BRICK):
+    for layout in (v.QUAD, v.OCTET):
         with pytest.raises(asc.AscError):
             v.set_layout(layout)
-    v.set_layout(v.LINEAR)
+    # rays from above the unit box ([0, 1] x [0, 1] x [-1, 0]) down through
+    # the middle of the slab
+    imv = np.zeros(12, np.float32)
+    imv[0], imv[5], imv[10] = 1.0, -1.0, 1.0
+    imv[3], imv[7], imv[11] = 0.5, 0.5, 0.5
+    vp = np.array([-0.3, -0.3, 0.6, 0.6])
+    out = ctx.alloc(4 * 64 * 64)
+    imgs = []
+    for layout in (v.BRICK, v.LINEAR):
+        v.set_layout(layout)
+        ctx.drr_render(v, imv, vp, 0.01, 1.0, 0.0, out, 64, 64)
+        imgs.append(out.download((64, 64)))
+    assert imgs[0].max() > 0 and np.array_equal(imgs[0], imgs[1])
     v.close()
 

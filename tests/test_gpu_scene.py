@@ -105,15 +105,39 @@ def test_background_mask_path(asc, ctx, orc, tiny):
     vol.close()
 
 
-def test_scene_follows_volume_layout(asc, ctx, tiny):
+def ray_axes(orc, tiny, poses):
+    """The BRICK copy each (pose, camera) item reads: the axis along which
+    the central ray advances most voxels (internal.hpp ray_axis)."""
+    prep = tiny["prep"]
+    W, H, D = prep["dims"]
+    RW, RH = tiny["render"]
+    axes = set()
+    for p in poses:
+        for cam in tiny["cams"]:
+            inview, imv, _, _ = orc.view_geometry(cam, p, prep["inv_scale"], prep["inv_trans"], RW, RH)
+            if inview:
+                axes.add(int(np.argmax([abs(imv[2]) * W, abs(imv[6]) * H, abs(imv[10]) * D])))
+    return axes
+
+
+def test_scene_follows_volume_layout(asc, ctx, orc, tiny):
+    """The fused batch gives the same bits in every layout, with poses whose
+    rays run along each volume axis (so each of BRICK's three copies is
+    read)."""
     sc, vol, _ = build_scene(asc, ctx, tiny)
     rng = np.random.default_rng(13)
     poses = tiny["wl"].gt_poses[0] + np.concatenate([rng.uniform(-2, 2, (4, 3)), rng.uniform(-4, 4, (4, 3))], 1)
-    c0, _ = sc.eval_poses(poses)
+    poses = np.concatenate([poses, tiny["wl"].gt_poses[0] + np.array([[0, 0, 0, 0, 90, 0], [0, 0, 0, 0, 0, 90]])])
+    assert ray_axes(orc, tiny, poses) == {0, 1, 2}
+    vol.set_layout(vol.LINEAR)
+    c0, p0 = sc.eval_poses(poses)
+    info0, _ = sc.last_info(len(poses) * 2)
     for layout in (vol.QUAD, vol.OCTET, vol.BRICK):
         vol.set_layout(layout)
-        c, _ = sc.eval_poses(poses)
-        assert np.array_equal(c, c0)
+        c, pc = sc.eval_poses(poses)
+        info, _ = sc.last_info(len(poses) * 2)
+        assert np.array_equal(c, c0) and np.array_equal(pc, p0)
+        assert np.array_equal(info, info0)
     sc.close()
     vol.close()
 

@@ -109,12 +109,14 @@ def test_frame_loop_matches_oracle(asc, ctx, orc, tiny, trial):
 
 
 def job_bytes(tiny, trial, particles):
-    """The server's device-memory estimate of a tiny job: linear + 8-corner
-    volume copies, float frames, one PSO epoch of DRR scratch."""
+    """The server's device-memory estimate of a tiny job: the linear volume
+    copy and three 8-corner BRICK copies (one per ray axis), float frames,
+    one PSO epoch of DRR scratch."""
     D, H, W = tiny["wl"].volume.shape
     RW, RH = tiny["render"]
     frames = sum(im.size for fr in trial["frames"] for im in fr) * 4
-    return (W + 3) * (H + 3) * (D + 3) * 2 * 9 + frames + particles * len(tiny["cams"]) * RW * RH * 4
+    vol = (W + 2) * (H + 2) * (D + 2) * 2 + (W + 8) * (H + 8) * (D + 8) * 2 * 8 * 3
+    return vol + frames + particles * len(tiny["cams"]) * RW * RH * 4
 
 
 @pytest.mark.parametrize("rule", ["active_jobs", "memory_budget"])
