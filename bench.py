@@ -873,6 +873,12 @@ def main(argv=None) -> int:
             out["plain_facade"]["speedup_of_fused_f16"] = f16["value"] / facade["reference_form"]
             if f16["graph_replay_value"]:
                 out["plain_facade"]["speedup_of_fused_f16_graph"] = f16["graph_replay_value"] / facade["reference_form"]
+        if probe:
+            # an F16 volume's march reads the same 16-byte cells as the u16
+            # one's, so the u16 gather probe is its gather roof too
+            f16 = out["storage_variants"]["f16"]
+            f16["roofline"] = {"bound": "gather", "frac": probe[0] / f16["kernel_ms"]["drr_tiles"],
+                               "probe_ms": probe[0], "measured_ms": f16["kernel_ms"]["drr_tiles"]}
     if args.frames > 0 and not multi:
         # Tracker's frame loop at this workload: PSO 100 particles x 100
         # epochs (every epoch runs: stall limit above the epoch count), then
