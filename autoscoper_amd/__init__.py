@@ -214,6 +214,9 @@ _SIGS = {
     "asc_scene_add_frame": [_vp, _i, _P(_vp), _P(_i)],
     "asc_scene_remove_frame": [_vp, _i],
     "asc_scene_eval_requests": [_vp, _i, _P(_i), _P(_i), _P(_i), _P(_d), _P(_d), _P(_d)],
+    "asc_scene_submit_requests": [_vp, _i, _P(_i), _P(_i), _P(_i), _P(_d), _P(_i)],
+    "asc_scene_finish_requests": [_vp, _i, _P(_d), _P(_d)],
+    "asc_scene_wait_requests": [_vp, _i],
     "asc_scene_use_graph": [_vp, _i],
     "asc_scene_set_debug": [_vp, _i],
     "asc_scene_last_info": [_vp, _P(_i), _i, _P(C.c_int64)],
@@ -233,6 +236,7 @@ _SIGS = {
     "asc_optimize_frame_fn": [_vp, _vp, _P(_f), _P(_d), _P(AscOptimizeOptions), _P(AscOptimizeStats)],
     "asc_frame_linear_extrap": [_P(_d), _P(_d), _P(_d)],
     "asc_server_create": [_vp, _sz, _i, _P(_vp)],
+    "asc_server_set_pipeline": [_vp, _i],
     "asc_server_submit": [_vp, _P(AscJobDesc), _P(_i)],
     "asc_server_wait": [_vp, _i, _P(_d), _P(_d), _P(AscJobStats)],
     "asc_server_stats": [_vp, _P(AscServeStats)],
@@ -514,20 +518,27 @@ class Server:
     """Registration server (asc_server_*): continuous batching of tracking
     jobs on one GPU, admitted by a device-memory budget."""
 
-    def __init__(self, ctx: "Context", budget_bytes: int, max_active_jobs: int = 8):
+    def __init__(self, ctx: "Context", budget_bytes: int, max_active_jobs: int = 8, pipeline: int = 2):
         h = C.c_void_p()
         _check(lib().asc_server_create(ctx.handle, budget_bytes, max_active_jobs, C.byref(h)))
         self.handle = h.value
+        if pipeline != 2:
+            self.set_pipeline(pipeline)
         self.ctx = ctx
         self._jobs = {}
         self._scene_keep = []  # volumes and images added to the server's scene by front ends
+
+    def set_pipeline(self, depth: int):
+        """Fused launches kept in flight: 2 queues the next while one runs,
+        1 waits for each (asc_server_set_pipeline)."""
+        _check(lib().asc_server_set_pipeline(self.handle, depth))
 
     @contextlib.contextmanager
     def scene(self):
         """The server's scene under asc_server_lock_scene, for a front end's
         volumes, cameras, camera sets and frames: a Scene that does not own
-        its handle; the batcher does not launch until the block ends (same
-        thread)."""
+        its handle; the batcher starts no launch until the block ends (same
+        thread), though one it started before may still run."""
         h = C.c_void_p()
         _check(lib().asc_server_lock_scene(self.handle, C.byref(h)))
         try:
@@ -1150,6 +1161,27 @@ class Scene:
         per = np.zeros((n, 8))
         _check(lib().asc_scene_eval_requests(self.handle, n, _ptr(v, _i), _ptr(st, _i), _ptr(fr, _i), _ptr(p, _d),
                                              _ptr(costs, _d), _ptr(per, _d)))
+        return costs, per
+
+    def submit_requests(self, volumes, sets, frames, poses) -> tuple[int, int]:
+        """Queue the fused launches of eval_requests' batch behind any batch
+        already submitted -> (ticket, n); at most two batches in flight."""
+        p = _f64(poses).reshape(-1, 6)
+        n = p.shape[0]
+        v = np.ascontiguousarray(volumes, np.int32)
+        st = np.ascontiguousarray(sets, np.int32)
+        fr = np.ascontiguousarray(frames if frames is not None else np.full(n, -1), np.int32)
+        t = C.c_int(-1)
+        _check(lib().asc_scene_submit_requests(self.handle, n, _ptr(v, _i), _ptr(st, _i), _ptr(fr, _i), _ptr(p, _d),
+                                               C.byref(t)))
+        return t.value, n
+
+    def finish_requests(self, ticket: tuple[int, int]):
+        """Wait for a submitted batch -> (costs[n], per_camera[n, 8])."""
+        slot, n = ticket
+        costs = np.zeros(n)
+        per = np.zeros((n, 8))
+        _check(lib().asc_scene_finish_requests(self.handle, slot, _ptr(costs, _d), _ptr(per, _d)))
         return costs, per
 
     def set_rad(self, camera: int, rad: Image):

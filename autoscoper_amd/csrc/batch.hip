@@ -785,8 +785,9 @@ struct asc_scene_s {
   DevBuf<Counters> d_counters;
   DevBuf<float> d_dbg;
   // host
-  PinBuf<unsigned char> h_batch;
-  PinBuf<double> h_results;
+  // pinned host buffers of the batch in each of the two submission slots
+  PinBuf<unsigned char> h_batch[2];
+  PinBuf<double> h_results[2];
   PinBuf<unsigned long long> h_samples;
   // items of the last batch in request order (request i, k-th camera of its
   // set at req_base[i] + k); kernel order is camera-major: korder[logical]
@@ -796,6 +797,21 @@ struct asc_scene_s {
   std::vector<int> req_base, req_ncam;
   int n_items = 0;
   size_t drr_total = 0;  // floats of DRR scratch used by the last batch
+  // Batches submitted and not yet finished (asc_scene_submit_requests): the
+  // host side of each (the fields above, moved here at submission and back
+  // at finish) and an event after its result copy. The device buffers are
+  // shared: the batches run one after another on the scene's stream.
+  struct InFlight {
+    bool busy = false;
+    int n = 0, stride = 0;
+    std::vector<ItemDesc> items;
+    std::vector<int> in_view, korder, req_base, req_ncam;
+    int n_items = 0;
+    size_t drr_total = 0;
+    bool timed = false;
+    hipEvent_t done = nullptr;
+  } fly[2];
+  int last_slot = 0;  // slot of the last finished batch (its h_results)
   int debug = 0;
   // graph
   bool use_graph = false;
@@ -926,7 +942,7 @@ void destroy_graph(asc_scene s)
 // Issue the batch (upload, three kernels, download) on the context stream,
 // with grid_drr and grid_post blocks for the two persistent kernels (any
 // grid of at least one block drains the whole tile list).
-int issue_batch(asc_scene s, size_t batch_bytes, int grid_drr, int grid_post)
+int issue_batch(asc_scene s, int slot, size_t batch_bytes, int grid_drr, int grid_post)
 {
   hipStream_t st = s->ctx->stream;
   const int n_items = s->n_items;
@@ -946,9 +962,11 @@ int issue_batch(asc_scene s, size_t batch_bytes, int grid_drr, int grid_post)
   a.dbg_rad = s->debug ? s->d_dbg.p + plane : nullptr;
   a.dbg_mask = s->debug ? s->d_dbg.p + 2 * plane : nullptr;
   a.wave_shift = s->wave_shift;
-  ASC_HIP(hipMemcpyAsync(s->d_batch.p, s->h_batch.p, batch_bytes, hipMemcpyHostToDevice, st));
+  ASC_HIP(hipMemcpyAsync(s->d_batch.p, s->h_batch[slot].p, batch_bytes, hipMemcpyHostToDevice, st));
   zero_u64<<<(n_items + 255) / 256, 256, 0, st>>>(s->d_samples.p, n_items);
-  const bool timed = s->timing && s->ev[0];
+  // the timing events are the scene's: a batch queued behind another does
+  // not record them
+  const bool timed = s->timing && s->ev[0] && !s->fly[0].busy && !s->fly[1].busy;
   if (timed) ASC_HIP(hipEventRecord(s->ev[0], st));
   if (s->gather_probe)
     drr_tiles<4, true><<<grid_drr, kBlock, 0, st>>>(a);
@@ -969,7 +987,8 @@ int issue_batch(asc_scene s, size_t batch_bytes, int grid_drr, int grid_post)
   ASC_HIP(hipGetLastError());
   // (the per-item sample counts stay on the device until asc_scene_last_info
   // asks for them: one copy less per batch)
-  ASC_HIP(hipMemcpyAsync(s->h_results.p, s->d_results.p, sizeof(double) * 10 * n_items, hipMemcpyDeviceToHost, st));
+  ASC_HIP(hipMemcpyAsync(s->h_results[slot].p, s->d_results.p, sizeof(double) * 10 * n_items, hipMemcpyDeviceToHost,
+                         st));
   return ASC_OK;
 }
 
@@ -980,12 +999,29 @@ struct Request {
   const double* pose;
 };
 
-// Geometry for every (request, camera) item, then the fused launches.
-// per_camera: n x stride values (camera k of request i at i*stride + k).
-int eval_requests(asc_scene s, const std::vector<Request>& reqs, double* costs, double* per_camera, int stride)
+// Geometry for every (request, camera) item, then the fused launches of
+// the batch in a free submission slot, queued on the scene's stream behind
+// any batch already submitted; the batch's host side moves to the slot
+// (finish_requests takes it back). `graph`: the synchronous form, which
+// may replay a recorded hipGraph (slot 0 only: the graph holds its
+// buffers).
+int submit_requests(asc_scene s, const std::vector<Request>& reqs, int stride, bool graph, int* slot_out)
 {
   // the calling thread may not have selected the scene's device yet
   ASC_HIP(hipSetDevice(s->ctx->device));
+  const bool other_busy = s->fly[0].busy || s->fly[1].busy;
+  ASC_CHECK(!(s->fly[0].busy && s->fly[1].busy), ASC_ERR_INVALID, "two batches are already submitted");
+  ASC_CHECK(!(graph && s->use_graph && other_busy), ASC_ERR_INVALID,
+            "a recorded graph runs with no other batch submitted");
+  ASC_CHECK(graph || !s->use_graph, ASC_ERR_UNSUPPORTED, "asynchronous batches do not replay recorded graphs");
+  const int slot = s->fly[0].busy ? 1 : 0;
+  hipStream_t st = s->ctx->stream;
+  // device buffers that grow are reallocated: let a submitted batch that
+  // uses them finish first
+  bool dirty = s->dirty;
+  for (const SceneVolume& sv : s->vols)
+    if (sv.in_use && sv.generation_seen != sv.vol->generation) dirty = true;
+  if (dirty && other_busy) ASC_HIP(hipStreamSynchronize(st));
   int rc = scene_upload_config(s);
   if (rc) return rc;
   const int n = (int)reqs.size();
@@ -1060,20 +1096,20 @@ int eval_requests(asc_scene s, const std::vector<Request>& reqs, double* costs, 
       if (count[c]) acams.push_back((int)c);
   }
   ASC_CHECK(acams.size() <= (size_t)kMaxCams, ASC_ERR_UNSUPPORTED, "more than 64 cameras in one batch");
-  std::vector<int> slot(s->cams.size(), -1);
-  for (size_t a = 0; a < acams.size(); a++) slot[acams[a]] = (int)a;
+  std::vector<int> cam_slot(s->cams.size(), -1);
+  for (size_t a = 0; a < acams.size(); a++) cam_slot[acams[a]] = (int)a;
   BatchHeader hdr;
   std::memset(&hdr, 0, sizeof hdr);
   hdr.n_items = n_items;
   hdr.debug = s->debug;
   hdr.n_acams = (int)acams.size();
-  for (int ii = 0; ii < n_items; ii++) hdr.cam_items[slot[s->items[ii].cam]]++;
+  for (int ii = 0; ii < n_items; ii++) hdr.cam_items[cam_slot[s->items[ii].cam]]++;
   for (int a = 1; a < hdr.n_acams; a++) hdr.item_base[a] = hdr.item_base[a - 1] + hdr.cam_items[a - 1];
   s->korder.assign(n_items, 0);
   {
     std::vector<int> fill(hdr.n_acams, 0);
     for (int ii = 0; ii < n_items; ii++) {
-      const int a = slot[s->items[ii].cam];
+      const int a = cam_slot[s->items[ii].cam];
       s->korder[ii] = hdr.item_base[a] + fill[a]++;
     }
   }
@@ -1113,12 +1149,17 @@ int eval_requests(asc_scene s, const std::vector<Request>& reqs, double* costs, 
   const size_t batch_bytes = sizeof(BatchHeader) + sizeof(ItemDesc) * (size_t)n_items;
   const void* before[6] = {s->d_batch.p, s->d_drr.p, s->d_partials.p, s->d_results.p, s->d_samples.p,
                            s->d_dbg.p};
+  if (other_busy && (batch_bytes > s->d_batch.cap || s->drr_total > s->d_drr.cap ||
+                     std::max<size_t>(part_total, 1) * 9 > s->d_partials.cap ||
+                     (size_t)n_items * 10 > s->d_results.cap || (size_t)n_items > s->d_samples.cap ||
+                     (s->debug && s->drr_total * 3 > s->d_dbg.cap)))
+    ASC_HIP(hipStreamSynchronize(st));
   if ((rc = s->d_batch.ensure(batch_bytes))) return rc;
-  if ((rc = s->h_batch.ensure(batch_bytes))) return rc;
+  if ((rc = s->h_batch[slot].ensure(batch_bytes))) return rc;
   if ((rc = s->d_drr.ensure(s->drr_total))) return rc;
   if ((rc = s->d_partials.ensure(std::max<size_t>(part_total, 1) * 9))) return rc;
   if ((rc = s->d_results.ensure((size_t)n_items * 10))) return rc;
-  if ((rc = s->h_results.ensure((size_t)n_items * 10))) return rc;
+  if ((rc = s->h_results[slot].ensure((size_t)n_items * 10))) return rc;
   if ((rc = s->d_samples.ensure((size_t)n_items))) return rc;
   if ((rc = s->h_samples.ensure((size_t)n_items))) return rc;
   if (!s->d_counters.p) {
@@ -1150,16 +1191,15 @@ int eval_requests(asc_scene s, const std::vector<Request>& reqs, double* costs, 
       s->grid_post = prop.multiProcessorCount * std::max(per_cu, 1);
     }
   }
-  std::memcpy(s->h_batch.p, &hdr, sizeof hdr);
-  std::memcpy(s->h_batch.p + sizeof hdr, kitems.data(), sizeof(ItemDesc) * n_items);
+  std::memcpy(s->h_batch[slot].p, &hdr, sizeof hdr);
+  std::memcpy(s->h_batch[slot].p + sizeof hdr, kitems.data(), sizeof(ItemDesc) * n_items);
 
-  hipStream_t st = s->ctx->stream;
-  if (s->use_graph) {
+  if (graph && s->use_graph) {
     if (!s->graph_exec || s->graph_items != n_items) {
       destroy_graph(s);
       ASC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       // a recorded graph replays batches of other tile counts: full grids
-      rc = issue_batch(s, batch_bytes, s->grid_persistent, s->grid_post);
+      rc = issue_batch(s, slot, batch_bytes, s->grid_persistent, s->grid_post);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(st, &g);
       if (rc) return rc;
@@ -1174,12 +1214,48 @@ int eval_requests(asc_scene s, const std::vector<Request>& reqs, double* costs, 
     // tiles, rounded up to the 8 partitions of the tile list
     const int gd = (int)std::min<long long>(s->grid_persistent, ((long long)hdr.drr.total + 7) / 8 * 8);
     const int gp = (int)std::min<long long>(s->grid_post, ((long long)hdr.post.total + 7) / 8 * 8);
-    if ((rc = issue_batch(s, batch_bytes, std::max(gd, 8), std::max(gp, 8)))) return rc;
+    if ((rc = issue_batch(s, slot, batch_bytes, std::max(gd, 8), std::max(gp, 8)))) return rc;
   }
-  ASC_HIP(hipStreamSynchronize(st));
-  if (s->timing && s->ev[0] && !s->use_graph)
-    for (int k = 0; k < 3; k++) ASC_HIP(hipEventElapsedTime(&s->last_ms[k], s->ev[k], s->ev[k + 1]));
+  auto& f = s->fly[slot];
+  if (!f.done) ASC_HIP(hipEventCreateWithFlags(&f.done, hipEventDisableTiming));
+  ASC_HIP(hipEventRecord(f.done, st));
+  f.busy = true;
+  f.n = (int)reqs.size();
+  f.stride = stride;
+  f.timed = !other_busy;  // issue_batch recorded the timing events
+  std::swap(f.items, s->items);
+  std::swap(f.in_view, s->in_view);
+  std::swap(f.korder, s->korder);
+  std::swap(f.req_base, s->req_base);
+  std::swap(f.req_ncam, s->req_ncam);
+  f.n_items = s->n_items;
+  f.drr_total = s->drr_total;
+  *slot_out = slot;
+  return ASC_OK;
+}
 
+// Wait for the batch of `slot`, take its host side back as the scene's last
+// batch and turn its moments into costs.
+int finish_requests(asc_scene s, int slot, double* costs, double* per_camera)
+{
+  ASC_CHECK(slot == 0 || slot == 1, ASC_ERR_INVALID, "bad batch ticket");
+  auto& f = s->fly[slot];
+  ASC_CHECK(f.busy, ASC_ERR_INVALID, "no batch submitted under this ticket");
+  ASC_HIP(hipSetDevice(s->ctx->device));
+  f.busy = false;
+  ASC_HIP(hipEventSynchronize(f.done));
+  std::swap(f.items, s->items);
+  std::swap(f.in_view, s->in_view);
+  std::swap(f.korder, s->korder);
+  std::swap(f.req_base, s->req_base);
+  std::swap(f.req_ncam, s->req_ncam);
+  s->n_items = f.n_items;
+  s->drr_total = f.drr_total;
+  s->last_slot = slot;
+  if (s->timing && s->ev[0] && !s->use_graph && f.timed)
+    for (int k = 0; k < 3; k++) ASC_HIP(hipEventElapsedTime(&s->last_ms[k], s->ev[k], s->ev[k + 1]));
+  const int n = f.n, stride = f.stride;
+  const double* res = s->h_results[slot].p;
   // Tracker.cpp:614-618 (9999 when not in view), :769 (1 - NCC),
   // :814-826 (sum over cameras; negative totals become 9999).
   for (int i = 0; i < n; i++) {
@@ -1187,7 +1263,7 @@ int eval_requests(asc_scene s, const std::vector<Request>& reqs, double* costs, 
     for (int k = 0; k < s->req_ncam[i]; k++) {
       const int ii = s->req_base[i] + k;
       const int ki = s->korder[ii];
-      const double corr = s->in_view[ii] ? 1.0 - s->h_results.p[(size_t)ki * 10 + 9] : 9999.0;
+      const double corr = s->in_view[ii] ? 1.0 - res[(size_t)ki * 10 + 9] : 9999.0;
       if (per_camera) per_camera[(size_t)i * stride + k] = corr;
       total = k == 0 ? corr : total + corr;
     }
@@ -1195,6 +1271,16 @@ int eval_requests(asc_scene s, const std::vector<Request>& reqs, double* costs, 
     if (costs) costs[i] = total;
   }
   return ASC_OK;
+}
+
+// Geometry, fused launches and costs of a batch, waiting for it.
+// per_camera: n x stride values (camera k of request i at i*stride + k).
+int eval_requests(asc_scene s, const std::vector<Request>& reqs, double* costs, double* per_camera, int stride)
+{
+  int slot = 0;
+  int rc = submit_requests(s, reqs, stride, true, &slot);
+  if (rc) return rc;
+  return finish_requests(s, slot, costs, per_camera);
 }
 
 // The all-camera form used by eval_poses / eval_manips.
@@ -1254,8 +1340,11 @@ ASC_API int asc_scene_destroy(asc_scene s)
   s->d_counters.release();
   s->d_dbg.release();
   s->p_bufs.release();
-  s->h_batch.release();
-  s->h_results.release();
+  for (int k = 0; k < 2; k++) {
+    s->h_batch[k].release();
+    s->h_results[k].release();
+    if (s->fly[k].done) hipEventDestroy(s->fly[k].done);
+  }
   s->h_samples.release();
   delete s;
   return ASC_OK;
@@ -1493,6 +1582,32 @@ ASC_API int asc_scene_eval_requests(asc_scene s, int n, const int* volumes, cons
   return eval_requests(s, reqs, costs, per_camera, kMaxSetCams);
 }
 
+ASC_API int asc_scene_submit_requests(asc_scene s, int n, const int* volumes, const int* sets, const int* frames,
+                                      const double* xyzypr, int* ticket)
+{
+  ASC_CHECK(s && volumes && sets && xyzypr && ticket && n > 0, ASC_ERR_INVALID, "bad argument");
+  std::vector<Request> reqs(n);
+  for (int i = 0; i < n; i++) reqs[i] = Request{volumes[i], sets[i], frames ? frames[i] : -1, xyzypr + 6 * i};
+  return submit_requests(s, reqs, kMaxSetCams, false, ticket);
+}
+
+ASC_API int asc_scene_wait_requests(asc_scene s, int ticket)
+{
+  ASC_CHECK(s && (ticket == 0 || ticket == 1), ASC_ERR_INVALID, "bad argument");
+  // reads only the slot's event, which submit created before handing out
+  // the ticket and which no call replaces while the slot is submitted
+  ASC_CHECK(s->fly[ticket].busy, ASC_ERR_INVALID, "no batch submitted under this ticket");
+  ASC_HIP(hipSetDevice(s->ctx->device));
+  ASC_HIP(hipEventSynchronize(s->fly[ticket].done));
+  return ASC_OK;
+}
+
+ASC_API int asc_scene_finish_requests(asc_scene s, int ticket, double* costs, double* per_camera)
+{
+  ASC_CHECK(s, ASC_ERR_INVALID, "null scene");
+  return finish_requests(s, ticket, costs, per_camera);
+}
+
 ASC_API int asc_scene_scratch_bytes(asc_scene s, size_t* bytes)
 {
   ASC_CHECK(s && bytes, ASC_ERR_INVALID, "bad argument");
@@ -1582,11 +1697,13 @@ ASC_API int asc_scene_last_timing(asc_scene s, float ms[3])
 ASC_API int asc_scene_last_info(asc_scene s, int* info, int capacity, int64_t* samples)
 {
   ASC_CHECK(s, ASC_ERR_INVALID, "null scene");
+  ASC_CHECK(!s->fly[0].busy && !s->fly[1].busy, ASC_ERR_INVALID, "finish the submitted batches first");
   // the last batch's counts, still in device memory (the batch has been
   // waited for; the next one overwrites them). A batch that failed before
   // its buffers were sized leaves no counts to read.
-  ASC_CHECK((size_t)s->n_items <= s->d_samples.cap && (size_t)s->n_items <= s->h_samples.cap, ASC_ERR_INVALID,
-            "no batch results to report");
+  ASC_CHECK((size_t)s->n_items <= s->d_samples.cap && (size_t)s->n_items <= s->h_samples.cap &&
+              (size_t)s->n_items * 10 <= s->h_results[s->last_slot].cap,
+            ASC_ERR_INVALID, "no batch results to report");
   if (s->n_items > 0) {
     ASC_HIP(hipStreamSynchronize(s->ctx->stream));
     ASC_HIP(hipMemcpy(s->h_samples.p, s->d_samples.p, sizeof(unsigned long long) * s->n_items,
@@ -1602,7 +1719,7 @@ ASC_API int asc_scene_last_info(asc_scene s, int* info, int capacity, int64_t* s
       o[0] = s->in_view[ii];
       o[1] = it.w;
       o[2] = it.h;
-      o[3] = (int)s->h_results.p[(size_t)ki * 10 + 3];
+      o[3] = (int)s->h_results[s->last_slot].p[(size_t)ki * 10 + 3];
       o[4] = it.tiles_x * it.tiles_y;
       o[5] = (int)s->h_samples.p[ki];
       o[6] = o[7] = 0;
@@ -1615,14 +1732,17 @@ ASC_API int asc_scene_last_info(asc_scene s, int* info, int capacity, int64_t* s
 ASC_API int asc_scene_last_moments(asc_scene s, double* moments, int capacity)
 {
   ASC_CHECK(s && moments, ASC_ERR_INVALID, "bad argument");
+  ASC_CHECK(!s->fly[0].busy && !s->fly[1].busy, ASC_ERR_INVALID, "finish the submitted batches first");
+  ASC_CHECK((size_t)s->n_items * 10 <= s->h_results[s->last_slot].cap, ASC_ERR_INVALID, "no batch results to report");
   for (int ii = 0; ii < s->n_items && (ii + 1) * 10 <= capacity; ii++)
-    for (int k = 0; k < 10; k++) moments[ii * 10 + k] = s->h_results.p[(size_t)s->korder[ii] * 10 + k];
+    for (int k = 0; k < 10; k++) moments[ii * 10 + k] = s->h_results[s->last_slot].p[(size_t)s->korder[ii] * 10 + k];
   return ASC_OK;
 }
 
 ASC_API int asc_scene_last_images(asc_scene s, int pose, int camera, float* drr, float* rad, float* mask)
 {
   ASC_CHECK(s && s->debug, ASC_ERR_INVALID, "enable debug images with asc_scene_set_debug first");
+  ASC_CHECK(!s->fly[0].busy && !s->fly[1].busy, ASC_ERR_INVALID, "finish the submitted batches first");
   ASC_CHECK(pose >= 0 && pose < (int)s->req_base.size() && camera >= 0 && camera < s->req_ncam[pose], ASC_ERR_INVALID,
             "bad item");
   const int ii = s->req_base[pose] + camera;

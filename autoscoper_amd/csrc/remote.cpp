@@ -257,8 +257,8 @@ struct Session {
   void close_trial()
   {
     if (srv && set >= 0) {
-      // out of the shared scene before the device copies go (the lock
-      // waits for a launch that may read them)
+      // out of the shared scene before the device copies go (no launch
+      // in flight reads them: the session's own requests are all done)
       asc_scene sc = nullptr;
       asc_server_lock_scene(srv, &sc);
       asc_scene_remove_camera_set(sc, set);

@@ -137,11 +137,27 @@ struct asc_server_s {
   std::deque<Job*> to_free;                  // out of the scene, device copies to free
   std::map<int, std::unique_ptr<Job>> jobs;  // all not yet collected by asc_server_wait
   std::vector<Request*> pending;
+  // submitted launches, oldest first (at most two: the scene's slots); the
+  // completer thread waits for each and hands its results out
+  struct Launch {
+    int ticket = 0;
+    std::vector<Request*> reqs;
+    size_t n = 0;
+    int bone_reqs = 0, ext_reqs = 0;
+    Clock::time_point t;
+  };
+  std::deque<Launch> launches;
+  int depth = 2;  // launches in flight (asc_server_set_pipeline)
+  bool stop_completer = false;
+  std::condition_variable cv_completer;
+  // the current run of launches in flight started at busy_from; the last
+  // one ended at busy_to
+  Clock::time_point busy_from, busy_to;
   int running_bones = 0;
   int streams = 0;  // front ends inside asc_server_stream_begin / _end
   int next_id = 1;
   bool stop = false, stop_stager = false;
-  std::thread batcher, stager;
+  std::thread batcher, stager, completer;
   Clock::time_point t0;
   // statistics
   size_t scene_scratch = 0;  // the scene's device scratch after the last launch
@@ -417,6 +433,69 @@ void reap(asc_server s, std::unique_lock<std::mutex>& lk)
   }
 }
 
+// Hand a launch's results to its requesters (lock held).
+void deliver(asc_server s, const std::vector<Request*>& reqs, const std::vector<double>& costs,
+             const std::vector<double>& per, int rc)
+{
+  size_t k = 0;
+  for (Request* r : reqs) {
+    for (int i = 0; i < r->n; i++, k++) {
+      if (rc != ASC_OK) continue;
+      r->costs[i] = costs[k];
+      if (r->per_camera) std::memcpy(r->per_camera + 8 * i, &per[k * 8], 8 * sizeof(double));
+    }
+    r->status = rc;
+    r->done = true;
+  }
+  s->cv_req.notify_all();
+}
+
+// Completer thread: waits for the oldest submitted launch (outside the
+// scene lock, so that the batcher can submit the next one meanwhile), takes
+// its costs and wakes its requesters.
+void completer_main(asc_server s)
+{
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    s->cv_completer.wait(lk, [&] { return s->stop_completer || !s->launches.empty(); });
+    if (s->launches.empty()) break;  // stopping
+    const int ticket = s->launches.front().ticket;
+    lk.unlock();
+    int rc = asc_scene_wait_requests(s->scene, ticket);
+    lk.lock();
+    asc_server_s::Launch b = std::move(s->launches.front());
+    lk.unlock();
+    std::vector<double> costs(b.n), per(b.n * 8, 0.0);
+    size_t scratch = 0;
+    {
+      std::lock_guard<std::mutex> sl(s->scene_mu);
+      // finish even after a failed wait: it frees the slot
+      const int frc = asc_scene_finish_requests(s->scene, ticket, costs.data(), per.data());
+      if (rc == ASC_OK) rc = frc;
+      asc_scene_scratch_bytes(s->scene, &scratch);
+    }
+    const Clock::time_point t = Clock::now();
+    lk.lock();
+    s->launches.pop_front();
+    s->batches++;
+    s->requests += (int64_t)b.reqs.size();
+    s->evaluations += (int64_t)b.n;
+    // busy: the union of the launches' submit-to-finish intervals
+    if (s->launches.empty()) {
+      s->busy_ms += ms_since(s->busy_from, t);
+      s->busy_to = t;
+    }
+    s->scene_scratch = scratch;
+    deliver(s, b.reqs, costs, per, rc);
+    s->cv_batcher.notify_one();  // a slot is free
+    s->cv_stager.notify_one();   // the scratch may have grown: re-check admission
+  }
+}
+
+// Poses a launch must hold before the batcher splits the requesters into
+// two cohorts (half of them start a launch).
+constexpr int kCohortPoses = 32;
+
 void batcher_main(asc_server s)
 {
   std::unique_lock<std::mutex> lk(s->mu);
@@ -424,16 +503,35 @@ void batcher_main(asc_server s)
     reap(s, lk);
     register_staged(s, lk);
     // a tick: wait until every running bone and every announced stream of
-    // external requests has a request pending (between two optimiser steps
-    // they do only host work), or briefly for stragglers; any other
-    // external request waits the same way
+    // external requests has a request pending or in flight (between two
+    // optimiser steps they do only host work), or briefly for stragglers;
+    // any other external request waits the same way. With two launches in
+    // flight and nothing in flight, half of them start a launch once it
+    // holds kCohortPoses poses, so that the requesters settle into two
+    // cohorts whose launches alternate: one runs while the other's
+    // requesters do their host work. (Requests of a few poses keep sharing
+    // one launch: there a launch's fixed cost outweighs the host work.)
     auto ready = [&] {
-      const int bone_reqs = (int)std::count_if(s->pending.begin(), s->pending.end(),
-                                               [](const Request* r) { return r->job != nullptr; });
+      if (s->stop) return true;
+      if ((int)s->launches.size() >= s->depth) return false;
+      int bone_reqs = 0, poses = 0;
+      for (const Request* r : s->pending) {
+        bone_reqs += r->job != nullptr;
+        poses += r->n;
+      }
       const int ext_reqs = (int)s->pending.size() - bone_reqs;
-      return s->stop ||
-             (!s->pending.empty() && bone_reqs >= s->running_bones && ext_reqs >= s->streams) ||
-             !s->staged.empty() || std::any_of(s->jobs.begin(), s->jobs.end(), [](const auto& kv) {
+      int fly_bone = 0, fly_ext = 0;
+      for (const auto& b : s->launches) {
+        fly_bone += b.bone_reqs;
+        fly_ext += b.ext_reqs;
+      }
+      int need_bone = s->running_bones - fly_bone, need_ext = s->streams - fly_ext;
+      if (s->depth > 1 && s->launches.empty() && poses >= kCohortPoses) {
+        need_bone = (need_bone + 1) / 2;
+        need_ext = (need_ext + 1) / 2;
+      }
+      return (!s->pending.empty() && bone_reqs >= need_bone && ext_reqs >= need_ext) || !s->staged.empty() ||
+             std::any_of(s->jobs.begin(), s->jobs.end(), [](const auto& kv) {
                return kv.second->registered && !kv.second->released && kv.second->bones_left == 0;
              });
     };
@@ -443,16 +541,25 @@ void batcher_main(asc_server s)
       s->cv_batcher.wait_for(lk, std::chrono::microseconds(500), ready);
     if (s->stop && s->pending.empty() && s->running_bones == 0) break;
     if (s->pending.empty()) continue;
-    std::vector<Request*> reqs;
-    reqs.swap(s->pending);
+    if ((int)s->launches.size() >= s->depth) {
+      // both slots in flight: wait for the completer (stragglers included)
+      s->cv_batcher.wait(lk, [&] { return (int)s->launches.size() < s->depth; });
+      continue;
+    }
+    asc_server_s::Launch b;
+    b.reqs.swap(s->pending);
     lk.unlock();
     // one fused launch for every pending request
     size_t n = 0;
-    for (Request* r : reqs) n += r->n;
+    for (Request* r : b.reqs) {
+      n += r->n;
+      (r->job ? b.bone_reqs : b.ext_reqs)++;
+    }
+    b.n = n;
     std::vector<int> vols(n), sets(n), frames(n);
-    std::vector<double> poses(n * 6), costs(n), per(n * 8);
+    std::vector<double> poses(n * 6);
     size_t k = 0;
-    for (Request* r : reqs) {
+    for (Request* r : b.reqs) {
       if (!r->job) {
         for (int i = 0; i < r->n; i++, k++) {
           vols[k] = r->ext_vols[i];
@@ -470,33 +577,21 @@ void batcher_main(asc_server s)
         asc_compose_pose(r->store, j.pivots[r->bone].data(), r->manips + 6 * i, &poses[k * 6]);
       }
     }
-    const Clock::time_point t = Clock::now();
+    b.t = Clock::now();
     int rc;
-    size_t scratch = 0;
     {
       std::lock_guard<std::mutex> sl(s->scene_mu);
-      rc = asc_scene_eval_requests(s->scene, (int)n, vols.data(), sets.data(), frames.data(), poses.data(),
-                                   costs.data(), per.data());
-      asc_scene_scratch_bytes(s->scene, &scratch);
+      rc = asc_scene_submit_requests(s->scene, (int)n, vols.data(), sets.data(), frames.data(), poses.data(),
+                                     &b.ticket);
     }
-    const double busy = ms_since(t, Clock::now());
     lk.lock();
-    s->batches++;
-    s->requests += (int64_t)reqs.size();
-    s->evaluations += (int64_t)n;
-    s->busy_ms += busy;
-    s->scene_scratch = scratch;
-    k = 0;
-    for (Request* r : reqs) {
-      for (int i = 0; i < r->n; i++, k++) {
-        r->costs[i] = costs[k];
-        if (r->per_camera) std::memcpy(r->per_camera + 8 * i, &per[k * 8], 8 * sizeof(double));
-      }
-      r->status = rc;
-      r->done = true;
+    if (rc != ASC_OK) {
+      deliver(s, b.reqs, {}, {}, rc);
+      continue;
     }
-    s->cv_req.notify_all();
-    s->cv_stager.notify_one();  // the scratch may have grown: re-check admission
+    if (s->launches.empty()) s->busy_from = std::max(b.t, s->busy_to);
+    s->launches.push_back(std::move(b));
+    s->cv_completer.notify_one();
   }
 }
 
@@ -528,6 +623,7 @@ ASC_API int asc_server_create(asc_context ctx, size_t hbm_budget, int max_active
     return rc;
   }
   s->batcher = std::thread(batcher_main, s);
+  s->completer = std::thread(completer_main, s);
   s->stager = std::thread(stager_main, s);
   *out = s;
   return ASC_OK;
@@ -633,6 +729,15 @@ ASC_API int asc_server_wait(asc_server s, int job_id, double* poses, double* fra
   }
   s->jobs.erase(it);
   if (status != ASC_OK) return asc::fail(status, "job failed");
+  return ASC_OK;
+}
+
+ASC_API int asc_server_set_pipeline(asc_server s, int depth)
+{
+  if (!s || depth < 1 || depth > 2) return asc::fail(ASC_ERR_INVALID, "pipeline depth is 1 or 2");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->depth = depth;
+  s->cv_batcher.notify_one();
   return ASC_OK;
 }
 
@@ -748,6 +853,12 @@ ASC_API int asc_server_destroy(asc_server s)
     s->cv_stager.notify_one();
   }
   s->batcher.join();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop_completer = true;
+    s->cv_completer.notify_one();
+  }
+  s->completer.join();
   s->stager.join();
   asc_scene_destroy(s->scene);
   asc_context_destroy(s->up_ctx);

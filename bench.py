@@ -210,13 +210,14 @@ def walk_frames(frames, n):
     return [frames[f % period if f % period < T else period - f % period] for f in range(n)]
 
 
-def serve_run(A, synth, ctx, trials, jobs, arrival=True):
+def serve_run(A, synth, ctx, trials, jobs, arrival=True, pipeline=2):
     """Serve `jobs` = [(arrival s, template, frames, particles, seed)] with one
-    server (continuous batching) -> result dict."""
+    server (continuous batching, `pipeline` fused launches in flight) ->
+    result dict."""
     import threading
 
     dflt, rflt = synth.default_filters()
-    srv = A.Server(ctx, 64 << 30, 16)
+    srv = A.Server(ctx, 64 << 30, 16, pipeline=pipeline)
     ids = [None] * len(jobs)
     t0 = time.perf_counter()
 
@@ -255,11 +256,14 @@ def serve_bench(A, synth, ctx, trials, loads, jobs_per_rate: int, max_frames: in
                  int(rng.integers(1 << 20))) for _ in range(n)]
 
     rng = np.random.default_rng(seed)
-    sat = serve_run(A, synth, ctx, trials, [(0.0, *j) for j in draw(rng, 8, 40)], arrival=False)
+    sat_jobs = [(0.0, *j) for j in draw(rng, 8, 40)]
+    sat = serve_run(A, synth, ctx, trials, sat_jobs, arrival=False)
     cap = sat["frames_per_s"]
     mean_frames = (20 + max_frames) / 2
     log(f"[serve] capacity {cap:.1f} frames/s (8 jobs at once, p99 {sat['latency_p99_ms']:.0f} ms)")
-    out = {"capacity": sat, "rates": []}
+    one = serve_run(A, synth, ctx, trials, sat_jobs, arrival=False, pipeline=1)
+    log(f"[serve] one launch in flight: {one['frames_per_s']:.1f} frames/s (p99 {one['latency_p99_ms']:.0f} ms)")
+    out = {"capacity": sat, "capacity_one_launch_in_flight": one, "rates": []}
     for ri, load in enumerate(loads):
         rate = load * cap / mean_frames  # jobs per second
         rng = np.random.default_rng(seed + 1 + ri)

@@ -316,6 +316,22 @@ ASC_API int asc_scene_remove_frame(asc_scene scene, int frame);
  * Tracker.cpp:614-618, 814-826). */
 ASC_API int asc_scene_eval_requests(asc_scene scene, int n, const int* volumes, const int* sets, const int* frames,
                                     const double* xyzypr, double* costs, double* per_camera);
+/* The same batch in two halves, so that a caller can prepare and queue the
+ * next batch while one runs (no counterpart in the reference; the
+ * registration server's double-buffered batcher): submit queues the fused
+ * launches on the scene's stream behind any batch already submitted and
+ * returns a ticket at once; finish waits for that batch and writes its costs
+ * and per-camera values as eval_requests does. At most two batches may be
+ * submitted and not finished; they finish in any order, and the last one
+ * finished is the "last batch" of asc_scene_last_*, which refuse while a
+ * batch is submitted. Submit refuses while graph replay is on. */
+ASC_API int asc_scene_submit_requests(asc_scene scene, int n, const int* volumes, const int* sets,
+                                      const int* frames, const double* xyzypr, int* ticket);
+ASC_API int asc_scene_finish_requests(asc_scene scene, int ticket, double* costs, double* per_camera);
+/* Blocks until the submitted batch's launches are done. It touches no other
+ * scene state, so a thread may wait here while another thread submits (the
+ * scene's other calls are not thread-safe). */
+ASC_API int asc_scene_wait_requests(asc_scene scene, int ticket);
 /* Device bytes the scene holds for batch scratch (DRR images, moment
  * partials, results, descriptors); they grow with the largest batch and are
  * kept for reuse. No counterpart in the reference (memory accounting of the
@@ -506,6 +522,15 @@ typedef struct {
 } asc_serve_stats;
 
 ASC_API int asc_server_create(asc_context ctx, size_t hbm_budget, int max_active_jobs, asc_server* out);
+/* Launches the batcher keeps in flight: 2 (default) queues the next fused
+ * launch while one runs. With nothing in flight it starts a launch once half
+ * of the running bones and streams have a request pending, if those hold at
+ * least 32 poses; the rest start the next launch while that one runs. The
+ * requesters so settle into two cohorts whose launches alternate, and the
+ * GPU does not wait for one cohort's host work (optimiser steps, batch
+ * preparation). 1 launches once every running bone has a request pending
+ * and waits for it before the next. Results are the same either way. */
+ASC_API int asc_server_set_pipeline(asc_server server, int depth);
 /* Copies the job's host data; returns at once. */
 ASC_API int asc_server_submit(asc_server server, const asc_job_desc* job, int* job_id);
 /* Blocks until the job is done: poses n_frames x n_volumes x 6, per-frame
@@ -517,7 +542,9 @@ ASC_API int asc_server_stats(asc_server server, asc_serve_stats* out);
  * asc_remote_create_shared; reference: the socket's getNCC and optimizeFrame
  * messages, Socket.cpp:282-302,345-372, which run one client's requests
  * alone there) share the server's scene and its fused launches. Under
- * asc_server_lock_scene the batcher does not launch: the front end adds or
+ * asc_server_lock_scene the batcher starts no launch (those it started may
+ * still run; they read no entry of a front end whose own requests are
+ * done): the front end adds or
  * removes its volumes, cameras, camera sets and frames with asc_scene_*
  * calls, then unlocks (same thread). asc_server_eval_requests evaluates n
  * requests as asc_scene_eval_requests does, in the batcher's next launch

@@ -126,3 +126,57 @@ def test_slots_are_reused(asc, ctx, orc, tiny, jobs):
     sc.close()
     for v in keep:
         v.close()
+
+
+def test_submitted_batches_equal_blocking_batches(asc, ctx, tiny, jobs):
+    """asc_scene_submit_requests / _finish_requests: two batches in flight
+    on the scene's stream, finished in either order, give the same bits as
+    asc_scene_eval_requests of each batch on another scene; the second batch
+    is larger than the first, so that the fresh scene's buffers grow while
+    the first is in flight; a third submit, graph replay and the last-batch
+    queries are refused while batches are in flight."""
+    ref, keep_r, *_ = build(asc, ctx, tiny, jobs)
+    sc, keep, (vA, vB), (sA, sB), (fA, fB), _ = build(asc, ctx, tiny, jobs)
+    wl, wl3 = tiny["wl"], jobs["wl3"]
+    rng = np.random.default_rng(91)
+
+    def batch(n):
+        job = rng.integers(0, 2, n)
+        frame = rng.integers(0, 2, n)
+        base = np.where(job[:, None] == 0, wl.gt_poses[0], wl3.gt_poses[0, 1])
+        poses = base + np.concatenate([rng.uniform(-2, 2, (n, 3)), rng.uniform(-4, 4, (n, 3))], 1)
+        vols = np.where(job == 0, vA, vB)
+        sets = np.where(job == 0, sA, sB)
+        frames = np.array([fA[f] if j == 0 else fB[f] for j, f in zip(job, frame)])
+        return vols, sets, frames, poses
+
+    small, large = batch(6), batch(40)
+    want = [ref.eval_requests(*small), ref.eval_requests(*large)]
+    for order in ((0, 1), (1, 0)):
+        t = [sc.submit_requests(*small), sc.submit_requests(*large)]
+        assert t[0][0] != t[1][0]
+        with pytest.raises(asc.AscError):
+            sc.submit_requests(*small)  # both slots in flight
+        with pytest.raises(asc.AscError):
+            sc.last_moments(4)
+        got = [None, None]
+        for k in order:
+            got[k] = sc.finish_requests(t[k])
+        for k in (0, 1):
+            assert np.array_equal(got[k][0], want[k][0]) and np.array_equal(got[k][1], want[k][1]), (order, k)
+    # the last batch finished is the scene's last batch
+    t = sc.submit_requests(*small)
+    got = sc.finish_requests(t)
+    assert np.array_equal(got[0], want[0][0])
+    ref.eval_requests(*small)
+    assert np.array_equal(sc.last_moments(6 * 2), ref.last_moments(6 * 2))
+    with pytest.raises(asc.AscError):
+        sc.finish_requests(t)  # already finished
+    sc.use_graph(True)
+    with pytest.raises(asc.AscError):
+        sc.submit_requests(*small)
+    sc.use_graph(False)
+    for s in (sc, ref):
+        s.close()
+    for v in keep + keep_r:
+        v.close()

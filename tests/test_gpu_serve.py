@@ -2,8 +2,9 @@
 served together with continuous batching.
 
 * A job's tracked poses are identical (bit for bit) run alone and run in a
-  mix with other jobs: a request's cost does not depend on the launch it
-  shares, and each bone's optimiser is deterministic given its costs.
+  mix with other jobs, with one fused launch in flight at a time or two: a
+  request's cost does not depend on the launch it shares, and each bone's
+  optimiser is deterministic given its costs.
 * A job's frame loop (guess 1 = previous frame) matches the oracle's
   optimize_frame run frame by frame from the same start: poses within
   0.05 mm / 0.05 deg.
@@ -71,11 +72,14 @@ def submit_bones(asc, server):
                          drr=(synth.DRR_STEP, synth.DRR_INTENSITY, 0.0), guess=2, **dict(OPTS, seed=3))
 
 
-def test_job_alone_equals_job_in_the_mix(asc, ctx, tiny, trial):
-    srv = asc.Server(ctx, 8 << 30, 8)
+@pytest.mark.parametrize("pipeline", [1, 2])
+def test_job_alone_equals_job_in_the_mix(asc, ctx, tiny, trial, pipeline):
+    """pipeline: fused launches the batcher keeps in flight (2: the next
+    one is queued while one runs, the bones in two alternating cohorts)."""
+    srv = asc.Server(ctx, 8 << 30, 8, pipeline=1)
     alone, _, _ = srv.wait(submit_tiny(asc, srv, tiny, trial))
     srv.close()
-    srv = asc.Server(ctx, 8 << 30, 8)
+    srv = asc.Server(ctx, 8 << 30, 8, pipeline=pipeline)
     ids = [submit_bones(asc, srv), submit_tiny(asc, srv, tiny, trial), submit_tiny(asc, srv, tiny, trial, seed=11),
            submit_bones(asc, srv)]
     res = [srv.wait(i) for i in ids]
