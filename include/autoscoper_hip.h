@@ -132,13 +132,15 @@ ASC_API int asc_volume_describe(asc_volume vol, asc_volume_desc* desc);
 /* Device layout used by the ray march: LINEAR (one texel per element),
  * QUAD (each element carries its 2x2 corner block in x/y: 4x memory, 2 loads
  * per sample), OCTET (each element carries its 2x2x2 cell: 8x memory, 1 load
- * per sample, x-fastest rows) or BRICK (OCTET elements grouped in 2x2x2
- * bricks of one 128-byte line, so the cells a wave samples share fewer
- * cache lines whatever the ray direction). Values and results are identical;
- * only the access pattern changes. asc_volume_create picks the densest that
- * fits (BRICK, else OCTET, QUAD, LINEAR): the march addresses with 24-bit
- * products, so a layout needs slice/brick-layer strides below 8 MiB and the
- * whole array below 2 GiB (ASC_ERR_TOO_LARGE otherwise). */
+ * per sample, x-fastest rows) or BRICK (three copies of the OCTET cells, one
+ * per ray axis, each grouped in bricks of 1 x 2 x 4 cells across that axis,
+ * so that the cells neighbouring rays sample share cache lines; a launch
+ * reads the copy of its rays' main axis: 24x memory). Values and results are
+ * identical; only the access pattern changes. asc_volume_create picks the
+ * densest that fits (BRICK, else OCTET, QUAD, LINEAR): OCTET and QUAD need
+ * slice strides below 8 MiB for the march's 24-bit address products, BRICK
+ * needs W + H + D + 3 <= 4096 for its offset tables, and every array must
+ * stay below 2 GiB (ASC_ERR_TOO_LARGE otherwise). */
 enum { ASC_LAYOUT_LINEAR = 0, ASC_LAYOUT_QUAD = 1, ASC_LAYOUT_OCTET = 2, ASC_LAYOUT_BRICK = 3 };
 ASC_API int asc_volume_set_layout(asc_volume vol, int layout);
 ASC_API int asc_volume_destroy(asc_volume vol);
