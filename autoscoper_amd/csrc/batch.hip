@@ -130,8 +130,9 @@ struct BatchArgs {
   const float* weights;
   float* drr;
   double* partials;
-  double* results;  // n_items * 10
+  double* results;  // n_items * 10, in the submission slot's pinned host buffer
   unsigned long long* samples;  // n_items
+  unsigned long long* samples_host;  // n_items, in the submission slot's pinned host buffer
   Counters* counters;
   float* dbg_drr;
   float* dbg_rad;
@@ -662,6 +663,7 @@ __global__ __launch_bounds__(kBlock) void finalize_items(BatchArgs a)
       double* r = a.results + (size_t)ii * 10;
       for (int j = 0; j < 9; j++) r[j] = v[j];
       r[9] = ncc_from_moments(v);
+      a.samples_host[ii] = a.samples[ii];  // drr_tiles has finished its counts
     }
     __syncthreads();
   }
@@ -671,10 +673,15 @@ __global__ __launch_bounds__(kBlock) void finalize_items(BatchArgs a)
   }
 }
 
-__global__ void zero_u64(unsigned long long* p, int n)
+// The batch's descriptors from pinned host memory to the device, and the
+// per-item sample counters cleared: one launch where a copy and a fill
+// would each be a stream operation of their own.
+__global__ void stage_batch(const uint4* __restrict__ src, uint4* __restrict__ dst, int n16,
+                            unsigned long long* __restrict__ samples, int n_items)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_items; i += stride) samples[i] = 0;
 }
 
 template <class T>
@@ -712,7 +719,9 @@ struct PinBuf {
     p = nullptr;
     cap = 0;
     const size_t want = std::max(n, (size_t)16);
-    ASC_HIP(hipHostMalloc((void**)&p, want * sizeof(T), hipHostMallocDefault));
+    // coherent (fine-grained): kernels read and write it directly, and no
+    // cache keeps a line of it across batches
+    ASC_HIP(hipHostMalloc((void**)&p, want * sizeof(T), hipHostMallocCoherent));
     cap = want;
     return ASC_OK;
   }
@@ -780,7 +789,6 @@ struct asc_scene_s {
   DevBuf<unsigned char> d_batch;  // header + items
   DevBuf<float> d_drr;
   DevBuf<double> d_partials;
-  DevBuf<double> d_results;
   DevBuf<unsigned long long> d_samples;
   DevBuf<Counters> d_counters;
   DevBuf<float> d_dbg;
@@ -788,7 +796,7 @@ struct asc_scene_s {
   // pinned host buffers of the batch in each of the two submission slots
   PinBuf<unsigned char> h_batch[2];
   PinBuf<double> h_results[2];
-  PinBuf<unsigned long long> h_samples;
+  PinBuf<unsigned long long> h_samples[2];
   // items of the last batch in request order (request i, k-th camera of its
   // set at req_base[i] + k); kernel order is camera-major: korder[logical]
   std::vector<ItemDesc> items;
@@ -954,16 +962,19 @@ int issue_batch(asc_scene s, int slot, size_t batch_bytes, int grid_drr, int gri
   a.weights = s->d_weights.p;
   a.drr = s->d_drr.p;
   a.partials = s->d_partials.p;
-  a.results = s->d_results.p;
+  a.results = s->h_results[slot].p;
   a.samples = s->d_samples.p;
+  a.samples_host = s->h_samples[slot].p;
   a.counters = s->d_counters.p;
   const size_t plane = s->drr_total;
   a.dbg_drr = s->debug ? s->d_dbg.p : nullptr;
   a.dbg_rad = s->debug ? s->d_dbg.p + plane : nullptr;
   a.dbg_mask = s->debug ? s->d_dbg.p + 2 * plane : nullptr;
   a.wave_shift = s->wave_shift;
-  ASC_HIP(hipMemcpyAsync(s->d_batch.p, s->h_batch[slot].p, batch_bytes, hipMemcpyHostToDevice, st));
-  zero_u64<<<(n_items + 255) / 256, 256, 0, st>>>(s->d_samples.p, n_items);
+  // batch_bytes: a multiple of 16 (both buffers hold that much)
+  const int n16 = (int)(batch_bytes / 16);
+  stage_batch<<<std::min((std::max(n16, n_items) + 255) / 256, 64), 256, 0, st>>>(
+    (const uint4*)s->h_batch[slot].p, (uint4*)s->d_batch.p, n16, s->d_samples.p, n_items);
   // the timing events are the scene's: a batch queued behind another does
   // not record them
   const bool timed = s->timing && s->ev[0] && !s->fly[0].busy && !s->fly[1].busy;
@@ -985,10 +996,9 @@ int issue_batch(asc_scene s, int slot, size_t batch_bytes, int grid_drr, int gri
   finalize_items<<<std::min(n_items, 4096), kBlock, 0, st>>>(a);
   if (timed) ASC_HIP(hipEventRecord(s->ev[3], st));
   ASC_HIP(hipGetLastError());
-  // (the per-item sample counts stay on the device until asc_scene_last_info
-  // asks for them: one copy less per batch)
-  ASC_HIP(hipMemcpyAsync(s->h_results[slot].p, s->d_results.p, sizeof(double) * 10 * n_items, hipMemcpyDeviceToHost,
-                         st));
+  // finalize_items wrote the results into the slot's pinned buffer; the
+  // per-item sample counts stay on the device until asc_scene_last_info
+  // asks for them
   return ASC_OK;
 }
 
@@ -1146,28 +1156,29 @@ int submit_requests(asc_scene s, const std::vector<Request>& reqs, int stride, b
     en.total = (int)base;
   }
   // buffers
-  const size_t batch_bytes = sizeof(BatchHeader) + sizeof(ItemDesc) * (size_t)n_items;
-  const void* before[6] = {s->d_batch.p, s->d_drr.p, s->d_partials.p, s->d_results.p, s->d_samples.p,
-                           s->d_dbg.p};
+  // (rounded up to whole 16-byte words for stage_batch)
+  const size_t batch_bytes = (sizeof(BatchHeader) + sizeof(ItemDesc) * (size_t)n_items + 15) & ~(size_t)15;
+  // a recorded graph names these buffers (slot 0's host buffers included)
+  const void* before[8] = {s->d_batch.p, s->d_drr.p, s->d_partials.p, s->h_batch[0].p, s->h_results[0].p,
+                           s->h_samples[0].p, s->d_samples.p, s->d_dbg.p};
   if (other_busy && (batch_bytes > s->d_batch.cap || s->drr_total > s->d_drr.cap ||
-                     std::max<size_t>(part_total, 1) * 9 > s->d_partials.cap ||
-                     (size_t)n_items * 10 > s->d_results.cap || (size_t)n_items > s->d_samples.cap ||
+                     std::max<size_t>(part_total, 1) * 9 > s->d_partials.cap || (size_t)n_items > s->d_samples.cap ||
                      (s->debug && s->drr_total * 3 > s->d_dbg.cap)))
     ASC_HIP(hipStreamSynchronize(st));
   if ((rc = s->d_batch.ensure(batch_bytes))) return rc;
   if ((rc = s->h_batch[slot].ensure(batch_bytes))) return rc;
   if ((rc = s->d_drr.ensure(s->drr_total))) return rc;
   if ((rc = s->d_partials.ensure(std::max<size_t>(part_total, 1) * 9))) return rc;
-  if ((rc = s->d_results.ensure((size_t)n_items * 10))) return rc;
   if ((rc = s->h_results[slot].ensure((size_t)n_items * 10))) return rc;
   if ((rc = s->d_samples.ensure((size_t)n_items))) return rc;
-  if ((rc = s->h_samples.ensure((size_t)n_items))) return rc;
+  if ((rc = s->h_samples[slot].ensure((size_t)n_items))) return rc;
   if (!s->d_counters.p) {
     if ((rc = s->d_counters.ensure(1))) return rc;
     ASC_HIP(hipMemset(s->d_counters.p, 0, sizeof(Counters)));
   }
   if (s->debug && (rc = s->d_dbg.ensure(s->drr_total * 3))) return rc;
-  const void* after[6] = {s->d_batch.p, s->d_drr.p, s->d_partials.p, s->d_results.p, s->d_samples.p, s->d_dbg.p};
+  const void* after[8] = {s->d_batch.p, s->d_drr.p, s->d_partials.p, s->h_batch[0].p, s->h_results[0].p,
+                          s->h_samples[0].p, s->d_samples.p, s->d_dbg.p};
   if (std::memcmp(before, after, sizeof before) != 0) destroy_graph(s);
   if (s->grid_persistent == 0 || s->grid_post == 0) {
     hipDeviceProp_t prop;
@@ -1335,7 +1346,6 @@ ASC_API int asc_scene_destroy(asc_scene s)
   s->d_batch.release();
   s->d_drr.release();
   s->d_partials.release();
-  s->d_results.release();
   s->d_samples.release();
   s->d_counters.release();
   s->d_dbg.release();
@@ -1343,9 +1353,9 @@ ASC_API int asc_scene_destroy(asc_scene s)
   for (int k = 0; k < 2; k++) {
     s->h_batch[k].release();
     s->h_results[k].release();
+    s->h_samples[k].release();
     if (s->fly[k].done) hipEventDestroy(s->fly[k].done);
   }
-  s->h_samples.release();
   delete s;
   return ASC_OK;
 }
@@ -1612,7 +1622,7 @@ ASC_API int asc_scene_scratch_bytes(asc_scene s, size_t* bytes)
 {
   ASC_CHECK(s && bytes, ASC_ERR_INVALID, "bad argument");
   *bytes = s->d_batch.cap + s->d_drr.cap * sizeof(float) + s->d_partials.cap * sizeof(double) +
-           s->d_results.cap * sizeof(double) + s->d_samples.cap * sizeof(unsigned long long) +
+           s->d_samples.cap * sizeof(unsigned long long) +
            s->d_dbg.cap * sizeof(float) + s->p_bufs.cap * sizeof(float);
   return ASC_OK;
 }
@@ -1698,30 +1708,26 @@ ASC_API int asc_scene_last_info(asc_scene s, int* info, int capacity, int64_t* s
 {
   ASC_CHECK(s, ASC_ERR_INVALID, "null scene");
   ASC_CHECK(!s->fly[0].busy && !s->fly[1].busy, ASC_ERR_INVALID, "finish the submitted batches first");
-  // the last batch's counts, still in device memory (the batch has been
-  // waited for; the next one overwrites them). A batch that failed before
-  // its buffers were sized leaves no counts to read.
-  ASC_CHECK((size_t)s->n_items <= s->d_samples.cap && (size_t)s->n_items <= s->h_samples.cap &&
-              (size_t)s->n_items * 10 <= s->h_results[s->last_slot].cap,
+  // the last finished batch's counts and moments, which finalize_items
+  // wrote into its slot's pinned buffers. A batch that failed before its
+  // buffers were sized leaves nothing to read.
+  const int sl = s->last_slot;
+  ASC_CHECK((size_t)s->n_items <= s->h_samples[sl].cap && (size_t)s->n_items * 10 <= s->h_results[sl].cap,
             ASC_ERR_INVALID, "no batch results to report");
-  if (s->n_items > 0) {
-    ASC_HIP(hipStreamSynchronize(s->ctx->stream));
-    ASC_HIP(hipMemcpy(s->h_samples.p, s->d_samples.p, sizeof(unsigned long long) * s->n_items,
-                      hipMemcpyDeviceToHost));
-  }
+  const unsigned long long* cnt = s->h_samples[sl].p;
   int64_t total = 0;
   for (int ii = 0; ii < s->n_items; ii++) {
-    total += (int64_t)s->h_samples.p[s->korder[ii]];
+    const int ki = s->korder[ii];
+    total += (int64_t)cnt[ki];
     if (info && (ii + 1) * 8 <= capacity) {
       int* o = info + 8 * ii;
       const ItemDesc& it = s->items[ii];
-      const int ki = s->korder[ii];
       o[0] = s->in_view[ii];
       o[1] = it.w;
       o[2] = it.h;
-      o[3] = (int)s->h_results[s->last_slot].p[(size_t)ki * 10 + 3];
+      o[3] = (int)s->h_results[sl].p[(size_t)ki * 10 + 3];
       o[4] = it.tiles_x * it.tiles_y;
-      o[5] = (int)s->h_samples.p[ki];
+      o[5] = (int)cnt[ki];
       o[6] = o[7] = 0;
     }
   }

@@ -279,35 +279,33 @@ def serve_bench(A, synth, ctx, trials, loads, jobs_per_rate: int, max_frames: in
 
 def time_fused(ctx, scene, batches, bone_of, ncams, args):
     """Warm up, then time args.steps fused batches (pose upload, the three
-    launches, result readback). Returns (pose-evals/s, ms per step, summed
-    per-kernel ms over the timed steps, summed ray samples)."""
+    launches, result readback), nothing else in the timed loop. Returns
+    (pose-evals/s, ms per step, summed per-kernel ms and summed ray samples
+    over an eager pass of the same batches with the kernel timers on)."""
     P = len(bone_of)
     nb = args.warmup + args.steps
-    scene.set_timing(not args.graph)
+    scene.set_timing(False)
     scene.use_graph(args.graph)
     log(f"[bench] warmup {args.warmup} x {P} poses")
     for k in range(args.warmup):
         scene.eval_poses(batches[k], volumes=bone_of)
     ctx.synchronize()
-    kern_ms = np.zeros(3)
-    samples = 0
     log(f"[bench] timing {args.steps} steps")
     t0 = time.perf_counter()
     for k in range(args.warmup, nb):
         scene.eval_poses(batches[k], volumes=bone_of)
-        if not args.graph:
-            kern_ms += scene.last_timing()
-            samples += scene.last_info(P * ncams)[1]
     ctx.synchronize()
     el = time.perf_counter() - t0
-    if args.graph:
-        # per-kernel time from an eager pass over the same batches
-        scene.use_graph(False)
-        scene.set_timing(True)
-        for k in range(args.warmup, nb):
-            scene.eval_poses(batches[k], volumes=bone_of)
-            kern_ms += scene.last_timing()
-            samples += scene.last_info(P * ncams)[1]
+    # per-kernel times (HIP events around each launch) and ray samples
+    kern_ms = np.zeros(3)
+    samples = 0
+    scene.use_graph(False)
+    scene.set_timing(True)
+    for k in range(args.warmup, nb):
+        scene.eval_poses(batches[k], volumes=bone_of)
+        kern_ms += scene.last_timing()
+        samples += scene.last_info(P * ncams)[1]
+    scene.set_timing(False)
     return P * args.steps / el, 1e3 * el / args.steps, kern_ms, samples
 
 
@@ -773,8 +771,7 @@ def main(argv=None) -> int:
     evals_s, ms_step, kern_ms, samples = time_fused(ctx, scene, batches, bone_of, len(cams), args)
     graph_replay = None
     if not args.graph:
-        # the same steps replayed from a hipGraph per batch (kernel times above
-        # come from the eager run, whose HIP events sit in its timed region)
+        # the same steps replayed from a hipGraph per batch
         g_args = argparse.Namespace(**{**vars(args), "graph": True})
         g_s, g_ms, _, _ = time_fused(ctx, scene, batches, bone_of, len(cams), g_args)
         scene.use_graph(False)
