@@ -825,6 +825,8 @@ struct asc_scene_s {
     hipEvent_t done = nullptr;
   } fly[2];
   int last_slot = 0;  // slot of the last finished batch (its h_results)
+  std::vector<CameraView> cam_views;  // per batch: camera_view of each camera in it
+  std::vector<char> cam_view_ok;
   int debug = 0;
   // graph
   bool use_graph = false;
@@ -1068,9 +1070,15 @@ int submit_requests(asc_scene s, const std::vector<Request>& reqs, int stride, b
   s->n_items = n_items;
   s->items.resize(n_items);
   s->in_view.assign(n_items, 0);
+  // each camera's inverse frame and film-plane bounds once per batch
+  std::vector<CameraView>& views = s->cam_views;
+  std::vector<char>& have_view = s->cam_view_ok;
+  views.resize(s->cams.size());
+  have_view.assign(s->cams.size(), 0);
   for (int i = 0; i < n; i++) {
     const Request& q = reqs[i];
     const asc_volume_desc& d = s->vols[q.vol].vol->desc;
+    const Frame pose = frame_from_xyzypr(q.pose);
     const std::vector<int>& cl = q.set < 0 ? all : s->sets[q.set].cams;
     const int RW = q.set < 0 ? s->RW : s->sets[q.set].RW, RH = q.set < 0 ? s->RH : s->sets[q.set].RH;
     for (int k = 0; k < s->req_ncam[i]; k++) {
@@ -1083,8 +1091,11 @@ int submit_requests(asc_scene s, const std::vector<Request>& reqs, int stride, b
       it.rad = rad->dev;
       double vp[4];
       unsigned sz[2] = {0, 0};
-      const int inview =
-        view_geometry(s->cams[c].cam, q.pose, d.inv_scale, d.inv_trans, RW, RH, it.imv, vp, sz);
+      if (!have_view[c]) {
+        views[c] = camera_view(s->cams[c].cam);
+        have_view[c] = 1;
+      }
+      const int inview = view_geometry(s->cams[c].cam, views[c], pose, d.inv_scale, d.inv_trans, RW, RH, it.imv, vp, sz);
       it.vol = q.vol;
       it.cam = c;
       it.axis = s->vols[q.vol].vol->dev.layout == ASC_LAYOUT_BRICK ? ray_axis(s->vols[q.vol].vol->dev, it.imv) : 0;

@@ -165,11 +165,32 @@ void inv_model_view(const double* m, const float* inv_scale, const float* inv_tr
   for (int col = 0; col < 4; col++) out[12 + col] = (float)m[12 + col];
 }
 
+// The camera's side of Tracker::calculate_viewport (Tracker.cpp:923-1032):
+// its inverse frame and the radiograph's film-plane (z = -2) bounds, the
+// projections of image-plane corners 1 and 3. They depend on the camera
+// alone, so a batch computes them once per camera.
+CameraView camera_view(const asc_camera& cam)
+{
+  Frame cf;
+  std::memcpy(cf.r, cam.rotation, sizeof cf.r);
+  std::memcpy(cf.t, cam.translation, sizeof cf.t);
+  CameraView cv;
+  cv.inv = frame_inverse(cf);
+  double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+  frame_point(cv.inv, cam.image_plane + 3, a);
+  frame_point(cv.inv, cam.image_plane + 9, b);
+  cv.r[0] = -2 * a[0] / a[2];
+  cv.r[1] = -2 * a[1] / a[2];
+  cv.r[2] = -2 * b[0] / b[2];
+  cv.r[3] = -2 * b[1] / b[2];
+  return cv;
+}
+
 // Tracker::calculate_viewport, Tracker.cpp:923-1032: film-plane (z = -2)
 // bounding box of the projected unit box, clipped to the radiograph.
 // Note the box starts at {1, 1, -1, -1}, as in the reference.
-bool calc_viewport(const Frame& mv, const asc_camera& cam, const float* inv_trans, const float* inv_scale,
-                   double* vp)
+static bool clip_viewport(const Frame& mv, const CameraView& cv, const float* inv_trans, const float* inv_scale,
+                          double* vp)
 {
   static const double unit_corners[24] = {0, 0, -1, 0, 0, 0, 0, 1, -1, 0, 1, 0,
                                           1, 0, -1, 1, 0, 0, 1, 1, -1, 1, 1, 0};
@@ -185,15 +206,7 @@ bool calc_viewport(const Frame& mv, const asc_camera& cam, const float* inv_tran
     if (hi_x < fx) hi_x = fx;
     if (hi_y < fy) hi_y = fy;
   }
-  Frame cinv;
-  std::memcpy(cinv.r, cam.rotation, sizeof cinv.r);
-  std::memcpy(cinv.t, cam.translation, sizeof cinv.t);
-  cinv = frame_inverse(cinv);
-  double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
-  frame_point(cinv, cam.image_plane + 3, a);
-  frame_point(cinv, cam.image_plane + 9, b);
-  const double r0 = -2 * a[0] / a[2], r1 = -2 * a[1] / a[2];
-  const double r2 = -2 * b[0] / b[2], r3 = -2 * b[1] / b[2];
+  const double r0 = cv.r[0], r1 = cv.r[1], r2 = cv.r[2], r3 = cv.r[3];
   if (lo_x > r2 && hi_x > r2) return false;
   if (lo_y > r3 && hi_y > r3) return false;
   if (lo_x < r0 && hi_x < r0) return false;
@@ -210,23 +223,34 @@ bool calc_viewport(const Frame& mv, const asc_camera& cam, const float* inv_tran
   return true;
 }
 
-// Tracker.cpp:604-629 for one camera.
-int view_geometry(const asc_camera& cam, const double* xyzypr, const float* inv_scale, const float* inv_trans,
-                  int render_w, int render_h, float* imv12, double* vp4, unsigned* size2)
+bool calc_viewport(const Frame& mv, const asc_camera& cam, const float* inv_trans, const float* inv_scale,
+                   double* vp)
 {
-  Frame cf;
-  std::memcpy(cf.r, cam.rotation, sizeof cf.r);
-  std::memcpy(cf.t, cam.translation, sizeof cf.t);
-  const Frame mv = frame_mul(frame_inverse(cf), frame_from_xyzypr(xyzypr));
+  return clip_viewport(mv, camera_view(cam), inv_trans, inv_scale, vp);
+}
+
+// Tracker.cpp:604-629 for one camera, from the camera's view and the
+// pose's frame (frame_from_xyzypr).
+int view_geometry(const asc_camera& cam, const CameraView& cv, const Frame& pose, const float* inv_scale,
+                  const float* inv_trans, int render_w, int render_h, float* imv12, double* vp4, unsigned* size2)
+{
+  const Frame mv = frame_mul(cv.inv, pose);
   double m16[16];
   frame_row_order(frame_inverse(mv), m16);
   float imv16[16];
   inv_model_view(m16, inv_scale, inv_trans, imv16);
   std::memcpy(imv12, imv16, 12 * sizeof(float));
-  if (!calc_viewport(mv, cam, inv_trans, inv_scale, vp4)) return 0;
+  if (!clip_viewport(mv, cv, inv_trans, inv_scale, vp4)) return 0;
   size2[0] = (unsigned)(vp4[2] * render_w / cam.viewport[2]);
   size2[1] = (unsigned)(vp4[3] * render_h / cam.viewport[3]);
   return 1;
+}
+
+int view_geometry(const asc_camera& cam, const double* xyzypr, const float* inv_scale, const float* inv_trans,
+                  int render_w, int render_h, float* imv12, double* vp4, unsigned* size2)
+{
+  return view_geometry(cam, camera_view(cam), frame_from_xyzypr(xyzypr), inv_scale, inv_trans, render_w, render_h,
+                       imv12, vp4, size2);
 }
 
 // Quat<float>(yaw, pitch, roll), Quaternion.hpp:80-94.

@@ -40,6 +40,17 @@ bool calc_viewport(const Frame& modelview, const asc_camera& cam, const float* i
 // Full per-camera geometry of one pose: returns 1 in view, 0 not in view.
 int view_geometry(const asc_camera& cam, const double* xyzypr, const float* inv_scale, const float* inv_trans,
                   int render_w, int render_h, float* imv12, double* vp4, unsigned* size2);
+// The camera's inverse frame and the radiograph's film-plane bounds
+// {left, bottom, right, top}, which view_geometry derives per camera.
+struct CameraView {
+  Frame inv;
+  double r[4];
+};
+CameraView camera_view(const asc_camera& cam);
+// view_geometry from camera_view(cam) and frame_from_xyzypr(xyzypr), for
+// callers that reuse them across cameras and poses (same results).
+int view_geometry(const asc_camera& cam, const CameraView& cv, const Frame& pose, const float* inv_scale,
+                  const float* inv_trans, int render_w, int render_h, float* imv12, double* vp4, unsigned* size2);
 
 struct PoseStore {
   float x, y, z;
