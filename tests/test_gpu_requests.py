@@ -180,3 +180,37 @@ def test_submitted_batches_equal_blocking_batches(asc, ctx, tiny, jobs):
         s.close()
     for v in keep + keep_r:
         v.close()
+
+
+def test_batches_with_no_item_in_view(asc, ctx, tiny, jobs):
+    """Edge cases of the fused launch: a batch whose poses are all out of
+    view (no tile to march or filter; each camera's value is the 9999
+    sentinel of Tracker.cpp:614-618, summed over the set's two cameras as
+    Tracker.cpp:814-826 sums them), blocking and submitted, then a batch of
+    one pose, each followed by a normal batch whose costs equal those of a
+    fresh scene (the work counters and slots are left clean)."""
+    sc, keep, (vA, vB), (sA, sB), (fA, fB), _ = build(asc, ctx, tiny, jobs)
+    ref, keep_r, *_ = build(asc, ctx, tiny, jobs)
+    wl = tiny["wl"]
+    rng = np.random.default_rng(5)
+    far = wl.gt_poses[0] + np.array([900.0, 900.0, 900.0, 0, 0, 0])
+    normal = wl.gt_poses[0] + np.concatenate([rng.uniform(-2, 2, (6, 3)), rng.uniform(-4, 4, (6, 3))], 1)
+    vols, sets, frames = np.full(6, vA), np.full(6, sA), np.full(6, fA[0])
+    want = ref.eval_requests(vols, sets, frames, normal)
+    costs, per = sc.eval_requests(vols[:3], sets[:3], frames[:3], np.repeat(far[None], 3, 0))
+    assert (costs == 2 * 9999.0).all() and (per[:, :2] == 9999.0).all()
+    assert sc.last_info(6)[1] == 0  # no ray sample
+    got = sc.eval_requests(vols, sets, frames, normal)
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    t = sc.submit_requests(vols[:2], sets[:2], frames[:2], np.repeat(far[None], 2, 0))
+    t2 = sc.submit_requests(vols, sets, frames, normal)
+    c_far, _ = sc.finish_requests(t)
+    got = sc.finish_requests(t2)
+    assert (c_far == 2 * 9999.0).all()
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    one = sc.eval_requests(vols[:1], sets[:1], frames[:1], normal[:1])
+    assert one[0][0] == want[0][0] and np.array_equal(one[1][0], want[1][0])
+    for s in (sc, ref):
+        s.close()
+    for v in keep + keep_r:
+        v.close()
