@@ -812,8 +812,9 @@ struct asc_scene_s {
   size_t drr_total = 0;  // floats of DRR scratch used by the last batch
   // Batches submitted and not yet finished (asc_scene_submit_requests): the
   // host side of each (the fields above, moved here at submission and back
-  // at finish) and an event after its result copy. The device buffers are
-  // shared: the batches run one after another on the scene's stream.
+  // at finish) and an event after its last kernel, finalize_items, which
+  // writes the slot's pinned results. The device buffers are shared: the
+  // batches run one after another on the scene's stream.
   struct InFlight {
     bool busy = false;
     int n = 0, stride = 0;
@@ -1003,9 +1004,8 @@ int issue_batch(asc_scene s, int slot, size_t batch_bytes, int grid_drr, int gri
   finalize_items<<<std::min(n_items, 4096), kBlock, 0, st>>>(a);
   if (timed) ASC_HIP(hipEventRecord(s->ev[3], st));
   ASC_HIP(hipGetLastError());
-  // finalize_items wrote the results into the slot's pinned buffer; the
-  // per-item sample counts stay on the device until asc_scene_last_info
-  // asks for them
+  // finalize_items writes the results and the per-item sample counts into
+  // the slot's pinned buffers: no copy follows
   return ASC_OK;
 }
 
