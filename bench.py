@@ -309,6 +309,28 @@ def time_fused(ctx, scene, batches, bone_of, ncams, args):
     return P * args.steps / el, 1e3 * el / args.steps, kern_ms, samples
 
 
+def time_two_in_flight(ctx, scene, batches, bone_of, args):
+    """The same batches with two in flight (asc_scene_submit_requests /
+    _finish_requests, as the registration server runs its cohorts): the host
+    prepares batch k+1 while batch k runs. -> (pose-evals/s, ms per step)."""
+    P = len(bone_of)
+    nb = args.warmup + args.steps
+    scene.use_graph(False)
+    scene.set_timing(False)
+    sets = np.full(P, -1, np.int32)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    pending = None
+    for k in range(args.warmup, nb):
+        ticket = scene.submit_requests(bone_of, sets, None, batches[k])
+        if pending is not None:
+            scene.finish_requests(pending)
+        pending = ticket
+    scene.finish_requests(pending)
+    el = time.perf_counter() - t0
+    return P * args.steps / el, 1e3 * el / args.steps
+
+
 def storage_variant(A, synth, ctx, args, batches, bone_of, storage):
     """The same workload and poses with another volume storage: pose-evals/s
     and the ray-march kernel's time (its error against the all-double oracle
@@ -769,7 +791,7 @@ def main(argv=None) -> int:
     if args.grid:
         scene.set_tuning(scene.TUNE_GRID, args.grid)
     evals_s, ms_step, kern_ms, samples = time_fused(ctx, scene, batches, bone_of, len(cams), args)
-    graph_replay = None
+    graph_replay = two_in_flight = None
     if not args.graph:
         # the same steps replayed from a hipGraph per batch
         g_args = argparse.Namespace(**{**vars(args), "graph": True})
@@ -778,6 +800,10 @@ def main(argv=None) -> int:
         scene.set_timing(True)
         graph_replay = {"value": g_s, "unit": "pose-evals/s", "ms_per_step": g_ms}
         log(f"[bench] hipGraph replay {g_s:.1f} evals/s")
+        p_s, p_ms = time_two_in_flight(ctx, scene, batches, bone_of, args)
+        two_in_flight = {"value": p_s, "unit": "pose-evals/s", "ms_per_step": p_ms}
+        log(f"[bench] two batches in flight {p_s:.1f} evals/s")
+        scene.set_timing(True)
     info, last_samples = scene.last_info(P * len(cams))
     drr_ms = kern_ms[0] / args.steps
     bpv = BYTES_PER_VOXEL[args.storage]
@@ -853,6 +879,7 @@ def main(argv=None) -> int:
                       "finalize_items": kern_ms[2] / args.steps},
         "cpu_baseline": cpu,
         "graph_replay": graph_replay,
+        "two_in_flight": two_in_flight,
         "plain_per_eval": {"value": plain_s, "unit": "pose-evals/s",
                            "speedup_of_fused": evals_s / plain_s if plain_s > 0 else None},
         "plain_facade": None if facade is None else {
