@@ -828,6 +828,8 @@ struct asc_scene_s {
   int last_slot = 0;  // slot of the last finished batch (its h_results)
   std::vector<CameraView> cam_views;  // per batch: camera_view of each camera in it
   std::vector<char> cam_view_ok;
+  std::vector<BoxCorners> vol_boxes;  // per batch: box_corners of each volume in it
+  std::vector<char> vol_box_ok;
   int debug = 0;
   // graph
   bool use_graph = false;
@@ -1075,6 +1077,11 @@ int submit_requests(asc_scene s, const std::vector<Request>& reqs, int stride, b
   std::vector<char>& have_view = s->cam_view_ok;
   views.resize(s->cams.size());
   have_view.assign(s->cams.size(), 0);
+  // and each volume's box corners
+  std::vector<BoxCorners>& boxes = s->vol_boxes;
+  std::vector<char>& have_box = s->vol_box_ok;
+  boxes.resize(s->vols.size());
+  have_box.assign(s->vols.size(), 0);
   for (int i = 0; i < n; i++) {
     const Request& q = reqs[i];
     const asc_volume_desc& d = s->vols[q.vol].vol->desc;
@@ -1095,7 +1102,12 @@ int submit_requests(asc_scene s, const std::vector<Request>& reqs, int stride, b
         views[c] = camera_view(s->cams[c].cam);
         have_view[c] = 1;
       }
-      const int inview = view_geometry(s->cams[c].cam, views[c], pose, d.inv_scale, d.inv_trans, RW, RH, it.imv, vp, sz);
+      if (!have_box[q.vol]) {
+        boxes[q.vol] = box_corners(d.inv_trans, d.inv_scale);
+        have_box[q.vol] = 1;
+      }
+      const int inview =
+        view_geometry(s->cams[c].cam, views[c], pose, boxes[q.vol], d.inv_scale, d.inv_trans, RW, RH, it.imv, vp, sz);
       it.vol = q.vol;
       it.cam = c;
       it.axis = s->vols[q.vol].vol->dev.layout == ASC_LAYOUT_BRICK ? ray_axis(s->vols[q.vol].vol->dev, it.imv) : 0;

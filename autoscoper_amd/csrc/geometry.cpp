@@ -186,19 +186,27 @@ CameraView camera_view(const asc_camera& cam)
   return cv;
 }
 
-// Tracker::calculate_viewport, Tracker.cpp:923-1032: film-plane (z = -2)
-// bounding box of the projected unit box, clipped to the radiograph.
-// Note the box starts at {1, 1, -1, -1}, as in the reference.
-static bool clip_viewport(const Frame& mv, const CameraView& cv, const float* inv_trans, const float* inv_scale,
-                          double* vp)
+// The volume's side of Tracker::calculate_viewport: the 8 corners of its
+// box in volume coordinates (the unit box undone by inv_trans, inv_scale).
+BoxCorners box_corners(const float* inv_trans, const float* inv_scale)
 {
   static const double unit_corners[24] = {0, 0, -1, 0, 0, 0, 0, 1, -1, 0, 1, 0,
                                           1, 0, -1, 1, 0, 0, 1, 1, -1, 1, 1, 0};
+  BoxCorners b;
+  for (int j = 0; j < 8; j++)
+    for (int a = 0; a < 3; a++) b.p[3 * j + a] = (unit_corners[3 * j + a] - inv_trans[a]) / inv_scale[a];
+  return b;
+}
+
+// Tracker::calculate_viewport, Tracker.cpp:923-1032: film-plane (z = -2)
+// bounding box of the projected unit box, clipped to the radiograph.
+// Note the box starts at {1, 1, -1, -1}, as in the reference.
+static bool clip_viewport(const Frame& mv, const CameraView& cv, const BoxCorners& box, double* vp)
+{
   double lo_x = 1.0, lo_y = 1.0, hi_x = -1.0, hi_y = -1.0;
   for (int j = 0; j < 8; j++) {
-    double corner[3], q[3];
-    for (int a = 0; a < 3; a++) corner[a] = (unit_corners[3 * j + a] - inv_trans[a]) / inv_scale[a];
-    frame_point(mv, corner, q);
+    double q[3];
+    frame_point(mv, box.p + 3 * j, q);
     const double fx = -2 * q[0] / q[2];
     const double fy = -2 * q[1] / q[2];
     if (lo_x > fx) lo_x = fx;
@@ -226,13 +234,14 @@ static bool clip_viewport(const Frame& mv, const CameraView& cv, const float* in
 bool calc_viewport(const Frame& mv, const asc_camera& cam, const float* inv_trans, const float* inv_scale,
                    double* vp)
 {
-  return clip_viewport(mv, camera_view(cam), inv_trans, inv_scale, vp);
+  return clip_viewport(mv, camera_view(cam), box_corners(inv_trans, inv_scale), vp);
 }
 
 // Tracker.cpp:604-629 for one camera, from the camera's view and the
 // pose's frame (frame_from_xyzypr).
-int view_geometry(const asc_camera& cam, const CameraView& cv, const Frame& pose, const float* inv_scale,
-                  const float* inv_trans, int render_w, int render_h, float* imv12, double* vp4, unsigned* size2)
+int view_geometry(const asc_camera& cam, const CameraView& cv, const Frame& pose, const BoxCorners& box,
+                  const float* inv_scale, const float* inv_trans, int render_w, int render_h, float* imv12,
+                  double* vp4, unsigned* size2)
 {
   const Frame mv = frame_mul(cv.inv, pose);
   double m16[16];
@@ -240,7 +249,7 @@ int view_geometry(const asc_camera& cam, const CameraView& cv, const Frame& pose
   float imv16[16];
   inv_model_view(m16, inv_scale, inv_trans, imv16);
   std::memcpy(imv12, imv16, 12 * sizeof(float));
-  if (!clip_viewport(mv, cv, inv_trans, inv_scale, vp4)) return 0;
+  if (!clip_viewport(mv, cv, box, vp4)) return 0;
   size2[0] = (unsigned)(vp4[2] * render_w / cam.viewport[2]);
   size2[1] = (unsigned)(vp4[3] * render_h / cam.viewport[3]);
   return 1;
@@ -249,8 +258,8 @@ int view_geometry(const asc_camera& cam, const CameraView& cv, const Frame& pose
 int view_geometry(const asc_camera& cam, const double* xyzypr, const float* inv_scale, const float* inv_trans,
                   int render_w, int render_h, float* imv12, double* vp4, unsigned* size2)
 {
-  return view_geometry(cam, camera_view(cam), frame_from_xyzypr(xyzypr), inv_scale, inv_trans, render_w, render_h,
-                       imv12, vp4, size2);
+  return view_geometry(cam, camera_view(cam), frame_from_xyzypr(xyzypr), box_corners(inv_trans, inv_scale),
+                       inv_scale, inv_trans, render_w, render_h, imv12, vp4, size2);
 }
 
 // Quat<float>(yaw, pitch, roll), Quaternion.hpp:80-94.

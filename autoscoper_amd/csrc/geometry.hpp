@@ -47,10 +47,18 @@ struct CameraView {
   double r[4];
 };
 CameraView camera_view(const asc_camera& cam);
-// view_geometry from camera_view(cam) and frame_from_xyzypr(xyzypr), for
-// callers that reuse them across cameras and poses (same results).
-int view_geometry(const asc_camera& cam, const CameraView& cv, const Frame& pose, const float* inv_scale,
-                  const float* inv_trans, int render_w, int render_h, float* imv12, double* vp4, unsigned* size2);
+// The 8 corners of a volume's box in volume coordinates, which
+// view_geometry derives per volume from inv_trans and inv_scale.
+struct BoxCorners {
+  double p[24];
+};
+BoxCorners box_corners(const float* inv_trans, const float* inv_scale);
+// view_geometry from camera_view(cam), frame_from_xyzypr(xyzypr) and
+// box_corners(inv_trans, inv_scale), for callers that reuse them across
+// cameras, poses and volumes (same results).
+int view_geometry(const asc_camera& cam, const CameraView& cv, const Frame& pose, const BoxCorners& box,
+                  const float* inv_scale, const float* inv_trans, int render_w, int render_h, float* imv12,
+                  double* vp4, unsigned* size2);
 
 struct PoseStore {
   float x, y, z;
