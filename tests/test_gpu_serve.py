@@ -92,6 +92,27 @@ def test_job_alone_equals_job_in_the_mix(asc, ctx, tiny, trial, pipeline):
     assert st["requests"] > st["batches"]  # requests of different jobs shared launches
 
 
+def test_pipeline_depth_changes_while_jobs_run(asc, ctx, tiny, trial):
+    """asc_server_set_pipeline between 1 and 2 while jobs run, with two
+    launches possibly in flight at the switch: every job finishes, with the
+    same poses as alone."""
+    import time
+
+    srv = asc.Server(ctx, 8 << 30, 8, pipeline=1)
+    alone, _, _ = srv.wait(submit_tiny(asc, srv, tiny, trial))
+    srv.close()
+    srv = asc.Server(ctx, 8 << 30, 8, pipeline=2)
+    ids = [submit_tiny(asc, srv, tiny, trial), submit_bones(asc, srv), submit_tiny(asc, srv, tiny, trial)]
+    for depth in (1, 2, 1, 2):
+        time.sleep(0.005)
+        srv.set_pipeline(depth)
+    res = [srv.wait(i) for i in ids]
+    with pytest.raises(asc.AscError):
+        srv.set_pipeline(3)
+    srv.close()
+    assert np.array_equal(res[0][0], alone) and np.array_equal(res[2][0], alone)
+
+
 def test_frame_loop_matches_oracle(asc, ctx, orc, tiny, trial):
     srv = asc.Server(ctx, 8 << 30, 4)
     poses, lat, st = srv.wait(submit_tiny(asc, srv, tiny, trial))
