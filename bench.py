@@ -684,7 +684,8 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=30,
+                    help="untimed steps before the timed ones (the host side of a step settles over the first few dozen)")
     ap.add_argument("--particles", type=int, default=100)
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--storage", default="u16", choices=sorted(STORAGES))
