@@ -823,8 +823,12 @@ struct asc_scene_s {
     int n_items = 0;
     size_t drr_total = 0;
     bool timed = false;
+    unsigned long long seq = 0;  // submission number
     hipEvent_t done = nullptr;
   } fly[2];
+  // submission numbers of the last batch submitted and of the last one
+  // finished: the debug images on the device are the last submitted batch's
+  unsigned long long submitted_seq = 0, finished_seq = 0;
   int last_slot = 0;  // slot of the last finished batch (its h_results)
   std::vector<CameraView> cam_views;  // per batch: camera_view of each camera in it
   std::vector<char> cam_view_ok;
@@ -1262,6 +1266,7 @@ int submit_requests(asc_scene s, const std::vector<Request>& reqs, int stride, b
   f.n = (int)reqs.size();
   f.stride = stride;
   f.timed = !other_busy;  // issue_batch recorded the timing events
+  f.seq = ++s->submitted_seq;
   std::swap(f.items, s->items);
   std::swap(f.in_view, s->in_view);
   std::swap(f.korder, s->korder);
@@ -1291,6 +1296,7 @@ int finish_requests(asc_scene s, int slot, double* costs, double* per_camera)
   s->n_items = f.n_items;
   s->drr_total = f.drr_total;
   s->last_slot = slot;
+  s->finished_seq = f.seq;
   if (s->timing && s->ev[0] && !s->use_graph && f.timed)
     for (int k = 0; k < 3; k++) ASC_HIP(hipEventElapsedTime(&s->last_ms[k], s->ev[k], s->ev[k + 1]));
   const int n = f.n, stride = f.stride;
@@ -1777,6 +1783,8 @@ ASC_API int asc_scene_last_images(asc_scene s, int pose, int camera, float* drr,
 {
   ASC_CHECK(s && s->debug, ASC_ERR_INVALID, "enable debug images with asc_scene_set_debug first");
   ASC_CHECK(!s->fly[0].busy && !s->fly[1].busy, ASC_ERR_INVALID, "finish the submitted batches first");
+  ASC_CHECK(s->finished_seq == s->submitted_seq, ASC_ERR_INVALID,
+            "the debug images are those of a later batch than the last one finished");
   ASC_CHECK(pose >= 0 && pose < (int)s->req_base.size() && camera >= 0 && camera < s->req_ncam[pose], ASC_ERR_INVALID,
             "bad item");
   const int ii = s->req_base[pose] + camera;

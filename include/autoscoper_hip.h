@@ -385,7 +385,9 @@ ASC_API int asc_scene_last_timing(asc_scene scene, float ms[3]);
 /* moments[n*ncam*10]: the nine NCC moments (asc_ncc order) + NCC per item. */
 ASC_API int asc_scene_last_moments(asc_scene scene, double* moments, int capacity);
 /* Copy the last batch's filtered, masked DRR and radiograph and the mask of
- * (pose i, camera c) to host buffers of w*h floats (any may be NULL). */
+ * (pose i, camera c) to host buffers of w*h floats (any may be NULL). With
+ * submitted batches, only when the batch finished last was also submitted
+ * last (the device holds the images of the last one submitted). */
 ASC_API int asc_scene_last_images(asc_scene scene, int pose, int camera, float* drr, float* rad, float* mask);
 /* Plain form: one pose evaluated camera by camera with one launch per
  * operator and a blocking NCC readback per camera, as trackFrame does. */

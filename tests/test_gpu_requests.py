@@ -214,3 +214,36 @@ def test_batches_with_no_item_in_view(asc, ctx, tiny, jobs):
         s.close()
     for v in keep + keep_r:
         v.close()
+
+
+def test_debug_images_follow_the_last_batch(asc, ctx, tiny, jobs):
+    """asc_scene_last_images after submitted batches: the images of the
+    batch finished last when it was also submitted last (the same as from a
+    blocking batch of its own), refused when an earlier batch finished last
+    (the device then holds a later batch's images)."""
+    sc, keep, (vA, vB), (sA, sB), (fA, fB), _ = build(asc, ctx, tiny, jobs)
+    wl = tiny["wl"]
+    rng = np.random.default_rng(8)
+    pa = wl.gt_poses[0] + np.concatenate([rng.uniform(-1, 1, (2, 3)), rng.uniform(-2, 2, (2, 3))], 1)
+    pb = wl.gt_poses[0] + np.concatenate([rng.uniform(-1, 1, (3, 3)), rng.uniform(-2, 2, (3, 3))], 1)
+    sc.set_debug(True)
+    sc.eval_requests([vA] * 3, [sA] * 3, [fA[0]] * 3, pb)
+    info, _ = sc.last_info(6)
+    w, h = int(info[0, 1]), int(info[0, 2])
+    want = sc.last_images(0, 0, w, h)
+    ta = sc.submit_requests([vA] * 2, [sA] * 2, [fA[1]] * 2, pa)
+    tb = sc.submit_requests([vA] * 3, [sA] * 3, [fA[0]] * 3, pb)
+    sc.finish_requests(ta)
+    sc.finish_requests(tb)  # submitted last, finished last
+    got = sc.last_images(0, 0, w, h)
+    assert all(np.array_equal(g, x) for g, x in zip(got, want))
+    ta = sc.submit_requests([vA] * 2, [sA] * 2, [fA[1]] * 2, pa)
+    tb = sc.submit_requests([vA] * 3, [sA] * 3, [fA[0]] * 3, pb)
+    sc.finish_requests(tb)
+    sc.finish_requests(ta)  # finished last, but b's images are on the device
+    with pytest.raises(asc.AscError):
+        sc.last_images(0, 0, w, h)
+    sc.set_debug(False)
+    sc.close()
+    for v in keep:
+        v.close()
