@@ -1556,6 +1556,8 @@ ASC_API int asc_scene_autotune(asc_scene s, int n, const int* volumes, const dou
                                asc_tune_result* result)
 {
   ASC_CHECK(s && poses && n > 0 && reps > 0, ASC_ERR_INVALID, "bad argument");
+  // the candidates' HIP-event timings need the stream to themselves
+  ASC_CHECK(!s->fly[0].busy && !s->fly[1].busy, ASC_ERR_INVALID, "finish the submitted batches first");
   const bool timing0 = s->timing, graph0 = s->use_graph;
   if (!s->ev[0])
     for (hipEvent_t& e : s->ev) ASC_HIP(hipEventCreate(&e));
