@@ -330,9 +330,10 @@ ASC_API int asc_scene_eval_requests(asc_scene scene, int n, const int* volumes, 
 ASC_API int asc_scene_submit_requests(asc_scene scene, int n, const int* volumes, const int* sets,
                                       const int* frames, const double* xyzypr, int* ticket);
 ASC_API int asc_scene_finish_requests(asc_scene scene, int ticket, double* costs, double* per_camera);
-/* Blocks until the submitted batch's launches are done. It touches no other
- * scene state, so a thread may wait here while another thread submits (the
- * scene's other calls are not thread-safe). */
+/* Blocks until the submitted batch's launches are done (no counterpart in
+ * the reference). It touches no other scene state, so a thread may wait here
+ * while another thread submits (the scene's other calls are not
+ * thread-safe). */
 ASC_API int asc_scene_wait_requests(asc_scene scene, int ticket);
 /* Device bytes the scene holds for batch scratch (DRR images, moment
  * partials, results, descriptors); they grow with the largest batch and are
@@ -526,7 +527,8 @@ typedef struct {
 } asc_serve_stats;
 
 ASC_API int asc_server_create(asc_context ctx, size_t hbm_budget, int max_active_jobs, asc_server* out);
-/* Launches the batcher keeps in flight: 2 (default) queues the next fused
+/* Launches the batcher keeps in flight (no counterpart in the reference,
+ * which evaluates one pose at a time): 2 (default) queues the next fused
  * launch while one runs. With nothing in flight it starts a launch once half
  * of the running bones and streams have a request pending, if those hold at
  * least 32 poses; the rest start the next launch while that one runs. The
